@@ -1,0 +1,177 @@
+/*
+ * xtfem.h -- C ABI of the MI355X space-time solve path (libxtfem_hip.so).
+ *
+ * Two groups of entry points:
+ *
+ * 1. Drop-in replacements for the reference subroutines on the path.  Same
+ *    argument meaning and order as the Fortran routines (implicit interfaces,
+ *    so Fortran passes every scalar by reference; these C entry points take
+ *    scalars by value and a thin iso_c_binding module adapts them), 1-based
+ *    CSR/MSR arrays, column-major q x q temporal matrices and temporal-major
+ *    space-time vectors x[j*p + dof].  Arrays are host memory; the work runs
+ *    on the current HIP device.  They return 0 on success and a negative
+ *    XT_E* code on a library failure; the reference's own ierr conventions
+ *    are reported through the ierr output arguments where the reference has
+ *    one.
+ *
+ * 2. Device-resident handle API (operators, preconditioners, batched slab
+ *    solves, damage state) for callers that keep data in HBM across slabs.
+ *    Vectors there use the interleaved layout X[dof][r*q + j] (all q*R
+ *    columns of one spatial DOF contiguous; r = right-hand side, j = temporal
+ *    function) and are device pointers.
+ *
+ * No function throws; no function frees caller memory.
+ */
+#ifndef XTFEM_H
+#define XTFEM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define XT_OK 0
+#define XT_E_ARG -1       /* invalid argument (sizes, null pointers)            */
+#define XT_E_HIP -2       /* HIP runtime error (allocation, launch, copy)       */
+#define XT_E_SINGULAR -3  /* singular q x q temporal matrix (reference: STOP)   */
+#define XT_E_NOMEM -4     /* host allocation failure                            */
+#define XT_E_UNSUPPORTED -5
+
+/* Last error message of the calling thread (never NULL). */
+const char *xt_last_error(void);
+
+/* ---- 1. reference-signature entry points ------------------------------------ */
+
+/* y = (A (x) K + B (x) M) x.  Replaces kronspmv, kronecker.f90:19-85. */
+int xt_kronspmv(const double *a, const double *b, int q,
+                const int *ki, const int *kj, const double *kv,
+                const int *mi, const int *mj, const double *mv,
+                int p, const double *x, double *y);
+
+/* y := alpha (A (x) K + B (x) M) x + beta y.  Replaces kronspgemv,
+ * kronecker.f90:88-127 (and mpikronspgemv, kronecker_mumps.f90:76-118, on one
+ * device). */
+int xt_kronspgemv(const double *a, const double *b, double alpha, double beta, int q,
+                  const int *ki, const int *kj, const double *kv,
+                  const int *mi, const int *mj, const double *mv,
+                  int p, const double *x, double *y);
+
+/* y = A x, CSR.  Replaces amux, ilut.f:3-44. */
+int xt_amux(int n, const double *x, double *y, const double *a, const int *ja,
+            const int *ia);
+
+/* Dual-threshold ILU of a CSR matrix into MSR alu/jlu/ju.  Replaces ilut,
+ * ilut.f:152-495 (host C++; same algorithm, same output format and ierr).
+ * Work arrays w (n+1) and jw (2n) as in the reference. */
+int xt_ilut(int n, const double *a, const int *ja, const int *ia, int lfil,
+            double droptol, double *alu, int *jlu, int *ju, int iwk, double *w,
+            int *jw, int *ierr);
+
+/* (LU) x = y with ilut's MSR factors.  Replaces lusol, ilut.f:2383-2432. */
+int xt_lusol(int n, const double *y, double *x, const double *alu, const int *jlu,
+             const int *ju);
+
+/* Solve (A (x) LU) y = x.  Replaces kronlusol, kronecker.f90:130-178.
+ * x and y may alias. */
+int xt_kronlusol(const double *a, int q, const double *alu, const int *jlu,
+                 const int *ju, int p, const double *x, double *y);
+
+/* Right-preconditioned restarted GMRES on (A (x) K + B (x) M) x = rhs with the
+ * Kronecker ILUT preconditioner A (x) LU.  Replaces kronpgmres,
+ * kronecker.f90:181-365: rhs is overwritten, sol is the initial guess on entry
+ * and the solution on exit, eps is relative to the initial residual, ierr is
+ * 0 (converged), 1 (maxits), -1 (zero initial residual).  The reference writes
+ * its residual history to Fortran unit iout; here it is returned in hist
+ * (its/ro0 ratios, up to hist_cap entries, count in *nhist) and the Fortran
+ * binding writes it in the reference's format. */
+int xt_kronpgmres(const double *a, const double *b, int q,
+                  const int *ki, const int *kj, const double *kv,
+                  const int *mi, const int *mj, const double *mv, int p,
+                  const double *alu, const int *jlu, const int *ju,
+                  double *rhs, double *sol, int im, double eps, int maxits,
+                  double *hist, int hist_cap, int *nhist, int *its, int *ierr);
+
+/* Two-scale fatigue damage over nip temporal points per Gauss point.
+ * Replaces kokkos_host_, kokkos_host.cpp:37-190 (functor kokkos_kernel.h:44-344):
+ * same arguments, same flattened layouts; state arrays are updated in place. */
+int xt_kokkos_host(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int ngpe,
+                   int ncmp, int nip, const int *nodes, const double *Nx,
+                   const double *Ny, const double *Nz, const double *Nt,
+                   const double *sol, const double *ym, const double *nu,
+                   double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
+                   double *p, double *D, double *ws, double *del_po, double *sig_d);
+
+/* Element status from damage: element e fails iff any of its ngpe Gauss points
+ * has D >= Dc.  Replaces chkelesta, delelement.f90:109-136 (device reduction).
+ * nefail receives the failed (1-based) element ids in increasing order. */
+int xt_chkelesta(int ne, int ngpe, double Dc, const double *D, int *elesta,
+                 int *gpsta, int *nef, int *nefail);
+
+/* ---- 2. device-resident handle API ------------------------------------------- */
+
+typedef struct xt_op_s *xt_op;        /* Kronecker operator sum_k T_k (x) S_k   */
+typedef struct xt_prec_s *xt_prec;    /* ILUT factors + level schedule          */
+
+/* Upload nterms spatial matrices (1-based CSR, host) of order p.  FP64 and FP32
+ * copies are kept on the device.  q <= 8, nterms <= 3. */
+int xt_op_create(int p, int q, int nterms, const int *const *rowptr,
+                 const int *const *colidx, const double *const *vals, xt_op *out);
+void xt_op_destroy(xt_op op);
+/* nnz of term k, and the device bytes one FP64/FP32 application streams. */
+long long xt_op_nnz(xt_op op, int k);
+
+/* Y := alpha * sum_k S_k X T_{k,r}^T + beta * Y on R right-hand sides.
+ * tmat: device array [R][nterms][q][q] (row-major T_{k,r}(i,j)), same dtype as
+ * x/y.  fp32 != 0 selects the FP32 values and float X/Y.  stream: hipStream_t. */
+int xt_op_apply(xt_op op, int R, const void *tmat, double alpha, double beta,
+                const void *x, void *y, int fp32, void *stream);
+
+/* Upload ILUT factors (1-based MSR from ilut) and build the level schedule. */
+int xt_prec_create(int p, const double *alu, const int *jlu, const int *ju, xt_prec *out);
+void xt_prec_destroy(xt_prec pr);
+int xt_prec_levels(xt_prec pr, int *nlev_lower, int *nlev_upper);
+
+/* Y = (LU)^-1 X Minv_r^T on R right-hand sides (Minv: device [R][q][q],
+ * row-major, the inverse temporal matrix; NULL = identity).  x and y must not
+ * alias.  fp32 selects the FP32 factor copy. */
+int xt_prec_apply(xt_prec pr, int R, int q, const void *minv, const void *x, void *y,
+                  int fp32, void *stream);
+
+/* Layout conversion between temporal-major x[r][j*p + dof] (R stacked
+ * reference vectors) and interleaved X[dof][r*q + j], FP64. */
+int xt_to_interleaved(int p, int q, int R, const double *src, double *dst, void *stream);
+int xt_from_interleaved(int p, int q, int R, const double *src, double *dst, void *stream);
+
+/* Batched slab solve.  Solves (A_r (x) K + B_r (x) M) x_r = b_r for r < R with
+ * GMRES(im) preconditioned by A_r (x) LU.
+ *   mode 0: FP64 throughout (plain FP64 GPU GMRES);
+ *   mode 1: FP32 operator and FP32 ILUT inside FP64 GMRES iterative refinement:
+ *           the true residual b - A x, the Arnoldi basis, orthogonalisation and
+ *           Givens least squares stay FP64.
+ * ab: host [R][2][q][q] column-major A_r, B_r (as passed to kronpgmres).
+ * b, x: device, interleaved, FP64.  x is the initial guess on entry.
+ * Outputs per r (host arrays of length R): its, ierr, relres (true
+ * ||b - A x|| / ||b||, FP64).  eps is relative to the initial residual. */
+int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
+                  double *x, int im, double eps, int maxits, int mode, int *its,
+                  int *ierr, double *relres, void *stream);
+
+/* Damage kernel on device-resident state (kokkos_kernel.h:44-344), one lane per
+ * Gauss point, nip temporal points per launch.  All pointers are device
+ * pointers with kokkos_host_'s layouts; sol is temporal-major FP64. */
+int xt_damage_device(int ne, int nn, int nip, const int *nodes, const double *Nx,
+                     const double *Ny, const double *Nz, const double *Nt,
+                     const double *sol, const double *ym, const double *nu,
+                     double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
+                     double *p, double *D, double *ws, double *del_po, double *sig_d,
+                     void *stream);
+
+/* Synchronise the device (all streams of this process). */
+int xt_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XTFEM_H */

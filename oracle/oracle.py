@@ -1,0 +1,208 @@
+"""ctypes front end of the C oracle (``liboracle.so``, built from xtfem_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and
+by the CPU-baseline leg of bench.py, always as the checker.  The product package
+(xtfem_amd) never imports it.
+
+Conventions are the reference's: 1-based CSR (int32), column-major q x q temporal
+matrices, temporal-major space-time vectors ``x[j*p + dof]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        ci, cd = C.c_int, C.c_double
+        L.or_amux.argtypes = [ci, _d, _d, _d, _i, _i]
+        L.or_kronspmv.argtypes = [_d, _d, ci, _i, _i, _d, _i, _i, _d, ci, _d, _d]
+        L.or_kronspgemv.argtypes = [_d, _d, cd, cd, ci, _i, _i, _d, _i, _i, _d, ci, _d, _d]
+        L.or_ilut.argtypes = [ci, _d, _i, _i, ci, cd, _d, _i, _i, ci, _d, _i]
+        L.or_ilut.restype = ci
+        L.or_lusol.argtypes = [ci, _d, _d, _d, _i, _i]
+        L.or_inverse.argtypes = [ci, _d, _d]
+        L.or_inverse.restype = ci
+        L.or_kronlusol.argtypes = [_d, ci, _d, _i, _i, ci, _d, _d]
+        L.or_kronlusol.restype = ci
+        L.or_kronpgmres.argtypes = [_d, _d, ci, _i, _i, _d, _i, _i, _d, ci, _d, _i, _i,
+                                    _d, _d, ci, cd, ci, _d, ci, C.POINTER(ci), C.POINTER(ci)]
+        L.or_kronpgmres.restype = ci
+        L.or_stcoeff_m.argtypes = [cd, cd, cd, _d, _d, _d, _d, _d, _d]
+        L.or_stcoeff_t.argtypes = [cd, cd, _d, _d, _d, _d, _d, _d]
+        L.or_getfextc_m.argtypes = [cd, cd, cd, _d, _d]
+        L.or_damage_nt.argtypes = [_d, cd, ci, _d]
+        L.or_kokkos_host.argtypes = [ci] * 9 + [_i, _d, _d, _d, _d, _d, _d, _d] + [_d] * 9
+        L.or_chkelesta.argtypes = [ci, ci, cd, _d, _i, _i, _i]
+        L.or_chkelesta.restype = ci
+        _LIB = L
+    return _LIB
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _i32(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+def _fmat(a):
+    """q x q matrix -> flat column-major float64 (Fortran order)."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).ravel()
+
+
+def amux(ia, ja, a, x):
+    """ilut.f:3-44"""
+    n = len(ia) - 1
+    y = np.zeros(n)
+    lib().or_amux(n, _f64(x), y, _f64(a), _i32(ja), _i32(ia))
+    return y
+
+
+def kronspmv(A, B, K, M, x):
+    """kronecker.f90:19-85. K, M are (ia, ja, a) 1-based CSR triples."""
+    q = A.shape[0]
+    p = len(K[0]) - 1
+    y = np.zeros(p * q)
+    lib().or_kronspmv(_fmat(A), _fmat(B), q, _i32(K[0]), _i32(K[1]), _f64(K[2]),
+                      _i32(M[0]), _i32(M[1]), _f64(M[2]), p, _f64(x), y)
+    return y
+
+
+def kronspgemv(A, B, alpha, beta, K, M, x, y):
+    """kronecker.f90:88-127"""
+    q = A.shape[0]
+    p = len(K[0]) - 1
+    y = _f64(y).copy()
+    lib().or_kronspgemv(_fmat(A), _fmat(B), alpha, beta, q, _i32(K[0]), _i32(K[1]), _f64(K[2]),
+                        _i32(M[0]), _i32(M[1]), _f64(M[2]), p, _f64(x), y)
+    return y
+
+
+def ilut(K, lfil, droptol, iwk=None):
+    """ilut.f:152-495. Returns (alu, jlu, ju, ierr) in 1-based MSR."""
+    ia, ja, a = K
+    n = len(ia) - 1
+    if iwk is None:
+        iwk = n + 2 + n * (2 * lfil + 1) + 16
+    alu = np.zeros(iwk)
+    jlu = np.zeros(iwk, dtype=np.int32)
+    ju = np.zeros(n, dtype=np.int32)
+    w = np.zeros(n + 1)
+    jw = np.zeros(2 * n, dtype=np.int32)
+    ierr = lib().or_ilut(n, _f64(a), _i32(ja), _i32(ia), lfil, droptol, alu, jlu, ju, iwk, w, jw)
+    nnz = int(jlu[n]) - 1 if ierr == 0 else 0
+    return alu[:max(nnz, n + 1)].copy(), jlu[:max(nnz, n + 1)].copy(), ju, ierr
+
+
+def lusol(alu, jlu, ju, y):
+    """ilut.f:2383-2432"""
+    n = len(ju)
+    x = np.zeros(n)
+    lib().or_lusol(n, _f64(y), x, _f64(alu), _i32(jlu), _i32(ju))
+    return x
+
+
+def inverse(A):
+    """kronecker.f90:368-395 (LAPACK DGETRF/DGETRI restated)."""
+    q = A.shape[0]
+    out = np.zeros(q * q)
+    if lib().or_inverse(q, _fmat(A), out):
+        raise np.linalg.LinAlgError("Matrix is numerically singular!")
+    return out.reshape(q, q).T.copy()
+
+
+def kronlusol(A, alu, jlu, ju, x):
+    """kronecker.f90:130-178"""
+    q = A.shape[0]
+    p = len(ju)
+    y = np.zeros(p * q)
+    if lib().or_kronlusol(_fmat(A), q, _f64(alu), _i32(jlu), _i32(ju), p, _f64(x), y):
+        raise np.linalg.LinAlgError("Matrix is numerically singular!")
+    return y
+
+
+def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
+    """kronecker.f90:181-365. prec = (alu, jlu, ju). Returns (sol, ierr, its, hist)."""
+    q = A.shape[0]
+    p = len(K[0]) - 1
+    rhs = _f64(rhs).copy()
+    sol = np.zeros(p * q) if sol0 is None else _f64(sol0).copy()
+    cap = maxits + 64
+    hist = np.zeros(cap)
+    nh = C.c_int(0)
+    its = C.c_int(0)
+    alu, jlu, ju = prec
+    ierr = lib().or_kronpgmres(_fmat(A), _fmat(B), q, _i32(K[0]), _i32(K[1]), _f64(K[2]),
+                               _i32(M[0]), _i32(M[1]), _f64(M[2]), p, _f64(alu), _i32(jlu),
+                               _i32(ju), rhs, sol, im, eps, maxits, hist, cap,
+                               C.byref(nh), C.byref(its))
+    return sol, ierr, its.value, hist[:nh.value].copy()
+
+
+def stcoeff_m(t1, dt, w):
+    """stcoeff.f90:62-356 -> dict of 6x6 arrays (k, m, ktn, mtn, ktn1, mtn1)."""
+    outs = [np.zeros(36) for _ in range(6)]
+    lib().or_stcoeff_m(t1, dt, w, *outs)
+    names = ("k", "m", "ktn", "mtn", "ktn1", "mtn1")
+    return {n: o.reshape(6, 6).T.copy() for n, o in zip(names, outs)}
+
+
+def stcoeff_t(t1, dt):
+    """stcoeff.f90:1-60 -> dict of 3x3 arrays."""
+    outs = [np.zeros(9) for _ in range(6)]
+    lib().or_stcoeff_t(t1, dt, *outs)
+    names = ("k", "m", "ktn", "mtn", "ktn1", "mtn1")
+    return {n: o.reshape(3, 3).T.copy() for n, o in zip(names, outs)}
+
+
+def getfextc_m(t1, w, dt):
+    """getfextc.f90:22-58 -> (fextc[6], fint[6])."""
+    f = np.zeros(6)
+    g = np.zeros(6)
+    lib().or_getfextc_m(t1, w, dt, f, g)
+    return f, g
+
+
+def damage_nt(tcoorde, w, nip):
+    """kokkos_damage.f90:36-47 -> Nt[nip, 6]."""
+    out = np.zeros(6 * nip)
+    lib().or_damage_nt(_f64(tcoorde), w, nip, out)
+    return out.reshape(nip, 6)
+
+
+def kokkos_host(ne, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu, state):
+    """kokkos_host.cpp:37-190 / kokkos_kernel.h:44-344 on the CPU.
+
+    state: dict with eps_old, eps_pl, sig_eff, sig_back ([ngp*6]) and p, D, ws,
+    del_po, sig_d ([ngp]); a new dict with updated copies is returned.
+    """
+    st = {k: _f64(v).copy() for k, v in state.items()}
+    lib().or_kokkos_host(0, ne, nn, 3, 8, 3, 8, 6, nip, _i32(nodes), _f64(Nx), _f64(Ny),
+                         _f64(Nz), _f64(Nt), _f64(sol), _f64(ym), _f64(nu),
+                         st["eps_old"], st["eps_pl"], st["sig_eff"], st["sig_back"],
+                         st["p"], st["D"], st["ws"], st["del_po"], st["sig_d"])
+    return st
+
+
+def chkelesta(ne, ngpe, Dc, D, elesta=None, gpsta=None):
+    """delelement.f90:109-136 -> (nefail list 1-based, elesta, gpsta)."""
+    el = np.ones(ne, dtype=np.int32) if elesta is None else _i32(elesta).copy()
+    gp = np.ones(ne * ngpe, dtype=np.int32) if gpsta is None else _i32(gpsta).copy()
+    nefail = np.zeros(ne, dtype=np.int32)
+    nef = lib().or_chkelesta(ne, ngpe, Dc, _f64(D), el, gp, nefail)
+    return nefail[:nef].copy(), el, gp

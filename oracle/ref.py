@@ -1,0 +1,233 @@
+"""ctypes front end of the reference build ``oracle/_ref/libxtfem_ref.so``.
+
+TEST INFRASTRUCTURE ONLY.  The library is the reference's own Fortran (kronecker.f90,
+ilut.f, stcoeff.f90, getfextc.f90, getdfinal.f and the FEM setup routines of
+elesubs.f90 / spmtx.f90 / mpisubs.f90 / procs.f90) compiled in place with AMD flang
+by ``make -C oracle ref``.  It exists only where /root/reference is mounted; every
+caller must treat it as optional (``available()``).
+
+All Fortran arguments are passed by reference; arrays are Fortran-ordered.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "_ref", "libxtfem_ref.so")
+_LIB = None
+
+
+def available() -> bool:
+    return os.path.exists(_PATH)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+        _LIB = C.CDLL(_PATH)
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ri(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _rd(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def _F(a, dtype=np.float64):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def stcoeff_m(t1, dt, w):
+    outs = [np.zeros((6, 6), order="F") for _ in range(6)]
+    lib().stcoeff_m_(_rd(t1), _rd(dt), _rd(w), *[_p(o) for o in outs])
+    return dict(zip(("k", "m", "ktn", "mtn", "ktn1", "mtn1"), [np.array(o) for o in outs]))
+
+
+def stcoeff_t(t1, dt):
+    outs = [np.zeros((3, 3), order="F") for _ in range(6)]
+    lib().stcoeff_t_(_rd(t1), _rd(dt), *[_p(o) for o in outs])
+    return dict(zip(("k", "m", "ktn", "mtn", "ktn1", "mtn1"), [np.array(o) for o in outs]))
+
+
+def getfextc_m(t1, w, dt):
+    f = np.zeros(6)
+    g = np.zeros(6)
+    lib().getfextc_m_(_rd(t1), _rd(w), _rd(dt), _p(f), _p(g))
+    return f, g
+
+
+def amux(ia, ja, a, x):
+    ia = _F(ia, np.int32); ja = _F(ja, np.int32); a = _F(a); x = _F(x)
+    n = len(ia) - 1
+    y = np.zeros(n)
+    lib().amux_(_ri(n), _p(x), _p(y), _p(a), _p(ja), _p(ia))
+    return y
+
+
+def _csr(S):
+    return _F(S[0], np.int32), _F(S[1], np.int32), _F(S[2])
+
+
+def kronspmv(A, B, K, M, x):
+    q = A.shape[0]
+    ki, kj, kv = _csr(K)
+    mi, mj, mv = _csr(M)
+    p = len(ki) - 1
+    x = _F(x).copy()
+    y = np.zeros(p * q)
+    lib().kronspmv_(_p(_F(A)), _p(_F(B)), _ri(q), _p(ki), _p(kj), _p(kv), _p(mi), _p(mj),
+                    _p(mv), _ri(p), _p(x), _p(y))
+    return y
+
+
+def kronspgemv(A, B, alpha, beta, K, M, x, y):
+    q = A.shape[0]
+    ki, kj, kv = _csr(K)
+    mi, mj, mv = _csr(M)
+    p = len(ki) - 1
+    x = _F(x).copy()
+    y = _F(y).copy()
+    lib().kronspgemv_(_p(_F(A)), _p(_F(B)), _rd(alpha), _rd(beta), _ri(q), _p(ki), _p(kj),
+                      _p(kv), _p(mi), _p(mj), _p(mv), _ri(p), _p(x), _p(y))
+    return y
+
+
+def ilut(K, lfil, droptol, iwk=None):
+    ia, ja, a = _csr(K)
+    n = len(ia) - 1
+    if iwk is None:
+        iwk = n + 2 + n * (2 * lfil + 1) + 16
+    alu = np.zeros(iwk)
+    jlu = np.zeros(iwk, dtype=np.int32)
+    ju = np.zeros(n, dtype=np.int32)
+    w = np.zeros(n + 1)
+    jw = np.zeros(2 * n, dtype=np.int32)
+    ierr = C.c_int(0)
+    lib().ilut_(_ri(n), _p(a), _p(ja), _p(ia), _ri(lfil), _rd(droptol), _p(alu), _p(jlu),
+                _p(ju), _ri(iwk), _p(w), _p(jw), C.byref(ierr))
+    nnz = int(jlu[n]) - 1 if ierr.value == 0 else 0
+    return alu[:max(nnz, n + 1)].copy(), jlu[:max(nnz, n + 1)].copy(), ju, ierr.value
+
+
+def lusol(alu, jlu, ju, y):
+    alu = _F(alu); jlu = _F(jlu, np.int32); ju = _F(ju, np.int32); y = _F(y).copy()
+    n = len(ju)
+    x = np.zeros(n)
+    lib().lusol_(_ri(n), _p(y), _p(x), _p(alu), _p(jlu), _p(ju))
+    return x
+
+
+def kronlusol(A, alu, jlu, ju, x):
+    q = A.shape[0]
+    alu = _F(alu); jlu = _F(jlu, np.int32); ju = _F(ju, np.int32)
+    p = len(ju)
+    x = _F(x).copy()
+    y = np.zeros(p * q)
+    lib().kronlusol_(_p(_F(A)), _ri(q), _p(alu), _p(jlu), _p(ju), _ri(p), _p(x), _p(y))
+    return y
+
+
+def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
+    q = A.shape[0]
+    ki, kj, kv = _csr(K)
+    mi, mj, mv = _csr(M)
+    p = len(ki) - 1
+    alu = _F(prec[0]); jlu = _F(prec[1], np.int32); ju = _F(prec[2], np.int32)
+    rhs = _F(rhs).copy()
+    sol = np.zeros(p * q) if sol0 is None else _F(sol0).copy()
+    ierr = C.c_int(0)
+    lib().kronpgmres_(_p(_F(A)), _p(_F(B)), _ri(q), _p(ki), _p(kj), _p(kv), _p(mi), _p(mj),
+                      _p(mv), _ri(p), _p(alu), _p(jlu), _p(ju), _p(rhs), _p(sol), _ri(im),
+                      _rd(eps), _ri(maxits), _ri(-1), C.byref(ierr))
+    return sol, ierr.value
+
+
+def getdfinal(strain_tgo, strain_tg, strain_pl, p, D, ws, sig_eff, backsig, del_po, sig_d,
+              cmat, dmat):
+    """getdfinal.f:10-397, one return-map step.  Array arguments are updated in place;
+    the scalars come back in a tuple (p, D, ws, del_po, sig_d)."""
+    sc = [C.c_double(float(v)) for v in (p, D, ws, del_po, sig_d)]
+    lib().getdfinal_(_p(strain_tgo), _p(strain_tg), _p(strain_pl), C.byref(sc[0]),
+                     C.byref(sc[1]), C.byref(sc[2]), _p(sig_eff), _p(backsig), C.byref(sc[3]),
+                     C.byref(sc[4]), _p(cmat), _p(dmat))
+    return tuple(s.value for s in sc)
+
+
+def chkelesta(nn, ne, ngpe, Dc, D):
+    elesta = np.ones(ne, dtype=np.int32)
+    gpsta = np.ones(ne * ngpe, dtype=np.int32)
+    nefail = np.zeros(ne, dtype=np.int32)
+    nef = C.c_int(0)
+    D = _F(D)
+    lib().chkelesta_(_ri(nn), _ri(ne), _ri(8), _ri(ngpe), _rd(Dc), _p(D), _p(elesta),
+                     _p(gpsta), C.byref(nef), _p(nefail))
+    return nefail[:nef.value].copy(), elesta, gpsta
+
+
+def assemble_m(xcoord, xconn, ym0, nu0, rho):
+    """mpiglbmtx_m (mpisubs.f90:69-139) over all elements, then tricsr
+    (spmtx.f90:359-407).  Returns 1-based CSR triples K, M and Nx, Ny, Nz [8, 8*ne]."""
+    L = lib()
+    xcoord = _F(xcoord)
+    xconn = _F(xconn, np.int32)
+    nn = xcoord.shape[0]
+    ne = xconn.shape[0]
+    ndofn, nnse, nquads, ngpe, ncmp, nnsd = 3, 8, 2, 8, 6, 2
+    wqs = np.zeros(nquads)
+    ns = np.zeros((ndofn * ngpe, ndofn * nnse), order="F")
+    dns = np.zeros((ndofn * ngpe, nnse), order="F")
+    dnxyz = np.zeros((ndofn ** 2 * ngpe, ndofn * nnse), order="F")
+    L.setshapefunc_m_(_ri(ndofn), _ri(nnsd), _ri(nnse), _ri(nquads), _ri(ngpe), _p(wqs),
+                      _p(ns), _p(dns), _p(dnxyz))
+    wqs_t = np.zeros(nquads)
+    ns_t = np.zeros((ngpe, nnse), order="F")
+    dns_t = np.zeros((ngpe, nnse, 3), order="F")
+    L.setshapefunc_t_(_ri(nnsd), _ri(nnse), _ri(nquads), _ri(ngpe), _p(wqs_t), _p(ns_t),
+                      _p(dns_t))
+    ele = np.arange(1, ne + 1, dtype=np.int32)
+    ym = np.full(ngpe * ne, float(ym0))
+    nu = np.full(ngpe * ne, float(nu0))
+    sd = np.zeros((6, 9), order="F")
+    stiff = np.zeros((6, 6, ne * ngpe), order="F")
+    Nx = np.zeros((nnse, ngpe * ne), order="F")
+    Ny = np.zeros_like(Nx)
+    Nz = np.zeros_like(Nx)
+    knz = ne * (nnse * ndofn) ** 2
+    ndof = nn * ndofn
+    nti = max(knz, ndof + 1)
+    ki = np.zeros(nti, dtype=np.int32); kj = np.zeros(knz, dtype=np.int32); kv = np.zeros(knz)
+    mi = np.zeros(nti, dtype=np.int32); mj = np.zeros(knz, dtype=np.int32); mv = np.zeros(knz)
+    vv = np.zeros(ndof)
+    tempst = np.zeros(nn)
+    L.mpiglbmtx_m_(_ri(ne), _p(ele), _ri(ne), _ri(nn), _ri(ndofn), _ri(nnse), _ri(nquads),
+                   _ri(ngpe), _ri(ncmp), _p(xconn), _p(xcoord), _p(ym), _p(nu), _rd(ym0),
+                   _rd(nu0), _rd(rho), _rd(0.0), _rd(0.0), _p(wqs), _p(ns_t), _p(ns), _p(dns_t),
+                   _p(dns), _p(dnxyz), _p(sd), _p(tempst), _p(stiff), _p(Nx), _p(Ny), _p(Nz),
+                   _p(ki), _p(kj), _p(kv), _ri(knz), _p(mi), _p(mj), _p(mv), _ri(knz), _p(vv))
+    out = []
+    for (ti, tj, tv) in ((ki, kj, kv), (mi, mj, mv)):
+        nz = C.c_int(knz)
+        L.tricsr_(_p(ti), _p(tj), _p(tv), _ri(ndof), C.byref(nz), _ri(nti))
+        out.append((ti[:ndof + 1].copy(), tj[:nz.value].copy(), tv[:nz.value].copy()))
+    return out[0], out[1], np.array(Nx), np.array(Ny), np.array(Nz)
+
+
+def csrapplybc(S, bc):
+    """spmtx.f90:268-307 (1-0 method, zero removal)."""
+    ia, ja, a = (np.array(S[0], dtype=np.int32), np.array(S[1], dtype=np.int32),
+                 np.array(S[2], dtype=np.float64))
+    n = len(ia) - 1
+    nz = C.c_int(len(a))
+    bc = np.asarray(bc, dtype=np.int32)
+    lib().csrapplybc_(_p(ia), _p(ja), _p(a), _ri(n), C.byref(nz), _p(bc))
+    return ia, ja[:nz.value].copy(), a[:nz.value].copy()

@@ -1,0 +1,165 @@
+"""Write the golden fixtures of tests/golden from the reference's own code.
+
+Run in a container where /root/reference is mounted and ``make -C oracle ref`` has
+built oracle/_ref/libxtfem_ref.so (the reference's Fortran, compiled in place):
+
+    python tests/golden/make_golden.py
+
+Outputs (numpy .npz, no pickles):
+  rod_325_192.npz   the mesh inp/prismaticrod-325-192.inp (input data: coordinates,
+                    connectivity)
+  golden_ref.npz    input/output vectors of the compiled reference routines
+                    (stcoeff_m, stcoeff_t, getfextc_m, mpiglbmtx_m+tricsr,
+                    csrapplybc, ilut, lusol, kronspmv, kronspgemv, kronlusol,
+                    kronpgmres, getdfinal, chkelesta) on seeded inputs.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+from xtfem_amd import fem  # noqa: E402
+
+SEED = 20261202
+INP = "/root/reference/inp/prismaticrod-325-192.inp"
+RHO_REF = float(np.float32(7.860e-6))   # main3d.f90:154, default-real literal
+YM_REF, NU_REF = 1.97e5, 0.3             # main3d.f90:152-153
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def strain_history(nsteps, amp, dirs, ph):
+    """Smooth cyclic strain path (6 Voigt components, 3 cycles) of amplitude amp.
+    The damage tests rebuild the same paths from (amp, dirs, ph)."""
+    t = np.arange(nsteps) / nsteps
+    return amp * dirs[None, :] * np.sin(2 * np.pi * 3 * t[:, None] + ph[None, :])
+
+
+def main():
+    if not ref.available():
+        raise SystemExit("oracle/_ref/libxtfem_ref.so missing: run `make -C oracle ref`")
+    mesh = fem.read_inp(INP)
+    np.savez_compressed(os.path.join(HERE, "rod_325_192.npz"), xcoord=mesh.xcoord,
+                        xconn=mesh.xconn)
+    rng = np.random.default_rng(SEED)
+    g = {}
+    # temporal matrices and load coefficients (stcoeff.f90, getfextc.f90)
+    pts = np.array([[0.0, 0.5, 2 * np.pi * 20], [0.0, 0.05, 2 * np.pi * 20],
+                    [0.3, 0.5, 2 * np.pi * 50], [0.0, 0.1, 2 * np.pi * 97.3]])
+    g["st_pts"] = pts
+    for i, (t1, dt, w) in enumerate(pts):
+        c = ref.stcoeff_m(t1, dt, w)
+        for k, v in c.items():
+            g[f"st{i}_{k}"] = v
+        f, h = ref.getfextc_m(t1, w, dt)
+        g[f"fx{i}_fextc"] = f
+        g[f"fx{i}_fint"] = h
+    ct = ref.stcoeff_t(0.0, 0.5)
+    for k, v in ct.items():
+        g[f"stt_{k}"] = v
+    # assembly of the rod (mpiglbmtx_m + tricsr) and 1-0 BCs clamping x = 0
+    K, M, Nx, Ny, Nz = ref.assemble_m(mesh.xcoord, mesh.xconn, YM_REF, NU_REF, RHO_REF)
+    g["asm_K_digest"] = np.array(digest(*K))
+    g["asm_M_digest"] = np.array(digest(*M))
+    g["asm_Nxyz_digest"] = np.array(digest(Nx.T, Ny.T, Nz.T))
+    g["asm_K_nnz"] = np.array(len(K[2]))
+    bc = fem.clamp_mask(mesh, 0, [0.0])
+    Kb = ref.csrapplybc(K, bc)
+    Mb = ref.csrapplybc(M, bc)
+    g["bc_mask"] = bc
+    g["Kb_digest"] = np.array(digest(*Kb))
+    g["Mb_digest"] = np.array(digest(*Mb))
+    # ILUT on the rod, lfil=20, droptol=1e-4 (BASELINE settings) and a tighter variant
+    alu, jlu, ju, ierr = ref.ilut(Kb, 20, 1e-4)
+    assert ierr == 0
+    g["ilut20_digest"] = np.array(digest(alu, jlu, ju))
+    g["ilut20_nnz"] = np.array(len(alu))
+    alu5, jlu5, ju5, ierr5 = ref.ilut(Kb, 5, 1e-3)
+    assert ierr5 == 0
+    g["ilut5_digest"] = np.array(digest(alu5, jlu5, ju5))
+    # triangular solve, Kronecker ops, preconditioner, GMRES
+    p = len(Kb[0]) - 1
+    y = rng.standard_normal(p)
+    g["lusol_y"] = y
+    g["lusol_x"] = ref.lusol(alu, jlu, ju, y)
+    c0 = ref.stcoeff_m(0.0, 0.5, 2 * np.pi * 20)
+    A = c0["k"] + c0["ktn"]
+    B = c0["m"] + c0["mtn"]
+    g["A"], g["B"] = A, B
+    x = rng.standard_normal(6 * p)
+    g["kron_x"] = x
+    g["kron_y"] = ref.kronspmv(A, B, Kb, Mb, x)
+    y0 = rng.standard_normal(6 * p)
+    g["gemv_y0"] = y0
+    g["gemv_y"] = ref.kronspgemv(A, B, 0.7, -1.3, Kb, Mb, x, y0)
+    g["klu_y"] = ref.kronlusol(A, alu, jlu, ju, x)
+    # a smooth slab right-hand side: axial traction on x = 3 at 20 Hz (t1 = 0, dt = 0.5)
+    rext = fem.traction_load(mesh, 0, 3.0, 0, 70.0 * 1.0)
+    f, h = ref.getfextc_m(0.0, 2 * np.pi * 20, 0.5)
+    rhs = np.concatenate([f[j] * rext for j in range(6)])
+    rhs[np.tile(bc, 6) != 0] = 0.0
+    g["gmres_rhs"] = rhs
+    for eps, tag in ((1e-8, "e8"), (1e-12, "e12")):
+        sol, ierr = ref.kronpgmres(A, B, Kb, Mb, (alu, jlu, ju), rhs, im=30, eps=eps, maxits=3000)
+        assert ierr == 0, ierr
+        g[f"gmres_{tag}_sol"] = sol
+    # two-scale damage return map (getdfinal.f) on seeded strain paths
+    ngp, nsteps = 24, 400
+    E, nu = YM_REF, NU_REF
+    G = E / (2 * (1 + nu))
+    cf = E / ((1 + nu) * (1 - 2 * nu))
+    dmat = np.zeros((6, 6), order="F")
+    cmat = np.zeros((6, 6), order="F")
+    for i in range(3):
+        dmat[i, i] = cf * (1 - nu)
+        dmat[i + 3, i + 3] = cf * (1 - 2 * nu) / 2
+        cmat[i, i] = 1 / E
+        cmat[i + 3, i + 3] = 2 * (1 + nu) / E
+        for j in range(3):
+            if i != j:
+                dmat[i, j] = cf * nu
+                cmat[i, j] = -nu / E
+    params = np.zeros((ngp, 13))
+    out6 = np.zeros((ngp, 4, 6))
+    out1 = np.zeros((ngp, 5))
+    for k in range(ngp):
+        amp = rng.uniform(0.0008, 0.004)
+        dirs = rng.standard_normal(6)
+        dirs /= np.linalg.norm(dirs)
+        ph = rng.uniform(0, 2 * np.pi, 6)
+        params[k] = np.r_[amp, dirs, ph]
+        path = strain_history(nsteps, amp, dirs, ph)
+        eo = np.zeros(6); ep = np.zeros(6); se = np.zeros(6); sb = np.zeros(6)
+        sc = (0.0, 0.0, 0.0, 0.0, 0.0)
+        for s in range(nsteps):
+            st = np.ascontiguousarray(path[s])
+            sc = ref.getdfinal(eo, st, ep, *sc[:3], se, sb, *sc[3:], cmat, dmat)
+        out6[k] = [eo, ep, se, sb]
+        out1[k] = sc
+    g["dmg_params"] = params
+    g["dmg_nsteps"] = np.array(nsteps)
+    g["dmg_out6"] = out6
+    g["dmg_out1"] = out1
+    # element status (chkelesta)
+    D = rng.uniform(0.0, 0.35, 8 * mesh.ne)
+    nefail, elesta, gpsta = ref.chkelesta(mesh.nn, mesh.ne, 8, 0.3, D)
+    g["chk_D"], g["chk_nefail"], g["chk_elesta"], g["chk_gpsta"] = D, nefail, elesta, gpsta
+    np.savez_compressed(os.path.join(HERE, "golden_ref.npz"), **g)
+    print("wrote", os.path.join(HERE, "golden_ref.npz"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+"""Host-side mirror of the reference subroutines on the space-time solve path.
+
+Same names, argument meaning and error behaviour as the reference routines; every
+call runs on the GPU through libxtfem_hip.so (C ABI declared in include/xtfem.h),
+except ``ilut``, whose factorisation is host C++ in the same library.
+
+Conventions (the reference's): CSR triples ``(ia, ja, a)`` 1-based int32 / float64,
+temporal matrices q x q numpy arrays, space-time vectors temporal-major
+``x[j*p + dof]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import numpy as np
+
+from ._lib import check, lib, XtError
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _i32(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+def _P(a):
+    return a.ctypes.data
+
+
+def _fmat(A):
+    """q x q -> flat column-major (Fortran storage of a(i,j))."""
+    return np.ascontiguousarray(np.asarray(A, dtype=np.float64).T).ravel()
+
+
+def kronspmv(A, B, K, M, x):
+    """y = (A (x) K + B (x) M) x.  kronecker.f90:19-85"""
+    q = A.shape[0]
+    ki, kj, kv = _i32(K[0]), _i32(K[1]), _f64(K[2])
+    mi, mj, mv = _i32(M[0]), _i32(M[1]), _f64(M[2])
+    p = len(ki) - 1
+    a, b, x = _fmat(A), _fmat(B), _f64(x)
+    y = np.zeros(p * q)
+    check(lib().xt_kronspmv(_P(a), _P(b), q, _P(ki), _P(kj), _P(kv), _P(mi), _P(mj), _P(mv), p,
+                            _P(x), _P(y)), "kronspmv")
+    return y
+
+
+def kronspgemv(A, B, alpha, beta, K, M, x, y):
+    """y := alpha (A (x) K + B (x) M) x + beta y.  kronecker.f90:88-127"""
+    q = A.shape[0]
+    ki, kj, kv = _i32(K[0]), _i32(K[1]), _f64(K[2])
+    mi, mj, mv = _i32(M[0]), _i32(M[1]), _f64(M[2])
+    p = len(ki) - 1
+    a, b, x = _fmat(A), _fmat(B), _f64(x)
+    y = _f64(y).copy()
+    check(lib().xt_kronspgemv(_P(a), _P(b), float(alpha), float(beta), q, _P(ki), _P(kj), _P(kv),
+                              _P(mi), _P(mj), _P(mv), p, _P(x), _P(y)), "kronspgemv")
+    return y
+
+
+def amux(ia, ja, a, x):
+    """y = A x (CSR).  ilut.f:3-44"""
+    ia, ja, a, x = _i32(ia), _i32(ja), _f64(a), _f64(x)
+    n = len(ia) - 1
+    y = np.zeros(n)
+    check(lib().xt_amux(n, _P(x), _P(y), _P(a), _P(ja), _P(ia)), "amux")
+    return y
+
+
+def ilut(K, lfil, droptol, iwk=None):
+    """Dual-threshold ILU -> (alu, jlu, ju, ierr), 1-based MSR.  ilut.f:152-495"""
+    ia, ja, a = _i32(K[0]), _i32(K[1]), _f64(K[2])
+    n = len(ia) - 1
+    if iwk is None:
+        iwk = n + 2 + n * (2 * lfil + 1) + 16
+    alu = np.zeros(iwk)
+    jlu = np.zeros(iwk, dtype=np.int32)
+    ju = np.zeros(n, dtype=np.int32)
+    w = np.zeros(n + 1)
+    jw = np.zeros(2 * n, dtype=np.int32)
+    ierr = C.c_int(0)
+    check(lib().xt_ilut(n, _P(a), _P(ja), _P(ia), int(lfil), float(droptol), _P(alu), _P(jlu),
+                        _P(ju), int(iwk), _P(w), _P(jw), C.addressof(ierr)), "ilut")
+    nnz = int(jlu[n]) - 1 if ierr.value == 0 else 0
+    return alu[:max(nnz, n + 1)].copy(), jlu[:max(nnz, n + 1)].copy(), ju, ierr.value
+
+
+def lusol(alu, jlu, ju, y):
+    """(LU) x = y with ilut's factors.  ilut.f:2383-2432"""
+    alu, jlu, ju, y = _f64(alu), _i32(jlu), _i32(ju), _f64(y)
+    n = len(ju)
+    x = np.zeros(n)
+    check(lib().xt_lusol(n, _P(y), _P(x), _P(alu), _P(jlu), _P(ju)), "lusol")
+    return x
+
+
+def kronlusol(A, alu, jlu, ju, x):
+    """Solve (A (x) LU) y = x.  kronecker.f90:130-178"""
+    q = A.shape[0]
+    alu, jlu, ju, x = _f64(alu), _i32(jlu), _i32(ju), _f64(x)
+    p = len(ju)
+    a = _fmat(A)
+    y = np.zeros(p * q)
+    rc = lib().xt_kronlusol(_P(a), q, _P(alu), _P(jlu), _P(ju), p, _P(x), _P(y))
+    if rc == -3:
+        raise np.linalg.LinAlgError("Matrix is numerically singular!")
+    check(rc, "kronlusol")
+    return y
+
+
+def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
+    """Right-preconditioned restarted GMRES with the Kronecker ILUT preconditioner.
+    kronecker.f90:181-365.  Returns (sol, ierr, its, hist) with hist the residual
+    history ro/ro0 that the reference writes to unit iout."""
+    q = A.shape[0]
+    ki, kj, kv = _i32(K[0]), _i32(K[1]), _f64(K[2])
+    mi, mj, mv = _i32(M[0]), _i32(M[1]), _f64(M[2])
+    p = len(ki) - 1
+    alu, jlu, ju = _f64(prec[0]), _i32(prec[1]), _i32(prec[2])
+    a, b = _fmat(A), _fmat(B)
+    rhs = _f64(rhs).copy()
+    sol = np.zeros(p * q) if sol0 is None else _f64(sol0).copy()
+    cap = maxits + 64
+    hist = np.zeros(cap)
+    nh, its, ierr = C.c_int(0), C.c_int(0), C.c_int(0)
+    check(lib().xt_kronpgmres(_P(a), _P(b), q, _P(ki), _P(kj), _P(kv), _P(mi), _P(mj), _P(mv), p,
+                              _P(alu), _P(jlu), _P(ju), _P(rhs), _P(sol), int(im), float(eps),
+                              int(maxits), _P(hist), cap, C.addressof(nh), C.addressof(its),
+                              C.addressof(ierr)), "kronpgmres")
+    return sol, ierr.value, its.value, hist[:nh.value].copy()
+
+
+def kokkos_host(ne, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu, state):
+    """Damage functor over all Gauss points.  kokkos_host.cpp:37-190 (kokkos_kernel.h:44-344).
+    Returns a new state dict (eps_old, eps_pl, sig_eff, sig_back, p, D, ws, del_po, sig_d)."""
+    st = {k: _f64(v).copy() for k, v in state.items()}
+    nodes, Nx, Ny, Nz, Nt = _i32(nodes), _f64(Nx), _f64(Ny), _f64(Nz), _f64(Nt)
+    sol, ym, nu = _f64(sol), _f64(ym), _f64(nu)
+    check(lib().xt_kokkos_host(0, ne, nn, 3, 8, 3, 8, 6, nip, _P(nodes), _P(Nx), _P(Ny), _P(Nz),
+                               _P(Nt), _P(sol), _P(ym), _P(nu), _P(st["eps_old"]),
+                               _P(st["eps_pl"]), _P(st["sig_eff"]), _P(st["sig_back"]),
+                               _P(st["p"]), _P(st["D"]), _P(st["ws"]), _P(st["del_po"]),
+                               _P(st["sig_d"])), "kokkos_host")
+    return st
+
+
+def kokkos_damage(mesh, Nx, Ny, Nz, ym, nu, w, tcoorde, sol, gpsta, nip, state):
+    """kokkos_damage.f90:1-112: temporal table Nt, flattening, kernel, then D = ws = 0 at
+    failed Gauss points.  Nx/Ny/Nz: [8*ne, 8] (Gauss point, local node)."""
+    from .slab import damage_nt
+    Nt = damage_nt(tcoorde, w, nip)
+    nodes = np.repeat(_i32(mesh.xconn), 8, axis=0)  # node ids per Gauss point
+    st = kokkos_host(mesh.ne, mesh.nn, nip, nodes.ravel(), Nx.ravel(), Ny.ravel(), Nz.ravel(),
+                     Nt.ravel(), sol, ym, nu, state)
+    failed = np.asarray(gpsta) == 0
+    st["D"][failed] = 0.0
+    st["ws"][failed] = 0.0
+    return st
+
+
+def chkelesta(ne, ngpe, Dc, D, elesta=None, gpsta=None):
+    """Element status from damage.  delelement.f90:109-136 -> (nefail, elesta, gpsta)."""
+    el = np.ones(ne, dtype=np.int32) if elesta is None else _i32(elesta).copy()
+    gp = np.ones(ne * ngpe, dtype=np.int32) if gpsta is None else _i32(gpsta).copy()
+    D = _f64(D)
+    nefail = np.zeros(max(ne, 1), dtype=np.int32)
+    nef = C.c_int(0)
+    check(lib().xt_chkelesta(ne, ngpe, float(Dc), _P(D), _P(el), _P(gp), C.addressof(nef),
+                             _P(nefail)), "chkelesta")
+    return nefail[:nef.value].copy(), el, gp
+
+
+__all__ = ["kronspmv", "kronspgemv", "amux", "ilut", "lusol", "kronlusol", "kronpgmres",
+           "kokkos_host", "kokkos_damage", "chkelesta", "XtError"]

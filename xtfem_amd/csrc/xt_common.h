@@ -1,0 +1,101 @@
+// Shared helpers of libxtfem_hip: error state, HIP checks, device buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/xtfem.h"
+
+namespace xt {
+
+// Thread-local text of the last failure; the C ABI returns codes only.
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+#define XT_HIP(...)                                                                     \
+    do {                                                                                \
+        hipError_t e_ = (__VA_ARGS__);                                                  \
+        if (e_ != hipSuccess) {                                                         \
+            ::xt::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #__VA_ARGS__,         \
+                            hipGetErrorString(e_));                                     \
+            return XT_E_HIP;                                                            \
+        }                                                                               \
+    } while (0)
+
+#define XT_TRY(...)                                                                     \
+    do {                                                                                \
+        int rc_ = (__VA_ARGS__);                                                               \
+        if (rc_ != XT_OK) return rc_;                                                   \
+    } while (0)
+
+#define XT_ARG(cond, msg)                                                               \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            ::xt::set_error("argument error: %s", msg);                                 \
+            return XT_E_ARG;                                                            \
+        }                                                                               \
+    } while (0)
+
+// Owning device allocation (hipMalloc / hipFree); move-only.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() { if (ptr) (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+    // Grow to at least n bytes (contents not preserved).
+    int reserve(size_t n) {
+        if (n <= bytes) return XT_OK;
+        release();
+        if (n == 0) return XT_OK;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+            return XT_E_HIP;
+        }
+        bytes = n;
+        return XT_OK;
+    }
+    int upload(const void *src, size_t n, hipStream_t s = 0) {
+        XT_TRY(reserve(n));
+        if (n) XT_HIP(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s));
+        return XT_OK;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace xt
+
+// Internal entry points shared between translation units.
+namespace xt {
+// CSR on the device, 0-based.
+struct DevCSR {
+    int n = 0;
+    long long nnz = 0;
+    DevBuf rowptr, col, val64, val32;
+};
+int csr_upload_1based(int n, const int *ia, const int *ja, const double *a, DevCSR &out,
+                      bool want32, hipStream_t s);
+
+// Sizes of an operator handle.
+int op_dims(xt_op op, int *p, int *q, int *nterms);
+// q x q row-major inverse with partial pivoting; 0 on success, 1 if singular.
+int small_inverse_rowmajor(int q, const double *A, double *Ainv);
+
+// Temporal mixing Y[dof][r*q+i] = sum_j M_r(i,j) X[dof][r*q+j] (M row-major [R][q][q]).
+template <typename TI, typename TO>
+int temporal_mix(int p, int q, int R, const TI *M, const TI *x, TO *y, hipStream_t s);
+}  // namespace xt

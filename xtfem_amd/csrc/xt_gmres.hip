@@ -1,0 +1,523 @@
+// Batched right-preconditioned restarted GMRES for space-time slabs.
+//
+// Restates kronpgmres (kronecker.f90:181-365) for R independent right-hand sides
+// that share the spatial matrices (load cases with their own temporal matrices
+// A_r, B_r).  Per right-hand side the algorithm is the reference's: residual
+// r0 = b - A x0, eps1 = eps*||r0||, Arnoldi with the preconditioner applied on
+// the right, Givens rotations on the Hessenberg column, stop when i = im or
+// |rs(i+1)| <= eps1, back substitution, x += P^-1 (V y), restart from the
+// recombined residual V rs'.  Orthogonalisation is classical Gram-Schmidt with
+// one re-orthogonalisation pass (CGS2) instead of the reference's modified
+// Gram-Schmidt; both are backward stable, CGS2 needs two fused passes over
+// the basis instead of i dependent ones.  The basis, dots and Givens
+// arithmetic are FP64.
+//
+// mode 1 (GMRES-IR): the inner GMRES applies the operator and the ILUT solves
+// in FP32 (FP32 values, FP32 accumulation) while keeping the FP64 basis; the
+// outer loop recomputes the true residual b - A x with the FP64 operator and
+// adds the inner correction, until ||b - A x|| <= eps * ||b - A x0||.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "xt_common.h"
+
+struct xt_op_s;
+struct xt_prec_s;
+
+namespace xt {
+template <typename T>
+int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
+                 hipStream_t s);
+
+static constexpr int JB = 8;  // basis vectors per multidot block row
+
+// partial[b][j][col] = sum over rows of chunk b of V_j[row][col] * w[row][col]
+__global__ void __launch_bounds__(256)
+multidot_partial(int p, int c, int k, int chunk, const double *__restrict__ V, size_t ldv,
+                 const double *__restrict__ w, double *__restrict__ partial) {
+    __shared__ double sh[256];
+    const int rpl = 256 / c;
+    const int lr = threadIdx.x / c;
+    const int col = threadIdx.x - lr * c;
+    const int j0 = blockIdx.y * JB;
+    const int rbeg = blockIdx.x * chunk;
+    const int rend = min(p, rbeg + chunk);
+    double acc[JB];
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) acc[jj] = 0.0;
+    if (lr < rpl) {
+        for (int row = rbeg + lr; row < rend; row += rpl) {
+            const size_t o = (size_t)row * c + col;
+            const double wv = w[o];
+#pragma unroll
+            for (int jj = 0; jj < JB; ++jj)
+                if (j0 + jj < k) acc[jj] += V[(size_t)(j0 + jj) * ldv + o] * wv;
+        }
+    }
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+        if (j0 + jj >= k) break;
+        sh[threadIdx.x] = (lr < rpl) ? acc[jj] : 0.0;
+        __syncthreads();
+        if (threadIdx.x < c) {
+            double s = 0.0;
+            for (int l = 0; l < rpl; ++l) s += sh[l * c + threadIdx.x];
+            partial[((size_t)blockIdx.x * k + j0 + jj) * c + threadIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// out[j][r] = sum_b sum_{i<q} partial[b][j][r*q + i]   (one block per j)
+__global__ void __launch_bounds__(256)
+reduce_partials(int nblk, int k, int c, int q, int R, const double *__restrict__ partial,
+                double *__restrict__ out) {
+    __shared__ double sh[256];
+    const int j = blockIdx.x;
+    const int rpl = 256 / c;
+    const int lr = threadIdx.x / c;
+    const int col = threadIdx.x - lr * c;
+    double s = 0.0;
+    if (lr < rpl)
+        for (int b = lr; b < nblk; b += rpl) s += partial[((size_t)b * k + j) * c + col];
+    sh[threadIdx.x] = (lr < rpl) ? s : 0.0;
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int r = threadIdx.x;
+        double t = 0.0;
+        for (int l = 0; l < rpl; ++l)
+            for (int i = 0; i < q; ++i) t += sh[l * c + r * q + i];
+        out[(size_t)j * R + r] = t;
+    }
+}
+
+// y = (yin ? yin : 0) + alpha * sum_j coef[j][r] V_j
+__global__ void __launch_bounds__(256)
+multiaxpy(long long n, int c, int q, int R, int k, double alpha, const double *__restrict__ coef,
+          const double *__restrict__ V, size_t ldv, const double *yin, double *y) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int col = (int)(idx % c);
+    const int r = col / q;
+    double s = yin ? yin[idx] : 0.0;
+    for (int j = 0; j < k; ++j) s += (alpha * coef[j * R + r]) * V[(size_t)j * ldv + idx];
+    y[idx] = s;
+}
+
+// v *= 1/sqrt(nrm2[r]) (0 if frozen or zero norm)
+__global__ void __launch_bounds__(256)
+scale_by_norm(long long n, int c, int q, const double *__restrict__ nrm2,
+              const int *__restrict__ frozen, double *v) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int r = (int)(idx % c) / q;
+    const double t = sqrt(nrm2[r]);
+    v[idx] = (frozen[r] || t == 0.0) ? 0.0 : v[idx] * (1.0 / t);
+}
+
+__global__ void __launch_bounds__(256) cvt_f64_f32(long long n, const double *a, float *b) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) b[idx] = (float)a[idx];
+}
+__global__ void __launch_bounds__(256) cvt_f32_f64(long long n, const float *a, double *b) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) b[idx] = (double)a[idx];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) add_into(long long n, const T *a, double *b) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) b[idx] += (double)a[idx];
+}
+__global__ void __launch_bounds__(256) sub_from(long long n, const double *b, double *r) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) r[idx] = b[idx] - r[idx];
+}
+
+// Partial-pivot LU inverse of a q x q row-major matrix (q <= 8). 0 on success.
+static int small_inverse(int q, const double *A, double *Ainv) {
+    double lu[64];
+    int piv[8];
+    std::memcpy(lu, A, sizeof(double) * q * q);
+    for (int k = 0; k < q; ++k) {
+        int pr = k;
+        double mx = std::fabs(lu[k * q + k]);
+        for (int i = k + 1; i < q; ++i)
+            if (std::fabs(lu[i * q + k]) > mx) { mx = std::fabs(lu[i * q + k]); pr = i; }
+        piv[k] = pr;
+        if (lu[pr * q + k] == 0.0) return 1;
+        if (pr != k)
+            for (int j = 0; j < q; ++j) std::swap(lu[k * q + j], lu[pr * q + j]);
+        const double rk = 1.0 / lu[k * q + k];
+        for (int i = k + 1; i < q; ++i) lu[i * q + k] *= rk;
+        for (int i = k + 1; i < q; ++i)
+            for (int j = k + 1; j < q; ++j) lu[i * q + j] -= lu[i * q + k] * lu[k * q + j];
+    }
+    for (int cc = 0; cc < q; ++cc) {
+        double e[8];
+        for (int i = 0; i < q; ++i) e[i] = (i == cc) ? 1.0 : 0.0;
+        for (int k = 0; k < q; ++k) if (piv[k] != k) std::swap(e[k], e[piv[k]]);
+        for (int i = 0; i < q; ++i) for (int k = 0; k < i; ++k) e[i] -= lu[i * q + k] * e[k];
+        for (int i = q - 1; i >= 0; --i) {
+            for (int k = i + 1; k < q; ++k) e[i] -= lu[i * q + k] * e[k];
+            e[i] /= lu[i * q + i];
+        }
+        for (int i = 0; i < q; ++i) Ainv[i * q + cc] = e[i];
+    }
+    return 0;
+}
+
+}  // namespace xt
+
+using namespace xt;
+
+namespace xt {
+int small_inverse_rowmajor(int q, const double *A, double *Ainv) { return small_inverse(q, A, Ainv); }
+static int op_apply_internal(xt_op op, int R, const void *tmat, double alpha, double beta,
+                             const void *x, void *y, int fp32, hipStream_t s) {
+    return xt_op_apply(op, R, tmat, alpha, beta, x, y, fp32, (void *)s);
+}
+}
+
+namespace {
+
+struct Solver {
+    xt_op op;
+    xt_prec pr;
+    int p, q, R, c, im;
+    long long n;
+    hipStream_t s;
+    DevBuf tm64, tm32, mi64, mi32;       // temporal matrices and inverses
+    DevBuf V;                            // (im+1) x n FP64 basis
+    DevBuf w32a, w32b;                   // FP32 work vectors
+    DevBuf z64, t64;                     // FP64 work vectors
+    DevBuf partial, hdev, hdev2, ndev, coef, frozen;
+    int nblk = 0, chunk = 0;
+    std::vector<double> *hist = nullptr;  // R == 1: ro/ro0 per iteration (kronpgmres log)
+    double hist_ro0 = 1.0;
+
+    double *Vj(int j) { return V.as<double>() + (size_t)j * n; }
+
+    int init() {
+        c = q * R;
+        n = (long long)p * c;
+        XT_ARG(c <= 256, "q*R must be <= 256 for the batched solver");
+        XT_TRY(V.reserve(sizeof(double) * n * (im + 1)));
+        XT_TRY(z64.reserve(sizeof(double) * n));
+        XT_TRY(t64.reserve(sizeof(double) * n));
+        XT_TRY(w32a.reserve(sizeof(float) * n));
+        XT_TRY(w32b.reserve(sizeof(float) * n));
+        const int rpl = 256 / c;
+        chunk = std::max(rpl, ((p + 1023) / 1024 + rpl - 1) / rpl * rpl);
+        nblk = ceil_div(p, chunk);
+        XT_TRY(partial.reserve(sizeof(double) * (size_t)nblk * (im + 1) * c));
+        XT_TRY(hdev.reserve(sizeof(double) * (im + 1) * R));
+        XT_TRY(hdev2.reserve(sizeof(double) * (im + 1) * R));
+        XT_TRY(ndev.reserve(sizeof(double) * R));
+        XT_TRY(coef.reserve(sizeof(double) * (im + 1) * R));
+        XT_TRY(frozen.reserve(sizeof(int) * R));
+        return XT_OK;
+    }
+
+    // out[j][r] = <V_j, w>_r for j < k
+    int dots(int k, const double *Vbase, const double *w, double *out) {
+        dim3 grid(nblk, ceil_div(k, JB));
+        hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, p, c, k, chunk, Vbase,
+                           (size_t)n, w, partial.as<double>());
+        hipLaunchKernelGGL(reduce_partials, dim3(k), dim3(256), 0, s, nblk, k, c, q, R,
+                           partial.as<double>(), out);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+    int maxpy(int k, double alpha, const double *cf, const double *Vbase, const double *yin,
+              double *y) {
+        hipLaunchKernelGGL(multiaxpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R, k,
+                           alpha, cf, Vbase, (size_t)n, yin, y);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+
+    // w = Op(P^-1 v) with FP64 or FP32 operator/preconditioner
+    int op_prec(const double *v, double *w, bool f32) {
+        if (!f32) {
+            XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), v, z64.as<double>(), false, s));
+            XT_TRY(op_apply_internal(op, R, tm64.ptr, 1.0, 0.0, z64.ptr, w, 0, s));
+            return XT_OK;
+        }
+        hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, v, w32a.as<float>());
+        XT_TRY(prec_apply_t<float>(pr, R, q, mi32.as<float>(), w32a.as<float>(), w32b.as<float>(), true, s));
+        XT_TRY(op_apply_internal(op, R, tm32.ptr, 1.0, 0.0, w32b.ptr, w32a.ptr, 1, s));
+        hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32a.as<float>(), w);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+    // x += P^-1 u
+    int prec_add(const double *u, double *x, bool f32) {
+        if (!f32) {
+            XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), u, z64.as<double>(), false, s));
+            hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, z64.as<double>(), x);
+        } else {
+            hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, u, w32a.as<float>());
+            XT_TRY(prec_apply_t<float>(pr, R, q, mi32.as<float>(), w32a.as<float>(), w32b.as<float>(), true, s));
+            hipLaunchKernelGGL(add_into<float>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), x);
+        }
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+    // r = b - A x (FP64 operator)
+    int residual(const double *b, const double *x, double *r) {
+        XT_TRY(op_apply_internal(op, R, tm64.ptr, 1.0, 0.0, x, r, 0, s));
+        hipLaunchKernelGGL(sub_from, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, b, r);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+    int norms2(const double *v, std::vector<double> &out) {
+        XT_TRY(dots(1, v, v, ndev.as<double>()));
+        out.resize(R);
+        XT_HIP(hipMemcpyAsync(out.data(), ndev.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+        XT_HIP(hipStreamSynchronize(s));
+        return XT_OK;
+    }
+
+    // GMRES on the residual system already stored in V_0 (unnormalised, = r0).
+    // x is updated in place.  eps1[r]: absolute target on |rs(i+1)|.
+    // active[r] in: RHS to solve; out: its_r, conv_r (1 converged, 0 not).
+    int gmres(double *x, bool f32, const std::vector<double> &eps1, int maxits,
+              std::vector<int> &active, std::vector<int> &its, std::vector<int> &conv,
+              std::vector<double> &ro_last) {
+        const int K = im;
+        std::vector<double> hh((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K),
+            rs((size_t)R * (K + 1)), hcol((size_t)(K + 1) * R), hcol2((size_t)(K + 1) * R);
+        std::vector<int> icyc(R, 0), fz(R, 0);
+        std::vector<double> nrm;
+        auto HH = [&](int r, int a, int b) -> double & { return hh[(size_t)r * (K + 1) * K + a + (size_t)b * (K + 1)]; };
+        for (int r = 0; r < R; ++r) conv[r] = 0;
+        for (;;) {
+            // start of a cycle: normalise V_0
+            XT_TRY(norms2(Vj(0), nrm));
+            bool any = false;
+            for (int r = 0; r < R; ++r) {
+                if (!active[r]) { fz[r] = 1; continue; }
+                const double ro = std::sqrt(nrm[r]);
+                ro_last[r] = ro;
+                if (hist && its[r] == 0) hist->push_back(ro / hist_ro0);
+                if (ro == 0.0 || ro <= eps1[r]) { conv[r] = 1; active[r] = 0; fz[r] = 1; continue; }
+                fz[r] = 0;
+                any = true;
+                rs[(size_t)r * (K + 1)] = ro;
+                icyc[r] = 0;
+            }
+            if (!any) break;
+            XT_HIP(hipMemcpyAsync(frozen.ptr, fz.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
+                               ndev.as<double>(), frozen.as<int>(), Vj(0));
+            int i = 0;
+            for (;;) {
+                // Arnoldi step i -> i+1
+                XT_TRY(op_prec(Vj(i), Vj(i + 1), f32));
+                XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
+                XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
+                XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>()));
+                XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
+                XT_TRY(dots(1, Vj(i + 1), Vj(i + 1), ndev.as<double>()));
+                // normalise V_{i+1} for every RHS still in its cycle at this step
+                // (the restart recombination reads it); zero it for the others
+                hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
+                                   ndev.as<double>(), frozen.as<int>(), Vj(i + 1));
+                XT_HIP(hipMemcpyAsync(hcol.data(), hdev.ptr, sizeof(double) * (i + 1) * R, hipMemcpyDeviceToHost, s));
+                XT_HIP(hipMemcpyAsync(hcol2.data(), hdev2.ptr, sizeof(double) * (i + 1) * R, hipMemcpyDeviceToHost, s));
+                nrm.resize(R);
+                XT_HIP(hipMemcpyAsync(nrm.data(), ndev.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+                XT_HIP(hipStreamSynchronize(s));
+                bool cont = false;
+                for (int r = 0; r < R; ++r) {
+                    if (fz[r]) continue;
+                    its[r]++;
+                    for (int j = 0; j <= i; ++j) HH(r, j, i) = hcol[j * R + r] + hcol2[j * R + r];
+                    HH(r, i + 1, i) = std::sqrt(nrm[r]);
+                    double *c_ = &cs[(size_t)r * K], *s_ = &sn[(size_t)r * K], *rs_ = &rs[(size_t)r * (K + 1)];
+                    for (int kk = 1; kk <= i; ++kk) {
+                        const double t = HH(r, kk - 1, i);
+                        HH(r, kk - 1, i) = c_[kk - 1] * t + s_[kk - 1] * HH(r, kk, i);
+                        HH(r, kk, i) = -s_[kk - 1] * t + c_[kk - 1] * HH(r, kk, i);
+                    }
+                    double gam = std::sqrt(HH(r, i, i) * HH(r, i, i) + HH(r, i + 1, i) * HH(r, i + 1, i));
+                    if (gam == 0.0) gam = 1.0e-16;
+                    c_[i] = HH(r, i, i) / gam;
+                    s_[i] = HH(r, i + 1, i) / gam;
+                    rs_[i + 1] = -s_[i] * rs_[i];
+                    rs_[i] = c_[i] * rs_[i];
+                    HH(r, i, i) = c_[i] * HH(r, i, i) + s_[i] * HH(r, i + 1, i);
+                    const double ro = std::fabs(rs_[i + 1]);
+                    ro_last[r] = ro;
+                    if (hist) hist->push_back(ro / hist_ro0);
+                    icyc[r] = i + 1;
+                    if (i + 1 < K && ro > eps1[r]) cont = true;
+                    else fz[r] = 1;  // this RHS ends its cycle here
+                }
+                if (!cont) break;
+                XT_HIP(hipMemcpyAsync(frozen.ptr, fz.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+                ++i;
+            }
+            // back substitution and solution update, per RHS with its own cycle length
+            const int kmax = i + 1;
+            std::vector<double> y((size_t)kmax * R, 0.0), rr((size_t)(kmax + 1) * R, 0.0);
+            for (int r = 0; r < R; ++r) {
+                if (!active[r] || icyc[r] == 0) continue;
+                const int ir = icyc[r];
+                double *rs_ = &rs[(size_t)r * (K + 1)];
+                std::vector<double> yy(rs_, rs_ + ir);
+                yy[ir - 1] = yy[ir - 1] / HH(r, ir - 1, ir - 1);
+                for (int kk = ir - 2; kk >= 0; --kk) {
+                    double t = yy[kk];
+                    for (int j = kk + 1; j < ir; ++j) t -= HH(r, kk, j) * yy[j];
+                    yy[kk] = t / HH(r, kk, kk);
+                }
+                for (int j = 0; j < ir; ++j) y[(size_t)j * R + r] = yy[j];
+            }
+            XT_HIP(hipMemcpyAsync(coef.ptr, y.data(), sizeof(double) * kmax * R, hipMemcpyHostToDevice, s));
+            XT_TRY(maxpy(kmax, 1.0, coef.as<double>(), Vj(0), nullptr, t64.as<double>()));
+            XT_TRY(prec_add(t64.as<double>(), x, f32));
+            // decide per RHS; build restart residual V_0 = sum_j rs'_j V_j
+            bool again = false;
+            for (int r = 0; r < R; ++r) {
+                if (!active[r]) continue;
+                const int ir = icyc[r];
+                if (ro_last[r] <= eps1[r]) { conv[r] = 1; active[r] = 0; continue; }
+                if (its[r] >= maxits) { active[r] = 0; continue; }
+                double *rs_ = &rs[(size_t)r * (K + 1)], *c_ = &cs[(size_t)r * K], *s_ = &sn[(size_t)r * K];
+                for (int j = 1; j <= ir; ++j) {
+                    const int jj = ir + 1 - j + 1;  // 1-based as the reference
+                    rs_[jj - 2] = -s_[jj - 2] * rs_[jj - 1];
+                    rs_[jj - 1] = c_[jj - 2] * rs_[jj - 1];
+                }
+                for (int j = 0; j <= ir; ++j) rr[(size_t)j * R + r] = rs_[j];
+                again = true;
+            }
+            XT_HIP(hipStreamSynchronize(s));
+            if (!again) break;
+            XT_HIP(hipMemcpyAsync(coef.ptr, rr.data(), sizeof(double) * (kmax + 1) * R, hipMemcpyHostToDevice, s));
+            XT_TRY(maxpy(kmax + 1, 1.0, coef.as<double>(), Vj(0), nullptr, t64.as<double>()));
+            XT_HIP(hipMemcpyAsync(Vj(0), t64.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            XT_HIP(hipStreamSynchronize(s));
+        }
+        return XT_OK;
+    }
+};
+
+}  // namespace
+
+namespace xt {
+int slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b, double *x,
+                    int im, double eps, int maxits, int mode, int *its, int *ierr,
+                    double *relres, hipStream_t stream, std::vector<double> *hist);
+}
+
+extern "C" int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
+                             double *x, int im, double eps, int maxits, int mode, int *its,
+                             int *ierr, double *relres, void *stream) {
+    return xt::slab_solve_impl(op, pr, R, ab, b, x, im, eps, maxits, mode, its, ierr, relres,
+                               (hipStream_t)stream, nullptr);
+}
+
+int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
+                        double *x, int im, double eps, int maxits, int mode, int *its,
+                        int *ierr, double *relres, hipStream_t stream,
+                        std::vector<double> *hist) {
+    XT_ARG(op && pr && R > 0 && ab && b && x && its && ierr && relres, "slab_solve arguments");
+    XT_ARG(im >= 1 && im <= 256 && maxits >= 1 && (mode == 0 || mode == 1), "slab_solve parameters");
+    Solver S;
+    S.op = op;
+    S.pr = pr;
+    S.R = R;
+    S.im = im;
+    S.s = stream;
+    S.hist = (R == 1 && mode == 0) ? hist : nullptr;
+    {
+        int p = 0, q = 0, nt = 0;
+        XT_TRY(op_dims(op, &p, &q, &nt));
+        XT_ARG(nt == 2, "slab_solve needs a two-term (K, M) operator");
+        S.p = p;
+        S.q = q;
+    }
+    XT_TRY(S.init());
+    const int q = S.q;
+    // temporal matrices: ab[r] = (A_r, B_r) column-major -> row-major T; inverse of A_r
+    std::vector<double> tm((size_t)R * 2 * q * q), mi((size_t)R * q * q);
+    for (int r = 0; r < R; ++r) {
+        for (int t = 0; t < 2; ++t)
+            for (int i = 0; i < q; ++i)
+                for (int j = 0; j < q; ++j)
+                    tm[(((size_t)r * 2 + t) * q + i) * q + j] = ab[((size_t)r * 2 + t) * q * q + i + j * q];
+        if (small_inverse(q, &tm[(size_t)r * 2 * q * q], &mi[(size_t)r * q * q])) {
+            set_error("temporal matrix A of right-hand side %d is numerically singular", r);
+            return XT_E_SINGULAR;
+        }
+    }
+    std::vector<float> tm32(tm.begin(), tm.end()), mi32(mi.begin(), mi.end());
+    XT_TRY(S.tm64.upload(tm.data(), sizeof(double) * tm.size(), S.s));
+    XT_TRY(S.mi64.upload(mi.data(), sizeof(double) * mi.size(), S.s));
+    XT_TRY(S.tm32.upload(tm32.data(), sizeof(float) * tm32.size(), S.s));
+    XT_TRY(S.mi32.upload(mi32.data(), sizeof(float) * mi32.size(), S.s));
+    XT_HIP(hipStreamSynchronize(S.s));
+
+    std::vector<int> active(R, 1), itv(R, 0), conv(R, 0);
+    std::vector<double> ro_last(R, 0.0), eps1(R, 0.0), r0(R), bn(R);
+    XT_TRY(S.norms2(b, bn));
+    S.hist_ro0 = std::sqrt(bn[0]);
+    // V_0 = b - A x0
+    XT_TRY(S.residual(b, x, S.Vj(0)));
+    XT_TRY(S.norms2(S.Vj(0), r0));
+    for (int r = 0; r < R; ++r) {
+        r0[r] = std::sqrt(r0[r]);
+        eps1[r] = eps * r0[r];
+        ierr[r] = 0;
+        if (r0[r] == 0.0) { ierr[r] = -1; active[r] = 0; }
+    }
+    if (mode == 0) {
+        XT_TRY(S.gmres(x, false, eps1, maxits, active, itv, conv, ro_last));
+        for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
+    } else {
+        // GMRES-IR: inner FP32 solves on the current residual, FP64 outer residual
+        const double inner_rel = 1.0e-4;
+        std::vector<double> rn(R);
+        DevBuf d;
+        XT_TRY(d.reserve(sizeof(double) * S.n));
+        for (int outer = 0; outer < 64; ++outer) {
+            XT_TRY(S.norms2(S.Vj(0), rn));
+            bool any = false;
+            std::vector<double> e1(R, 0.0);
+            std::vector<int> act(R, 0);
+            for (int r = 0; r < R; ++r) {
+                if (ierr[r] == -1) continue;
+                const double nr = std::sqrt(rn[r]);
+                if (nr <= eps1[r]) { conv[r] = 1; continue; }
+                if (itv[r] >= maxits) continue;
+                act[r] = 1;
+                any = true;
+                e1[r] = std::max(inner_rel * nr, 0.5 * eps1[r]);
+            }
+            if (!any) break;
+            // inner solve for the correction d: x0 = 0, V_0 = residual (already there)
+            XT_HIP(hipMemsetAsync(d.ptr, 0, sizeof(double) * S.n, S.s));
+            std::vector<int> cin(R, 0);
+            std::vector<int> act_in = act;
+            XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act_in, itv, cin, ro_last));
+            hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n,
+                               d.as<double>(), x);
+            XT_HIP(hipGetLastError());
+            XT_TRY(S.residual(b, x, S.Vj(0)));
+            XT_HIP(hipStreamSynchronize(S.s));
+        }
+        for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
+    }
+    // true residual
+    std::vector<double> rn;
+    XT_TRY(S.residual(b, x, S.t64.as<double>()));
+    XT_TRY(S.norms2(S.t64.as<double>(), rn));
+    for (int r = 0; r < R; ++r) {
+        its[r] = itv[r];
+        relres[r] = bn[r] > 0 ? std::sqrt(rn[r]) / std::sqrt(bn[r]) : 0.0;
+    }
+    return XT_OK;
+}

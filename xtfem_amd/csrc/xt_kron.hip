@@ -1,0 +1,298 @@
+// Kronecker space-time operator Y = alpha * sum_k S_k X T_k^T + beta * Y on the GPU.
+//
+// Restates kronspmv / kronspgemv (kronecker.f90:19-127): the reference forms
+// u_j = S x_j column by column with amux and then mixes y_i += T(i,j) u_j with
+// q^2 dense vector passes.  Here one pass over the CSR rows produces, for each
+// row, the q*R partial products Z_k(row, :) = sum_e S_k(row, e) X(e, :) in
+// registers, exchanges them through LDS inside the workgroup and applies the
+// q x q temporal mix before the single store of Y(row, :).  X and Y use the
+// interleaved layout X[dof][r*q + j] so every gathered row of X is one
+// contiguous run of q*R values.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "xt_common.h"
+
+namespace xt {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+const char *get_error() { return g_err; }
+
+// ---------------------------------------------------------------------------
+// CSR upload: 1-based host arrays -> 0-based device arrays (+ FP32 copy).
+// ---------------------------------------------------------------------------
+int csr_upload_1based(int n, const int *ia, const int *ja, const double *a, DevCSR &out,
+                      bool want32, hipStream_t s) {
+    XT_ARG(n >= 0 && ia && (n == 0 || (ja && a)), "csr arrays");
+    long long nnz = (long long)ia[n] - ia[0];
+    XT_ARG(nnz >= 0 && ia[0] == 1, "CSR must be 1-based with ia(1) = 1");
+    std::vector<int> rp(n + 1), cj((size_t)nnz);
+    for (int i = 0; i <= n; ++i) rp[i] = ia[i] - 1;
+    for (long long k = 0; k < nnz; ++k) {
+        int c = ja[k] - 1;
+        XT_ARG(c >= 0 && c < n, "column index out of range");
+        cj[k] = c;
+    }
+    out.n = n;
+    out.nnz = nnz;
+    XT_TRY(out.rowptr.upload(rp.data(), sizeof(int) * (n + 1), s));
+    XT_TRY(out.col.upload(cj.data(), sizeof(int) * nnz, s));
+    XT_TRY(out.val64.upload(a, sizeof(double) * nnz, s));
+    if (want32) {
+        std::vector<float> v32((size_t)nnz);
+        for (long long k = 0; k < nnz; ++k) v32[k] = (float)a[k];
+        XT_TRY(out.val32.upload(v32.data(), sizeof(float) * nnz, s));
+    }
+    // host vectors die at return: make the async copies complete first
+    XT_HIP(hipStreamSynchronize(s));
+    return XT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused multi-term CSR SpMM + temporal mix.
+//   block = 256 lanes = rows_per_block rows x ct columns (ct = column tile,
+//   a multiple of q holding whole right-hand sides); lane (lr, col) accumulates
+//   Z_k(row, col) for every term k, the workgroup exchanges Z through LDS and
+//   lane (lr, col = r*q + i) writes Y(row, col) = sum_k sum_j T_{k,r}(i,j) Z_k(row, r*q + j).
+// ---------------------------------------------------------------------------
+template <typename T, int NT>
+struct KronArgs {
+    const int *rp[NT];
+    const int *ci[NT];
+    const T *v[NT];
+};
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(256)
+kron_csr_kernel(KronArgs<T, NT> A, int p, int c, int q, int col0, int ct, int rpb,
+                const T *__restrict__ tmat, T alpha, T beta, const T *__restrict__ x,
+                T *__restrict__ y) {
+    extern __shared__ unsigned char smem_raw[];
+    T *z = reinterpret_cast<T *>(smem_raw);  // [NT][rpb][ct]
+    const int lr = threadIdx.x / ct;
+    const int col = threadIdx.x - lr * ct;
+    const int row = blockIdx.x * rpb + lr;
+    const bool inblk = lr < rpb;
+    const bool active = inblk && row < p;
+    T acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = T(0);
+    if (active) {
+        const T *xc = x + col0 + col;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int e0 = A.rp[t][row], e1 = A.rp[t][row + 1];
+            T s = T(0);
+            for (int e = e0; e < e1; ++e) s += A.v[t][e] * xc[(size_t)A.ci[t][e] * c];
+            acc[t] = s;
+        }
+    }
+    if (inblk) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) z[(t * rpb + lr) * ct + col] = acc[t];
+    }
+    __syncthreads();
+    if (active) {
+        const int gcol = col0 + col;
+        const int r = gcol / q;
+        const int i = gcol - r * q;
+        const int zb = col - i;  // first column of this right-hand side in the tile
+        T out = T(0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const T *tm = tmat + ((size_t)(r * NT + t) * q + i) * q;
+            const T *zr = z + (t * rpb + lr) * ct + zb;
+            for (int j = 0; j < q; ++j) out += tm[j] * zr[j];
+        }
+        const size_t o = (size_t)row * c + gcol;
+        y[o] = (beta == T(0)) ? alpha * out : alpha * out + beta * y[o];
+    }
+}
+
+template <typename T, int NT>
+static int launch_kron(const DevCSR *S, int p, int q, int R, const T *tmat, double alpha,
+                       double beta, const T *x, T *y, bool fp32, hipStream_t s) {
+    KronArgs<T, NT> A;
+    for (int t = 0; t < NT; ++t) {
+        A.rp[t] = S[t].rowptr.as<int>();
+        A.ci[t] = S[t].col.as<int>();
+        A.v[t] = fp32 ? (const T *)S[t].val32.ptr : (const T *)S[t].val64.ptr;
+    }
+    const int c = q * R;
+    const int tile = (256 / q) * q;  // whole right-hand sides per tile
+    for (int col0 = 0; col0 < c; col0 += tile) {
+        const int ct = (c - col0) < tile ? (c - col0) : tile;
+        const int rpb = 256 / ct;
+        const size_t lds = sizeof(T) * NT * rpb * ct;
+        const int grid = ceil_div(p, rpb);
+        if (grid == 0) break;
+        hipLaunchKernelGGL((kron_csr_kernel<T, NT>), dim3(grid), dim3(256), lds, s, A, p, c, q,
+                           col0, ct, rpb, tmat, (T)alpha, (T)beta, x, y);
+        XT_HIP(hipGetLastError());
+    }
+    return XT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Temporal mix Y[dof][r*q+i] = sum_j M_r(i,j) X[dof][r*q+j]
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256)
+mix_kernel(long long n, int c, int q, const TI *__restrict__ M, const TI *__restrict__ x,
+           TO *__restrict__ y) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const long long row = idx / c;
+    const int col = (int)(idx - row * c);
+    const int r = col / q, i = col - r * q;
+    const TI *m = M + ((size_t)r * q + i) * q;
+    const TI *xr = x + row * c + r * q;
+    TI s = TI(0);
+    for (int j = 0; j < q; ++j) s += m[j] * xr[j];
+    y[idx] = (TO)s;
+}
+
+template <typename TI, typename TO>
+int temporal_mix(int p, int q, int R, const TI *M, const TI *x, TO *y, hipStream_t s) {
+    long long n = (long long)p * q * R;
+    if (n == 0) return XT_OK;
+    hipLaunchKernelGGL((mix_kernel<TI, TO>), dim3(ceil_div(n, 256)), dim3(256), 0, s, n, q * R,
+                       q, M, x, y);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
+template int temporal_mix<double, double>(int, int, int, const double *, const double *,
+                                          double *, hipStream_t);
+template int temporal_mix<float, float>(int, int, int, const float *, const float *, float *,
+                                        hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Layout conversion: temporal-major x[r][j*p + dof] <-> interleaved X[dof][r*q + j]
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+to_inter_kernel(long long n, int p, int q, int R, const double *__restrict__ src,
+                double *__restrict__ dst) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int c = q * R;
+    const long long dof = idx / c;
+    const int col = (int)(idx - dof * c);
+    const int r = col / q, j = col - r * q;
+    dst[idx] = src[(size_t)r * q * p + (size_t)j * p + dof];
+}
+
+__global__ void __launch_bounds__(256)
+from_inter_kernel(long long n, int p, int q, int R, const double *__restrict__ src,
+                  double *__restrict__ dst) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // temporal-major index
+    if (idx >= n) return;
+    const long long per = (long long)q * p;
+    const int r = (int)(idx / per);
+    const long long rem = idx - r * per;
+    const int j = (int)(rem / p);
+    const long long dof = rem - (long long)j * p;
+    dst[idx] = src[dof * (q * R) + r * q + j];
+}
+
+}  // namespace xt
+
+using namespace xt;
+
+struct xt_op_s {
+    int p = 0, q = 0, nterms = 0;
+    DevCSR S[3];
+};
+
+namespace xt {
+int op_dims(xt_op op, int *p, int *q, int *nterms) {
+    XT_ARG(op, "null operator");
+    *p = op->p;
+    *q = op->q;
+    *nterms = op->nterms;
+    return XT_OK;
+}
+}  // namespace xt
+
+extern "C" {
+
+const char *xt_last_error(void) { return get_error(); }
+
+int xt_device_sync(void) {
+    XT_HIP(hipDeviceSynchronize());
+    return XT_OK;
+}
+
+int xt_op_create(int p, int q, int nterms, const int *const *rowptr, const int *const *colidx,
+                 const double *const *vals, xt_op *out) {
+    XT_ARG(out && p > 0 && q > 0 && q <= 8 && nterms >= 1 && nterms <= 3, "op sizes");
+    XT_ARG(rowptr && colidx && vals, "op arrays");
+    xt_op op = new (std::nothrow) xt_op_s;
+    if (!op) return XT_E_NOMEM;
+    op->p = p;
+    op->q = q;
+    op->nterms = nterms;
+    for (int t = 0; t < nterms; ++t) {
+        int rc = csr_upload_1based(p, rowptr[t], colidx[t], vals[t], op->S[t], true, 0);
+        if (rc != XT_OK) { delete op; return rc; }
+    }
+    *out = op;
+    return XT_OK;
+}
+
+void xt_op_destroy(xt_op op) { delete op; }
+
+long long xt_op_nnz(xt_op op, int k) {
+    if (!op || k < 0 || k >= op->nterms) return -1;
+    return op->S[k].nnz;
+}
+
+int xt_op_apply(xt_op op, int R, const void *tmat, double alpha, double beta, const void *x,
+                void *y, int fp32, void *stream) {
+    XT_ARG(op && R > 0 && tmat && x && y, "op_apply arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int nt = op->nterms;
+    if (fp32) {
+        const float *tm = (const float *)tmat;
+        const float *xx = (const float *)x;
+        float *yy = (float *)y;
+        if (nt == 1) return launch_kron<float, 1>(op->S, op->p, op->q, R, tm, alpha, beta, xx, yy, true, s);
+        if (nt == 2) return launch_kron<float, 2>(op->S, op->p, op->q, R, tm, alpha, beta, xx, yy, true, s);
+        return launch_kron<float, 3>(op->S, op->p, op->q, R, tm, alpha, beta, xx, yy, true, s);
+    }
+    const double *tm = (const double *)tmat;
+    const double *xx = (const double *)x;
+    double *yy = (double *)y;
+    if (nt == 1) return launch_kron<double, 1>(op->S, op->p, op->q, R, tm, alpha, beta, xx, yy, false, s);
+    if (nt == 2) return launch_kron<double, 2>(op->S, op->p, op->q, R, tm, alpha, beta, xx, yy, false, s);
+    return launch_kron<double, 3>(op->S, op->p, op->q, R, tm, alpha, beta, xx, yy, false, s);
+}
+
+int xt_to_interleaved(int p, int q, int R, const double *src, double *dst, void *stream) {
+    XT_ARG(p >= 0 && q > 0 && R > 0 && src && dst && src != dst, "layout arguments");
+    long long n = (long long)p * q * R;
+    if (!n) return XT_OK;
+    hipLaunchKernelGGL(to_inter_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, p, q, R, src, dst);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
+
+int xt_from_interleaved(int p, int q, int R, const double *src, double *dst, void *stream) {
+    XT_ARG(p >= 0 && q > 0 && R > 0 && src && dst && src != dst, "layout arguments");
+    long long n = (long long)p * q * R;
+    if (!n) return XT_OK;
+    hipLaunchKernelGGL(from_inter_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, p, q, R, src, dst);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+"""Device-resident handles over the C ABI (operators, ILUT preconditioners, slab solves,
+damage state).  torch supplies device memory and the current HIP stream; all compute
+is in libxtfem_hip.so.  Vectors on the device use the interleaved layout
+X[dof][r*q + j] (r = load case, j = temporal function)."""
+from __future__ import annotations
+
+import ctypes as C
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from . import api
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise api.XtError("no HIP device visible: the space-time solve runs on the GPU only")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _colmajor(Ts):
+    """[..., q, q] -> flat column-major per matrix (Fortran storage)."""
+    Ts = np.asarray(Ts, dtype=np.float64)
+    return np.ascontiguousarray(np.swapaxes(Ts, -1, -2)).reshape(Ts.shape[:-2] + (-1,))
+
+
+class KronOperator:
+    """Y = sum_k S_k X T_k^T on the device (kronspmv/kronspgemv, kronecker.f90:19-127)."""
+
+    def __init__(self, mats, q=6):
+        self.q = q
+        self.nterms = len(mats)
+        self.p = len(mats[0][0]) - 1
+        keep = [(np.ascontiguousarray(ia, dtype=np.int32), np.ascontiguousarray(ja, dtype=np.int32),
+                 np.ascontiguousarray(a, dtype=np.float64)) for ia, ja, a in mats]
+        rp = (C.c_void_p * self.nterms)(*[k[0].ctypes.data for k in keep])
+        cj = (C.c_void_p * self.nterms)(*[k[1].ctypes.data for k in keep])
+        vv = (C.c_void_p * self.nterms)(*[k[2].ctypes.data for k in keep])
+        h = C.c_void_p()
+        _dev()
+        check(lib().xt_op_create(self.p, q, self.nterms, rp, cj, vv, C.addressof(h)), "op_create")
+        self.h = h
+        self.nnz = [int(lib().xt_op_nnz(h, k)) for k in range(self.nterms)]
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().xt_op_destroy(h)
+            self.h = None
+
+    def algorithmic_bytes(self, R, fp32, beta=False):
+        """Bytes one application moves in the stored CSR format, read once and written
+        once: values + column indices + row pointers of every term, X read, Y written."""
+        sv = 4 if fp32 else 8
+        n = self.p * self.q * R
+        b = sum(nz * (sv + 4) + 4 * (self.p + 1) for nz in self.nnz) + 2 * n * sv
+        return b + (n * sv if beta else 0)
+
+    def flops(self, R):
+        c = self.q * R
+        return 2 * sum(self.nnz) * c + 2 * self.nterms * self.q * self.p * c
+
+    def tmat(self, Ts, fp32):
+        """Per-load-case temporal matrices [R][nterms][q][q] (row-major) on the device."""
+        t = torch.as_tensor(np.ascontiguousarray(Ts, dtype=np.float64), device=_dev())
+        return t.float().contiguous() if fp32 else t.contiguous()
+
+    def apply(self, tmat, x, y, R, alpha=1.0, beta=0.0, fp32=False):
+        check(lib().xt_op_apply(self.h, R, tmat.data_ptr(), float(alpha), float(beta),
+                                x.data_ptr(), y.data_ptr(), int(fp32), _stream()), "op_apply")
+
+    def apply_host(self, A, B, X, fp32=False):
+        """Y_r = (A_r (x) K + B_r (x) M) X_r for host temporal-major X [R, q*p]."""
+        A = np.asarray(A).reshape(-1, self.q, self.q)
+        B = np.asarray(B).reshape(-1, self.q, self.q)
+        X = np.atleast_2d(X)
+        R = X.shape[0]
+        xi = to_interleaved(X, self.p, self.q)
+        T = self.tmat(np.stack([A, B], axis=1), fp32)
+        dt = torch.float32 if fp32 else torch.float64
+        xin = xi.to(dt).contiguous()
+        y = torch.empty_like(xin)
+        self.apply(T, xin, y, R, fp32=fp32)
+        return from_interleaved(y.to(torch.float64).contiguous(), self.p, self.q, R)
+
+
+def to_interleaved(X, p, q):
+    """host temporal-major [R, q*p] -> device interleaved FP64 [p*q*R]."""
+    X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.float64)
+    R = X.shape[0]
+    src = torch.as_tensor(X, device=_dev()).contiguous()
+    dst = torch.empty(p * q * R, dtype=torch.float64, device=src.device)
+    check(lib().xt_to_interleaved(p, q, R, src.data_ptr(), dst.data_ptr(), _stream()),
+          "to_interleaved")
+    return dst
+
+
+def from_interleaved(y, p, q, R):
+    dst = torch.empty(p * q * R, dtype=torch.float64, device=y.device)
+    check(lib().xt_from_interleaved(p, q, R, y.data_ptr(), dst.data_ptr(), _stream()),
+          "from_interleaved")
+    return dst.cpu().numpy().reshape(R, q * p)
+
+
+class IlutPreconditioner:
+    """ILUT factors of K (ilut.f:152-495, host) with the level-scheduled GPU solves of
+    kronlusol/lusol (kronecker.f90:130-178, ilut.f:2383-2432)."""
+
+    def __init__(self, K, lfil=20, droptol=1e-4):
+        alu, jlu, ju, ierr = api.ilut(K, lfil, droptol)
+        if ierr != 0:
+            raise api.XtError(f"ilut failed with ierr={ierr}")
+        self.p = len(ju)
+        self.factors = (alu, jlu, ju)
+        self.nnz = int(jlu[self.p]) - 1 - (self.p + 1)
+        h = C.c_void_p()
+        _dev()
+        check(lib().xt_prec_create(self.p, alu.ctypes.data, jlu.ctypes.data, ju.ctypes.data,
+                                   C.addressof(h)), "prec_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().xt_prec_destroy(h)
+            self.h = None
+
+    def levels(self):
+        lo, up = C.c_int(0), C.c_int(0)
+        check(lib().xt_prec_levels(self.h, C.addressof(lo), C.addressof(up)), "prec_levels")
+        return lo.value, up.value
+
+    def apply(self, minv, x, y, R, q=6, fp32=False):
+        check(lib().xt_prec_apply(self.h, R, q, minv.data_ptr() if minv is not None else None,
+                                  x.data_ptr(), y.data_ptr(), int(fp32), _stream()), "prec_apply")
+
+
+def slab_solve(op, pr, A, B, rhs, x0=None, im=30, eps=1e-8, maxits=500, mode=1):
+    """Solve (A_r (x) K + B_r (x) M) x_r = rhs_r for every load case r on the GPU.
+
+    mode 0: FP64 GMRES(im) with the FP64 Kronecker ILUT preconditioner.
+    mode 1: FP32 operator and ILUT solves inside FP64 GMRES iterative refinement.
+    rhs: host [R, q*p] temporal-major.  Returns (x [R, q*p], its, ierr, relres)."""
+    A = np.asarray(A, dtype=np.float64).reshape(-1, op.q, op.q)
+    B = np.asarray(B, dtype=np.float64).reshape(-1, op.q, op.q)
+    rhs = np.atleast_2d(rhs)
+    R = rhs.shape[0]
+    b = to_interleaved(rhs, op.p, op.q)
+    x = torch.zeros_like(b) if x0 is None else to_interleaved(x0, op.p, op.q)
+    x, its, ierr, relres = slab_solve_device(op, pr, A, B, b, x, R, im, eps, maxits, mode)
+    return from_interleaved(x, op.p, op.q, R), its, ierr, relres
+
+
+def slab_solve_device(op, pr, A, B, b, x, R, im=30, eps=1e-8, maxits=500, mode=1):
+    """Device-vector form of slab_solve (b, x interleaved FP64 device tensors; x in/out)."""
+    ab = np.ascontiguousarray(np.stack([_colmajor(A), _colmajor(B)], axis=1))
+    its = np.zeros(R, dtype=np.int32)
+    ierr = np.zeros(R, dtype=np.int32)
+    relres = np.zeros(R)
+    check(lib().xt_slab_solve(op.h, pr.h, R, ab.ctypes.data, b.data_ptr(), x.data_ptr(), int(im),
+                              float(eps), int(maxits), int(mode), its.ctypes.data,
+                              ierr.ctypes.data, relres.ctypes.data, _stream()), "slab_solve")
+    return x, its, ierr, relres
+
+
+class DamageState:
+    """Per-Gauss-point damage state resident in HBM across slabs, with the C3D8 shape
+    derivative tables; ``step`` runs the damage kernel (kokkos_kernel.h:44-344) for one
+    slab and ``failed_elements`` applies chkelesta (delelement.f90:109-136)."""
+
+    FIELDS6 = ("eps_old", "eps_pl", "sig_eff", "sig_back")
+    FIELDS1 = ("p", "D", "ws", "del_po", "sig_d")
+
+    def __init__(self, mesh, Nx, Ny, Nz, ym, nu):
+        dev = _dev()
+        self.ne, self.nn = mesh.ne, mesh.nn
+        ngp = 8 * mesh.ne
+        t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), device=dev).to(dt).contiguous()
+        self.nodes = t(np.repeat(mesh.xconn, 8, axis=0).ravel(), torch.int32)
+        self.Nx, self.Ny, self.Nz = t(np.ravel(Nx)), t(np.ravel(Ny)), t(np.ravel(Nz))
+        self.ym = t(np.broadcast_to(ym, (ngp,)))
+        self.nu = t(np.broadcast_to(nu, (ngp,)))
+        self.s = {k: torch.zeros(6 * ngp, dtype=torch.float64, device=dev) for k in self.FIELDS6}
+        self.s.update({k: torch.zeros(ngp, dtype=torch.float64, device=dev) for k in self.FIELDS1})
+        self.gpsta = torch.ones(ngp, dtype=torch.int32, device=dev)
+
+    def step(self, sol_dev, Nt_dev, nip):
+        """sol_dev: temporal-major FP64 device vector [6*3*nn]; Nt_dev: [nip*6]."""
+        s = self.s
+        check(lib().xt_damage_device(self.ne, self.nn, nip, self.nodes.data_ptr(), self.Nx.data_ptr(),
+                                     self.Ny.data_ptr(), self.Nz.data_ptr(), Nt_dev.data_ptr(),
+                                     sol_dev.data_ptr(), self.ym.data_ptr(), self.nu.data_ptr(),
+                                     *[s[k].data_ptr() for k in self.FIELDS6],
+                                     *[s[k].data_ptr() for k in self.FIELDS1], _stream()),
+              "damage_device")
+        # kokkos_damage.f90:102-107: failed Gauss points carry D = ws = 0
+        dead = self.gpsta == 0
+        s["D"].masked_fill_(dead, 0.0)
+        s["ws"].masked_fill_(dead, 0.0)
+
+    def host_state(self):
+        return {k: v.cpu().numpy() for k, v in self.s.items()}

@@ -1,0 +1,296 @@
+"""Spatial FEM setup feeding the space-time solve: C3D8 meshes, K/M assembly, 1-0 BCs.
+
+This is host-side input construction (it runs once per mesh), restated from the
+reference's setup path so that the matrices handed to the GPU solve are the ones
+the reference would build:
+
+* Gauss points and weights: ``gaussquadrature`` n=2 (procs.f90:517-548), with the
+  reference's 15-digit literal 0.577350269189626.
+* C3D8 shape functions and derivatives in the reference's local node order
+  (``shape3x``/``shape3dx``, procs.f90:1298-1414).
+* Element matrices ``elemtx_m`` (elesubs.f90:2-58): ke = sum_l B^T D B det(J),
+  me = sum_l N^T rho N det(J); D from ``setsdstiff`` (elesubs.f90:374-401);
+  Nx/Ny/Nz tables (shape-function x/y/z derivatives at every Gauss point).
+* Triplet -> CSR as ``tricsr`` (spmtx.f90:359-407): rows and columns sorted,
+  duplicates summed in element order, exact zeros removed.
+* 1-0 boundary conditions ``csrapplybc`` (spmtx.f90:268-307).
+* Traction lumping per element-node incidence (main3d.f90:508-531).
+
+All CSR output is 1-based int32 (the reference's convention); DOF numbering is
+``3*(node-1) + comp`` (mpisubs.f90:111-116).
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+GAUSS2 = np.array([-0.577350269189626, 0.577350269189626])
+
+# local (xi, eta, zeta) sign of each C3D8 node, procs.f90:1320-1327
+NODE_SIGNS = np.array([
+    [-1, -1, +1], [-1, -1, -1], [-1, +1, -1], [-1, +1, +1],
+    [+1, -1, +1], [+1, -1, -1], [+1, +1, -1], [+1, +1, +1]], dtype=np.float64)
+
+
+@dataclasses.dataclass
+class Mesh:
+    xcoord: np.ndarray      # [nn, 3] float64
+    xconn: np.ndarray       # [ne, 8] int32, 1-based node ids
+
+    @property
+    def nn(self):
+        return self.xcoord.shape[0]
+
+    @property
+    def ne(self):
+        return self.xconn.shape[0]
+
+
+def read_inp(path) -> Mesh:
+    """Mesh file reader (main3d.f90:220-229): nn, nn x (id x y z), ne, ne x (id n1..n8)."""
+    with open(path) as f:
+        toks = f.read().split()
+    pos = 0
+    nn = int(toks[pos]); pos += 1
+    xc = np.zeros((nn, 3))
+    for _ in range(nn):
+        i = int(toks[pos]); xc[i - 1] = [float(t) for t in toks[pos + 1:pos + 4]]; pos += 4
+    ne = int(toks[pos]); pos += 1
+    xe = np.zeros((ne, 8), dtype=np.int32)
+    for _ in range(ne):
+        i = int(toks[pos]); xe[i - 1] = [int(t) for t in toks[pos + 1:pos + 9]]; pos += 9
+    return Mesh(xc, xe)
+
+
+def structured_hex(nx, ny, nz, lx, ly, lz, origin=(0.0, 0.0, 0.0), keep=None) -> Mesh:
+    """Structured C3D8 grid of nx*ny*nz elements over [0,lx]x[0,ly]x[0,lz].
+
+    Nodes are numbered z fastest, then y, then x.  ``keep`` is an optional boolean
+    array [nx, ny, nz] of elements to retain (notches); orphan nodes are dropped and
+    the survivors renumbered in the same order.  Local node order follows the
+    reference's shape3x so det(J) > 0.
+    """
+    xs = origin[0] + np.linspace(0.0, lx, nx + 1)
+    ys = origin[1] + np.linspace(0.0, ly, ny + 1)
+    zs = origin[2] + np.linspace(0.0, lz, nz + 1)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    xcoord = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+    def nid(i, j, k):
+        return (i * (ny + 1) + j) * (nz + 1) + k
+
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    if keep is not None:
+        m = np.asarray(keep, dtype=bool).ravel()
+        I, J, K = I[m], J[m], K[m]
+    s = NODE_SIGNS
+    conn = np.stack([nid(I + (s[a, 0] > 0), J + (s[a, 1] > 0), K + (s[a, 2] > 0))
+                     for a in range(8)], axis=1)
+    used = np.zeros(xcoord.shape[0], dtype=bool)
+    used[conn.ravel()] = True
+    newid = np.cumsum(used) - 1
+    return Mesh(xcoord[used].copy(), (newid[conn] + 1).astype(np.int32))
+
+
+def _shape_tables():
+    """N[g, a] and dN/dxi[g, m, a] at the 8 Gauss points (l = 4i + 2j + k)."""
+    pts = np.array([[GAUSS2[i], GAUSS2[j], GAUSS2[k]]
+                    for i in range(2) for j in range(2) for k in range(2)])
+    s = NODE_SIGNS
+    f = 1.0 + s[None, :, :] * pts[:, None, :]            # [8g, 8a, 3]
+    N = 0.125 * f[:, :, 0] * f[:, :, 1] * f[:, :, 2]
+    dN = np.empty((8, 3, 8))
+    dN[:, 0, :] = (0.125 * s[None, :, 0]) * f[:, :, 1] * f[:, :, 2]
+    dN[:, 1, :] = (0.125 * s[None, :, 1]) * f[:, :, 0] * f[:, :, 2]
+    dN[:, 2, :] = (0.125 * s[None, :, 2]) * f[:, :, 0] * f[:, :, 1]
+    return N, dN
+
+
+def _det3(A):
+    """procs.f90:476-478 (explicit 3x3 determinant, same term order)."""
+    a = lambda i, j: A[..., i - 1, j - 1]
+    return (a(1, 1) * a(2, 2) * a(3, 3) + a(1, 2) * a(2, 3) * a(3, 1) + a(1, 3) * a(2, 1) * a(3, 2)
+            - a(1, 1) * a(2, 3) * a(3, 2) - a(1, 2) * a(2, 1) * a(3, 3) - a(1, 3) * a(2, 2) * a(3, 1))
+
+
+def _inv3(A, det):
+    """procs.f90:300-310 (cofactor inverse)."""
+    a = lambda i, j: A[..., i - 1, j - 1]
+    R = np.empty_like(A)
+    R[..., 0, 0] = a(2, 2) * a(3, 3) - a(2, 3) * a(3, 2)
+    R[..., 0, 1] = a(1, 3) * a(3, 2) - a(1, 2) * a(3, 3)
+    R[..., 0, 2] = a(1, 2) * a(2, 3) - a(1, 3) * a(2, 2)
+    R[..., 1, 0] = a(2, 3) * a(3, 1) - a(2, 1) * a(3, 3)
+    R[..., 1, 1] = a(1, 1) * a(3, 3) - a(1, 3) * a(3, 1)
+    R[..., 1, 2] = a(1, 3) * a(2, 1) - a(1, 1) * a(2, 3)
+    R[..., 2, 0] = a(2, 1) * a(3, 2) - a(2, 2) * a(3, 1)
+    R[..., 2, 1] = a(1, 2) * a(3, 1) - a(1, 1) * a(3, 2)
+    R[..., 2, 2] = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1)
+    return R / det[..., None, None]
+
+
+def elastic_dmat(ym, nu):
+    """setsdstiff (elesubs.f90:374-401): isotropic D in Voigt xx,yy,zz,xy,yz,xz."""
+    ym = np.asarray(ym, dtype=np.float64)
+    nu = np.asarray(nu, dtype=np.float64)
+    c = ym / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    D = np.zeros(ym.shape + (6, 6))
+    for i in range(3):
+        for j in range(3):
+            D[..., i, j] = c * nu
+        D[..., i, i] = c * (1.0 - nu)
+        D[..., i + 3, i + 3] = c * (0.5 - nu)
+    return D
+
+
+def element_matrices(mesh: Mesh, ym, nu, rho, chunk=8192):
+    """elemtx_m (elesubs.f90:2-58) for every element, vectorised over elements.
+
+    Each MATMUL of the reference is evaluated as a sequential sum in index order
+    (jac = dns*X, bmat = (sd*gam)*dnxyz, ket = bmat^T*(stiff*bmat),
+    met = ns^T*(rho*ns)) and ke/me accumulate over the 8 Gauss points in order, so
+    the products and sums round exactly as the compiled reference does.
+
+    ym, nu: scalars or per-Gauss-point arrays [8*ne].
+    Returns ke, me [ne, 24, 24] (DOF order 3*a + comp) and Nx, Ny, Nz [ne*8, 8]
+    (row g = Gauss point, column a = local node).
+    """
+    ne = mesh.ne
+    N, dN = _shape_tables()
+    ymg = np.broadcast_to(np.asarray(ym, dtype=np.float64), (ne * 8,)).reshape(ne, 8)
+    nug = np.broadcast_to(np.asarray(nu, dtype=np.float64), (ne * 8,)).reshape(ne, 8)
+    ke = np.zeros((ne, 24, 24))
+    me = np.zeros((ne, 24, 24))
+    Gall = np.zeros((ne, 8, 3, 8))
+    for e0 in range(0, ne, chunk):
+        e1 = min(ne, e0 + chunk)
+        X = mesh.xcoord[mesh.xconn[e0:e1] - 1]             # [E, 8a, 3p]
+        E = e1 - e0
+        for l in range(8):
+            jac = np.zeros((E, 3, 3))
+            for a in range(8):                              # jac(m,p) = sum_a dns(m,a) X(a,p)
+                jac = jac + dN[l, :, a][None, :, None] * X[:, a, None, :]
+            det = _det3(jac)
+            if np.any(det <= 0.0):
+                raise ValueError("non-positive Jacobian: element node order does not match C3D8")
+            gam = _inv3(jac, det)
+            G = np.zeros((E, 3, 8))                         # dN_a/dx_p = sum_m gam(p,m) dN(m,a)
+            for m in range(3):
+                G = G + gam[:, :, m, None] * dN[l, m, :][None, None, :]
+            Gall[e0:e1, l] = G
+            B = np.zeros((E, 6, 24))
+            gx, gy, gz = G[:, 0, :], G[:, 1, :], G[:, 2, :]
+            B[:, 0, 0::3] = gx
+            B[:, 1, 1::3] = gy
+            B[:, 2, 2::3] = gz
+            B[:, 3, 0::3] = gy
+            B[:, 3, 1::3] = gx
+            B[:, 4, 1::3] = gz
+            B[:, 4, 2::3] = gy
+            B[:, 5, 0::3] = gz
+            B[:, 5, 2::3] = gx
+            D = elastic_dmat(ymg[e0:e1, l], nug[e0:e1, l])  # [E, 6, 6]
+            DB = np.zeros((E, 6, 24))
+            for j in range(6):
+                DB = DB + D[:, :, j, None] * B[:, j, None, :]
+            ket = np.zeros((E, 24, 24))
+            for i in range(6):
+                ket = ket + B[:, i, :, None] * DB[:, i, None, :]
+            ke[e0:e1] = ke[e0:e1] + ket * det[:, None, None]
+            mm = N[l, :, None] * (rho * N[l, None, :])      # [8, 8]
+            met = np.zeros((24, 24))
+            for c in range(3):
+                met[c::3, c::3] = mm
+            me[e0:e1] = me[e0:e1] + met[None] * det[:, None, None]
+    Nx = Gall[:, :, 0, :].reshape(ne * 8, 8).copy()
+    Ny = Gall[:, :, 1, :].reshape(ne * 8, 8).copy()
+    Nz = Gall[:, :, 2, :].reshape(ne * 8, 8).copy()
+    return ke, me, Nx, Ny, Nz
+
+
+def _triplet_to_csr(rows, cols, vals, n):
+    """tricsr: sort by (row, col), sum duplicates sequentially in input (element)
+    order, drop exact zeros.  rows/cols 0-based in, 1-based CSR out."""
+    key = rows.astype(np.int64) * n + cols.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    vals = vals[order]
+    start = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    cnt = np.diff(np.r_[start, len(key)])
+    acc = vals[start].copy()
+    for k in range(1, int(cnt.max()) if len(cnt) else 1):
+        m = cnt > k
+        acc[m] = acc[m] + vals[start[m] + k]
+    ukey = key[start]
+    nz = acc != 0.0
+    ukey, acc = ukey[nz], acc[nz]
+    r = (ukey // n).astype(np.int64)
+    c = (ukey % n).astype(np.int32)
+    ia = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(ia, r + 1, 1)
+    ia = np.cumsum(ia)
+    return (ia + 1).astype(np.int32), (c + 1).astype(np.int32), acc
+
+
+def assemble(mesh: Mesh, ym, nu, rho):
+    """Global K, M (1-based CSR) and Nx/Ny/Nz as mpiglbmtx_m + tricsr build them."""
+    ke, me, Nx, Ny, Nz = element_matrices(mesh, ym, nu, rho)
+    ne = mesh.ne
+    n = 3 * mesh.nn
+    dofs = (3 * (mesh.xconn.astype(np.int64) - 1)[:, :, None] + np.arange(3)[None, None, :]).reshape(ne, 24)
+    rows = np.repeat(dofs[:, :, None], 24, axis=2).ravel()
+    cols = np.repeat(dofs[:, None, :], 24, axis=1).ravel()
+    K = _triplet_to_csr(rows, cols, ke.ravel(), n)
+    M = _triplet_to_csr(rows, cols, me.ravel(), n)
+    return K, M, Nx, Ny, Nz
+
+
+def applybc(S, bc):
+    """csrapplybc (spmtx.f90:268-307): 1-0 method on rows/cols with bc != 0."""
+    ia, ja, a = S
+    n = len(ia) - 1
+    bc = np.asarray(bc) != 0
+    row = np.repeat(np.arange(n), np.diff(ia))
+    col = ja - 1
+    a = a.copy()
+    hit = bc[row] | bc[col]
+    a[hit] = 0.0
+    a[hit & (row == col)] = 1.0
+    keep = a != 0.0
+    cnt = np.bincount(row[keep], minlength=n)
+    ia2 = np.r_[0, np.cumsum(cnt)].astype(np.int32) + 1
+    return ia2, ja[keep].copy(), a[keep].copy()
+
+
+def face_nodes(mesh: Mesh, axis, value, tol=1e-9):
+    return np.flatnonzero(np.abs(mesh.xcoord[:, axis] - value) <= tol)
+
+
+def traction_load(mesh: Mesh, axis, value, direction, total_force, tol=1e-9):
+    """Lumped traction as main3d.f90:508-531: each element-node incidence on the
+    loaded face receives total_force / (number of such incidences)."""
+    on = np.abs(mesh.xcoord[:, axis] - value) <= tol
+    inc = on[mesh.xconn - 1]
+    nt = inc.sum()
+    rext = np.zeros(3 * mesh.nn)
+    if nt == 0:
+        return rext
+    fnode = total_force / float(nt)
+    nodes = (mesh.xconn - 1)[inc]
+    np.add.at(rext, 3 * nodes + direction, fnode)
+    return rext
+
+
+def clamp_mask(mesh: Mesh, axis, values, tol=1e-9):
+    """Spatial DOF mask (length 3*nn) fixing all 3 components of nodes on the
+    planes x_axis == v for v in values (main3d.f90:482-488)."""
+    m = np.zeros(mesh.nn, dtype=bool)
+    for v in np.atleast_1d(values):
+        m |= np.abs(mesh.xcoord[:, axis] - v) <= tol
+    return np.repeat(m, 3).astype(np.int32)
+
+
+def node_degree(mesh: Mesh):
+    """Number of elements sharing each node (main3d.f90:496-501)."""
+    return np.bincount((mesh.xconn - 1).ravel(), minlength=mesh.nn)

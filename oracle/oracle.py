@@ -46,6 +46,7 @@ def lib():
         L.or_getfextc_m.argtypes = [cd, cd, cd, _d, _d]
         L.or_damage_nt.argtypes = [_d, cd, ci, _d]
         L.or_kokkos_host.argtypes = [ci] * 9 + [_i, _d, _d, _d, _d, _d, _d, _d] + [_d] * 9
+        L.or_damage_step.argtypes = [_d, cd, cd, _d, _d, _d, _d, _d]
         L.or_chkelesta.argtypes = [ci, ci, cd, _d, _i, _i, _i]
         L.or_chkelesta.restype = ci
         _LIB = L
@@ -197,6 +198,14 @@ def kokkos_host(ne, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu, state):
                          st["eps_old"], st["eps_pl"], st["sig_eff"], st["sig_back"],
                          st["p"], st["D"], st["ws"], st["del_po"], st["sig_d"])
     return st
+
+
+def damage_step(strain, ym, nu, eo, ep, se, sb, sc):
+    """One return-map step (kokkos_kernel.h:85-341) on copies of the state; returns
+    (eo, ep, se, sb, sc) with sc = (p, D, ws, del_po, sig_d)."""
+    eo, ep, se, sb, sc = (_f64(v).copy() for v in (eo, ep, se, sb, sc))
+    lib().or_damage_step(_f64(strain), float(ym), float(nu), eo, ep, se, sb, sc)
+    return eo, ep, se, sb, sc
 
 
 def chkelesta(ne, ngpe, Dc, D, elesta=None, gpsta=None):

@@ -681,17 +681,140 @@ void or_damage_nt(const double *tcoorde, double w, int nip, double *Nt)
 #define D_EPSD 0.08
 #define D_WD ((D_SIGU - D_SIGF) * D_EPSD)
 
+/* One return-map step of the functor for a given strain (kokkos_kernel.h:85-341).
+ * State: eo/ep/se/sb[6] = eps_old, eps_pl, sig_eff, sig_back; sc[5] = p, D, ws,
+ * del_po, sig_d.  Same operation order as the reference. */
+void or_damage_step(const double *strain, double ym, double nu, double *eo, double *ep,
+                    double *se, double *sb, double *sc)
+{
+    double *p = &sc[0], *d = &sc[1], *ws = &sc[2], *del_po = &sc[3], *sig_d = &sc[4];
+    double a = (1.0 + nu) / (3.0 * (1.0 - nu));
+    double b = 2.0 * (4.0 - 5.0 * nu) / (15.0 * (1.0 - nu));
+    double G = ym / (2.0 * (1.0 + nu));
+    double Kb = ym / (3.0 * (1.0 - 2.0 * nu));
+    double coef = ym / ((1. + nu) * (1. - 2. * nu));
+    double del_p = 0.0;
+    double dmat[6][6] = {{0.0}};
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) dmat[i][j] = coef * nu;
+    for (int i = 0; i < 3; ++i) {
+        dmat[i][i] = coef * (1.0 - nu);
+        dmat[i + 3][i + 3] = coef * (1.0 - 2.0 * nu) / 2.0;
+    }
+    double dd = *d;
+    double c1 = 1.0 / (1.0 - b * dd);
+    double c2 = (a - b) * dd / ((1.0 - b * dd) * (1.0 - a * dd));
+    double c3 = b * (1.0 - dd) / (1.0 - b * dd);
+    double tel[6], tee[6];
+    for (int i = 0; i < 6; ++i) tel[i] = c1 * strain[i] + c3 * ep[i];
+    double tr = 0.0;
+    for (int i = 0; i < 3; ++i) tr += strain[i];
+    tr = c2 * tr / 3.0;
+    for (int i = 0; i < 3; ++i) tel[i] += tr;
+    for (int i = 0; i < 6; ++i) tee[i] = tel[i] - ep[i];
+    double tse[6] = {0.0};
+    for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) tse[i] += dmat[i][j] * tee[j];
+    double trs = 0.0;
+    for (int i = 0; i < 3; ++i) trs += tse[i];
+    trs = trs / 3.0;
+    double tdev[6] = {tse[0] - trs, tse[1] - trs, tse[2] - trs, tse[3], tse[4], tse[5]};
+    double tsig[6];
+    for (int i = 0; i < 6; ++i) tsig[i] = tdev[i] - sb[i];
+    double teq = 0.0;
+    for (int i = 0; i < 3; ++i) teq += 1.5 * tsig[i] * tsig[i];
+    for (int i = 3; i < 6; ++i) teq += 3.0 * tsig[i] * tsig[i];
+    teq = sqrt(teq);
+    if (teq <= D_SIGF) {                                        /* elastic :163 */
+        for (int i = 0; i < 6; ++i) { se[i] = tse[i]; eo[i] = strain[i]; }
+        *sig_d = teq;
+        return;
+    }
+    /* plastic corrector :175 */
+    double dstrain[6], dstress[6] = {0.0};
+    for (int i = 0; i < 6; ++i) dstrain[i] = strain[i] - eo[i];
+    for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j < 6; ++j) dstress[i] += dmat[i][j] * dstrain[j];
+        dstress[i] = dstress[i] / (1.0 - b * dd);
+    }
+    double c4 = Kb * (a - b) * dd / ((1.0 - b * dd) * (1.0 - a * dd));
+    double trd = 0.0;
+    for (int i = 0; i < 3; ++i) trd += dstrain[i];
+    for (int i = 0; i < 3; ++i) dstress[i] += c4 * trd;
+    double Qs[6];
+    for (int i = 0; i < 6; ++i) Qs[i] = sb[i] - se[i] - dstress[i];
+    double gamma = D_CY * (1.0 - dd) + 3.0 * G * (1.0 - b) / (1.0 - b * dd);
+    double trq = 0.0;
+    for (int i = 0; i < 3; ++i) trq += Qs[i];
+    trq = trq / 3.0;
+    double dq[6] = {Qs[0] - trq, Qs[1] - trq, Qs[2] - trq, Qs[3], Qs[4], Qs[5]};
+    double qeq = 0.0;
+    for (int i = 0; i < 3; ++i) qeq += 1.5 * dq[i] * dq[i];
+    for (int i = 3; i < 6; ++i) qeq += 3.0 * dq[i] * dq[i];
+    qeq = sqrt(qeq);
+    double tr_sigs = -trq;
+    del_p = (qeq - D_SIGF) / gamma;
+    *p += del_p;
+    double c5 = 1.0 + gamma * del_p / D_SIGF;
+    double ds[6];
+    for (int i = 0; i < 6; ++i) ds[i] = -dq[i] / c5;
+    double sigs[6] = {ds[0] + tr_sigs, ds[1] + tr_sigs, ds[2] + tr_sigs, ds[3], ds[4], ds[5]};
+    double dpl[6];
+    for (int i = 0; i < 6; ++i) {
+        dpl[i] = 3.0 * ds[i] * del_p / (2.0 * D_SIGF);
+        ep[i] += dpl[i];
+    }
+    for (int i = 0; i < 6; ++i) sb[i] += 2.0 * D_CY * (1.0 - dd) * dpl[i] / 3.0;
+    for (int i = 0; i < 6; ++i) se[i] = sigs[i] + sb[i];
+    double tse2 = 0.0;
+    for (int i = 0; i < 3; ++i) tse2 += se[i];
+    tse2 = tse2 / 3.0;
+    double dse[6];
+    for (int i = 0; i < 3; ++i) dse[i] = se[i] - tse2;
+    for (int i = 3; i < 6; ++i) dse[i] = se[i];
+    double seq = 0.0;
+    for (int i = 0; i < 3; ++i) seq += 1.5 * dse[i] * dse[i];
+    for (int i = 3; i < 6; ++i) seq += 3.0 * dse[i] * dse[i];
+    seq = sqrt(seq);
+    double sig_new = fabs(seq - D_SIGF);
+    double sig_old = fabs(*sig_d - D_SIGF);
+    *ws += (sig_new * del_p + sig_old * *del_po) / 2.0;
+    if (*ws >= D_WD) {                                          /* :293 */
+        double sp[3] = {0.0}, sn[3] = {0.0};
+        for (int i = 0; i < 3; ++i) {
+            if (se[i] < 0.0) sn[i] = se[i];
+            else sp[i] = se[i];
+        }
+        double hr = (1.0 - dd) / (1.0 - D_H * dd);
+        double Yp = (1.0 + nu) / (2.0 * ym) *
+                    (sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2] +
+                     D_H * hr * hr * (sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]));
+        double trt = 0.0;
+        for (int i = 0; i < 3; ++i) trt += se[i];
+        double Yn;
+        if (trt >= 0.0) Yn = nu / (2.0 * ym) * trt * trt;
+        else Yn = nu / (2.0 * ym) * D_H * hr * hr * trt * trt;
+        double Y = Yp - Yn;
+        /* (Y/S)^s with s = 0.5 (damage.h:13) evaluated correctly rounded:
+         * libm pow differs from the exact power in the last bit (glibc
+         * 0.52 ulp) and the return map amplifies last-bit differences at
+         * load reversals, so the platform-independent value is used. */
+        *d += (D_SEXP == 0.5 ? sqrt(Y / D_S) : pow(Y / D_S, D_SEXP)) * del_p;
+    } else {
+        *d = 0.0;
+    }
+    for (int i = 0; i < 6; ++i) eo[i] = strain[i];
+    *sig_d = seq;
+    *del_po = del_p;
+}
+
+/* One Gauss point over its nip time points: displacement interpolation and strain
+ * (kokkos_kernel.h:58-84), then the return map. */
 static void or_damage_gp(int ii, int nn, int ndofn, int nnse, int nnte, int ncmp, int nip,
                          const int *nodes, const double *Nx, const double *Ny, const double *Nz,
                          const double *Nt, const double *sol, const double *ym, const double *nu,
                          double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
                          double *p, double *d, double *ws, double *del_po, double *sig_d)
 {
-    double a = (1.0 + nu[ii]) / (3.0 * (1.0 - nu[ii]));
-    double b = 2.0 * (4.0 - 5.0 * nu[ii]) / (15.0 * (1.0 - nu[ii]));
-    double G = ym[ii] / (2.0 * (1.0 + nu[ii]));
-    double Kb = ym[ii] / (3.0 * (1.0 - 2.0 * nu[ii]));
-    double coef = ym[ii] / ((1. + nu[ii]) * (1. - 2. * nu[ii]));
+    double sc[5] = {p[ii], d[ii], ws[ii], del_po[ii], sig_d[ii]};
     for (int jj = 0; jj < nip; ++jj) {
         double disp[24] = {0.0};
         for (int i = 0; i < nnse; ++i) {
@@ -712,117 +835,10 @@ static void or_damage_gp(int ii, int nn, int ndofn, int nnse, int nnte, int ncmp
             strain[4] += nz * disp[i * ndofn + 1] + ny * disp[i * ndofn + 2];
             strain[5] += nz * disp[i * ndofn + 0] + nx * disp[i * ndofn + 2];
         }
-        double del_p = 0.0;
-        double dmat[6][6] = {{0.0}};
-        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) dmat[i][j] = coef * nu[ii];
-        for (int i = 0; i < 3; ++i) {
-            dmat[i][i] = coef * (1.0 - nu[ii]);
-            dmat[i + 3][i + 3] = coef * (1.0 - 2.0 * nu[ii]) / 2.0;
-        }
-        double dd = d[ii];
-        double c1 = 1.0 / (1.0 - b * dd);
-        double c2 = (a - b) * dd / ((1.0 - b * dd) * (1.0 - a * dd));
-        double c3 = b * (1.0 - dd) / (1.0 - b * dd);
-        double tel[6], tee[6];
-        for (int i = 0; i < 6; ++i) tel[i] = c1 * strain[i] + c3 * eps_pl[ii * ncmp + i];
-        double tr = 0.0;
-        for (int i = 0; i < 3; ++i) tr += strain[i];
-        tr = c2 * tr / 3.0;
-        for (int i = 0; i < 3; ++i) tel[i] += tr;
-        for (int i = 0; i < 6; ++i) tee[i] = tel[i] - eps_pl[ii * ncmp + i];
-        double tse[6] = {0.0};
-        for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) tse[i] += dmat[i][j] * tee[j];
-        double trs = 0.0;
-        for (int i = 0; i < 3; ++i) trs += tse[i];
-        trs = trs / 3.0;
-        double tdev[6] = {tse[0] - trs, tse[1] - trs, tse[2] - trs, tse[3], tse[4], tse[5]};
-        double tsig[6];
-        for (int i = 0; i < 6; ++i) tsig[i] = tdev[i] - sig_back[ii * ncmp + i];
-        double teq = 0.0;
-        for (int i = 0; i < 3; ++i) teq += 1.5 * tsig[i] * tsig[i];
-        for (int i = 3; i < 6; ++i) teq += 3.0 * tsig[i] * tsig[i];
-        teq = sqrt(teq);
-        if (teq <= D_SIGF) {                                    /* elastic :163 */
-            for (int i = 0; i < 6; ++i) { sig_eff[ii * ncmp + i] = tse[i]; eps_old[ii * ncmp + i] = strain[i]; }
-            sig_d[ii] = teq;
-        } else {                                                 /* plastic :175 */
-            double dstrain[6], dstress[6] = {0.0};
-            for (int i = 0; i < 6; ++i) dstrain[i] = strain[i] - eps_old[ii * ncmp + i];
-            for (int i = 0; i < 6; ++i) {
-                for (int j = 0; j < 6; ++j) dstress[i] += dmat[i][j] * dstrain[j];
-                dstress[i] = dstress[i] / (1.0 - b * dd);
-            }
-            double c4 = Kb * (a - b) * dd / ((1.0 - b * dd) * (1.0 - a * dd));
-            double trd = 0.0;
-            for (int i = 0; i < 3; ++i) trd += dstrain[i];
-            for (int i = 0; i < 3; ++i) dstress[i] += c4 * trd;
-            double Qs[6];
-            for (int i = 0; i < 6; ++i) Qs[i] = sig_back[ii * ncmp + i] - sig_eff[ii * ncmp + i] - dstress[i];
-            double gamma = D_CY * (1.0 - dd) + 3.0 * G * (1.0 - b) / (1.0 - b * dd);
-            double trq = 0.0;
-            for (int i = 0; i < 3; ++i) trq += Qs[i];
-            trq = trq / 3.0;
-            double dq[6] = {Qs[0] - trq, Qs[1] - trq, Qs[2] - trq, Qs[3], Qs[4], Qs[5]};
-            double qeq = 0.0;
-            for (int i = 0; i < 3; ++i) qeq += 1.5 * dq[i] * dq[i];
-            for (int i = 3; i < 6; ++i) qeq += 3.0 * dq[i] * dq[i];
-            qeq = sqrt(qeq);
-            double tr_sigs = -trq;
-            del_p = (qeq - D_SIGF) / gamma;
-            p[ii] += del_p;
-            double c5 = 1.0 + gamma * del_p / D_SIGF;
-            double ds[6];
-            for (int i = 0; i < 6; ++i) ds[i] = -dq[i] / c5;
-            double sigs[6] = {ds[0] + tr_sigs, ds[1] + tr_sigs, ds[2] + tr_sigs, ds[3], ds[4], ds[5]};
-            double dpl[6];
-            for (int i = 0; i < 6; ++i) {
-                dpl[i] = 3.0 * ds[i] * del_p / (2.0 * D_SIGF);
-                eps_pl[ii * ncmp + i] += dpl[i];
-            }
-            for (int i = 0; i < 6; ++i) sig_back[ii * ncmp + i] += 2.0 * D_CY * (1.0 - dd) * dpl[i] / 3.0;
-            for (int i = 0; i < 6; ++i) sig_eff[ii * ncmp + i] = sigs[i] + sig_back[ii * ncmp + i];
-            double tse2 = 0.0;
-            for (int i = 0; i < 3; ++i) tse2 += sig_eff[ii * ncmp + i];
-            tse2 = tse2 / 3.0;
-            double dse[6];
-            for (int i = 0; i < 3; ++i) dse[i] = sig_eff[ii * ncmp + i] - tse2;
-            for (int i = 3; i < 6; ++i) dse[i] = sig_eff[ii * ncmp + i];
-            double seq = 0.0;
-            for (int i = 0; i < 3; ++i) seq += 1.5 * dse[i] * dse[i];
-            for (int i = 3; i < 6; ++i) seq += 3.0 * dse[i] * dse[i];
-            seq = sqrt(seq);
-            double sig_new = fabs(seq - D_SIGF);
-            double sig_old = fabs(sig_d[ii] - D_SIGF);
-            ws[ii] += (sig_new * del_p + sig_old * del_po[ii]) / 2.0;
-            if (ws[ii] >= D_WD) {                               /* :293 */
-                double sp[3] = {0.0}, sn[3] = {0.0};
-                for (int i = 0; i < 3; ++i) {
-                    if (sig_eff[ii * ncmp + i] < 0.0) sn[i] = sig_eff[ii * ncmp + i];
-                    else sp[i] = sig_eff[ii * ncmp + i];
-                }
-                double hr = (1.0 - dd) / (1.0 - D_H * dd);
-                double Yp = (1.0 + nu[ii]) / (2.0 * ym[ii]) *
-                            (sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2] +
-                             D_H * hr * hr * (sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]));
-                double trt = 0.0;
-                for (int i = 0; i < 3; ++i) trt += sig_eff[ii * ncmp + i];
-                double Yn;
-                if (trt >= 0.0) Yn = nu[ii] / (2.0 * ym[ii]) * trt * trt;
-                else Yn = nu[ii] / (2.0 * ym[ii]) * D_H * hr * hr * trt * trt;
-                double Y = Yp - Yn;
-                /* (Y/S)^s with s = 0.5 (damage.h:13) evaluated correctly rounded:
-                 * libm pow differs from the exact power in the last bit (glibc
-                 * 0.52 ulp) and the return map amplifies last-bit differences at
-                 * load reversals, so the platform-independent value is used. */
-                d[ii] += (D_SEXP == 0.5 ? sqrt(Y / D_S) : pow(Y / D_S, D_SEXP)) * del_p;
-            } else {
-                d[ii] = 0.0;
-            }
-            for (int i = 0; i < 6; ++i) eps_old[ii * ncmp + i] = strain[i];
-            sig_d[ii] = seq;
-            del_po[ii] = del_p;
-        }
+        or_damage_step(strain, ym[ii], nu[ii], eps_old + ii * ncmp, eps_pl + ii * ncmp,
+                       sig_eff + ii * ncmp, sig_back + ii * ncmp, sc);
     }
+    p[ii] = sc[0]; d[ii] = sc[1]; ws[ii] = sc[2]; del_po[ii] = sc[3]; sig_d[ii] = sc[4];
 }
 
 /* kokkos_host_ (kokkos_host.cpp:37-190): run the functor over all ngpe*ne    */

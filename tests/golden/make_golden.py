@@ -117,7 +117,7 @@ def main():
         assert ierr == 0, ierr
         g[f"gmres_{tag}_sol"] = sol
     # two-scale damage return map (getdfinal.f) on seeded strain paths
-    ngp, nsteps = 24, 400
+    ngp, nsteps = 24, 1200
     E, nu = YM_REF, NU_REF
     G = E / (2 * (1 + nu))
     cf = E / ((1 + nu) * (1 - 2 * nu))
@@ -133,10 +133,10 @@ def main():
                 dmat[i, j] = cf * nu
                 cmat[i, j] = -nu / E
     params = np.zeros((ngp, 13))
-    out6 = np.zeros((ngp, 4, 6))
-    out1 = np.zeros((ngp, 5))
+    in6, in1, strains, out6, out1 = [], [], [], [], []
     for k in range(ngp):
-        amp = rng.uniform(0.0008, 0.004)
+        lo, hi = ((0.0008, 0.004), (0.01, 0.03), (0.08, 0.15))[k % 3]
+        amp = rng.uniform(lo, hi)
         dirs = rng.standard_normal(6)
         dirs /= np.linalg.norm(dirs)
         ph = rng.uniform(0, 2 * np.pi, 6)
@@ -144,15 +144,22 @@ def main():
         path = strain_history(nsteps, amp, dirs, ph)
         eo = np.zeros(6); ep = np.zeros(6); se = np.zeros(6); sb = np.zeros(6)
         sc = (0.0, 0.0, 0.0, 0.0, 0.0)
-        for s in range(nsteps):
-            st = np.ascontiguousarray(path[s])
+        for s_ in range(nsteps):
+            st = np.ascontiguousarray(path[s_])
+            rec = s_ % 48 == 47
+            if rec:
+                in6.append(np.array([eo, ep, se, sb]))
+                in1.append(np.array(sc))
+                strains.append(st.copy())
             sc = ref.getdfinal(eo, st, ep, *sc[:3], se, sb, *sc[3:], cmat, dmat)
-        out6[k] = [eo, ep, se, sb]
-        out1[k] = sc
+            if rec:
+                out6.append(np.array([eo, ep, se, sb]))
+                out1.append(np.array(sc))
+    # single return-map steps of getdfinal sampled along its own trajectories
     g["dmg_params"] = params
     g["dmg_nsteps"] = np.array(nsteps)
-    g["dmg_out6"] = out6
-    g["dmg_out1"] = out1
+    g["dmg_in6"], g["dmg_in1"], g["dmg_strain"] = np.array(in6), np.array(in1), np.array(strains)
+    g["dmg_out6"], g["dmg_out1"] = np.array(out6), np.array(out1)
     # element status (chkelesta)
     D = rng.uniform(0.0, 0.35, 8 * mesh.ne)
     nefail, elesta, gpsta = ref.chkelesta(mesh.nn, mesh.ne, 8, 0.3, D)

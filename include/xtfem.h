@@ -119,8 +119,15 @@ typedef struct xt_prec_s *xt_prec;    /* ILUT factors + level schedule          
 int xt_op_create(int p, int q, int nterms, const int *const *rowptr,
                  const int *const *colidx, const double *const *vals, xt_op *out);
 void xt_op_destroy(xt_op op);
-/* nnz of term k, and the device bytes one FP64/FP32 application streams. */
+/* nnz of term k (CSR form). */
 long long xt_op_nnz(xt_op op, int k);
+/* Kernel selection: 0 (default) = node-block form when the operator has one (two
+ * terms K, M of a 3-DOF mesh with diagonal M blocks, q = 6), 1 = CSR form. */
+int xt_op_set_format(xt_op op, int format);
+/* Format used by xt_op_apply: 0 node blocks, 1 CSR. */
+int xt_op_format(xt_op op);
+/* Stored node blocks (real != 0: without slice padding); 0 without node-block form. */
+long long xt_op_blocks(xt_op op, int real);
 
 /* Y := alpha * sum_k S_k X T_{k,r}^T + beta * Y on R right-hand sides.
  * tmat: device array [R][nterms][q][q] (row-major T_{k,r}(i,j)), same dtype as

@@ -183,8 +183,42 @@ def test_kron_fp32_operator(xt, oracle_lib):
     B = c["m"] + c["mtn"]
     X = rng.standard_normal((R, 6 * S["p"]))
     op = KronOperator([S["Kb"], S["Mb"]], q=6)
-    for fp32, tol in ((False, 1e-12), (True, 2e-5)):
-        Y = op.apply_host(A, B, X, fp32=fp32)
-        for r in range(R):
-            ref = oracle_lib.kronspmv(A[r], B[r], S["Kb"], S["Mb"], X[r])
-            assert rel(Y[r], ref) < tol, (fp32, r, rel(Y[r], ref))
+    assert op.format == "blocks" and op.blocks_real > 0
+    for fmt in ("blocks", "csr"):
+        op.format = fmt
+        for fp32, tol in ((False, 1e-12), (True, 2e-5)):
+            Y = op.apply_host(A, B, X, fp32=fp32)
+            for r in range(R):
+                ref = oracle_lib.kronspmv(A[r], B[r], S["Kb"], S["Mb"], X[r])
+                assert rel(Y[r], ref) < tol, (fmt, fp32, r, rel(Y[r], ref))
+
+
+@pytest.mark.parametrize("R", [1, 3, 8, 33, 70])
+def test_kron_blocks_many_load_cases(xt, oracle_lib, R):
+    """Node-block kernel with lanes-per-node 1..64 and the chunking above 64 load
+    cases; beta != 0 (kronspgemv form) on the notched plate geometry."""
+    from xtfem_amd import fem, slab
+    from xtfem_amd.device import KronOperator
+    keep = np.ones((14, 9, 2), bool)
+    keep[0:5, 4, :] = False
+    mesh = fem.structured_hex(14, 9, 2, 14.0, 9.0, 2.0, keep=keep)
+    K, M, *_ = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+    bc = fem.clamp_mask(mesh, 1, [0.0])
+    Kb, Mb = fem.applybc(K, bc), fem.applybc(M, bc)
+    p = 3 * mesh.nn
+    rng = np.random.default_rng(R)
+    c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))
+    A, B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+    X = rng.standard_normal((R, 6 * p))
+    Y0 = rng.standard_normal((R, 6 * p))
+    op = KronOperator([Kb, Mb], q=6)
+    assert op.format == "blocks"
+    import torch
+    from xtfem_amd.device import to_interleaved, from_interleaved
+    T = op.tmat(np.stack([A, B], axis=1), False)
+    x, y = to_interleaved(X, p, 6), to_interleaved(Y0, p, 6)
+    op.apply(T, x, y, R, alpha=0.5, beta=-2.0)
+    Y = from_interleaved(y, p, 6, R)
+    for r in range(R):
+        ref = oracle_lib.kronspgemv(A[r], B[r], 0.5, -2.0, Kb, Mb, X[r], Y0[r])
+        assert rel(Y[r], ref) < 1e-12, r

@@ -41,6 +41,9 @@ _SIGS = {
     "xt_op_create": (ci, [ci, ci, ci, vp, vp, vp, vp]),
     "xt_op_destroy": (None, [vp]),
     "xt_op_nnz": (C.c_longlong, [vp, ci]),
+    "xt_op_set_format": (ci, [vp, ci]),
+    "xt_op_format": (ci, [vp]),
+    "xt_op_blocks": (C.c_longlong, [vp, ci]),
     "xt_op_apply": (ci, [vp, ci, vp, cd, cd, vp, vp, ci, vp]),
     "xt_prec_create": (ci, [ci, vp, vp, vp, vp]),
     "xt_prec_destroy": (None, [vp]),

@@ -19,7 +19,7 @@ OUT = os.path.join(HERE, "libxtfem_hip.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("XTFEM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["xt_kron.hip", "xt_prec.hip", "xt_gmres.hip", "xt_damage.hip", "xt_capi.hip",
+SOURCES = ["xt_kron.hip", "xt_bsr.hip", "xt_prec.hip", "xt_gmres.hip", "xt_damage.hip", "xt_capi.hip",
            "xt_ilut.cpp"]
 
 # Packed FP32 instructions are disabled for all device code (-packed-fp32-ops).

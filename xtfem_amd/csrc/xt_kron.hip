@@ -206,9 +206,24 @@ from_inter_kernel(long long n, int p, int q, int R, const double *__restrict__ s
 
 using namespace xt;
 
+namespace xt {
+struct BsrPlan;
+int bsr_build(int p, const int *krp, const int *kci, const double *kv, const int *mrp,
+              const int *mci, const double *mv, BsrPlan &P);
+template <typename T>
+int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta, const T *x,
+              T *y, bool f32, hipStream_t s);
+BsrPlan *bsr_new();
+void bsr_delete(BsrPlan *);
+long long bsr_blocks(const BsrPlan *, int real);
+}  // namespace xt
+
 struct xt_op_s {
     int p = 0, q = 0, nterms = 0;
     DevCSR S[3];
+    xt::BsrPlan *bsr = nullptr;  // node-block form (3-DOF mechanics, q = 6), if applicable
+    int format = 0;              // 0: node blocks when available, 1: CSR
+    ~xt_op_s() { if (bsr) xt::bsr_delete(bsr); }
 };
 
 namespace xt {
@@ -243,8 +258,42 @@ int xt_op_create(int p, int q, int nterms, const int *const *rowptr, const int *
         int rc = csr_upload_1based(p, rowptr[t], colidx[t], vals[t], op->S[t], true, 0);
         if (rc != XT_OK) { delete op; return rc; }
     }
+    if (nterms == 2 && q == 6 && p % 3 == 0) {
+        std::vector<int> krp(p + 1), mrp(p + 1);
+        for (int i = 0; i <= p; ++i) { krp[i] = rowptr[0][i] - 1; mrp[i] = rowptr[1][i] - 1; }
+        const long long nk = krp[p], nm = mrp[p];
+        std::vector<int> kci(nk), mci(nm);
+        for (long long e = 0; e < nk; ++e) kci[e] = colidx[0][e] - 1;
+        for (long long e = 0; e < nm; ++e) mci[e] = colidx[1][e] - 1;
+        op->bsr = bsr_new();
+        int rc = bsr_build(p, krp.data(), kci.data(), vals[0], mrp.data(), mci.data(), vals[1],
+                           *op->bsr);
+        if (rc == XT_E_UNSUPPORTED) {
+            bsr_delete(op->bsr);
+            op->bsr = nullptr;
+        } else if (rc != XT_OK) {
+            delete op;
+            return rc;
+        }
+    }
     *out = op;
     return XT_OK;
+}
+
+int xt_op_set_format(xt_op op, int format) {
+    XT_ARG(op && (format == 0 || format == 1), "op format");
+    op->format = format;
+    return XT_OK;
+}
+
+int xt_op_format(xt_op op) {
+    if (!op) return -1;
+    return (op->bsr && op->format == 0) ? 0 : 1;
+}
+
+long long xt_op_blocks(xt_op op, int real) {
+    if (!op || !op->bsr) return 0;
+    return bsr_blocks(op->bsr, real);
 }
 
 void xt_op_destroy(xt_op op) { delete op; }
@@ -259,6 +308,13 @@ int xt_op_apply(xt_op op, int R, const void *tmat, double alpha, double beta, co
     XT_ARG(op && R > 0 && tmat && x && y, "op_apply arguments");
     hipStream_t s = (hipStream_t)stream;
     const int nt = op->nterms;
+    if (op->bsr && op->format == 0) {
+        if (fp32)
+            return bsr_apply<float>(*op->bsr, R, (const float *)tmat, alpha, beta,
+                                    (const float *)x, (float *)y, true, s);
+        return bsr_apply<double>(*op->bsr, R, (const double *)tmat, alpha, beta,
+                                 (const double *)x, (double *)y, false, s);
+    }
     if (fp32) {
         const float *tm = (const float *)tmat;
         const float *xx = (const float *)x;

@@ -45,6 +45,17 @@ class KronOperator:
         check(lib().xt_op_create(self.p, q, self.nterms, rp, cj, vv, C.addressof(h)), "op_create")
         self.h = h
         self.nnz = [int(lib().xt_op_nnz(h, k)) for k in range(self.nterms)]
+        self.blocks = int(lib().xt_op_blocks(h, 0))
+        self.blocks_real = int(lib().xt_op_blocks(h, 1))
+
+    @property
+    def format(self):
+        """'blocks' (node-block slices) or 'csr'."""
+        return "blocks" if lib().xt_op_format(self.h) == 0 else "csr"
+
+    @format.setter
+    def format(self, name):
+        check(lib().xt_op_set_format(self.h, 0 if name == "blocks" else 1), "op_set_format")
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -52,13 +63,21 @@ class KronOperator:
             lib().xt_op_destroy(h)
             self.h = None
 
-    def algorithmic_bytes(self, R, fp32, beta=False):
-        """Bytes one application moves in the stored CSR format, read once and written
-        once: values + column indices + row pointers of every term, X read, Y written."""
+    def algorithmic_bytes(self, R, fp32, beta=False, fmt=None, padded=True):
+        """Bytes one application moves in the stored format, each read once and written
+        once: matrix (values + indices + row/slice pointers), X read, Y written.
+        CSR: nnz_k (s_v + 4) + 4 (p + 1) per term.  Node blocks: per stored block 12
+        values (3x3 K + diagonal M) + one int32 column node, plus slice tables."""
         sv = 4 if fp32 else 8
         n = self.p * self.q * R
-        b = sum(nz * (sv + 4) + 4 * (self.p + 1) for nz in self.nnz) + 2 * n * sv
-        return b + (n * sv if beta else 0)
+        fmt = fmt or self.format
+        if fmt == "blocks":
+            nb = self.blocks if padded else self.blocks_real
+            nsl = -(-(self.p // 3) // 64)
+            mat = nb * (12 * sv + 4) + nsl * (64 * 4 + 8)
+        else:
+            mat = sum(nz * (sv + 4) + 4 * (self.p + 1) for nz in self.nnz)
+        return mat + 2 * n * sv + (n * sv if beta else 0)
 
     def flops(self, R):
         c = self.q * R

@@ -114,6 +114,29 @@ def test_structured_mesh_and_notch():
     assert abs(me[:, 0::3, 0::3].sum() - vol) < 1e-12 * vol
 
 
+def test_node_orderings_are_permutations():
+    """x-fastest and z-fastest numberings give the same stiffness up to a node
+    permutation (matched through the coordinates)."""
+    from xtfem_amd import fem
+    keep = np.ones((6, 4, 2), bool)
+    keep[0:2, 2, :] = False
+    ma = fem.structured_hex(6, 4, 2, 6.0, 4.0, 2.0, keep=keep, order="zyx")
+    mb = fem.structured_hex(6, 4, 2, 6.0, 4.0, 2.0, keep=keep, order="xyz")
+    assert ma.nn == mb.nn and ma.ne == mb.ne
+    key = lambda m: np.lexsort(m.xcoord.T[::-1])
+    perm = np.empty(ma.nn, int)
+    perm[key(ma)] = key(mb)                      # node i of a is node perm[i] of b
+    assert np.array_equal(ma.xcoord, mb.xcoord[perm])
+    Ka = fem.assemble(ma, 2e5, 0.3, 1.0)[0]
+    Kb = fem.assemble(mb, 2e5, 0.3, 1.0)[0]
+    import scipy.sparse as sp
+    dense = lambda S, n: sp.csr_matrix((S[2], S[1] - 1, S[0] - 1), shape=(n, n)).toarray()
+    n = 3 * ma.nn
+    d = (3 * perm[:, None] + np.arange(3)).ravel()
+    A, B = dense(Ka, n), dense(Kb, n)[np.ix_(d, d)]
+    assert np.max(np.abs(A - B)) <= 1e-9 * np.max(np.abs(A))
+
+
 def test_traction_lumping_sums_to_total_force():
     from xtfem_amd import fem
     m = fem.structured_hex(4, 3, 2, 4.0, 3.0, 2.0)

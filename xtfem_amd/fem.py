@@ -62,10 +62,12 @@ def read_inp(path) -> Mesh:
     return Mesh(xc, xe)
 
 
-def structured_hex(nx, ny, nz, lx, ly, lz, origin=(0.0, 0.0, 0.0), keep=None) -> Mesh:
+def structured_hex(nx, ny, nz, lx, ly, lz, origin=(0.0, 0.0, 0.0), keep=None,
+                   order="zyx") -> Mesh:
     """Structured C3D8 grid of nx*ny*nz elements over [0,lx]x[0,ly]x[0,lz].
 
-    Nodes are numbered z fastest, then y, then x.  ``keep`` is an optional boolean
+    ``order`` names the node numbering from fastest to slowest axis ("zyx": z
+    fastest, then y, then x; "xyz": x fastest).  ``keep`` is an optional boolean
     array [nx, ny, nz] of elements to retain (notches); orphan nodes are dropped and
     the survivors renumbered in the same order.  Local node order follows the
     reference's shape3x so det(J) > 0.
@@ -73,11 +75,20 @@ def structured_hex(nx, ny, nz, lx, ly, lz, origin=(0.0, 0.0, 0.0), keep=None) ->
     xs = origin[0] + np.linspace(0.0, lx, nx + 1)
     ys = origin[1] + np.linspace(0.0, ly, ny + 1)
     zs = origin[2] + np.linspace(0.0, lz, nz + 1)
-    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
-    xcoord = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    n1 = {"x": nx + 1, "y": ny + 1, "z": nz + 1}
+    ax = {"x": 0, "y": 1, "z": 2}
+    o_fast, o_mid, o_slow = order[0], order[1], order[2]
 
     def nid(i, j, k):
-        return (i * (ny + 1) + j) * (nz + 1) + k
+        ijk = (i, j, k)
+        return (ijk[ax[o_slow]] * n1[o_mid] + ijk[ax[o_mid]]) * n1[o_fast] + ijk[ax[o_fast]]
+
+    I0, J0, K0 = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    xcoord = np.zeros(((nx + 1) * (ny + 1) * (nz + 1), 3))
+    ids = nid(I0.ravel(), J0.ravel(), K0.ravel())
+    xcoord[ids, 0] = xs[I0.ravel()]
+    xcoord[ids, 1] = ys[J0.ravel()]
+    xcoord[ids, 2] = zs[K0.ravel()]
 
     I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     I, J, K = I.ravel(), J.ravel(), K.ravel()

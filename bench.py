@@ -1,0 +1,334 @@
+#!/usr/bin/env python3
+"""Space-time slab solves on one MI355X.
+
+Workload: the single-edge-notched plate, C3D8 grid 100 x 46 x 4 elements over
+100 x 46 x 4 mm with a 20 x 1 element strip removed at mid-height (x < 20 mm,
+23 mm < y < 24 mm), 3 displacement DOF per node (p = 71,205 before the notch),
+clamped at y = 0, tensile traction on y = 46, linear elastic steel (E = 2e5 MPa,
+nu = 0.3, rho = 7.85e-9 t/mm^3), n_t = 6.  R load cases: case 0 is 80 MPa at
+20 Hz, the others draw amplitudes U[20, 120] MPa and frequencies U[10, 100] Hz from a
+fixed seed.  Each case's slab spans a whole number of load periods closest to 0.5 s
+(dt_r = round(f_r / 2) / f_r; 0.5 s = 10 periods for case 0), the regime in which the
+reference's t1 = 0 temporal matrices (main3d.f90:816) describe every slab.
+
+One step = one space-time slab for every load case, all on the device:
+  jump term   b = (ktn1_r (x) K + mtn1_r (x) M) x_prev_r          (kronspgemv)
+  load        b += fextc_r (x) amp_r rext, fixed DOFs zeroed       (main3d.f90:803-823)
+  solve       (A_r (x) K + B_r (x) M) x_r = b_r, x0 = 0, GMRES(30) to 1e-8 relative
+              residual, right preconditioner A_r^-1 (x) K^-1 (exact banded factor).
+value: slab solves per second of the batched FP32-IR solve (FP32 operator and factor
+sweeps inside FP64 GMRES iterative refinement), every solve checked at true FP64
+relative residual <= 1e-8.  The plain FP64 GPU GMRES (one load case per solve) runs
+the same slabs for the comparison.  Setup (assembly, upload, factorization) is
+outside the timed region and reported separately.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+SEED = 20261202
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cases", type=int, default=32, help="load cases per batched solve")
+    ap.add_argument("--grid", type=int, nargs=3, default=[100, 46, 4])
+    ap.add_argument("--notch", type=int, default=20)
+    ap.add_argument("--block", type=int, default=0, help="factor block rows (0: bandwidth)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
+    return ap.parse_args()
+
+
+class Problem:
+    """Mesh, spatial matrices and per-case slab data of the notched plate."""
+
+    def __init__(self, nx, ny, nz, notch, R):
+        from xtfem_amd import fem, slab
+        keep = np.ones((nx, ny, nz), bool)
+        keep[0:notch, ny // 2, :] = False
+        self.mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz), keep=keep)
+        K, M, *_ = fem.assemble(self.mesh, 2.0e5, 0.3, 7.85e-9)
+        self.bc = fem.clamp_mask(self.mesh, 1, [0.0])
+        self.Kb, self.Mb = fem.applybc(K, self.bc), fem.applybc(M, self.bc)
+        self.p = 3 * self.mesh.nn
+        rng = np.random.default_rng(SEED)
+        self.amp = np.concatenate([[80.0], rng.uniform(20.0, 120.0, R - 1)])
+        self.freq = np.concatenate([[20.0], rng.uniform(10.0, 100.0, R - 1)])
+        self.w = 2 * np.pi * self.freq
+        self.dt = np.round(0.5 * self.freq) / self.freq
+        # t1 = 0 in stcoeff_m, as the reference's call passes it (main3d.f90:816)
+        c = slab.stcoeff_m(0.0, self.dt, self.w)
+        self.A, self.B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+        self.J1, self.J2 = c["ktn1"], c["mtn1"]
+        # unit traction on y = ny: total force = area of the loaded face
+        self.rext = fem.traction_load(self.mesh, 1, float(ny), 1, float(nx * nz))
+
+    def coef(self, step, cases):
+        """fextc_r(j) * amp_r for slab `step` (tcoorde(1) = step * dt_r, main3d.f90:806)."""
+        from xtfem_amd import slab
+        fx, _ = slab.getfextc_m(step * self.dt[cases], self.w[cases], self.dt[cases])
+        return (np.atleast_2d(fx) * self.amp[cases, None]).ravel()
+
+
+class Runner:
+    """Device state of one slab sequence (R load cases) and its step."""
+
+    def __init__(self, prob, op, pr, cases, mode, nsteps):
+        import torch
+        from xtfem_amd import device
+        self.prob, self.op, self.pr, self.mode = prob, op, pr, mode
+        self.cases = np.asarray(cases)
+        self.R = len(self.cases)
+        p, R = prob.p, self.R
+        dev = torch.device("cuda")
+        self.tj = op.tmat(np.stack([prob.J1[self.cases], prob.J2[self.cases]], axis=1), False)
+        self.rext = torch.as_tensor(prob.rext, device=dev)
+        self.fixed = torch.as_tensor(prob.bc.astype(np.uint8), device=dev)
+        self.coefs = torch.as_tensor(np.stack([prob.coef(n, self.cases) for n in range(nsteps)]),
+                                     device=dev)
+        self.b = torch.zeros(p * 6 * R, dtype=torch.float64, device=dev)
+        self.x = torch.zeros_like(self.b)
+        self.xprev = torch.zeros_like(self.b)
+        self.A, self.B = prob.A[self.cases], prob.B[self.cases]
+        self.device = device
+        self.n = 0
+        self.its, self.relres, self.ierr = [], [], []
+
+    def step(self):
+        from xtfem_amd._lib import check, lib
+        import ctypes as C
+        import torch
+        s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.op.apply(self.tj, self.xprev, self.b, self.R, alpha=1.0, beta=0.0)
+        check(lib().xt_slab_rhs(self.prob.p, 6, self.R, self.coefs[self.n].data_ptr(),
+                                self.rext.data_ptr(), self.fixed.data_ptr(), self.b.data_ptr(), s),
+              "slab_rhs")
+        self.x.zero_()
+        _, its, ierr, relres = self.device.slab_solve_device(
+            self.op, self.pr, self.A, self.B, self.b, self.x, self.R, im=30, eps=1e-8,
+            maxits=300, mode=self.mode)
+        self.its.append(its)
+        self.ierr.append(ierr)
+        self.relres.append(relres)
+        self.xprev, self.x = self.x, self.xprev
+        self.n += 1
+
+
+def timed(run, K):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        run.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(run, K, mask):
+    """Second pass over K steps with per-launch HIP events on the launching stream."""
+    import ctypes as C
+    from xtfem_amd._lib import check, lib
+    check(lib().xt_prof_enable(mask), "prof_enable")
+    timed(run, K)
+    out = {}
+    for kind, name in ((0, "kron_spmm"), (1, "band_sweep"), (3, "damage")):
+        if not (mask >> kind) & 1:
+            continue
+        ms, n, by, fl = C.c_double(), C.c_longlong(), C.c_double(), C.c_double()
+        check(lib().xt_prof_read(kind, C.addressof(ms), C.addressof(n), C.addressof(by),
+                                 C.addressof(fl)), "prof_read")
+        if n.value:
+            out[name] = dict(ms=ms.value, launches=n.value, bytes=by.value, flops=fl.value,
+                             avg_us=1e3 * ms.value / n.value,
+                             gbs=by.value / (ms.value * 1e-3) / 1e9)
+    check(lib().xt_prof_enable(0), "prof_enable")
+    return out
+
+
+def spmm_microbench(op, p, reps=20, warm=3):
+    """Kron-SpMM alone, R = 1, FP32: median per-launch time over reps after warm-ups."""
+    import ctypes as C
+    import torch
+    from xtfem_amd import slab
+    from xtfem_amd._lib import check, lib
+    c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * 20.0)
+    T = op.tmat(np.stack([c["k"] + c["ktn"], c["m"] + c["mtn"]])[None], True)
+    g = torch.Generator(device="cuda").manual_seed(SEED)
+    x = torch.rand(p * 6, device="cuda", generator=g, dtype=torch.float32) * 2 - 1
+    y = torch.empty_like(x)
+    for _ in range(warm):
+        op.apply(T, x, y, 1, fp32=True)
+    times = []
+    for _ in range(reps):
+        check(lib().xt_prof_enable(1), "prof_enable")
+        op.apply(T, x, y, 1, fp32=True)
+        ms, n, by, fl = C.c_double(), C.c_longlong(), C.c_double(), C.c_double()
+        check(lib().xt_prof_read(0, C.addressof(ms), C.addressof(n), C.addressof(by),
+                                 C.addressof(fl)), "prof_read")
+        times.append(ms.value)
+    check(lib().xt_prof_enable(0), "prof_enable")
+    med = float(np.median(times))
+    return dict(R=1, dtype="f32", median_us=1e3 * med, bytes=by.value,
+                gbs=by.value / (med * 1e-3) / 1e9, frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS)
+
+
+def cpu_baseline(prob, budget_s=15.0):
+    """The oracle on the host: FP64 restatement of kronpgmres (kronecker.f90:181-365,
+    MGS, one core) preconditioned by A^-1 (x) K^-1 with the exact band Cholesky factor
+    of K (the factor the GPU path applies), on consecutive slabs of load case 0.  The
+    factorization is timed separately and excluded, as on the GPU."""
+    from oracle import oracle
+    from xtfem_amd import slab
+    p = prob.p
+    t0 = time.perf_counter()
+    bf = oracle.BandFactor(prob.Kb)
+    t_fact = time.perf_counter() - t0
+    if bf.info != 0:
+        return None
+    fixed = np.tile(prob.bc, 6)
+    x = np.zeros(6 * p)
+    n, t_solve, its_all = 0, 0.0, []
+    while n == 0 or (t_solve < budget_s and n < 8):
+        t0 = time.perf_counter()
+        jump = oracle.kronspmv(prob.J1[0], prob.J2[0], prob.Kb, prob.Mb, x)
+        rhs = slab.slab_rhs(p, prob.coef(n, np.array([0])), None, prob.rext, jump=jump,
+                            fixed=fixed)
+        x, ie, its, _ = oracle.kronpgmres(prob.A[0], prob.B[0], prob.Kb, prob.Mb, bf, rhs,
+                                          im=30, eps=1e-8, maxits=300)
+        t_solve += time.perf_counter() - t0
+        its_all.append(its)
+        n += 1
+    return dict(value=n / t_solve, unit="slab solves/s", cores=1, kind="port",
+                sample=f"{n} consecutive slab(s) of load case 0 (80 MPa, 20 Hz): jump term, "
+                       f"RHS and oracle FP64 GMRES(30) to 1e-8 with the exact band Cholesky "
+                       f"factor (half-bandwidth {bf.w}; factorization {t_fact:.1f} s excluded), "
+                       f"its {its_all}, {t_solve:.1f} s on one host core")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
+              "(independent load-case groups or meshes would shard across GPUs).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from xtfem_amd import device
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    R, K, W = args.cases, args.steps, args.warmup
+    nx, ny, nz = args.grid
+    t0 = time.perf_counter()
+    prob = Problem(nx, ny, nz, args.notch, R)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    op = device.KronOperator([prob.Kb, prob.Mb], q=6)
+    pr = device.BandPreconditioner(prob.Kb, block=args.block)
+    torch.cuda.synchronize()
+    t_fac = time.perf_counter() - t0
+    print(f"# setup: p={prob.p} assembly {t_asm:.1f}s, upload+factor {t_fac:.1f}s "
+          f"(bandwidth {pr.bandwidth}, {pr.nblocks} blocks of {pr.block})", file=sys.stderr,
+          flush=True)
+
+    nsteps = W + K
+    run32 = Runner(prob, op, pr, np.arange(R), mode=1, nsteps=2 * nsteps)
+    for _ in range(W):
+        run32.step()
+    t32 = timed(run32, K)
+    its32 = np.array(run32.its[W:])
+    rr32 = np.array(run32.relres[W:])
+    ok32 = bool(np.all(np.array(run32.ierr[W:]) == 0) and np.all(rr32 <= 1e-8))
+    print(f"# fp32-ir batched: {K} steps x {R} cases in {t32:.3f}s, its max {its32.max()}, "
+          f"relres max {rr32.max():.2e}", file=sys.stderr, flush=True)
+    from xtfem_amd.device import from_interleaved
+    print(f"# fp32-ir its per case (last slab): {run32.its[-1].tolist()}", file=sys.stderr)
+    x32 = from_interleaved(run32.xprev, prob.p, 6, R)          # every case after W + K slabs
+    prof = kernel_profile(run32, K, mask=0b11)
+
+    res = {}
+    if not args.no_fp64:
+        # plain FP64 GPU GMRES on the same slabs, one load case per solve
+        runs = [Runner(prob, op, pr, np.array([r]), mode=0, nsteps=nsteps) for r in range(R)]
+        for _ in range(W):
+            for rn in runs:
+                rn.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            for rn in runs:
+                rn.step()
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        rr64 = np.array([rn.relres[W:] for rn in runs])
+        its64 = np.array([rn.its[W:] for rn in runs])
+        res["fp64_slabs_per_s"] = K * R / t64
+        res["fp64_ms_per_slab"] = 1e3 * t64 / (K * R)
+        res["fp64_its_mean"] = float(its64.mean())
+        res["fp64_its_max"] = int(its64.max())
+        res["relres_fp64_max"] = float(rr64.max())
+        # every case of the batched FP32-IR run vs its FP64 run, both after W + K slabs
+        x64 = np.stack([from_interleaved(rn.xprev, prob.p, 6, 1)[0] for rn in runs])
+        err = np.linalg.norm(x32 - x64, axis=1) / np.linalg.norm(x64, axis=1)
+        res["rel_err_vs_fp64_max"] = float(err.max())
+        print(f"# fp64 one case per solve: {K}x{R} slabs in {t64:.3f}s, its mean "
+              f"{its64.mean():.1f} max {its64.max()}, relres max {rr64.max():.2e}, "
+              f"rel err fp32-ir vs fp64 max {err.max():.2e}", file=sys.stderr, flush=True)
+    spmm = spmm_microbench(op, prob.p)
+
+    value = K * R / t32
+    dom_name = max(prof, key=lambda k: prof[k]["ms"])
+    dom = prof[dom_name]
+    out = {
+        "metric": "slab solves/s at rel.res 1e-8 (FP32-IR batched vs FP64 GPU); Kron-SpMM GB/s",
+        "value": value,
+        "unit": "slab solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * t32 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 operator/factor sweeps in f64 GMRES-IR",
+        "data": "synthetic (structured notched-plate mesh and seeded load cases)",
+        "config": {"workload": f"notched plate {nx}x{ny}x{nz} C3D8, notch {args.notch}x1, "
+                               f"3 DOF/node, p={prob.p}, n_t=6, {R} load cases per solve, "
+                               f"GMRES(30) eps 1e-8, exact banded factor "
+                               f"({pr.nblocks} blocks of {pr.block})",
+                   "cases": R, "p": prob.p},
+        "all_converged": ok32,
+        "relres_max": float(rr32.max()),
+        "its_max": int(its32.max()),
+        "its_mean": float(its32.mean()),
+        "speedup_vs_fp64": (value / res["fp64_slabs_per_s"]) if "fp64_slabs_per_s" in res else None,
+        **res,
+        "kron_spmm": spmm,
+        "kernels": prof,
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbs"],
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["gbs"] / PEAK_HBM_GBS,
+                     "traffic": None},
+        "setup_s": {"assembly": t_asm, "upload_factor": t_fac},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prob)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -138,6 +138,19 @@ int xt_op_apply(xt_op op, int R, const void *tmat, double alpha, double beta,
 /* Upload ILUT factors (1-based MSR from ilut) and build the level schedule. */
 int xt_prec_create(int p, const double *alu, const int *jlu, const int *ju, xt_prec *out);
 void xt_prec_destroy(xt_prec pr);
+/* Exact factor K^-1 of a symmetric positive definite banded K (1-based CSR, host;
+ * the lower triangle is used), applied by block sweeps on the device.  Stands in
+ * for the reference's live preconditioner, the exact MUMPS factor of K
+ * (mumps_solver, mumps.f90:22-145; mpilusol, kronecker_mumps.f90:120-183), and is
+ * ilut's lfil >= n, droptol = 0 limit (ilut.f:2169-2170).  K is cut into diagonal
+ * blocks of max(block, bandwidth) rows rounded up to a multiple of 64 (at most
+ * 2048: number the mesh along its longest axis last); factorization in FP64 on the
+ * device.  XT_E_SINGULAR if K is not positive definite. */
+int xt_prec_create_band(int p, const int *rowptr, const int *colidx, const double *vals,
+                        int block, xt_prec *out);
+/* Block size, number of blocks and bandwidth of a banded-factor preconditioner. */
+int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth);
+/* Sequential steps of the two sweeps: ILUT levels, or blocks of a banded factor. */
 int xt_prec_levels(xt_prec pr, int *nlev_lower, int *nlev_upper);
 
 /* Y = (LU)^-1 X Minv_r^T on R right-hand sides (Minv: device [R][q][q],
@@ -152,11 +165,14 @@ int xt_to_interleaved(int p, int q, int R, const double *src, double *dst, void 
 int xt_from_interleaved(int p, int q, int R, const double *src, double *dst, void *stream);
 
 /* Batched slab solve.  Solves (A_r (x) K + B_r (x) M) x_r = b_r for r < R with
- * GMRES(im) preconditioned by A_r (x) LU.
- *   mode 0: FP64 throughout (plain FP64 GPU GMRES);
- *   mode 1: FP32 operator and FP32 ILUT inside FP64 GMRES iterative refinement:
- *           the true residual b - A x, the Arnoldi basis, orthogonalisation and
- *           Givens least squares stay FP64.
+ * GMRES(im) right-preconditioned by A_r (x) P, P the ILUT factors or the exact banded
+ * factor of K (preconditioner handle).
+ *   mode 0: FP64 throughout (plain FP64 GPU GMRES, the reference's algorithm);
+ *   mode 1: GMRES iterative refinement: FP64 true residual b - A x outside, one
+ *           flexible GMRES(im) cycle per refinement step inside with the FP32
+ *           operator and FP32 preconditioner sweeps (FP64 Arnoldi basis,
+ *           orthogonalisation and Givens least squares);
+ *   mode 2: FP64 flexible GMRES(im) with the FP32 preconditioner sweeps.
  * ab: host [R][2][q][q] column-major A_r, B_r (as passed to kronpgmres).
  * b, x: device, interleaved, FP64.  x is the initial guess on entry.
  * Outputs per r (host arrays of length R): its, ierr, relres (true
@@ -174,6 +190,27 @@ int xt_damage_device(int ne, int nn, int nip, const int *nodes, const double *Nx
                      double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
                      double *p, double *D, double *ws, double *del_po, double *sig_d,
                      void *stream);
+
+/* Right-hand side of a slab for R load cases (main3d.f90:803-823):
+ *   b[dof][r*q + j] := fixed[dof] ? 0 : b[dof][r*q + j] + coef[r*q + j] * rext[dof]
+ * i.e. fextc_r(j) * amplitude_r * rext added to the jump term already in b (from
+ * xt_op_apply with the ktn1/mtn1 temporal matrices), then the fixed DOFs zeroed.
+ * coef: device [R*q]; rext: device [p]; fixed: device uint8 [p] or NULL.  FP64. */
+int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext,
+                const unsigned char *fixed, double *b, void *stream);
+
+/* ---- kernel timing ----------------------------------------------------------- */
+#define XT_PROF_SPMM 0     /* Kronecker operator (node-block or CSR kernel)         */
+#define XT_PROF_BAND 1     /* banded-factor block sweeps                            */
+#define XT_PROF_TRSV 2     /* ILUT level-scheduled triangular solves                */
+#define XT_PROF_DAMAGE 3   /* damage kernel                                         */
+#define XT_PROF_NKINDS 4
+/* Enable per-launch HIP-event timing for the kinds in mask (bit k = kind k) and
+ * clear the counters; 0 disables.  Process-wide; one host thread. */
+int xt_prof_enable(unsigned mask);
+/* Sum of per-launch kernel durations (ms), number of launches, and the launches'
+ * algorithmic bytes and flops since xt_prof_enable.  Synchronises on the last event. */
+int xt_prof_read(int kind, double *ms, long long *launches, double *bytes, double *flops);
 
 /* Synchronise the device (all streams of this process). */
 int xt_device_sync(void);

@@ -41,6 +41,15 @@ def lib():
         L.or_kronpgmres.argtypes = [_d, _d, ci, _i, _i, _d, _i, _i, _d, ci, _d, _i, _i,
                                     _d, _d, ci, cd, ci, _d, ci, C.POINTER(ci), C.POINTER(ci)]
         L.or_kronpgmres.restype = ci
+        L.or_bandchol.argtypes = [ci, ci, _d]
+        L.or_bandchol.restype = ci
+        L.or_bandsol.argtypes = [ci, ci, _d, _d, _d]
+        L.or_kronbandsol.argtypes = [_d, ci, _d, ci, ci, _d, _d]
+        L.or_kronbandsol.restype = ci
+        L.or_kronpgmres_band.argtypes = [_d, _d, ci, _i, _i, _d, _i, _i, _d, ci, _d, ci,
+                                         _d, _d, ci, cd, ci, _d, ci, C.POINTER(ci),
+                                         C.POINTER(ci)]
+        L.or_kronpgmres_band.restype = ci
         L.or_stcoeff_m.argtypes = [cd, cd, cd, _d, _d, _d, _d, _d, _d]
         L.or_stcoeff_t.argtypes = [cd, cd, _d, _d, _d, _d, _d, _d]
         L.or_getfextc_m.argtypes = [cd, cd, cd, _d, _d]
@@ -138,7 +147,8 @@ def kronlusol(A, alu, jlu, ju, x):
 
 
 def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
-    """kronecker.f90:181-365. prec = (alu, jlu, ju). Returns (sol, ierr, its, hist)."""
+    """kronecker.f90:181-365.  prec = (alu, jlu, ju) ILUT factors, or a BandFactor
+    (exact K^-1).  Returns (sol, ierr, its, hist)."""
     q = A.shape[0]
     p = len(K[0]) - 1
     rhs = _f64(rhs).copy()
@@ -147,12 +157,45 @@ def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
     hist = np.zeros(cap)
     nh = C.c_int(0)
     its = C.c_int(0)
-    alu, jlu, ju = prec
-    ierr = lib().or_kronpgmres(_fmat(A), _fmat(B), q, _i32(K[0]), _i32(K[1]), _f64(K[2]),
-                               _i32(M[0]), _i32(M[1]), _f64(M[2]), p, _f64(alu), _i32(jlu),
-                               _i32(ju), rhs, sol, im, eps, maxits, hist, cap,
-                               C.byref(nh), C.byref(its))
+    args = (_fmat(A), _fmat(B), q, _i32(K[0]), _i32(K[1]), _f64(K[2]),
+            _i32(M[0]), _i32(M[1]), _f64(M[2]), p)
+    tail = (rhs, sol, im, eps, maxits, hist, cap, C.byref(nh), C.byref(its))
+    if isinstance(prec, BandFactor):
+        ierr = lib().or_kronpgmres_band(*args, prec.lb, prec.w, *tail)
+    else:
+        alu, jlu, ju = prec
+        ierr = lib().or_kronpgmres(*args, _f64(alu), _i32(jlu), _i32(ju), *tail)
     return sol, ierr, its.value, hist[:nh.value].copy()
+
+
+class BandFactor:
+    """Band Cholesky K = L L^T of an SPD CSR matrix (lower band, 1-based CSR)."""
+
+    def __init__(self, K):
+        ia, ja, a = _i32(K[0]), _i32(K[1]), _f64(K[2])
+        n = len(ia) - 1
+        rows = np.repeat(np.arange(n), np.diff(ia))
+        cols = ja - 1
+        low = cols <= rows
+        self.n = n
+        self.w = int(np.max(rows[low] - cols[low])) if n else 0
+        ld = self.w + 1
+        self.lb = np.zeros(ld * n)
+        self.lb[(rows[low] - cols[low]) + cols[low] * ld] = a[low]
+        self.info = lib().or_bandchol(n, self.w, self.lb)
+
+    def solve(self, y):
+        x = np.zeros(self.n)
+        lib().or_bandsol(self.n, self.w, self.lb, _f64(y), x)
+        return x
+
+    def kronsol(self, A, x):
+        """(A (x) K)^-1 x in kronlusol's order."""
+        q = A.shape[0]
+        y = np.zeros(self.n * q)
+        if lib().or_kronbandsol(_fmat(A), q, self.lb, self.w, self.n, _f64(x), y):
+            raise np.linalg.LinAlgError("Matrix is numerically singular!")
+        return y
 
 
 def stcoeff_m(t1, dt, w):

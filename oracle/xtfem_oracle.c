@@ -322,12 +322,23 @@ static void or_daxpy(size_t n, double t, const double *x, double *y)
 /* most hist_cap entries, *nhist set to the count).  Returns ierr:            */
 /* 0 converged, 1 maxits reached, -1 zero initial residual, 2 singular A.     */
 /* ------------------------------------------------------------------------ */
-int or_kronpgmres(const double *a, const double *b, int q,
-                  const int *ki, const int *kj, const double *kv,
-                  const int *mi, const int *mj, const double *mv, int p,
-                  const double *alu, const int *jlu, const int *ju,
-                  double *rhs, double *sol, int im, double eps, int maxits,
-                  double *hist, int hist_cap, int *nhist, int *its_out)
+/* Preconditioner of the GMRES restatement: y = (A (x) P)^-1 x, x may alias y. */
+typedef int (*or_prec_fn)(const double *a, int q, const void *ctx, int p, const double *x,
+                          double *y);
+struct or_ilut_ctx { const double *alu; const int *jlu, *ju; };
+static int or_prec_ilut(const double *a, int q, const void *ctx, int p, const double *x,
+                        double *y)
+{
+    const struct or_ilut_ctx *c = (const struct or_ilut_ctx *)ctx;
+    return or_kronlusol(a, q, c->alu, c->jlu, c->ju, p, x, y);
+}
+
+static int or_kronpgmres_impl(const double *a, const double *b, int q,
+                              const int *ki, const int *kj, const double *kv,
+                              const int *mi, const int *mj, const double *mv, int p,
+                              or_prec_fn prec, const void *pctx,
+                              double *rhs, double *sol, int im, double eps, int maxits,
+                              double *hist, int hist_cap, int *nhist, int *its_out)
 {
     size_t n = (size_t)p * q;
     double *hh = (double *)calloc((size_t)(im + 1) * im, sizeof(double));
@@ -355,7 +366,7 @@ int or_kronpgmres(const double *a, const double *b, int q,
         for (;;) {                                               /* label 4 */
             i = i + 1; its = its + 1;
             int i1 = i + 1;
-            if (or_kronlusol(a, q, alu, jlu, ju, p, VV(i - 1), rhs)) { ierr = 2; goto done; }
+            if (prec(a, q, pctx, p, VV(i - 1), rhs)) { ierr = 2; goto done; }
             or_kronspmv(a, b, q, ki, kj, kv, mi, mj, mv, p, rhs, VV(i1 - 1));
             for (int j = 1; j <= i; ++j) {                       /* MGS :277 */
                 t = or_ddot(n, VV(j - 1), VV(i1 - 1));
@@ -400,7 +411,7 @@ int or_kronpgmres(const double *a, const double *b, int q,
             t = rs[j - 1];
             for (size_t k = 0; k < n; ++k) rhs[k] = rhs[k] + t * VV(j - 1)[k];
         }
-        if (or_kronlusol(a, q, alu, jlu, ju, p, rhs, rhs)) { ierr = 2; goto done; }
+        if (prec(a, q, pctx, p, rhs, rhs)) { ierr = 2; goto done; }
         for (size_t k = 0; k < n; ++k) sol[k] = sol[k] + rhs[k];
         if (ro <= eps1) { ierr = 0; break; }                     /* :341 */
         if (its >= maxits) { ierr = 1; break; }
@@ -422,6 +433,110 @@ done:
 #undef HH
 #undef VV
     return ierr;
+}
+
+int or_kronpgmres(const double *a, const double *b, int q,
+                  const int *ki, const int *kj, const double *kv,
+                  const int *mi, const int *mj, const double *mv, int p,
+                  const double *alu, const int *jlu, const int *ju,
+                  double *rhs, double *sol, int im, double eps, int maxits,
+                  double *hist, int hist_cap, int *nhist, int *its_out)
+{
+    struct or_ilut_ctx c = {alu, jlu, ju};
+    return or_kronpgmres_impl(a, b, q, ki, kj, kv, mi, mj, mv, p, or_prec_ilut, &c, rhs, sol,
+                              im, eps, maxits, hist, hist_cap, nhist, its_out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact factor of K for the live-path semantics.  The reference's live       */
+/* solver preconditions with the exact MUMPS factor of K (mumps.f90:22-145,   */
+/* applied by mpilusol, kronecker_mumps.f90:120-183); ilut with lfil >= n and */
+/* droptol = 0 is the same exact factor (ilut.f:2169-2170).  For a banded SPD */
+/* K (half-bandwidth w) this is the textbook band Cholesky K = L L^T          */
+/* (right-looking, column by column).  Band storage: lb[(i-j) + j*(w+1)] =    */
+/* L(i,j) for 0 <= i-j <= w, filled on entry with the lower band of K.        */
+/* Returns 0, or j+1 if the leading minor of order j+1 is not positive.       */
+/* ------------------------------------------------------------------------ */
+int or_bandchol(int n, int w, double *lb)
+{
+    const size_t ld = (size_t)w + 1;
+    for (int j = 0; j < n; ++j) {
+        double *cj = lb + (size_t)j * ld;
+        if (!(cj[0] > 0.0)) return j + 1;
+        const double d = sqrt(cj[0]);
+        cj[0] = d;
+        const int m = (n - 1 - j) < w ? (n - 1 - j) : w;
+        for (int t = 1; t <= m; ++t) cj[t] = cj[t] / d;
+        for (int k = 1; k <= m; ++k) {          /* column j+k -= L(j+k,j) L(j+k..,j) */
+            double *ck = lb + (size_t)(j + k) * ld;
+            const double lkj = cj[k];
+            for (int t = k; t <= m; ++t) ck[t - k] = ck[t - k] - cj[t] * lkj;
+        }
+    }
+    return 0;
+}
+
+/* (L L^T) x = y with or_bandchol's factor; x may alias y. */
+void or_bandsol(int n, int w, const double *lb, const double *y, double *x)
+{
+    const size_t ld = (size_t)w + 1;
+    if (x != y) memcpy(x, y, sizeof(double) * (size_t)n);
+    for (int j = 0; j < n; ++j) {                /* L z = y, column oriented */
+        const double *cj = lb + (size_t)j * ld;
+        x[j] = x[j] / cj[0];
+        const int m = (n - 1 - j) < w ? (n - 1 - j) : w;
+        for (int t = 1; t <= m; ++t) x[j + t] = x[j + t] - cj[t] * x[j];
+    }
+    for (int j = n - 1; j >= 0; --j) {           /* L^T x = z, dot oriented */
+        const double *cj = lb + (size_t)j * ld;
+        const int m = (n - 1 - j) < w ? (n - 1 - j) : w;
+        double s = x[j];
+        for (int t = 1; t <= m; ++t) s = s - cj[t] * x[j + t];
+        x[j] = s / cj[0];
+    }
+}
+
+/* (A (x) K)^-1 x with the band factor, in kronlusol's order (kronecker.f90:130-178):
+   w_j = K^-1 x_j, then y_i = sum_j inva(i,j) w_j.  x and y may alias. */
+struct or_band_ctx { const double *lb; int w; };
+int or_kronbandsol(const double *a, int q, const double *lb, int w, int p, const double *x,
+                   double *y)
+{
+    double inva[64];
+    if (or_inverse(q, a, inva)) return 1;
+    size_t n = (size_t)p * q;
+    double *wv = (double *)calloc(n, sizeof(double));
+    for (int i = 0; i < q; ++i) or_bandsol(p, w, lb, x + (size_t)i * p, wv + (size_t)i * p);
+    for (size_t k = 0; k < n; ++k) y[k] = 0.0;
+    for (int i = 0; i < q; ++i)
+        for (int j = 0; j < q; ++j) {
+            double s = inva[i + j * q];
+            double *yi = y + (size_t)i * p;
+            const double *wj = wv + (size_t)j * p;
+            for (int k = 0; k < p; ++k) yi[k] = yi[k] + s * wj[k];
+        }
+    free(wv);
+    return 0;
+}
+static int or_prec_band(const double *a, int q, const void *ctx, int p, const double *x,
+                        double *y)
+{
+    const struct or_band_ctx *c = (const struct or_band_ctx *)ctx;
+    return or_kronbandsol(a, q, c->lb, c->w, p, x, y);
+}
+
+/* kronpgmres (kronecker.f90:181-365) with the exact band factor in place of the
+   ILUT factors: the live path's mpipgmres semantics with K^-1 exact. */
+int or_kronpgmres_band(const double *a, const double *b, int q,
+                       const int *ki, const int *kj, const double *kv,
+                       const int *mi, const int *mj, const double *mv, int p,
+                       const double *lb, int w,
+                       double *rhs, double *sol, int im, double eps, int maxits,
+                       double *hist, int hist_cap, int *nhist, int *its_out)
+{
+    struct or_band_ctx c = {lb, w};
+    return or_kronpgmres_impl(a, b, q, ki, kj, kv, mi, mj, mv, p, or_prec_band, &c, rhs, sol,
+                              im, eps, maxits, hist, hist_cap, nhist, its_out);
 }
 
 /* ------------------------------------------------------------------------ */

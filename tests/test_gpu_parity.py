@@ -109,7 +109,7 @@ def _slab_problem(S, R, seed):
     return A, B, np.array(rhs)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_slab_solve_batched(xt, mode, oracle_lib):
     import torch
     from xtfem_amd.device import KronOperator, IlutPreconditioner, slab_solve

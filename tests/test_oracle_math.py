@@ -148,6 +148,49 @@ def test_gmres_exact_preconditioner_one_iteration(oracle_lib, tiny):
     assert rel(np.kron(A, dense(S["Kb"])) @ sol, rhs) < 1e-10
 
 
+def test_band_cholesky_is_the_exact_factor(oracle_lib, tiny):
+    """The band Cholesky (the exact factor of the live path) equals numpy's Cholesky
+    of K, solves K x = y, and gives the same solution as ilut's exact limit
+    (lfil >= n, droptol = 0), which is pinned against the compiled reference."""
+    for S in tiny + [structured_system(6, 2, 2, 6.0, 2.0, 2.0)]:
+        Kd = dense(S["Kb"])
+        bf = oracle_lib.BandFactor(S["Kb"])
+        assert bf.info == 0
+        n, w = bf.n, bf.w
+        Lb = np.zeros((n, n))
+        for j in range(n):
+            for t in range(min(w, n - 1 - j) + 1):
+                Lb[j + t, j] = bf.lb[t + j * (w + 1)]
+        assert rel(Lb, np.linalg.cholesky(Kd)) < 1e-12
+        y = np.random.default_rng(n).standard_normal(n)
+        x = bf.solve(y)
+        assert rel(Kd @ x, y) < 1e-10
+        alu, jlu, ju, _ = oracle_lib.ilut(S["Kb"], n, 0.0)
+        assert rel(x, oracle_lib.lusol(alu, jlu, ju, y)) < 1e-10
+        A, _ = temporal()
+        X = np.random.default_rng(2).standard_normal(6 * n)
+        assert rel(bf.kronsol(A, np.kron(A, Kd) @ X), X) < 1e-9
+    # not positive definite: the failing minor is reported
+    S = tiny[0]
+    neg = (S["Kb"][0], S["Kb"][1], -S["Kb"][2])
+    assert oracle_lib.BandFactor(neg).info == 1
+
+
+def test_gmres_with_band_factor_matches_ilut_exact(oracle_lib, tiny):
+    """kronpgmres with the band factor and with ilut's exact factor: same iterations
+    and the same solution (both are K^-1 exactly)."""
+    S = tiny[1]
+    A, B = temporal()
+    rhs = np.random.default_rng(9).standard_normal(6 * S["p"])
+    s1, e1, i1, h1 = oracle_lib.kronpgmres(A, B, S["Kb"], S["Mb"], oracle_lib.BandFactor(S["Kb"]),
+                                           rhs, eps=1e-12)
+    s2, e2, i2, h2 = oracle_lib.kronpgmres(A, B, S["Kb"], S["Mb"],
+                                           oracle_lib.ilut(S["Kb"], S["p"], 0.0)[:3], rhs,
+                                           eps=1e-12)
+    assert e1 == e2 == 0 and i1 == i2
+    assert rel(s1, s2) < 1e-9 and rel(h1, h2) < 1e-6
+
+
 def test_gmres_error_codes_and_residual(oracle_lib, tiny):
     S = tiny[1]
     A, B = temporal()

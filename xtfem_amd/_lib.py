@@ -46,6 +46,8 @@ _SIGS = {
     "xt_op_blocks": (C.c_longlong, [vp, ci]),
     "xt_op_apply": (ci, [vp, ci, vp, cd, cd, vp, vp, ci, vp]),
     "xt_prec_create": (ci, [ci, vp, vp, vp, vp]),
+    "xt_prec_create_band": (ci, [ci, vp, vp, vp, ci, vp]),
+    "xt_prec_band_info": (ci, [vp, vp, vp, vp]),
     "xt_prec_destroy": (None, [vp]),
     "xt_prec_levels": (ci, [vp, vp, vp]),
     "xt_prec_apply": (ci, [vp, ci, ci, vp, vp, vp, ci, vp]),
@@ -53,6 +55,9 @@ _SIGS = {
     "xt_from_interleaved": (ci, [ci, ci, ci, vp, vp, vp]),
     "xt_slab_solve": (ci, [vp, vp, ci, vp, vp, vp, ci, cd, ci, ci, vp, vp, vp, vp]),
     "xt_damage_device": (ci, [ci, ci, ci] + [vp] * 18),
+    "xt_slab_rhs": (ci, [ci, ci, ci, vp, vp, vp, vp, vp]),
+    "xt_prof_enable": (ci, [C.c_uint]),
+    "xt_prof_read": (ci, [ci, vp, vp, vp, vp]),
     "xt_device_sync": (ci, []),
 }
 

@@ -19,8 +19,10 @@ OUT = os.path.join(HERE, "libxtfem_hip.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("XTFEM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["xt_kron.hip", "xt_bsr.hip", "xt_prec.hip", "xt_gmres.hip", "xt_damage.hip", "xt_capi.hip",
-           "xt_ilut.cpp"]
+SOURCES = ["xt_kron.hip", "xt_bsr.hip", "xt_prec.hip", "xt_band.hip", "xt_gmres.hip",
+           "xt_damage.hip", "xt_capi.hip", "xt_prof.hip", "xt_ilut.cpp"]
+# rocSOLVER / rocBLAS: dense block factorization of the banded preconditioner (setup only).
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 # Packed FP32 instructions are disabled for all device code (-packed-fp32-ops).
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -51,7 +53,7 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, *LIBS, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -298,6 +298,7 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
     hipLaunchKernelGGL((kron_bsr3_kernel<T, N>), dim3(grid), dim3(256), 0, s, P.nslices,    \
                        P.slot_node.as<int>(), P.slice_base.as<int>(), P.slice_len.as<int>(), \
                        P.bcol.as<int>(), bv, R, r0, c, tmat, (T)alpha, (T)beta, x, y)
+        if (prof_on(XT_PROF_SPMM)) prof_begin(XT_PROF_SPMM, s);
         switch (lpn) {
             case 1: XT_BSR_LAUNCH(1); break;
             case 2: XT_BSR_LAUNCH(2); break;
@@ -308,6 +309,16 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
             default: XT_BSR_LAUNCH(64); break;
         }
 #undef XT_BSR_LAUNCH
+        if (prof_on(XT_PROF_SPMM)) {
+            // 12 values (3x3 K block + diagonal of the M block) + 1 column index per
+            // stored block, slice tables, X read, Y written (and read when beta != 0)
+            const double sv = sizeof(T);
+            const double vec = (double)P.nn * 3 * 6 * Rc * sv;
+            const double bytes = (double)P.nblocks * (12 * sv + 4) +
+                                 (double)P.nslices * (kSlice * 4 + 8) + vec * (beta != 0 ? 3 : 2);
+            const double flops = 2.0 * 12 * P.nreal * 6 * Rc + 2.0 * 2 * 6 * (3.0 * P.nn) * 6 * Rc;
+            prof_end(XT_PROF_SPMM, s, bytes, flops);
+        }
         XT_HIP(hipGetLastError());
     }
     return XT_OK;

@@ -95,6 +95,19 @@ int op_dims(xt_op op, int *p, int *q, int *nterms);
 // q x q row-major inverse with partial pivoting; 0 on success, 1 if singular.
 int small_inverse_rowmajor(int q, const double *A, double *Ainv);
 
+// Kernel timing (xt_prof.hip): kinds are the XT_PROF_* codes of xtfem.h.
+bool prof_on(int kind);
+void prof_begin(int kind, hipStream_t s);
+void prof_end(int kind, hipStream_t s, double bytes, double flops);
+
+// Exact block-tridiagonal factor of a banded SPD matrix (xt_band.hip).
+struct BandPrec;
+int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out);
+void band_delete(BandPrec *bp);
+void band_dims(const BandPrec *bp, int *b, int *nb, int *bw);
+template <typename T>
+int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s);
+
 // Temporal mixing Y[dof][r*q+i] = sum_j M_r(i,j) X[dof][r*q+j] (M row-major [R][q][q]).
 template <typename TI, typename TO>
 int temporal_mix(int p, int q, int R, const TI *M, const TI *x, TO *y, hipStream_t s);

@@ -276,9 +276,14 @@ extern "C" int xt_damage_device(int ne, int nn, int nip, const int *nodes, const
            "damage arrays");
     const int ngp = 8 * ne;
     if (ngp == 0 || nip == 0) return XT_OK;
+    const bool prof = prof_on(XT_PROF_DAMAGE);
+    if (prof) prof_begin(XT_PROF_DAMAGE, (hipStream_t)stream);
     hipLaunchKernelGGL(damage_kernel, dim3(ceil_div(ngp, 64)), dim3(64), 0,
                        (hipStream_t)stream, ngp, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu,
                        eps_old, eps_pl, sig_eff, sig_back, p, D, ws, del_po, sig_d);
+    if (prof)   // state 31 FP64 read + written, 24 derivatives + 8 node ids, Nt, 24 disp gathers per step
+        prof_end(XT_PROF_DAMAGE, (hipStream_t)stream,
+                 (double)ngp * (31 * 16 + 24 * 8 + 8 * 4) + 8.0 * 6 * nip + 8.0 * 18 * nn, 0.0);
     XT_HIP(hipGetLastError());
     return XT_OK;
 }

@@ -189,6 +189,7 @@ struct Solver {
     hipStream_t s;
     DevBuf tm64, tm32, mi64, mi32;       // temporal matrices and inverses
     DevBuf V;                            // (im+1) x n FP64 basis
+    DevBuf Z;                            // im x n preconditioned basis (flexible variant)
     DevBuf w32a, w32b;                   // FP32 work vectors
     DevBuf z64, t64;                     // FP64 work vectors
     DevBuf partial, hdev, hdev2, ndev, coef, frozen;
@@ -197,12 +198,14 @@ struct Solver {
     double hist_ro0 = 1.0;
 
     double *Vj(int j) { return V.as<double>() + (size_t)j * n; }
+    double *Zj(int j) { return Z.as<double>() + (size_t)j * n; }
 
-    int init() {
+    int init(bool flexible) {
         c = q * R;
         n = (long long)p * c;
         XT_ARG(c <= 256, "q*R must be <= 256 for the batched solver");
         XT_TRY(V.reserve(sizeof(double) * n * (im + 1)));
+        if (flexible) XT_TRY(Z.reserve(sizeof(double) * n * im));
         XT_TRY(z64.reserve(sizeof(double) * n));
         XT_TRY(t64.reserve(sizeof(double) * n));
         XT_TRY(w32a.reserve(sizeof(float) * n));
@@ -251,6 +254,27 @@ struct Solver {
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
+    // Flexible step: Z_i = P^-1 v (FP32 or FP64 preconditioner, stored FP64), w = Op Z_i
+    // (FP32 or FP64 operator).
+    int op_prec_flex(int i, const double *v, double *w, bool f32prec, bool f32op) {
+        double *z = Zj(i);
+        if (f32prec) {
+            hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, v, w32a.as<float>());
+            XT_TRY(prec_apply_t<float>(pr, R, q, mi32.as<float>(), w32a.as<float>(), w32b.as<float>(), true, s));
+            hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), z);
+        } else {
+            XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), v, z, false, s));
+        }
+        if (f32op) {
+            hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, z, w32a.as<float>());
+            XT_TRY(op_apply_internal(op, R, tm32.ptr, 1.0, 0.0, w32a.ptr, w32b.ptr, 1, s));
+            hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), w);
+        } else {
+            XT_TRY(op_apply_internal(op, R, tm64.ptr, 1.0, 0.0, z, w, 0, s));
+        }
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
     // x += P^-1 u
     int prec_add(const double *u, double *x, bool f32) {
         if (!f32) {
@@ -282,10 +306,16 @@ struct Solver {
     // GMRES on the residual system already stored in V_0 (unnormalised, = r0).
     // x is updated in place.  eps1[r]: absolute target on |rs(i+1)|.
     // active[r] in: RHS to solve; out: its_r, conv_r (1 converged, 0 not).
+    // max_cycles bounds the restart cycles of this call (GMRES-IR inner solves run one).
+    // Precision: f32 = FP32 operator and preconditioner; flexible (FGMRES) stores
+    // Z_i = P^-1 V_i and updates x += Z y, with the preconditioner in FP32 when f32prec
+    // and the operator in FP32 when f32.
     int gmres(double *x, bool f32, const std::vector<double> &eps1, int maxits,
               std::vector<int> &active, std::vector<int> &its, std::vector<int> &conv,
-              std::vector<double> &ro_last) {
+              std::vector<double> &ro_last, int max_cycles = 1 << 30, bool flexible = false,
+              bool f32prec = false) {
         const int K = im;
+        std::vector<int> ncyc(R, 0);
         std::vector<double> hh((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K),
             rs((size_t)R * (K + 1)), hcol((size_t)(K + 1) * R), hcol2((size_t)(K + 1) * R);
         std::vector<int> icyc(R, 0), fz(R, 0);
@@ -314,7 +344,8 @@ struct Solver {
             int i = 0;
             for (;;) {
                 // Arnoldi step i -> i+1
-                XT_TRY(op_prec(Vj(i), Vj(i + 1), f32));
+                if (flexible) XT_TRY(op_prec_flex(i, Vj(i), Vj(i + 1), f32prec, f32));
+                else XT_TRY(op_prec(Vj(i), Vj(i + 1), f32));
                 XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
                 XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
                 XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>()));
@@ -376,15 +407,19 @@ struct Solver {
                 for (int j = 0; j < ir; ++j) y[(size_t)j * R + r] = yy[j];
             }
             XT_HIP(hipMemcpyAsync(coef.ptr, y.data(), sizeof(double) * kmax * R, hipMemcpyHostToDevice, s));
-            XT_TRY(maxpy(kmax, 1.0, coef.as<double>(), Vj(0), nullptr, t64.as<double>()));
-            XT_TRY(prec_add(t64.as<double>(), x, f32));
+            if (flexible) {
+                XT_TRY(maxpy(kmax, 1.0, coef.as<double>(), Zj(0), x, x));
+            } else {
+                XT_TRY(maxpy(kmax, 1.0, coef.as<double>(), Vj(0), nullptr, t64.as<double>()));
+                XT_TRY(prec_add(t64.as<double>(), x, f32));
+            }
             // decide per RHS; build restart residual V_0 = sum_j rs'_j V_j
             bool again = false;
             for (int r = 0; r < R; ++r) {
                 if (!active[r]) continue;
                 const int ir = icyc[r];
                 if (ro_last[r] <= eps1[r]) { conv[r] = 1; active[r] = 0; continue; }
-                if (its[r] >= maxits) { active[r] = 0; continue; }
+                if (its[r] >= maxits || ++ncyc[r] >= max_cycles) { active[r] = 0; continue; }
                 double *rs_ = &rs[(size_t)r * (K + 1)], *c_ = &cs[(size_t)r * K], *s_ = &sn[(size_t)r * K];
                 for (int j = 1; j <= ir; ++j) {
                     const int jj = ir + 1 - j + 1;  // 1-based as the reference
@@ -425,7 +460,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                         int *ierr, double *relres, hipStream_t stream,
                         std::vector<double> *hist) {
     XT_ARG(op && pr && R > 0 && ab && b && x && its && ierr && relres, "slab_solve arguments");
-    XT_ARG(im >= 1 && im <= 256 && maxits >= 1 && (mode == 0 || mode == 1), "slab_solve parameters");
+    XT_ARG(im >= 1 && im <= 256 && maxits >= 1 && mode >= 0 && mode <= 2, "slab_solve parameters");
     Solver S;
     S.op = op;
     S.pr = pr;
@@ -440,7 +475,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         S.p = p;
         S.q = q;
     }
-    XT_TRY(S.init());
+    XT_TRY(S.init(mode != 0));
     const int q = S.q;
     // temporal matrices: ab[r] = (A_r, B_r) column-major -> row-major T; inverse of A_r
     std::vector<double> tm((size_t)R * 2 * q * q), mi((size_t)R * q * q);
@@ -474,35 +509,51 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         ierr[r] = 0;
         if (r0[r] == 0.0) { ierr[r] = -1; active[r] = 0; }
     }
-    if (mode == 0) {
-        XT_TRY(S.gmres(x, false, eps1, maxits, active, itv, conv, ro_last));
+    if (mode == 0 || mode == 2) {
+        // mode 2: FP64 flexible GMRES, FP32 preconditioner
+        XT_TRY(S.gmres(x, false, eps1, maxits, active, itv, conv, ro_last, 1 << 30, mode == 2,
+                       mode == 2));
         for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
     } else {
-        // GMRES-IR: inner FP32 solves on the current residual, FP64 outer residual
-        const double inner_rel = 1.0e-4;
-        std::vector<double> rn(R);
+        // GMRES-IR: one flexible GMRES(im) cycle per refinement step on the current
+        // residual with the FP32 operator and FP32 preconditioner (FP64 basis, stored
+        // preconditioned vectors), FP64 outer residual.  The inner target starts at 1e-4
+        // of the residual and is relaxed to the reduction the last step achieved when
+        // FP32 rounding is the floor (ill-conditioned temporal matrices); a right-hand
+        // side whose step reduced its residual by less than 2x continues with inner
+        // cycles on the FP64 operator (FP32 preconditioner).
+        std::vector<double> rn(R), irel(R, 1.0e-4), prev(R, 0.0);
+        std::vector<int> fb(R, 0);
         DevBuf d;
         XT_TRY(d.reserve(sizeof(double) * S.n));
         for (int outer = 0; outer < 64; ++outer) {
             XT_TRY(S.norms2(S.Vj(0), rn));
-            bool any = false;
+            bool any32 = false, any64 = false;
             std::vector<double> e1(R, 0.0);
-            std::vector<int> act(R, 0);
+            std::vector<int> act32(R, 0), act64(R, 0);
             for (int r = 0; r < R; ++r) {
                 if (ierr[r] == -1) continue;
                 const double nr = std::sqrt(rn[r]);
                 if (nr <= eps1[r]) { conv[r] = 1; continue; }
                 if (itv[r] >= maxits) continue;
-                act[r] = 1;
-                any = true;
-                e1[r] = std::max(inner_rel * nr, 0.5 * eps1[r]);
+                if (prev[r] > 0.0) {
+                    const double rho = nr / prev[r];
+                    if (rho > 0.5) fb[r] = 1;
+                    else if (rho > 10.0 * irel[r]) irel[r] = std::min(0.1, rho);
+                }
+                prev[r] = nr;
+                e1[r] = std::max(irel[r] * nr, 0.5 * eps1[r]);
+                if (fb[r]) { act64[r] = 1; any64 = true; }
+                else { act32[r] = 1; any32 = true; }
             }
-            if (!any) break;
-            // inner solve for the correction d: x0 = 0, V_0 = residual (already there)
+            if (!any32 && !any64) break;
+            // inner solves for the correction d: x0 = 0, V_0 = residual (already there)
             XT_HIP(hipMemsetAsync(d.ptr, 0, sizeof(double) * S.n, S.s));
             std::vector<int> cin(R, 0);
-            std::vector<int> act_in = act;
-            XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act_in, itv, cin, ro_last));
+            if (any32)
+                XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act32, itv, cin, ro_last, 1, true, true));
+            if (any64)
+                XT_TRY(S.gmres(d.as<double>(), false, e1, maxits, act64, itv, cin, ro_last, 1, true, true));
             hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n,
                                d.as<double>(), x);
             XT_HIP(hipGetLastError());

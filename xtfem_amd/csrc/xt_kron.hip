@@ -134,8 +134,17 @@ static int launch_kron(const DevCSR *S, int p, int q, int R, const T *tmat, doub
         const size_t lds = sizeof(T) * NT * rpb * ct;
         const int grid = ceil_div(p, rpb);
         if (grid == 0) break;
+        const bool prof = prof_on(XT_PROF_SPMM);
+        if (prof) prof_begin(XT_PROF_SPMM, s);
         hipLaunchKernelGGL((kron_csr_kernel<T, NT>), dim3(grid), dim3(256), lds, s, A, p, c, q,
                            col0, ct, rpb, tmat, (T)alpha, (T)beta, x, y);
+        if (prof) {   // CSR values + column indices + row pointers per term, X, Y
+            double nnz = 0.0;
+            for (int t = 0; t < NT; ++t) nnz += (double)S[t].nnz;
+            const double sv = sizeof(T), vec = (double)p * ct * sv;
+            const double bytes = nnz * (sv + 4) + NT * 4.0 * (p + 1) + vec * (beta != 0 ? 3 : 2);
+            prof_end(XT_PROF_SPMM, s, bytes, 2.0 * nnz * ct + 2.0 * NT * q * (double)p * ct);
+        }
         XT_HIP(hipGetLastError());
     }
     return XT_OK;
@@ -173,6 +182,20 @@ template int temporal_mix<double, double>(int, int, int, const double *, const d
                                           double *, hipStream_t);
 template int temporal_mix<float, float>(int, int, int, const float *, const float *, float *,
                                         hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Slab right-hand side (main3d.f90:803-823): b += coef (x) rext, fixed DOFs zeroed.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+slab_rhs_kernel(long long n, int c, const double *__restrict__ coef,
+                const double *__restrict__ rext, const unsigned char *__restrict__ fixed,
+                double *__restrict__ b) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const long long dof = idx / c;
+    const int col = (int)(idx - dof * c);
+    b[idx] = (fixed && fixed[dof]) ? 0.0 : b[idx] + coef[col] * rext[dof];
+}
 
 // ---------------------------------------------------------------------------
 // Layout conversion: temporal-major x[r][j*p + dof] <-> interleaved X[dof][r*q + j]
@@ -297,6 +320,16 @@ long long xt_op_blocks(xt_op op, int real) {
 }
 
 void xt_op_destroy(xt_op op) { delete op; }
+
+int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext,
+                const unsigned char *fixed, double *b, void *stream) {
+    XT_ARG(p > 0 && q > 0 && R > 0 && coef && rext && b, "slab_rhs arguments");
+    const long long n = (long long)p * q * R;
+    hipLaunchKernelGGL(slab_rhs_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, q * R, coef, rext, fixed, b);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
 
 long long xt_op_nnz(xt_op op, int k) {
     if (!op || k < 0 || k >= op->nterms) return -1;

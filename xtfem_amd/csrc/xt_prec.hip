@@ -61,6 +61,7 @@ struct Tri {
     DevCSR m;                    // strict triangle, 0-based, reference entry order
     DevBuf lev_rows;             // rows sorted by level
     std::vector<int> lev_ptr;    // host copy of level boundaries
+    std::vector<long long> lev_nnz;  // entries per level (algorithmic bytes)
 };
 
 }  // namespace xt
@@ -69,9 +70,11 @@ using namespace xt;
 
 struct xt_prec_s {
     int p = 0;
-    Tri L, U;
+    Tri L, U;                  // ILUT factors (level-scheduled solves)
     DevBuf dinv64, dinv32;
     DevBuf work;
+    BandPrec *band = nullptr;  // or: exact banded factor (block sweeps)
+    ~xt_prec_s() { if (band) band_delete(band); }
 };
 
 namespace xt {
@@ -87,6 +90,10 @@ static int upload_tri(int n, const std::vector<int> &rp, const std::vector<int> 
     XT_TRY(T.m.val32.upload(v32.data(), sizeof(float) * v32.size()));
     std::vector<int> lrows;
     build_levels(n, rp, ci, forward, T.lev_ptr, lrows);
+    T.lev_nnz.assign(T.lev_ptr.size() - 1, 0);
+    for (size_t l = 0; l + 1 < T.lev_ptr.size(); ++l)
+        for (int k = T.lev_ptr[l]; k < T.lev_ptr[l + 1]; ++k)
+            T.lev_nnz[l] += rp[lrows[k] + 1] - rp[lrows[k]];
     XT_TRY(T.lev_rows.upload(lrows.data(), sizeof(int) * n));
     XT_HIP(hipDeviceSynchronize());
     return XT_OK;
@@ -101,9 +108,17 @@ static int run_tri(const Tri &tri, const T *vals, const T *dinv, int c, const T 
         const int r0 = tri.lev_ptr[l], nr = tri.lev_ptr[l + 1] - r0;
         if (!nr) continue;
         long long n = (long long)nr * c;
+        const bool prof = prof_on(XT_PROF_TRSV);
+        if (prof) prof_begin(XT_PROF_TRSV, s);
         hipLaunchKernelGGL(trsv_level_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, nr,
                            rows + r0, tri.m.rowptr.as<int>(), tri.m.col.as<int>(), vals, dinv,
                            c, b, x);
+        if (prof) {   // level rows: values + column indices of their entries, row data, x
+            const double nnz = (double)(tri.lev_nnz[l]);
+            prof_end(XT_PROF_TRSV, s, nnz * (sizeof(T) + 4) + nr * 12.0 +
+                                          (double)nr * c * sizeof(T) * (b ? 2 : 1) +
+                                          nnz * c * sizeof(T), 2.0 * nnz * c);
+        }
     }
     XT_HIP(hipGetLastError());
     return XT_OK;
@@ -112,6 +127,7 @@ static int run_tri(const Tri &tri, const T *vals, const T *dinv, int c, const T 
 template <typename T>
 int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
                  hipStream_t s) {
+    if (pr->band) return band_apply<T>(pr->band, R, q, minv, x, y, s);
     const int c = q * R;
     const T *b = x;
     if (minv) {
@@ -177,8 +193,33 @@ int xt_prec_create(int p, const double *alu, const int *jlu, const int *ju, xt_p
 
 void xt_prec_destroy(xt_prec pr) { delete pr; }
 
+int xt_prec_create_band(int p, const int *ia, const int *ja, const double *a, int block,
+                        xt_prec *out) {
+    XT_ARG(out && p > 0 && ia && ja && a, "prec_create_band arguments");
+    xt_prec pr = new (std::nothrow) xt_prec_s;
+    if (!pr) return XT_E_NOMEM;
+    pr->p = p;
+    int rc = band_build(p, ia, ja, a, block, &pr->band);
+    if (rc != XT_OK) { delete pr; return rc; }
+    *out = pr;
+    return XT_OK;
+}
+
+int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth) {
+    XT_ARG(pr && pr->band, "not a banded-factor preconditioner");
+    band_dims(pr->band, block, nblocks, bandwidth);
+    return XT_OK;
+}
+
 int xt_prec_levels(xt_prec pr, int *nlev_lower, int *nlev_upper) {
     XT_ARG(pr, "prec");
+    if (pr->band) {   // sequential block steps of each sweep
+        int nb = 0;
+        band_dims(pr->band, nullptr, &nb, nullptr);
+        if (nlev_lower) *nlev_lower = nb;
+        if (nlev_upper) *nlev_upper = nb;
+        return XT_OK;
+    }
     if (nlev_lower) *nlev_lower = (int)pr->L.lev_ptr.size() - 1;
     if (nlev_upper) *nlev_upper = (int)pr->U.lev_ptr.size() - 1;
     return XT_OK;

@@ -1,0 +1,85 @@
+// Per-launch kernel timing with HIP events on the launching stream.
+//
+// When a kind is enabled, its launch sites bracket each kernel launch with an event
+// pair and add the launch's algorithmic bytes and flops (each matrix and vector
+// element the kernel's format needs, read once and written once).  xt_prof_read
+// synchronises on the last event and sums the per-launch durations.  Process-wide
+// state: enable, run, read from one host thread.
+#include <vector>
+
+#include "xt_common.h"
+
+namespace xt {
+namespace {
+struct Kind {
+    std::vector<hipEvent_t> ev;   // pairs: ev[2i] start, ev[2i+1] stop
+    size_t used = 0;              // events recorded since enable
+    double bytes = 0.0, flops = 0.0;
+};
+unsigned g_mask = 0;
+Kind g_kind[XT_PROF_NKINDS];
+
+hipEvent_t next_event(Kind &k) {
+    if (k.used == k.ev.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        k.ev.push_back(e);
+    }
+    return k.ev[k.used++];
+}
+}  // namespace
+
+bool prof_on(int kind) { return (g_mask >> kind) & 1u; }
+
+void prof_begin(int kind, hipStream_t s) {
+    if (!prof_on(kind)) return;
+    Kind &k = g_kind[kind];
+    if (k.used % 2) return;                      // unmatched begin: keep the pair structure
+    hipEvent_t e = next_event(k);
+    if (e) (void)hipEventRecord(e, s);
+}
+
+void prof_end(int kind, hipStream_t s, double bytes, double flops) {
+    if (!prof_on(kind)) return;
+    Kind &k = g_kind[kind];
+    if (k.used % 2 == 0) return;
+    hipEvent_t e = next_event(k);
+    if (!e) { --k.used; return; }
+    (void)hipEventRecord(e, s);
+    k.bytes += bytes;
+    k.flops += flops;
+}
+}  // namespace xt
+
+using namespace xt;
+
+extern "C" {
+
+int xt_prof_enable(unsigned mask) {
+    g_mask = mask;
+    for (auto &k : g_kind) {
+        k.used = 0;
+        k.bytes = k.flops = 0.0;
+    }
+    return XT_OK;
+}
+
+int xt_prof_read(int kind, double *ms, long long *launches, double *bytes, double *flops) {
+    XT_ARG(kind >= 0 && kind < XT_PROF_NKINDS, "profile kind");
+    Kind &k = g_kind[kind];
+    const size_t pairs = k.used / 2;
+    double total = 0.0;
+    if (pairs) XT_HIP(hipEventSynchronize(k.ev[2 * pairs - 1]));
+    for (size_t i = 0; i < pairs; ++i) {
+        float t = 0.0f;
+        XT_HIP(hipEventElapsedTime(&t, k.ev[2 * i], k.ev[2 * i + 1]));
+        total += t;
+    }
+    if (ms) *ms = total;
+    if (launches) *launches = (long long)pairs;
+    if (bytes) *bytes = k.bytes;
+    if (flops) *flops = k.flops;
+    return XT_OK;
+}
+
+}  // extern "C"

@@ -46,9 +46,11 @@ def exact_ilut(oracle_lib, K):
     return alu, jlu, ju
 
 
-@pytest.mark.parametrize("shape,block,R", [((20, 2, 2, 0), 0, 3),     # many 64-row blocks
-                                           ((12, 9, 2, 4), 0, 12),    # c = 72: two column chunks
-                                           ((12, 9, 2, 4), 256, 1)])  # one padded block
+@pytest.mark.parametrize("shape,block,R", [((20, 2, 2, 0), 0, 3),     # many 64-row blocks, c = 18
+                                           ((20, 2, 2, 0), 0, 2),     # c = 12: 16-column rows kernel
+                                           ((12, 9, 2, 4), 0, 12),    # c = 72: column tiles
+                                           ((12, 9, 2, 4), 0, 32),    # c = 192
+                                           ((12, 9, 2, 4), 256, 1)])  # one padded block, c = 6
 def test_band_sweeps_match_exact_lu(dev, oracle_lib, shape, block, R):
     from xtfem_amd import slab
     S = notched_plate(*shape)

@@ -14,8 +14,8 @@
 //   backward  z_nb = D^T y,       z_i     = D_i^T y_i     - H_i z_{i+1}, H_i = (E_i D_i)^T
 //
 // The D x and D^T y products of all blocks run in one batched launch; each recurrence
-// step is one launch over b/8 row tiles.  All four matrix sets are stored row-major
-// (the row of the operator a tile needs is contiguous) in FP64 and FP32.
+// step is one launch.  All four matrix sets are stored row-major (the row of the
+// operator a tile needs is contiguous) in FP64 and FP32.
 //
 // The factorization runs once per K on the device with rocSOLVER potrf/trtri and
 // rocBLAS syrk/trsm/gemm (setup, not on the per-iteration path).
@@ -36,201 +36,295 @@ struct BandPrec {
     // G, H: recurrence matrices (nb - 1 of each).
     DevBuf Dl[2], Du[2], G[2], H[2];
     DevBuf w1, w2;
+    DevBuf part;   // split-k partial tiles of a recurrence step: [8][b][c]
 };
 
 // ---------------------------------------------------------------------------
-// out_blk[r][col] = (add ? add_blk[r][col] - s : s),  s = sum_k A_blk[r][k] V_blk[k][col]
+// Block products of the sweeps:
+//   out_blk[r][col] = (add ? add_blk[r][col] - s : s),  s = sum_k A_blk[r][k] V_blk[k][col]
 // A_blk: b x b row-major; V, add, out: b x c row-major (c columns of one row contiguous).
-// tri = 1: A lower triangular (k <= r), tri = 2: upper (k >= r); skipped parts are zero.
-// Grid: x = b / 8 row tiles, y = column chunks of cw (<= 64), z = blocks of the batch.
-// The k range is walked in chunks of KC rows.  Each thread loads its share of a chunk
-// (the 8 x KC slice of A and the KC x cw slice of V) into registers with every load in
-// flight at once, the chunk goes to LDS, and the loads of the next chunk are issued
-// before the current one is consumed.  Thread t accumulates column t % cw over the
-// k-slice t / cw; the slices are summed through LDS at the end.
+// tri = 1: A lower triangular (k <= r), tri = 2: upper (k >= r); the stored zeros of
+// the other triangle are skipped where a whole tile lies there.  Grid z runs over the
+// blocks of a batch (a_stride, v_stride, ... between consecutive blocks).
 // ---------------------------------------------------------------------------
-constexpr int kTM = 8;
 
-template <typename T>
-struct BandTile {
-    static constexpr int KC = sizeof(T) == 4 ? 256 : 128;   // k rows per chunk
-    static constexpr int VEC = 16 / sizeof(T);               // elements per 16-byte load
-    static constexpr int AV = KC * kTM / VEC / 256;          // A vectors per thread
-    static constexpr int VV = KC * 64 / VEC / 256;           // V vectors per thread (max)
-    static constexpr size_t smem = sizeof(T) * ((size_t)KC * kTM + (size_t)KC * 64 + kTM * 256);
-};
+// Few columns (c <= CM <= 16, V block in LDS): a workgroup owns RW rows.  It first
+// copies the rows k0..k1-1 of V (one contiguous span) into LDS with 16-byte loads;
+// its 256 threads then stride over k, loading the RW A rows (coalesced, every load
+// of a group of GK k issued before the products, indices clamped and out-of-range
+// terms multiplied by zero) and reading V from LDS.  The RW * CM <= 16 partial sums
+// of the 256 threads are summed through LDS: 16 threads per output add 16 partials
+// each, then a 16-lane butterfly (one value per lane).  Grid x = b / RW.
+constexpr size_t kRowsLdsMax = 96 * 1024;
 
-template <typename T>
+template <typename T, int CM, int RW>
 __global__ void __launch_bounds__(256)
-band_gemm_kernel(int b, int c, int cw, int tri, const T *__restrict__ A, long long a_stride,
+band_rows_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_stride,
                  const T *__restrict__ V, long long v_stride, const T *add, long long add_stride,
                  T *out, long long out_stride) {
-    using BT = BandTile<T>;
-    constexpr int KC = BT::KC, VEC = BT::VEC, AV = BT::AV, VV = BT::VV;
+    constexpr int VEC = 16 / sizeof(T);
     typedef T vec_t __attribute__((ext_vector_type(VEC)));
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *As = reinterpret_cast<T *>(smem_raw);   // [KC][kTM]
-    T *Vs = As + KC * kTM;                      // [KC][cw]
-    T *red = Vs + KC * 64;                      // [kTM][ks][cw]
+    static_assert(RW * CM <= 16, "at most 16 outputs per workgroup");
+    T *Vsh = reinterpret_cast<T *>(smem_raw);   // [k1 - k0][c]
+    __shared__ T red[16][256];                   // [output][thread]
     const int tid = threadIdx.x;
-    const int r0 = blockIdx.x * kTM;
-    const int n0 = blockIdx.y * cw;
+    const int r0 = blockIdx.x * RW;
     const long long z = blockIdx.z;
     A += z * a_stride;
     V += z * v_stride;
     out += z * out_stride;
     if (add) add += z * add_stride;
-
     int k0 = 0, k1 = b;
-    if (tri == 1) k1 = min(b, r0 + kTM);
+    if (tri == 1) k1 = r0 + RW;
     if (tri == 2) k0 = r0;
-    // V layouts: one contiguous span when the chunk holds every column (c <= cw);
-    // otherwise cw-wide row segments, vector loads when rows stay 16-byte aligned.
-    const bool span = c <= cw;
-    const bool vrow = !span && (c % VEC == 0);
-    const int segv = cw / VEC;                  // vectors per row segment
-
-    vec_t ra[AV], rv[VV];
-    T rs[VV * VEC];
-    auto load = [&](int kc0) {
-        const int nk = min(KC, k1 - kc0);      // a multiple of 8
-        const int nvr = nk / VEC;
-#pragma unroll
-        for (int j = 0; j < AV; ++j) {
-            const int e = tid + j * 256;
-            if (e < kTM * nvr) {
-                const int r = e / nvr, kv = e - r * nvr;
-                ra[j] = *reinterpret_cast<const vec_t *>(A + (size_t)(r0 + r) * b + kc0 + kv * VEC);
-            }
-        }
-        if (span) {
-            const T *src = V + (size_t)kc0 * c;
-            const int nv = nk * c / VEC;
-#pragma unroll
-            for (int j = 0; j < VV; ++j) {
-                const int e = tid + j * 256;
-                if (e < nv) rv[j] = *reinterpret_cast<const vec_t *>(src + (size_t)e * VEC);
-            }
-        } else if (vrow) {
-#pragma unroll
-            for (int j = 0; j < VV; ++j) {
-                const int e = tid + j * 256;
-                const int kk = e / segv, cv = e - kk * segv;
-                if (kk < nk && n0 + cv * VEC < c)
-                    rv[j] = *reinterpret_cast<const vec_t *>(V + (size_t)(kc0 + kk) * c + n0 + cv * VEC);
-            }
+    {
+        const int n = (k1 - k0) * c;
+        const T *src = V + (size_t)k0 * c;
+        if ((((size_t)k0 * c) % VEC) == 0 && (n % VEC) == 0) {
+            for (int e = tid; e < n / VEC; e += 256)
+                *reinterpret_cast<vec_t *>(Vsh + (size_t)e * VEC) =
+                    *reinterpret_cast<const vec_t *>(src + (size_t)e * VEC);
         } else {
+            for (int e = tid; e < n; e += 256) Vsh[e] = src[e];
+        }
+    }
+    __syncthreads();
+    T acc[RW][CM];
 #pragma unroll
-            for (int j = 0; j < VV * VEC; ++j) {
-                const int e = tid + j * 256;
-                const int kk = e / cw, cl = e - kk * cw;
-                rs[j] = (kk < nk && n0 + cl < c) ? V[(size_t)(kc0 + kk) * c + n0 + cl] : T(0);
+    for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+        for (int j = 0; j < CM; ++j) acc[rr][j] = T(0);
+    const T *a0 = A + (size_t)r0 * b;
+    constexpr int GK = RW >= 8 ? 2 : 4;
+    for (int kb = k0; kb < k1; kb += GK * 256) {
+        T av[GK][RW];
+#pragma unroll
+        for (int it = 0; it < GK; ++it) {
+            const int kk = min(kb + tid + 256 * it, k1 - 1);
+#pragma unroll
+            for (int rr = 0; rr < RW; ++rr) av[it][rr] = a0[(size_t)rr * b + kk];
+        }
+#pragma unroll
+        for (int it = 0; it < GK; ++it) {
+            const int k = kb + tid + 256 * it;
+            const int kk = min(k, k1 - 1);
+            const T m = (k < k1) ? T(1) : T(0);
+            T vv[CM];
+#pragma unroll
+            for (int j = 0; j < CM; ++j) vv[j] = Vsh[(kk - k0) * c + min(j, c - 1)];
+#pragma unroll
+            for (int rr = 0; rr < RW; ++rr) {
+                const T a = av[it][rr] * m;
+#pragma unroll
+                for (int j = 0; j < CM; ++j) acc[rr][j] = fma(a, vv[j], acc[rr][j]);
             }
         }
-    };
-    auto store = [&](int kc0) {
-        const int nk = min(KC, k1 - kc0);
-        const int nvr = nk / VEC;
-#pragma unroll
-        for (int j = 0; j < AV; ++j) {
-            const int e = tid + j * 256;
-            if (e < kTM * nvr) {
-                const int r = e / nvr, kv = e - r * nvr;
-#pragma unroll
-                for (int u = 0; u < VEC; ++u) As[(kv * VEC + u) * kTM + r] = ra[j][u];
-            }
-        }
-        if (span) {
-            const int nv = nk * c / VEC;
-#pragma unroll
-            for (int j = 0; j < VV; ++j) {
-                const int e = tid + j * 256;
-                if (e < nv) {
-#pragma unroll
-                    for (int u = 0; u < VEC; ++u) {
-                        const int i = e * VEC + u, kk = i / c;
-                        Vs[kk * cw + (i - kk * c)] = rv[j][u];
-                    }
-                }
-            }
-        } else if (vrow) {
-#pragma unroll
-            for (int j = 0; j < VV; ++j) {
-                const int e = tid + j * 256;
-                const int kk = e / segv, cv = e - kk * segv;
-                if (kk < nk) {
-                    const bool ok = n0 + cv * VEC < c;
-#pragma unroll
-                    for (int u = 0; u < VEC; ++u) Vs[kk * cw + cv * VEC + u] = ok ? rv[j][u] : T(0);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < VV * VEC; ++j) {
-                const int e = tid + j * 256;
-                if (e < nk * cw) Vs[e] = rs[j];
-            }
-        }
-    };
-
-    const int ks_n = 256 / cw;
-    const int col = tid % cw, ks = tid / cw;
-    const int cc = n0 + col;
-    T acc[kTM];
-#pragma unroll
-    for (int r = 0; r < kTM; ++r) acc[r] = T(0);
-
-    if (k0 < k1) load(k0);
-    for (int kc0 = k0; kc0 < k1; kc0 += KC) {
-        store(kc0);
-        __syncthreads();
-        if (kc0 + KC < k1) load(kc0 + KC);     // next chunk in flight during the compute
-        const int nk = min(KC, k1 - kc0);
-        if (cc < c) {
-#pragma unroll 4
-            for (int k = ks; k < nk; k += ks_n) {
-                const T v = Vs[k * cw + col];
-                const T *a = As + k * kTM;
-#pragma unroll
-                for (int r = 0; r < kTM; ++r) acc[r] = fma(a[r], v, acc[r]);
-            }
-        }
-        __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < kTM; ++r) red[(r * ks_n + ks) * cw + col] = acc[r];
+    for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+        for (int j = 0; j < CM; ++j) red[rr * CM + j][tid] = acc[rr][j];
     __syncthreads();
-    for (int o = tid; o < kTM * cw; o += 256) {
-        const int r = o / cw, cl = o - r * cw;
-        if (n0 + cl >= c) continue;
-        T s = T(0);
-        for (int j = 0; j < ks_n; ++j) s += red[(r * ks_n + j) * cw + cl];
-        const size_t idx = (size_t)(r0 + r) * c + n0 + cl;
-        out[idx] = add ? add[idx] - s : s;
+    {
+        const int o = tid >> 4, part = tid & 15;   // output o, partials part + 16 i
+        T sum = T(0);
+        if (o < RW * CM) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) sum += red[o][part + 16 * i];
+        }
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 16);
+        const int rr = o / CM, j = o - rr * CM;
+        if (part == 0 && o < RW * CM && j < c) {
+            const size_t idx = (size_t)(r0 + rr) * c + j;
+            out[idx] = add ? add[idx] - sum : sum;
+        }
     }
 }
 
-static int col_chunk(int c) {
-    int cw = 1;
-    while (cw < c && cw < 64) cw *= 2;
-    return cw;
+// Many columns (c > 16): 16 x 32 output tiles.  The k range is split into four
+// contiguous quarters, one per wave; each wave walks its quarter in chunks of 64 k
+// staged in its own LDS slice, with the next chunk's loads in flight (registers)
+// during the current chunk's products, and owns a 2 x 4 micro-tile per lane.  The
+// four partial tiles are summed through LDS.  Grid x = b / 16, y = ceil(c / 32).
+constexpr int kBT_M = 16, kBT_N = 32;
+
+template <typename T>
+struct TileSmem {   // per wave: A chunk [K][16] + V chunk [K][32]; K = 64 (FP32), 32 (FP64)
+    static constexpr int K = sizeof(T) == 4 ? 64 : 32;
+    static constexpr size_t elems = 4 * (size_t)K * (kBT_M + kBT_N);
+    static constexpr size_t bytes = sizeof(T) * elems;
+};
+
+// Split k (ks_n > 1, single block, tri = 0): workgroup z = ks covers the ks-th of
+// ks_n contiguous k ranges and writes its partial tile to part[ks][b][c]; the
+// partials are summed by band_splitk_reduce.
+template <typename T>
+__global__ void __launch_bounds__(256)
+band_tile_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_stride,
+                 const T *__restrict__ V, long long v_stride, const T *add, long long add_stride,
+                 T *out, long long out_stride, int ks_n, T *part) {
+    constexpr int kBT_K = TileSmem<T>::K, VJ = kBT_K / 2;   // V loads per lane
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *sm = reinterpret_cast<T *>(smem_raw);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T *As = sm + (size_t)wave * kBT_K * (kBT_M + kBT_N);   // [K][16]
+    T *Vs = As + kBT_K * kBT_M;                              // [K][32]
+    const int r0 = blockIdx.x * kBT_M, n0 = blockIdx.y * kBT_N;
+    const long long z = ks_n > 1 ? 0 : blockIdx.z;
+    A += z * a_stride;
+    V += z * v_stride;
+    out += z * out_stride;
+    if (add) add += z * add_stride;
+    int k0 = 0, k1 = b;
+    if (tri == 1) k1 = r0 + kBT_M;
+    if (tri == 2) k0 = r0;
+    if (ks_n > 1) {   // this workgroup's share of the k range, in whole chunks
+        const int per = (b / kBT_K + ks_n - 1) / ks_n * kBT_K;
+        k0 = blockIdx.z * per;
+        k1 = min(b, k0 + per);
+    }
+    // quarter of the k range (in whole chunks) for this wave; all waves run nch chunks
+    const int nch_all = (k1 - k0 + kBT_K - 1) / kBT_K;
+    const int nch = (nch_all + 3) / 4;
+    const int kw0 = k0 + wave * nch * kBT_K;
+    const int kw1 = min(k1, kw0 + nch * kBT_K);
+    // loads: A (row e / K, kk = e % K) and V (kk = e / 32, cc = e % 32), e = lane + 64 j
+    constexpr int AJ = kBT_M * kBT_K / 64;                   // A loads per lane
+    // Indices are clamped into range and out-of-range values zeroed at the LDS store,
+    // so every load of a chunk issues without a branch between them.
+    T ra[AJ], rv[VJ];
+    int kcur = kw0;
+    auto load = [&](int kc0) {
+        kcur = kc0;
+        const int klast = max(kw1 - 1, 0);
+#pragma unroll
+        for (int j = 0; j < AJ; ++j) {
+            const int e = lane + 64 * j, r = e / kBT_K, k = min(kc0 + (e % kBT_K), klast);
+            ra[j] = A[(size_t)(r0 + r) * b + k];
+        }
+#pragma unroll
+        for (int j = 0; j < VJ; ++j) {
+            const int e = lane + 64 * j, k = min(kc0 + (e >> 5), klast);
+            const int cc = min(n0 + (e & 31), c - 1);
+            rv[j] = V[(size_t)k * c + cc];
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < AJ; ++j) {
+            const int e = lane + 64 * j;
+            As[(e % kBT_K) * kBT_M + e / kBT_K] = (kcur + (e % kBT_K) < kw1) ? ra[j] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < VJ; ++j) {
+            const int e = lane + 64 * j;
+            const bool ok = kcur + (e >> 5) < kw1 && n0 + (e & 31) < c;
+            Vs[e] = ok ? rv[j] : T(0);
+        }
+    };
+    const int tr = (lane >> 3) * 2, tc = (lane & 7) * 4;
+    T acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+    load(kw0);
+    for (int ch = 0; ch < nch; ++ch) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (ch + 1 < nch) load(kw0 + (ch + 1) * kBT_K);
+#pragma unroll 8
+        for (int k = 0; k < kBT_K; ++k) {
+            const T a0 = As[k * kBT_M + tr], a1 = As[k * kBT_M + tr + 1];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const T v = Vs[k * kBT_N + tc + j];
+                acc[0][j] = fma(a0, v, acc[0][j]);
+                acc[1][j] = fma(a1, v, acc[1][j]);
+            }
+        }
+    }
+    __syncthreads();
+    T *red = sm;   // [4][16][32]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[(wave * kBT_M + tr + i) * kBT_N + tc + j] = acc[i][j];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int o = tid + 256 * u, r = o >> 5, cl = o & 31;
+        if (n0 + cl < c) {
+            const T sum = (red[(0 * kBT_M + r) * kBT_N + cl] + red[(1 * kBT_M + r) * kBT_N + cl]) +
+                          (red[(2 * kBT_M + r) * kBT_N + cl] + red[(3 * kBT_M + r) * kBT_N + cl]);
+            const size_t idx = (size_t)(r0 + r) * c + n0 + cl;
+            if (ks_n > 1) part[(size_t)blockIdx.z * b * c + idx] = sum;
+            else out[idx] = add ? add[idx] - sum : sum;
+        }
+    }
+}
+
+// out = (add ? add - sum_ks part[ks] : sum_ks part[ks]) over one b x c block.
+template <typename T>
+__global__ void __launch_bounds__(256)
+band_splitk_reduce(long long n, int ks_n, const T *__restrict__ part, const T *add, T *out) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n) return;
+    T sum = T(0);
+    for (int k = 0; k < ks_n; ++k) sum += part[(size_t)k * n + idx];
+    out[idx] = add ? add[idx] - sum : sum;
+}
+
+// Tile kernel launch.  A single-block product (a recurrence step) splits k over
+// workgroups so that each wave walks one chunk: the few workgroups of one block
+// would otherwise each run a long chain of dependent chunks.
+template <typename T>
+static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long as,
+                       const T *V, long long vs, const T *add, long long ads, T *out,
+                       long long os, int nblk, hipStream_t s) {
+    const int b = bp.b;
+    int ks_n = 1;
+    if (nblk == 1 && tri == 0) ks_n = std::min(8, std::max(1, b / (4 * TileSmem<T>::K)));
+    dim3 grid(b / kBT_M, ceil_div(c, kBT_N), ks_n > 1 ? ks_n : nblk);
+    T *part = ks_n > 1 ? bp.part.as<T>() : nullptr;
+    hipLaunchKernelGGL(band_tile_kernel<T>, grid, dim3(256), TileSmem<T>::bytes, s, b, c, tri, A,
+                       as, V, vs, add, ads, out, os, ks_n, part);
+    if (ks_n > 1) {
+        const long long n = (long long)b * c;
+        hipLaunchKernelGGL(band_splitk_reduce<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                           ks_n, (const T *)part, add, out);
+    }
+    return XT_OK;
 }
 
 template <typename T>
 static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long as,
                        const T *V, long long vs, const T *add, long long ads, T *out,
                        long long os, int nblk, hipStream_t s) {
-    const int cw = col_chunk(c);
     const int b = bp.b;
-    dim3 grid(b / kTM, ceil_div(c, cw), nblk);
     const bool prof = prof_on(XT_PROF_BAND);
     if (prof) prof_begin(XT_PROF_BAND, s);
-    hipLaunchKernelGGL(band_gemm_kernel<T>, grid, dim3(256), BandTile<T>::smem, s, b, c, cw, tri,
-                       A, as, V, vs, add, ads, out, os);
+    const size_t vlds = sizeof(T) * (size_t)b * c;
+#define XT_ROWS(CM, RW)                                                                        \
+    hipLaunchKernelGGL((band_rows_kernel<T, CM, RW>), dim3(b / RW, 1, nblk), dim3(256), vlds, s, \
+                       b, c, tri, A, as, V, vs, add, ads, out, os)
+    if (vlds > kRowsLdsMax) {
+        XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
+    } else if (c <= 8) {
+        XT_ROWS(8, 2);
+    } else if (c <= 16) {
+        XT_ROWS(16, 1);
+    } else {
+        XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
+    }
+#undef XT_ROWS
     if (prof) {
-        // matrix elements the tiles stage (triangular: rows r0..r0+7 up to/from the
-        // diagonal tile), V read, out written, add read
-        const double mat = tri ? (double)b * (b + kTM) / 2 : (double)b * b;
+        // matrix elements of the stored operator (triangular: the lower or upper half
+        // and the diagonal), V read, out written, add read
+        const double mat = tri ? (double)b * (b + 1) / 2 : (double)b * b;
         const double vec = (double)b * c;
         const double bytes = sizeof(T) * nblk * (mat + vec * (add ? 3 : 2));
         prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c);
@@ -377,12 +471,18 @@ static int factor(BandPrec &bp, std::vector<double> &kd, std::vector<double> &ko
 
 int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out) {
     XT_ARG(p > 0 && ia && ja && a && out, "band prec arguments");
-    XT_HIP(hipFuncSetAttribute((const void *)band_gemm_kernel<double>,
+    XT_HIP(hipFuncSetAttribute((const void *)band_tile_kernel<double>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)BandTile<double>::smem));
-    XT_HIP(hipFuncSetAttribute((const void *)band_gemm_kernel<float>,
+                               (int)TileSmem<double>::bytes));
+    XT_HIP(hipFuncSetAttribute((const void *)band_tile_kernel<float>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)BandTile<float>::smem));
+                               (int)TileSmem<float>::bytes));
+#define XT_ROWS_ATTR(T, CM, RW)                                                                \
+    XT_HIP(hipFuncSetAttribute((const void *)band_rows_kernel<T, CM, RW>,                      \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRowsLdsMax))
+    XT_ROWS_ATTR(double, 8, 2); XT_ROWS_ATTR(double, 16, 1);
+    XT_ROWS_ATTR(float, 8, 2); XT_ROWS_ATTR(float, 16, 1);
+#undef XT_ROWS_ATTR
     XT_ARG(ia[0] == 1, "CSR must be 1-based");
     int bw = 0;
     for (int i = 0; i < p; ++i)
@@ -440,6 +540,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     const long long npad = (long long)nb * b * c, nreal = (long long)bp->p * c;
     XT_TRY(bp->w1.reserve(sizeof(T) * npad));
     XT_TRY(bp->w2.reserve(sizeof(T) * npad));
+    XT_TRY(bp->part.reserve(sizeof(T) * 8 * (size_t)b * c));
     T *w1 = bp->w1.as<T>(), *w2 = bp->w2.as<T>();
     if (minv) XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w1, s));
     else XT_HIP(hipMemcpyAsync(w1, x, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));

@@ -5,11 +5,15 @@ Workload: the single-edge-notched plate, C3D8 grid 100 x 46 x 4 elements over
 100 x 46 x 4 mm with a 20 x 1 element strip removed at mid-height (x < 20 mm,
 23 mm < y < 24 mm), 3 displacement DOF per node (p = 71,205 before the notch),
 clamped at y = 0, tensile traction on y = 46, linear elastic steel (E = 2e5 MPa,
-nu = 0.3, rho = 7.85e-9 t/mm^3), n_t = 6.  R load cases: case 0 is 80 MPa at
-20 Hz, the others draw amplitudes U[20, 120] MPa and frequencies U[10, 100] Hz from a
-fixed seed.  Each case's slab spans a whole number of load periods closest to 0.5 s
-(dt_r = round(f_r / 2) / f_r; 0.5 s = 10 periods for case 0), the regime in which the
-reference's t1 = 0 temporal matrices (main3d.f90:816) describe every slab.
+nu = 0.3, rho = 7.85e-9 t/mm^3), n_t = 6, slabs of dt = 0.5 s (10 load periods).
+R load cases on the specimen, all at the plate's 20 Hz: case 0 is the uniform
+80 MPa traction; the others draw an amplitude U[20, 120] MPa and a traction profile
+1 + a sin(2 pi m x / 100 + phi) along the loaded edge (a U[0, 0.5], m in {1, 2, 3},
+phi U[0, 2 pi]) from a fixed seed.
+--cases-kind sweep instead draws frequencies U[10, 100] Hz (uniform traction), each
+case's slab spanning the whole number of periods closest to 0.5 s
+(dt_r = round(f_r / 2) / f_r), the regime in which the reference's t1 = 0 temporal
+matrices (main3d.f90:816) describe every slab.
 
 One step = one space-time slab for every load case, all on the device:
   jump term   b = (ktn1_r (x) K + mtn1_r (x) M) x_prev_r          (kronspgemv)
@@ -49,6 +53,7 @@ def parse():
     ap.add_argument("--grid", type=int, nargs=3, default=[100, 46, 4])
     ap.add_argument("--notch", type=int, default=20)
     ap.add_argument("--block", type=int, default=0, help="factor block rows (0: bandwidth)")
+    ap.add_argument("--cases-kind", choices=("plate", "sweep"), default="plate")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     return ap.parse_args()
@@ -57,7 +62,7 @@ def parse():
 class Problem:
     """Mesh, spatial matrices and per-case slab data of the notched plate."""
 
-    def __init__(self, nx, ny, nz, notch, R):
+    def __init__(self, nx, ny, nz, notch, R, kind="plate"):
         from xtfem_amd import fem, slab
         keep = np.ones((nx, ny, nz), bool)
         keep[0:notch, ny // 2, :] = False
@@ -68,15 +73,28 @@ class Problem:
         self.p = 3 * self.mesh.nn
         rng = np.random.default_rng(SEED)
         self.amp = np.concatenate([[80.0], rng.uniform(20.0, 120.0, R - 1)])
-        self.freq = np.concatenate([[20.0], rng.uniform(10.0, 100.0, R - 1)])
+        if kind == "sweep":
+            self.freq = np.concatenate([[20.0], rng.uniform(10.0, 100.0, R - 1)])
+        else:
+            self.freq = np.full(R, 20.0)
         self.w = 2 * np.pi * self.freq
         self.dt = np.round(0.5 * self.freq) / self.freq
         # t1 = 0 in stcoeff_m, as the reference's call passes it (main3d.f90:816)
         c = slab.stcoeff_m(0.0, self.dt, self.w)
         self.A, self.B = c["k"] + c["ktn"], c["m"] + c["mtn"]
         self.J1, self.J2 = c["ktn1"], c["mtn1"]
-        # unit traction on y = ny: total force = area of the loaded face
-        self.rext = fem.traction_load(self.mesh, 1, float(ny), 1, float(nx * nz))
+        # unit traction on y = ny (total force = area of the loaded face), then each
+        # case's profile along x; rows of self.rext are per case [R, p]
+        unit = fem.traction_load(self.mesh, 1, float(ny), 1, float(nx * nz))
+        xnode = np.repeat(self.mesh.xcoord[:, 0], 3)
+        self.rext = np.tile(unit, (R, 1))
+        if kind == "plate":
+            a = rng.uniform(0.0, 0.5, R)
+            m = rng.integers(1, 4, R)
+            phi = rng.uniform(0.0, 2 * np.pi, R)
+            a[0] = 0.0
+            for r in range(R):
+                self.rext[r] *= 1.0 + a[r] * np.sin(2 * np.pi * m[r] * xnode / nx + phi[r])
 
     def coef(self, step, cases):
         """fextc_r(j) * amp_r for slab `step` (tcoorde(1) = step * dt_r, main3d.f90:806)."""
@@ -97,7 +115,7 @@ class Runner:
         p, R = prob.p, self.R
         dev = torch.device("cuda")
         self.tj = op.tmat(np.stack([prob.J1[self.cases], prob.J2[self.cases]], axis=1), False)
-        self.rext = torch.as_tensor(prob.rext, device=dev)
+        self.rext = torch.as_tensor(np.ascontiguousarray(prob.rext[self.cases]), device=dev)
         self.fixed = torch.as_tensor(prob.bc.astype(np.uint8), device=dev)
         self.coefs = torch.as_tensor(np.stack([prob.coef(n, self.cases) for n in range(nsteps)]),
                                      device=dev)
@@ -116,8 +134,8 @@ class Runner:
         s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         self.op.apply(self.tj, self.xprev, self.b, self.R, alpha=1.0, beta=0.0)
         check(lib().xt_slab_rhs(self.prob.p, 6, self.R, self.coefs[self.n].data_ptr(),
-                                self.rext.data_ptr(), self.fixed.data_ptr(), self.b.data_ptr(), s),
-              "slab_rhs")
+                                self.rext.data_ptr(), self.prob.p, self.fixed.data_ptr(),
+                                self.b.data_ptr(), s), "slab_rhs")
         self.x.zero_()
         _, its, ierr, relres = self.device.slab_solve_device(
             self.op, self.pr, self.A, self.B, self.b, self.x, self.R, im=30, eps=1e-8,
@@ -206,7 +224,7 @@ def cpu_baseline(prob, budget_s=15.0):
     while n == 0 or (t_solve < budget_s and n < 8):
         t0 = time.perf_counter()
         jump = oracle.kronspmv(prob.J1[0], prob.J2[0], prob.Kb, prob.Mb, x)
-        rhs = slab.slab_rhs(p, prob.coef(n, np.array([0])), None, prob.rext, jump=jump,
+        rhs = slab.slab_rhs(p, prob.coef(n, np.array([0])), None, prob.rext[0], jump=jump,
                             fixed=fixed)
         x, ie, its, _ = oracle.kronpgmres(prob.A[0], prob.B[0], prob.Kb, prob.Mb, bf, rhs,
                                           im=30, eps=1e-8, maxits=300)
@@ -235,7 +253,7 @@ def main():
     R, K, W = args.cases, args.steps, args.warmup
     nx, ny, nz = args.grid
     t0 = time.perf_counter()
-    prob = Problem(nx, ny, nz, args.notch, R)
+    prob = Problem(nx, ny, nz, args.notch, R, args.cases_kind)
     t_asm = time.perf_counter() - t0
     t0 = time.perf_counter()
     op = device.KronOperator([prob.Kb, prob.Mb], q=6)
@@ -259,10 +277,19 @@ def main():
     from xtfem_amd.device import from_interleaved
     print(f"# fp32-ir its per case (last slab): {run32.its[-1].tolist()}", file=sys.stderr)
     x32 = from_interleaved(run32.xprev, prob.p, 6, R)          # every case after W + K slabs
-    prof = kernel_profile(run32, K, mask=0b11)
 
     res = {}
     if not args.no_fp64:
+        # FP64 GMRES on the same batch (all cases in one solve)
+        runb = Runner(prob, op, pr, np.arange(R), mode=0, nsteps=nsteps)
+        for _ in range(W):
+            runb.step()
+        tb = timed(runb, K)
+        res["fp64_batched_slabs_per_s"] = K * R / tb
+        res["fp64_batched_its_max"] = int(np.max(runb.its[W:]))
+        print(f"# fp64 batched: {K} steps x {R} cases in {tb:.3f}s, its max "
+              f"{np.max(runb.its[W:])}", file=sys.stderr, flush=True)
+        del runb
         # plain FP64 GPU GMRES on the same slabs, one load case per solve
         runs = [Runner(prob, op, pr, np.array([r]), mode=0, nsteps=nsteps) for r in range(R)]
         for _ in range(W):
@@ -290,6 +317,9 @@ def main():
               f"{its64.mean():.1f} max {its64.max()}, relres max {rr64.max():.2e}, "
               f"rel err fp32-ir vs fp64 max {err.max():.2e}", file=sys.stderr, flush=True)
     spmm = spmm_microbench(op, prob.p)
+    # per-kernel durations: the batched FP32-IR steps once more, every launch of the
+    # operator and sweep kernels bracketed by HIP events (after the timed legs)
+    prof = kernel_profile(run32, K, mask=0b11)
 
     value = K * R / t32
     dom_name = max(prof, key=lambda k: prof[k]["ms"])
@@ -308,7 +338,8 @@ def main():
         "dtype": "f32 operator/factor sweeps in f64 GMRES-IR",
         "data": "synthetic (structured notched-plate mesh and seeded load cases)",
         "config": {"workload": f"notched plate {nx}x{ny}x{nz} C3D8, notch {args.notch}x1, "
-                               f"3 DOF/node, p={prob.p}, n_t=6, {R} load cases per solve, "
+                               f"3 DOF/node, p={prob.p}, n_t=6, {R} load cases per solve "
+                               f"({args.cases_kind}), "
                                f"GMRES(30) eps 1e-8, exact banded factor "
                                f"({pr.nblocks} blocks of {pr.block})",
                    "cases": R, "p": prob.p},

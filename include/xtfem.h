@@ -192,11 +192,12 @@ int xt_damage_device(int ne, int nn, int nip, const int *nodes, const double *Nx
                      void *stream);
 
 /* Right-hand side of a slab for R load cases (main3d.f90:803-823):
- *   b[dof][r*q + j] := fixed[dof] ? 0 : b[dof][r*q + j] + coef[r*q + j] * rext[dof]
- * i.e. fextc_r(j) * amplitude_r * rext added to the jump term already in b (from
+ *   b[dof][r*q + j] := fixed[dof] ? 0 : b[dof][r*q + j] + coef[r*q + j] * rext_r[dof]
+ * i.e. fextc_r(j) * amplitude_r * rext_r added to the jump term already in b (from
  * xt_op_apply with the ktn1/mtn1 temporal matrices), then the fixed DOFs zeroed.
- * coef: device [R*q]; rext: device [p]; fixed: device uint8 [p] or NULL.  FP64. */
-int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext,
+ * coef: device [R*q]; rext: device, rext_r = rext + r * rext_ld (rext_ld = 0: one
+ * load vector for every case, else >= p); fixed: device uint8 [p] or NULL.  FP64. */
+int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext, long long rext_ld,
                 const unsigned char *fixed, double *b, void *stream);
 
 /* ---- kernel timing ----------------------------------------------------------- */
@@ -206,7 +207,8 @@ int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext,
 #define XT_PROF_DAMAGE 3   /* damage kernel                                         */
 #define XT_PROF_NKINDS 4
 /* Enable per-launch HIP-event timing for the kinds in mask (bit k = kind k) and
- * clear the counters; 0 disables.  Process-wide; one host thread. */
+ * clear the counters; 0 disables and releases the events.  Process-wide; one host
+ * thread. */
 int xt_prof_enable(unsigned mask);
 /* Sum of per-launch kernel durations (ms), number of launches, and the launches'
  * algorithmic bytes and flops since xt_prof_enable.  Synchronises on the last event. */

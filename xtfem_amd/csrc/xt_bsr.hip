@@ -212,7 +212,7 @@ kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
     constexpr int NPW = 64 / LPN;
     const int lane = threadIdx.x & 63;
     const long long blk = xcd_block(blockIdx.x, gridDim.x);
-    const long long wave = blk * 4 + (threadIdx.x >> 6);
+    const long long wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const long long slot = wave * NPW + lane / LPN;
     const int s = (int)(slot / kSlice);
     if (s >= nslices) return;
@@ -293,9 +293,11 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
         while (lpn < Rc) lpn <<= 1;
         const long long slots = (long long)P.nslices * kSlice;
         const long long waves = slots * lpn / 64;
-        const int grid = (int)(((waves + 3) / 4 + 7) / 8 * 8);
+        // one wave per workgroup when four-wave workgroups would leave CUs idle
+        const int wpb = waves >= 4 * 1024 ? 4 : 1;
+        const int grid = (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
 #define XT_BSR_LAUNCH(N)                                                                   \
-    hipLaunchKernelGGL((kron_bsr3_kernel<T, N>), dim3(grid), dim3(256), 0, s, P.nslices,    \
+    hipLaunchKernelGGL((kron_bsr3_kernel<T, N>), dim3(grid), dim3(64 * wpb), 0, s, P.nslices, \
                        P.slot_node.as<int>(), P.slice_base.as<int>(), P.slice_len.as<int>(), \
                        P.bcol.as<int>(), bv, R, r0, c, tmat, (T)alpha, (T)beta, x, y)
         if (prof_on(XT_PROF_SPMM)) prof_begin(XT_PROF_SPMM, s);

@@ -69,27 +69,26 @@ multidot_partial(int p, int c, int k, int chunk, const double *__restrict__ V, s
     }
 }
 
-// out[j][r] = sum_b sum_{i<q} partial[b][j][r*q + i]   (one block per j)
+// out[j][r] = sum_b sum_{i<q} partial[b][j][r*q + i]: one workgroup per (j, r), its
+// threads stride over the nblk * q terms, then a tree sum in LDS.
 __global__ void __launch_bounds__(256)
 reduce_partials(int nblk, int k, int c, int q, int R, const double *__restrict__ partial,
                 double *__restrict__ out) {
     __shared__ double sh[256];
-    const int j = blockIdx.x;
-    const int rpl = 256 / c;
-    const int lr = threadIdx.x / c;
-    const int col = threadIdx.x - lr * c;
+    const int j = blockIdx.x, r = blockIdx.y;
+    const int n = nblk * q;
     double s = 0.0;
-    if (lr < rpl)
-        for (int b = lr; b < nblk; b += rpl) s += partial[((size_t)b * k + j) * c + col];
-    sh[threadIdx.x] = (lr < rpl) ? s : 0.0;
-    __syncthreads();
-    if (threadIdx.x < R) {
-        const int r = threadIdx.x;
-        double t = 0.0;
-        for (int l = 0; l < rpl; ++l)
-            for (int i = 0; i < q; ++i) t += sh[l * c + r * q + i];
-        out[(size_t)j * R + r] = t;
+    for (int t = threadIdx.x; t < n; t += 256) {
+        const int b = t / q, i = t - b * q;
+        s += partial[((size_t)b * k + j) * c + r * q + i];
     }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)j * R + r] = sh[0];
 }
 
 // y = (yin ? yin : 0) + alpha * sum_j coef[j][r] V_j
@@ -227,7 +226,7 @@ struct Solver {
         dim3 grid(nblk, ceil_div(k, JB));
         hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, p, c, k, chunk, Vbase,
                            (size_t)n, w, partial.as<double>());
-        hipLaunchKernelGGL(reduce_partials, dim3(k), dim3(256), 0, s, nblk, k, c, q, R,
+        hipLaunchKernelGGL(reduce_partials, dim3(k, R), dim3(256), 0, s, nblk, k, c, q, R,
                            partial.as<double>(), out);
         XT_HIP(hipGetLastError());
         return XT_OK;

@@ -187,14 +187,15 @@ template int temporal_mix<float, float>(int, int, int, const float *, const floa
 // Slab right-hand side (main3d.f90:803-823): b += coef (x) rext, fixed DOFs zeroed.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-slab_rhs_kernel(long long n, int c, const double *__restrict__ coef,
-                const double *__restrict__ rext, const unsigned char *__restrict__ fixed,
-                double *__restrict__ b) {
+slab_rhs_kernel(long long n, int c, int q, const double *__restrict__ coef,
+                const double *__restrict__ rext, long long rext_ld,
+                const unsigned char *__restrict__ fixed, double *__restrict__ b) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const long long dof = idx / c;
     const int col = (int)(idx - dof * c);
-    b[idx] = (fixed && fixed[dof]) ? 0.0 : b[idx] + coef[col] * rext[dof];
+    const double re = rext[(long long)(col / q) * rext_ld + dof];
+    b[idx] = (fixed && fixed[dof]) ? 0.0 : b[idx] + coef[col] * re;
 }
 
 // ---------------------------------------------------------------------------
@@ -321,12 +322,13 @@ long long xt_op_blocks(xt_op op, int real) {
 
 void xt_op_destroy(xt_op op) { delete op; }
 
-int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext,
+int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext, long long rext_ld,
                 const unsigned char *fixed, double *b, void *stream) {
     XT_ARG(p > 0 && q > 0 && R > 0 && coef && rext && b, "slab_rhs arguments");
+    XT_ARG(rext_ld == 0 || rext_ld >= p, "slab_rhs: rext_ld must be 0 or >= p");
     const long long n = (long long)p * q * R;
     hipLaunchKernelGGL(slab_rhs_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
-                       (hipStream_t)stream, n, q * R, coef, rext, fixed, b);
+                       (hipStream_t)stream, n, q * R, q, coef, rext, rext_ld, fixed, b);
     XT_HIP(hipGetLastError());
     return XT_OK;
 }

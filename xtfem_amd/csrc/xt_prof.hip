@@ -60,6 +60,10 @@ int xt_prof_enable(unsigned mask) {
     for (auto &k : g_kind) {
         k.used = 0;
         k.bytes = k.flops = 0.0;
+        if (!mask) {   // disabling releases the event pool
+            for (hipEvent_t e : k.ev) (void)hipEventDestroy(e);
+            k.ev.clear();
+        }
     }
     return XT_OK;
 }

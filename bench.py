@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--notch", type=int, default=20)
     ap.add_argument("--block", type=int, default=0, help="factor block rows (0: bandwidth)")
     ap.add_argument("--cases-kind", choices=("plate", "sweep"), default="plate")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     return ap.parse_args()
@@ -238,8 +240,36 @@ def cpu_baseline(prob, budget_s=15.0):
                        f"its {its_all}, {t_solve:.1f} s on one host core")
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "band_tile_kernel_notched_plate_pmc.md")
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    import re
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PMC_SUMMARY).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_probe():
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 1 << 28                                     # 2^28 floats = 1 GiB each way
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
               "(independent load-case groups or meshes would shard across GPUs).",
@@ -353,7 +383,9 @@ def main():
         "kernels": prof,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbs"],
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["gbs"] / PEAK_HBM_GBS,
-                     "traffic": None},
+                     "traffic": pmc_traffic(),
+                     "algorithmic_bytes_per_launch": dom["bytes"] / dom["launches"],
+                     "avg_launch_us": dom["avg_us"]},
         "setup_s": {"assembly": t_asm, "upload_factor": t_fac},
     }
     if not args.no_cpu:

@@ -103,6 +103,19 @@ int xt_kokkos_host(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int 
                    double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
                    double *p, double *D, double *ws, double *del_po, double *sig_d);
 
+/* The reference's own C symbols of the damage boundary (kokkos_host.cpp:14-24,
+ * 37-41), with its argument list (scalars by value, as kokkos_damage.f90:86-89 passes
+ * them with %VAL): kokkos_host_ runs xt_kokkos_host and aborts with a message on a
+ * library failure (the reference has no error return); kokkos_init_ does nothing,
+ * kokkos_finish_ synchronises the device. */
+void kokkos_init_(void);
+void kokkos_finish_(void);
+void kokkos_host_(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int ngpe, int ncmp,
+                  int nip, int *nodes, double *Nx, double *Ny, double *Nz, double *Nt,
+                  double *sol, double *ym, double *nu, double *eps_old, double *eps_pl,
+                  double *sig_eff, double *sig_back, double *p, double *D, double *ws,
+                  double *del_po, double *sig_d);
+
 /* Element status from damage: element e fails iff any of its ngpe Gauss points
  * has D >= Dc.  Replaces chkelesta, delelement.f90:109-136 (device reduction).
  * nefail receives the failed (1-based) element ids in increasing order. */

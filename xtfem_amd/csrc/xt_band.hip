@@ -278,6 +278,16 @@ band_splitk_reduce(long long n, int ks_n, const T *__restrict__ part, const T *a
     out[idx] = add ? add[idx] - sum : sum;
 }
 
+// Algorithmic bytes and flops of one sweep kernel launch: the matrix elements of the
+// stored operator (triangular: the lower or upper half and the diagonal), V read and
+// out_elems elements written or read besides, per block of the batch.
+template <typename T>
+static void band_prof_end(int b, int c, int tri, int nblk, double out_elems, hipStream_t s) {
+    const double mat = tri ? (double)b * (b + 1) / 2 : (double)b * b;
+    const double bytes = sizeof(T) * nblk * (mat + (double)b * c + out_elems);
+    prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c);
+}
+
 // Tile kernel launch.  A single-block product (a recurrence step) splits k over
 // workgroups so that each wave walks one chunk: the few workgroups of one block
 // would otherwise each run a long chain of dependent chunks.
@@ -290,8 +300,15 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
     if (nblk == 1 && tri == 0) ks_n = std::min(8, std::max(1, b / (4 * TileSmem<T>::K)));
     dim3 grid(b / kBT_M, ceil_div(c, kBT_N), ks_n > 1 ? ks_n : nblk);
     T *part = ks_n > 1 ? bp.part.as<T>() : nullptr;
+    const bool prof = prof_on(XT_PROF_BAND);
+    if (prof) prof_begin(XT_PROF_BAND, s);
     hipLaunchKernelGGL(band_tile_kernel<T>, grid, dim3(256), TileSmem<T>::bytes, s, b, c, tri, A,
                        as, V, vs, add, ads, out, os, ks_n, part);
+    if (prof) {   // split k: partial tiles written instead of out (add read by the reduction)
+        const double vec = (double)b * c;
+        const double outb = ks_n > 1 ? ks_n * vec : vec * (add ? 2 : 1);
+        band_prof_end<T>(b, c, tri, nblk, outb, s);
+    }
     if (ks_n > 1) {
         const long long n = (long long)b * c;
         hipLaunchKernelGGL(band_splitk_reduce<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
@@ -306,11 +323,12 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
                        long long os, int nblk, hipStream_t s) {
     const int b = bp.b;
     const bool prof = prof_on(XT_PROF_BAND);
-    if (prof) prof_begin(XT_PROF_BAND, s);
     const size_t vlds = sizeof(T) * (size_t)b * c;
 #define XT_ROWS(CM, RW)                                                                        \
+    if (prof) prof_begin(XT_PROF_BAND, s);                                                     \
     hipLaunchKernelGGL((band_rows_kernel<T, CM, RW>), dim3(b / RW, 1, nblk), dim3(256), vlds, s, \
-                       b, c, tri, A, as, V, vs, add, ads, out, os)
+                       b, c, tri, A, as, V, vs, add, ads, out, os);                            \
+    if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s)
     if (vlds > kRowsLdsMax) {
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
     } else if (c <= 8) {
@@ -321,14 +339,6 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
     }
 #undef XT_ROWS
-    if (prof) {
-        // matrix elements of the stored operator (triangular: the lower or upper half
-        // and the diagonal), V read, out written, add read
-        const double mat = tri ? (double)b * (b + 1) / 2 : (double)b * b;
-        const double vec = (double)b * c;
-        const double bytes = sizeof(T) * nblk * (mat + vec * (add ? 3 : 2));
-        prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c);
-    }
     return XT_OK;
 }
 

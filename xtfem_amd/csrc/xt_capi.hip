@@ -8,6 +8,8 @@
 #include <cmath>
 #include <vector>
 
+#include <cstdlib>
+
 #include "xt_common.h"
 
 namespace xt {
@@ -219,6 +221,24 @@ int xt_kokkos_host(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int 
     for (int k = 0; k < 5; ++k)
         XT_HIP(hipMemcpy(st1[k], d1[k]->ptr, sizeof(double) * ngp, hipMemcpyDeviceToHost));
     return XT_OK;
+}
+
+// The reference's own C symbols of the damage boundary, so kokkos_damage.f90 and
+// main3d.f90 link against this library unchanged.
+void kokkos_init_(void) {}
+void kokkos_finish_(void) { (void)hipDeviceSynchronize(); }
+void kokkos_host_(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int ngpe, int ncmp,
+                  int nip, int *nodes, double *Nx, double *Ny, double *Nz, double *Nt,
+                  double *sol, double *ym, double *nu, double *eps_old, double *eps_pl,
+                  double *sig_eff, double *sig_back, double *p, double *D, double *ws,
+                  double *del_po, double *sig_d) {
+    const int rc = xt_kokkos_host(rank, ne, nn, ndofn, nnse, nnte, ngpe, ncmp, nip, nodes, Nx,
+                                  Ny, Nz, Nt, sol, ym, nu, eps_old, eps_pl, sig_eff, sig_back, p,
+                                  D, ws, del_po, sig_d);
+    if (rc != XT_OK) {   // the reference has no error return here: fail loudly
+        std::fprintf(stderr, "kokkos_host_: %s (code %d)\n", get_error(), rc);
+        std::abort();
+    }
 }
 
 int xt_chkelesta(int ne, int ngpe, double Dc, const double *D, int *elesta, int *gpsta, int *nef,

@@ -151,9 +151,10 @@ band_rows_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
 constexpr int kBT_M = 16, kBT_N = 32;
 
 template <typename T>
-struct TileSmem {   // per wave: A chunk [K][16] + V chunk [K][32]; K = 64 (FP32), 32 (FP64)
+struct TileSmem {   // per wave: A chunk [K][16 + 1] + V chunk [K][32]; K = 64 (FP32), 32 (FP64)
     static constexpr int K = sizeof(T) == 4 ? 64 : 32;
-    static constexpr size_t elems = 4 * (size_t)K * (kBT_M + kBT_N);
+    static constexpr int LDA = kBT_M + 1;   // padded: the transposing stores avoid bank conflicts
+    static constexpr size_t elems = 4 * (size_t)K * (LDA + kBT_N);
     static constexpr size_t bytes = sizeof(T) * elems;
 };
 
@@ -169,8 +170,9 @@ band_tile_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    T *As = sm + (size_t)wave * kBT_K * (kBT_M + kBT_N);   // [K][16]
-    T *Vs = As + kBT_K * kBT_M;                              // [K][32]
+    constexpr int LDA = TileSmem<T>::LDA;
+    T *As = sm + (size_t)wave * kBT_K * (LDA + kBT_N);      // [K][LDA]
+    T *Vs = As + kBT_K * LDA;                                // [K][32]
     const int r0 = blockIdx.x * kBT_M, n0 = blockIdx.y * kBT_N;
     const long long z = ks_n > 1 ? 0 : blockIdx.z;
     A += z * a_stride;
@@ -215,7 +217,7 @@ band_tile_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
 #pragma unroll
         for (int j = 0; j < AJ; ++j) {
             const int e = lane + 64 * j;
-            As[(e % kBT_K) * kBT_M + e / kBT_K] = (kcur + (e % kBT_K) < kw1) ? ra[j] : T(0);
+            As[(e % kBT_K) * LDA + e / kBT_K] = (kcur + (e % kBT_K) < kw1) ? ra[j] : T(0);
         }
 #pragma unroll
         for (int j = 0; j < VJ; ++j) {
@@ -238,7 +240,7 @@ band_tile_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
         if (ch + 1 < nch) load(kw0 + (ch + 1) * kBT_K);
 #pragma unroll 8
         for (int k = 0; k < kBT_K; ++k) {
-            const T a0 = As[k * kBT_M + tr], a1 = As[k * kBT_M + tr + 1];
+            const T a0 = As[k * LDA + tr], a1 = As[k * LDA + tr + 1];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const T v = Vs[k * kBT_N + tc + j];

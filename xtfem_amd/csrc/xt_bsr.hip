@@ -228,12 +228,20 @@ kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
 #pragma unroll
         for (int j = 0; j < 6; ++j) { aK[i][j] = T(0); aM[i][j] = T(0); }
     const int rc = act ? r * 6 : 0;
-    for (int e = 0; e < L; ++e) {
-        const long long idx = b0 + (long long)e * kSlice + k;
-        const int m = bcol[idx];
-        T kv[12];
-        load_block(bval + idx * 12, kv);
-        T xv[3][6];
+    // UNR entries per pass: their column indices, blocks and X rows are all loaded
+    // before the products, so one pass costs two dependent memory latencies (column
+    // index, then X) instead of two per entry.
+    constexpr int UNR = sizeof(T) == 4 ? 4 : 2;
+    auto accumulate = [&](const T (&kv)[12], const T (&xv)[3][6]) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                aK[i][j] += kv[3 * i + 0] * xv[0][j] + kv[3 * i + 1] * xv[1][j] + kv[3 * i + 2] * xv[2][j];
+                aM[i][j] += kv[9 + i] * xv[i][j];
+            }
+    };
+    auto gather = [&](int m, T (&xv)[3][6]) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const T *xr = x + (size_t)(3 * m + d) * c + rc;
@@ -244,13 +252,28 @@ kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
                 xv[d][j + 1] = v.y;
             }
         }
+    };
+    int e = 0;
+    for (; e + UNR <= L; e += UNR) {
+        int m[UNR];
+        T kv[UNR][12], xv[UNR][3][6];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int u = 0; u < UNR; ++u) {
+            const long long idx = b0 + (long long)(e + u) * kSlice + k;
+            m[u] = bcol[idx];
+            load_block(bval + idx * 12, kv[u]);
+        }
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                aK[i][j] += kv[3 * i + 0] * xv[0][j] + kv[3 * i + 1] * xv[1][j] + kv[3 * i + 2] * xv[2][j];
-                aM[i][j] += kv[9 + i] * xv[i][j];
-            }
+        for (int u = 0; u < UNR; ++u) gather(m[u], xv[u]);
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) accumulate(kv[u], xv[u]);
+    }
+    for (; e < L; ++e) {
+        const long long idx = b0 + (long long)e * kSlice + k;
+        T kv[12], xv[3][6];
+        load_block(bval + idx * 12, kv);
+        gather(bcol[idx], xv);
+        accumulate(kv, xv);
     }
     if (!act) return;
     const T *TA = tmat + (size_t)(r * 2 + 0) * 36;

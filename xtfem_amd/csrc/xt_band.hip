@@ -36,7 +36,6 @@ struct BandPrec {
     // G, H: recurrence matrices (nb - 1 of each).
     DevBuf Dl[2], Du[2], G[2], H[2];
     DevBuf w1, w2;
-    DevBuf part;   // split-k partial tiles of a recurrence step: [8][b][c]
 };
 
 // ---------------------------------------------------------------------------
@@ -143,141 +142,100 @@ band_rows_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
     }
 }
 
-// Many columns (c > 16): 16 x 32 output tiles.  The k range is split into four
-// contiguous quarters, one per wave; each wave walks its quarter in chunks of 64 k
-// staged in its own LDS slice, with the next chunk's loads in flight (registers)
-// during the current chunk's products, and owns a 2 x 4 micro-tile per lane.  The
-// four partial tiles are summed through LDS.  Grid x = b / 16, y = ceil(c / 32).
-constexpr int kBT_M = 16, kBT_N = 32;
+// Many columns (c > 16) on the matrix cores: a workgroup computes one 16 x 16 output
+// tile; its 4 waves split the k range into quarters (whole chunks) and each runs
+// v_mfma_*_16x16x4 steps over its quarter; the 4 partial tiles are summed through
+// LDS in a fixed order (deterministic, one launch).  Lane l = 16 g + t feeds row t of
+// A and column t of V; the k index of a step is assigned per lane group g (group g
+// holds KG consecutive k of each chunk of 4 KG, loaded as one vector), which any
+// order of the k sum allows, so A rows are read with vector loads.  Operands come
+// straight from global memory (L2), UNR chunks of loads in flight per pass.  Output
+// layout of the 4 accumulators per lane: f32 row 4 g + i, f64 row g + 4 i, column t.
+// Grid x = b / 16, y = ceil(c / 16), z = blocks of the batch.
+typedef float xt_f4m __attribute__((ext_vector_type(4)));
+typedef double xt_d4m __attribute__((ext_vector_type(4)));
 
 template <typename T>
-struct TileSmem {   // per wave: A chunk [K][16 + 1] + V chunk [K][32]; K = 64 (FP32), 32 (FP64)
-    static constexpr int K = sizeof(T) == 4 ? 64 : 32;
-    static constexpr int LDA = kBT_M + 1;   // padded: the transposing stores avoid bank conflicts
-    static constexpr size_t elems = 4 * (size_t)K * (LDA + kBT_N);
-    static constexpr size_t bytes = sizeof(T) * elems;
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef xt_f4m acc_t;
+    static constexpr int KG = 4, UNR = 4;
+    __device__ static acc_t step(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int g, int i) { return 4 * g + i; }
+};
+template <>
+struct Mfma<double> {
+    typedef xt_d4m acc_t;
+    static constexpr int KG = 2, UNR = 8;
+    __device__ static acc_t step(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int g, int i) { return g + 4 * i; }
 };
 
-// Split k (ks_n > 1, single block, tri = 0): workgroup z = ks covers the ks-th of
-// ks_n contiguous k ranges and writes its partial tile to part[ks][b][c]; the
-// partials are summed by band_splitk_reduce.
 template <typename T>
 __global__ void __launch_bounds__(256)
-band_tile_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_stride,
+band_mfma_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_stride,
                  const T *__restrict__ V, long long v_stride, const T *add, long long add_stride,
-                 T *out, long long out_stride, int ks_n, T *part) {
-    constexpr int kBT_K = TileSmem<T>::K, VJ = kBT_K / 2;   // V loads per lane
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *sm = reinterpret_cast<T *>(smem_raw);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int LDA = TileSmem<T>::LDA;
-    T *As = sm + (size_t)wave * kBT_K * (LDA + kBT_N);      // [K][LDA]
-    T *Vs = As + kBT_K * LDA;                                // [K][32]
-    const int r0 = blockIdx.x * kBT_M, n0 = blockIdx.y * kBT_N;
-    const long long z = ks_n > 1 ? 0 : blockIdx.z;
+                 T *out, long long out_stride) {
+    using M = Mfma<T>;
+    constexpr int KG = M::KG, CH = 4 * KG, UNR = M::UNR;
+    typedef T vec_t __attribute__((ext_vector_type(KG)));
+    __shared__ T red[3][64][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, t = lane & 15;
+    const int r0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+    const long long z = blockIdx.z;
     A += z * a_stride;
     V += z * v_stride;
     out += z * out_stride;
     if (add) add += z * add_stride;
     int k0 = 0, k1 = b;
-    if (tri == 1) k1 = r0 + kBT_M;
+    if (tri == 1) k1 = r0 + 16;
     if (tri == 2) k0 = r0;
-    if (ks_n > 1) {   // this workgroup's share of the k range, in whole chunks
-        const int per = (b / kBT_K + ks_n - 1) / ks_n * kBT_K;
-        k0 = blockIdx.z * per;
-        k1 = min(b, k0 + per);
-    }
-    // quarter of the k range (in whole chunks) for this wave; all waves run nch chunks
-    const int nch_all = (k1 - k0 + kBT_K - 1) / kBT_K;
-    const int nch = (nch_all + 3) / 4;
-    const int kw0 = k0 + wave * nch * kBT_K;
-    const int kw1 = min(k1, kw0 + nch * kBT_K);
-    // loads: A (row e / K, kk = e % K) and V (kk = e / 32, cc = e % 32), e = lane + 64 j
-    constexpr int AJ = kBT_M * kBT_K / 64;                   // A loads per lane
-    // Indices are clamped into range and out-of-range values zeroed at the LDS store,
-    // so every load of a chunk issues without a branch between them.
-    T ra[AJ], rv[VJ];
-    int kcur = kw0;
-    auto load = [&](int kc0) {
-        kcur = kc0;
-        const int klast = max(kw1 - 1, 0);
+    // this wave's quarter of [k0, k1), in whole chunks
+    const int per = ((k1 - k0) / CH + 3) / 4 * CH;
+    const int kw0 = k0 + wave * per, kw1 = min(k1, kw0 + per);
+    const T *arow = A + (size_t)(r0 + t) * b;
+    const int cc = min(n0 + t, c - 1);
+    typename M::acc_t acc = {T(0), T(0), T(0), T(0)};
+    for (int kc = kw0; kc < kw1; kc += CH * UNR) {
+        vec_t ra[UNR];
+        T rb[UNR][KG];
 #pragma unroll
-        for (int j = 0; j < AJ; ++j) {
-            const int e = lane + 64 * j, r = e / kBT_K, k = min(kc0 + (e % kBT_K), klast);
-            ra[j] = A[(size_t)(r0 + r) * b + k];
-        }
+        for (int u = 0; u < UNR; ++u) {
+            const int kk = kc + u * CH + KG * g;
+            const bool ok = kk < kw1;
+            const int kl = ok ? kk : kw0;
+            ra[u] = *reinterpret_cast<const vec_t *>(arow + kl);
 #pragma unroll
-        for (int j = 0; j < VJ; ++j) {
-            const int e = lane + 64 * j, k = min(kc0 + (e >> 5), klast);
-            const int cc = min(n0 + (e & 31), c - 1);
-            rv[j] = V[(size_t)k * c + cc];
-        }
-    };
-    auto store = [&]() {
+            for (int s2 = 0; s2 < KG; ++s2) rb[u][s2] = V[(size_t)(kl + s2) * c + cc];
+            if (!ok) {
 #pragma unroll
-        for (int j = 0; j < AJ; ++j) {
-            const int e = lane + 64 * j;
-            As[(e % kBT_K) * LDA + e / kBT_K] = (kcur + (e % kBT_K) < kw1) ? ra[j] : T(0);
-        }
-#pragma unroll
-        for (int j = 0; j < VJ; ++j) {
-            const int e = lane + 64 * j;
-            const bool ok = kcur + (e >> 5) < kw1 && n0 + (e & 31) < c;
-            Vs[e] = ok ? rv[j] : T(0);
-        }
-    };
-    const int tr = (lane >> 3) * 2, tc = (lane & 7) * 4;
-    T acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-    load(kw0);
-    for (int ch = 0; ch < nch; ++ch) {
-        __syncthreads();
-        store();
-        __syncthreads();
-        if (ch + 1 < nch) load(kw0 + (ch + 1) * kBT_K);
-#pragma unroll 8
-        for (int k = 0; k < kBT_K; ++k) {
-            const T a0 = As[k * LDA + tr], a1 = As[k * LDA + tr + 1];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const T v = Vs[k * kBT_N + tc + j];
-                acc[0][j] = fma(a0, v, acc[0][j]);
-                acc[1][j] = fma(a1, v, acc[1][j]);
+                for (int s2 = 0; s2 < KG; ++s2) ra[u][s2] = T(0);
             }
         }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int s2 = 0; s2 < KG; ++s2) acc = M::step(ra[u][s2], rb[u][s2], acc);
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[wave - 1][lane][i] = acc[i];
     }
     __syncthreads();
-    T *red = sm;   // [4][16][32]
+    if (wave == 0 && n0 + t < c) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) red[(wave * kBT_M + tr + i) * kBT_N + tc + j] = acc[i][j];
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int o = tid + 256 * u, r = o >> 5, cl = o & 31;
-        if (n0 + cl < c) {
-            const T sum = (red[(0 * kBT_M + r) * kBT_N + cl] + red[(1 * kBT_M + r) * kBT_N + cl]) +
-                          (red[(2 * kBT_M + r) * kBT_N + cl] + red[(3 * kBT_M + r) * kBT_N + cl]);
-            const size_t idx = (size_t)(r0 + r) * c + n0 + cl;
-            if (ks_n > 1) part[(size_t)blockIdx.z * b * c + idx] = sum;
-            else out[idx] = add ? add[idx] - sum : sum;
+        for (int i = 0; i < 4; ++i) {
+            const T sum = ((acc[i] + red[0][lane][i]) + red[1][lane][i]) + red[2][lane][i];
+            const size_t idx = (size_t)(r0 + M::row(g, i)) * c + n0 + t;
+            out[idx] = add ? add[idx] - sum : sum;
         }
     }
-}
-
-// out = (add ? add - sum_ks part[ks] : sum_ks part[ks]) over one b x c block.
-template <typename T>
-__global__ void __launch_bounds__(256)
-band_splitk_reduce(long long n, int ks_n, const T *__restrict__ part, const T *add, T *out) {
-    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= n) return;
-    T sum = T(0);
-    for (int k = 0; k < ks_n; ++k) sum += part[(size_t)k * n + idx];
-    out[idx] = add ? add[idx] - sum : sum;
 }
 
 // Algorithmic bytes and flops of one sweep kernel launch: the matrix elements of the
@@ -290,32 +248,17 @@ static void band_prof_end(int b, int c, int tri, int nblk, double out_elems, hip
     prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c);
 }
 
-// Tile kernel launch.  A single-block product (a recurrence step) splits k over
-// workgroups so that each wave walks one chunk: the few workgroups of one block
-// would otherwise each run a long chain of dependent chunks.
+// Matrix-core launch for many columns.
 template <typename T>
 static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long as,
                        const T *V, long long vs, const T *add, long long ads, T *out,
                        long long os, int nblk, hipStream_t s) {
     const int b = bp.b;
-    int ks_n = 1;
-    if (nblk == 1 && tri == 0) ks_n = std::min(8, std::max(1, b / (4 * TileSmem<T>::K)));
-    dim3 grid(b / kBT_M, ceil_div(c, kBT_N), ks_n > 1 ? ks_n : nblk);
-    T *part = ks_n > 1 ? bp.part.as<T>() : nullptr;
     const bool prof = prof_on(XT_PROF_BAND);
     if (prof) prof_begin(XT_PROF_BAND, s);
-    hipLaunchKernelGGL(band_tile_kernel<T>, grid, dim3(256), TileSmem<T>::bytes, s, b, c, tri, A,
-                       as, V, vs, add, ads, out, os, ks_n, part);
-    if (prof) {   // split k: partial tiles written instead of out (add read by the reduction)
-        const double vec = (double)b * c;
-        const double outb = ks_n > 1 ? ks_n * vec : vec * (add ? 2 : 1);
-        band_prof_end<T>(b, c, tri, nblk, outb, s);
-    }
-    if (ks_n > 1) {
-        const long long n = (long long)b * c;
-        hipLaunchKernelGGL(band_splitk_reduce<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
-                           ks_n, (const T *)part, add, out);
-    }
+    hipLaunchKernelGGL(band_mfma_kernel<T>, dim3(b / 16, ceil_div(c, 16), nblk), dim3(256), 0, s,
+                       b, c, tri, A, as, V, vs, add, ads, out, os);
+    if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s);
     return XT_OK;
 }
 
@@ -483,12 +426,6 @@ static int factor(BandPrec &bp, std::vector<double> &kd, std::vector<double> &ko
 
 int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out) {
     XT_ARG(p > 0 && ia && ja && a && out, "band prec arguments");
-    XT_HIP(hipFuncSetAttribute((const void *)band_tile_kernel<double>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)TileSmem<double>::bytes));
-    XT_HIP(hipFuncSetAttribute((const void *)band_tile_kernel<float>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)TileSmem<float>::bytes));
 #define XT_ROWS_ATTR(T, CM, RW)                                                                \
     XT_HIP(hipFuncSetAttribute((const void *)band_rows_kernel<T, CM, RW>,                      \
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRowsLdsMax))
@@ -552,7 +489,6 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     const long long npad = (long long)nb * b * c, nreal = (long long)bp->p * c;
     XT_TRY(bp->w1.reserve(sizeof(T) * npad));
     XT_TRY(bp->w2.reserve(sizeof(T) * npad));
-    XT_TRY(bp->part.reserve(sizeof(T) * 8 * (size_t)b * c));
     T *w1 = bp->w1.as<T>(), *w2 = bp->w2.as<T>();
     if (minv) XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w1, s));
     else XT_HIP(hipMemcpyAsync(w1, x, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));

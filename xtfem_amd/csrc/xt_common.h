@@ -100,6 +100,16 @@ bool prof_on(int kind);
 void prof_begin(int kind, hipStream_t s);
 void prof_end(int kind, hipStream_t s, double bytes, double flops);
 
+// Device workspace of the batched slab solver, owned by the operator handle and kept
+// across calls (buffers only grow), so a solve allocates nothing once warm.
+struct SolverWorkspace {
+    DevBuf tm64, tm32, mi64, mi32;   // temporal matrices and inverses
+    DevBuf V, Z;                     // Arnoldi basis, preconditioned basis (flexible)
+    DevBuf w32a, w32b, z64, t64, d;  // work vectors
+    DevBuf partial, hdev, hdev2, ndev, coef, frozen;
+};
+SolverWorkspace *op_workspace(xt_op op);
+
 // Exact block-tridiagonal factor of a banded SPD matrix (xt_band.hip).
 struct BandPrec;
 int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out);

@@ -186,12 +186,17 @@ struct Solver {
     int p, q, R, c, im;
     long long n;
     hipStream_t s;
-    DevBuf tm64, tm32, mi64, mi32;       // temporal matrices and inverses
-    DevBuf V;                            // (im+1) x n FP64 basis
-    DevBuf Z;                            // im x n preconditioned basis (flexible variant)
-    DevBuf w32a, w32b;                   // FP32 work vectors
-    DevBuf z64, t64;                     // FP64 work vectors
-    DevBuf partial, hdev, hdev2, ndev, coef, frozen;
+    // buffers live in the operator's workspace (reused across calls)
+    DevBuf &tm64, &tm32, &mi64, &mi32;   // temporal matrices and inverses
+    DevBuf &V;                           // (im+1) x n FP64 basis
+    DevBuf &Z;                           // im x n preconditioned basis (flexible variant)
+    DevBuf &w32a, &w32b;                 // FP32 work vectors
+    DevBuf &z64, &t64;                   // FP64 work vectors
+    DevBuf &partial, &hdev, &hdev2, &ndev, &coef, &frozen;
+    explicit Solver(SolverWorkspace &w)
+        : tm64(w.tm64), tm32(w.tm32), mi64(w.mi64), mi32(w.mi32), V(w.V), Z(w.Z), w32a(w.w32a),
+          w32b(w.w32b), z64(w.z64), t64(w.t64), partial(w.partial), hdev(w.hdev),
+          hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), frozen(w.frozen) {}
     int nblk = 0, chunk = 0;
     std::vector<double> *hist = nullptr;  // R == 1: ro/ro0 per iteration (kronpgmres log)
     double hist_ro0 = 1.0;
@@ -460,7 +465,9 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                         std::vector<double> *hist) {
     XT_ARG(op && pr && R > 0 && ab && b && x && its && ierr && relres, "slab_solve arguments");
     XT_ARG(im >= 1 && im <= 256 && maxits >= 1 && mode >= 0 && mode <= 2, "slab_solve parameters");
-    Solver S;
+    SolverWorkspace *ws = op_workspace(op);
+    if (!ws) return XT_E_NOMEM;
+    Solver S(*ws);
     S.op = op;
     S.pr = pr;
     S.R = R;
@@ -523,7 +530,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         // cycles on the FP64 operator (FP32 preconditioner).
         std::vector<double> rn(R), irel(R, 1.0e-4), prev(R, 0.0);
         std::vector<int> fb(R, 0);
-        DevBuf d;
+        DevBuf &d = ws->d;
         XT_TRY(d.reserve(sizeof(double) * S.n));
         for (int outer = 0; outer < 64; ++outer) {
             XT_TRY(S.norms2(S.Vj(0), rn));

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <new>
+
 #include "xt_common.h"
 
 namespace xt {
@@ -247,10 +249,18 @@ struct xt_op_s {
     DevCSR S[3];
     xt::BsrPlan *bsr = nullptr;  // node-block form (3-DOF mechanics, q = 6), if applicable
     int format = 0;              // 0: node blocks when available, 1: CSR
-    ~xt_op_s() { if (bsr) xt::bsr_delete(bsr); }
+    xt::SolverWorkspace *ws = nullptr;   // slab-solver buffers, created on first use
+    ~xt_op_s() {
+        if (bsr) xt::bsr_delete(bsr);
+        delete ws;
+    }
 };
 
 namespace xt {
+SolverWorkspace *op_workspace(xt_op op) {
+    if (!op->ws) op->ws = new (std::nothrow) SolverWorkspace;
+    return op->ws;
+}
 int op_dims(xt_op op, int *p, int *q, int *nterms) {
     XT_ARG(op, "null operator");
     *p = op->p;

@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--notch", type=int, default=20)
     ap.add_argument("--block", type=int, default=0, help="factor block rows (0: bandwidth)")
     ap.add_argument("--cases-kind", choices=("plate", "sweep"), default="plate")
+    ap.add_argument("--no-damage", action="store_true", help="skip the damage-kernel leg")
+    ap.add_argument("--damage-nip", type=int, default=2560,
+                    help="temporal points per slab of the damage leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -208,6 +211,67 @@ def spmm_microbench(op, p, reps=20, warm=3):
                 gbs=by.value / (med * 1e-3) / 1e9, frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS)
 
 
+# Algorithmic FP64 operations of one Gauss point and time point of the damage kernel
+# in its elastic branch (the count of the kernel's arithmetic: 36 for the shared
+# nodal interpolation, 144 strain, 15 damage factors, 31 trial strain, 24 D_el
+# product, 13 deviator, 19 von Mises incl. sqrt); the plastic corrector adds ~235 and
+# the damage increment ~30, so this is a lower bound when points yield.
+DAMAGE_FLOPS_ELASTIC = 282
+
+
+_DAMAGE = {}
+
+
+def damage_problem():
+    """The damage leg's block, shape-derivative tables and seeded field (built once)."""
+    if not _DAMAGE:
+        from xtfem_amd import fem
+        mesh = fem.structured_hex(60, 60, 12, 60.0, 60.0, 12.0)
+        _, _, Nx, Ny, Nz = fem.assemble(mesh, 1.97e5, 0.3, 7.86e-6)
+        rng = np.random.default_rng(SEED + 4)
+        xi = mesh.xcoord / np.array([60.0, 60.0, 12.0])
+        w = rng.uniform(0.5, 1.0, 6) * np.array([1, 1, 1, 0.6, 0.6, 0.6])
+        eps0 = 7.0e-4
+        sol = np.zeros((6, mesh.nn, 3))
+        for k in range(6):
+            sol[k, :, 0] = eps0 * w[k] * 60.0 * xi[:, 0] * (1 + 0.3 * xi[:, 1])
+            sol[k, :, 1] = (-0.3 * eps0 * w[k] * 60.0 * xi[:, 1]
+                            + 0.2 * eps0 * w[k] * 60.0 * xi[:, 0] ** 2)
+            sol[k, :, 2] = -0.3 * eps0 * w[k] * 12.0 * xi[:, 2]
+        _DAMAGE.update(mesh=mesh, Nx=Nx, Ny=Ny, Nz=Nz, sol=sol.ravel())
+    return _DAMAGE
+
+
+def damage_leg(nip, slabs=3):
+    """Damage kernel on the 60 x 60 x 12 C3D8 block (43,200 elements, 345,600 Gauss
+    points), nip temporal points per slab, seeded smooth stretch/bend field (axial
+    strain 7e-4, peak von Mises about the 180 MPa threshold)."""
+    import torch
+    from xtfem_amd import slab
+    from xtfem_amd.device import DamageState
+    dp = damage_problem()
+    mesh = dp["mesh"]
+    st = DamageState(mesh, dp["Nx"], dp["Ny"], dp["Nz"], 1.97e5, 0.3)
+    sol_d = torch.as_tensor(dp["sol"], device="cuda")
+    Nt = torch.as_tensor(slab.damage_nt(np.array([0.0, 0.25, 0.5]), 2 * np.pi * 20.0, nip).ravel(),
+                         device="cuda")
+    st.step(sol_d, Nt, nip)                       # warm-up slab
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(slabs):
+        st.step(sol_d, Nt, nip)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / slabs
+    ngp = 8 * mesh.ne
+    gps = ngp * nip / t
+    D = st.s["D"].cpu().numpy()
+    yielded = float(np.mean(st.s["p"].cpu().numpy() > 0))
+    return dict(workload=f"C3D8 block 60x60x12, {ngp} Gauss points, nip {nip}", ms_per_slab=1e3 * t,
+                gp_steps_per_s=gps, tflops_elastic_count=gps * DAMAGE_FLOPS_ELASTIC / 1e12,
+                frac_fp64_peak=gps * DAMAGE_FLOPS_ELASTIC / 78.6e12, yielded_fraction=yielded,
+                max_D=float(D.max()))
+
+
 def cpu_baseline(prob, budget_s=15.0):
     """The oracle on the host: FP64 restatement of kronpgmres (kronecker.f90:181-365,
     MGS, one core) preconditioned by A^-1 (x) K^-1 with the exact band Cholesky factor
@@ -234,11 +298,35 @@ def cpu_baseline(prob, budget_s=15.0):
         t_solve += time.perf_counter() - t0
         its_all.append(its)
         n += 1
-    return dict(value=n / t_solve, unit="slab solves/s", cores=1, kind="port",
+    out = dict(value=n / t_solve, unit="slab solves/s", cores=1, kind="port",
                 sample=f"{n} consecutive slab(s) of load case 0 (80 MPa, 20 Hz): jump term, "
                        f"RHS and oracle FP64 GMRES(30) to 1e-8 with the exact band Cholesky "
                        f"factor (half-bandwidth {bf.w}; factorization {t_fact:.1f} s excluded), "
                        f"its {its_all}, {t_solve:.1f} s on one host core")
+    # the damage functor on the host: the oracle's kokkos_host over the first 1000
+    # elements of the damage leg's block, same field and temporal points
+    out["damage"] = cpu_damage_sample()
+    return out
+
+
+def cpu_damage_sample(ne_s=1000, nip=2560):
+    from oracle import oracle
+    from xtfem_amd import slab
+    dp = damage_problem()
+    mesh, Nx, Ny, Nz, sol = dp["mesh"], dp["Nx"], dp["Ny"], dp["Nz"], dp["sol"]
+    ngp = 8 * ne_s
+    nodes = np.repeat(mesh.xconn[:ne_s], 8, axis=0).ravel()
+    Nt = slab.damage_nt(np.array([0.0, 0.25, 0.5]), 2 * np.pi * 20.0, nip).ravel()
+    st = {k: np.zeros(6 * ngp) for k in ("eps_old", "eps_pl", "sig_eff", "sig_back")}
+    st.update({k: np.zeros(ngp) for k in ("p", "D", "ws", "del_po", "sig_d")})
+    t0 = time.perf_counter()
+    oracle.kokkos_host(ne_s, mesh.nn, nip, nodes, np.ravel(Nx)[:8 * ngp], np.ravel(Ny)[:8 * ngp],
+                       np.ravel(Nz)[:8 * ngp], Nt, sol, np.full(ngp, 1.97e5),
+                       np.full(ngp, 0.3), st)
+    t = time.perf_counter() - t0
+    return dict(gp_steps_per_s=ngp * nip / t, cores=1, kind="port",
+                sample=f"oracle kokkos_host, {ngp} Gauss points x {nip} temporal points, "
+                       f"{t:.1f} s on one host core")
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "band_mfma_kernel_notched_plate_pmc.md")
@@ -379,6 +467,10 @@ def main():
               f"{its64.mean():.1f} max {its64.max()}, relres max {rr64.max():.2e}, "
               f"rel err fp32-ir vs fp64 max {err.max():.2e}", file=sys.stderr, flush=True)
     spmm = spmm_microbench(op, prob.p)
+    if not args.no_damage:
+        res["damage"] = damage_leg(args.damage_nip)
+        print(f"# damage: {res['damage']['ms_per_slab']:.1f} ms per slab, "
+              f"{res['damage']['gp_steps_per_s']:.3e} GP-steps/s", file=sys.stderr, flush=True)
     # per-kernel durations: the batched FP32-IR steps once more, every launch of the
     # operator and sweep kernels bracketed by HIP events (after the timed legs)
     prof = kernel_profile(run32, K, mask=0b11)

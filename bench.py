@@ -109,6 +109,46 @@ class Problem:
         return (np.atleast_2d(fx) * self.amp[cases, None]).ravel()
 
 
+class RodProblem(Problem):
+    """The prismatic rod: C3D8 4 x 4 x 12 elements over 10 x 10 x 30 mm (325 nodes),
+    clamped at z = 0, axial traction 100 MPa at 50 Hz on z = 30, E = 2e5 MPa,
+    nu = 0.3, rho = 7.85e-9 t/mm^3, slabs of 0.5 s (25 periods), one load case."""
+
+    def __init__(self):
+        from xtfem_amd import fem, slab
+        self.mesh = fem.structured_hex(4, 4, 12, 10.0, 10.0, 30.0)
+        K, M, *_ = fem.assemble(self.mesh, 2.0e5, 0.3, 7.85e-9)
+        self.bc = fem.clamp_mask(self.mesh, 2, [0.0])
+        self.Kb, self.Mb = fem.applybc(K, self.bc), fem.applybc(M, self.bc)
+        self.p = 3 * self.mesh.nn
+        self.amp = np.array([100.0])
+        self.freq = np.array([50.0])
+        self.w = 2 * np.pi * self.freq
+        self.dt = np.array([0.5])
+        c = slab.stcoeff_m(0.0, self.dt, self.w)
+        self.A, self.B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+        self.J1, self.J2 = c["ktn1"], c["mtn1"]
+        self.rext = fem.traction_load(self.mesh, 2, 30.0, 2, 100.0)[None]
+
+
+def rod_leg(K):
+    """The rod with the reference's ILUT path: ILUT(20, 1e-4), GMRES(30) to 1e-8, one
+    slab per solve, FP64 and FP32-IR on the GPU."""
+    from xtfem_amd import device
+    prob = RodProblem()
+    op = device.KronOperator([prob.Kb, prob.Mb], q=6)
+    pr = device.IlutPreconditioner(prob.Kb, lfil=20, droptol=1e-4)
+    out = {"workload": f"rod 4x4x12 C3D8 (p={prob.p}), ILUT(20, 1e-4), GMRES(30), eps 1e-8"}
+    for mode, name in ((0, "fp64"), (1, "fp32_ir")):
+        run = Runner(prob, op, pr, np.array([0]), mode=mode, nsteps=K + 2)
+        run.step()
+        t = timed(run, K)
+        rr = np.array(run.relres[1:])
+        out[name] = dict(slabs_per_s=K / t, ms_per_slab=1e3 * t / K,
+                         its_max=int(np.max(run.its[1:])), relres_max=float(rr.max()))
+    return out
+
+
 class Runner:
     """Device state of one slab sequence (R load cases) and its step."""
 
@@ -306,7 +346,31 @@ def cpu_baseline(prob, budget_s=15.0):
     # the damage functor on the host: the oracle's kokkos_host over the first 1000
     # elements of the damage leg's block, same field and temporal points
     out["damage"] = cpu_damage_sample()
+    out["rod_ilut"] = cpu_rod_sample()
     return out
+
+
+def cpu_rod_sample(slabs=5):
+    """The oracle's FP64 kronpgmres with its ILUT(20, 1e-4) factors on the rod, one core."""
+    from oracle import oracle
+    from xtfem_amd import slab
+    prob = RodProblem()
+    p = prob.p
+    prec = oracle.ilut(prob.Kb, 20, 1e-4)[:3]
+    fixed = np.tile(prob.bc, 6)
+    x = np.zeros(6 * p)
+    t0 = time.perf_counter()
+    its_all = []
+    for n in range(slabs):
+        jump = oracle.kronspmv(prob.J1[0], prob.J2[0], prob.Kb, prob.Mb, x)
+        rhs = slab.slab_rhs(p, prob.coef(n, np.array([0])), None, prob.rext[0], jump=jump,
+                            fixed=fixed)
+        x, ie, its, _ = oracle.kronpgmres(prob.A[0], prob.B[0], prob.Kb, prob.Mb, prec, rhs,
+                                          im=30, eps=1e-8, maxits=500)
+        its_all.append(its)
+    t = time.perf_counter() - t0
+    return dict(slabs_per_s=slabs / t, cores=1, kind="port",
+                sample=f"{slabs} slabs, oracle FP64 kronpgmres + ILUT(20, 1e-4), its {its_all}")
 
 
 def cpu_damage_sample(ne_s=1000, nip=2560):
@@ -467,6 +531,9 @@ def main():
               f"{its64.mean():.1f} max {its64.max()}, relres max {rr64.max():.2e}, "
               f"rel err fp32-ir vs fp64 max {err.max():.2e}", file=sys.stderr, flush=True)
     spmm = spmm_microbench(op, prob.p)
+    res["rod_ilut"] = rod_leg(K)
+    print(f"# rod ILUT: fp64 {res['rod_ilut']['fp64']['ms_per_slab']:.2f} ms/slab, fp32-ir "
+          f"{res['rod_ilut']['fp32_ir']['ms_per_slab']:.2f} ms/slab", file=sys.stderr, flush=True)
     if not args.no_damage:
         res["damage"] = damage_leg(args.damage_nip)
         print(f"# damage: {res['damage']['ms_per_slab']:.1f} ms per slab, "

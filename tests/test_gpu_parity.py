@@ -222,3 +222,28 @@ def test_kron_blocks_many_load_cases(xt, oracle_lib, R):
     for r in range(R):
         ref = oracle_lib.kronspgemv(A[r], B[r], 0.5, -2.0, Kb, Mb, X[r], Y0[r])
         assert rel(Y[r], ref) < 1e-12, r
+
+
+@pytest.mark.parametrize("R", [1, 4])
+def test_ilut_sweeps_single_and_multi_launch(xt, oracle_lib, R):
+    """Kronecker ILUT solves (kronlusol, kronecker.f90:130-178) on a bar large enough
+    that R = 4 takes the one-launch-per-level path and R = 1 the single-workgroup path."""
+    import torch
+    from xtfem_amd import slab
+    from xtfem_amd.device import IlutPreconditioner, from_interleaved, to_interleaved
+    S = structured_system(30, 10, 10, 60.0, 20.0, 20.0)
+    p = S["p"]
+    pr = IlutPreconditioner(S["Kb"], lfil=20, droptol=1e-4)
+    alu, jlu, ju = pr.factors
+    rng = np.random.default_rng(40 + R)
+    c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))
+    A = c["k"] + c["ktn"]
+    X = rng.standard_normal((R, 6 * p))
+    minv = torch.as_tensor(np.linalg.inv(A), device="cuda").contiguous()
+    x = to_interleaved(X, p, 6)
+    y = torch.empty_like(x)
+    pr.apply(minv, x, y, R)
+    Y = from_interleaved(y, p, 6, R)
+    for r in range(R):
+        ref = oracle_lib.kronlusol(A[r], alu, jlu, ju, X[r])
+        assert rel(Y[r], ref) < 1e-12, (R, r, rel(Y[r], ref))

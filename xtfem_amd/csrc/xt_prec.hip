@@ -34,6 +34,50 @@ trsv_level_kernel(int nrows, const int *__restrict__ rows, const int *__restrict
     x[o] = s;
 }
 
+// Both sweeps in one workgroup (small factors): the levels are walked in order with a
+// barrier between levels instead of one launch per level, then the upper levels.
+// Same per-row arithmetic and order as trsv_level_kernel.  XL: the solution vector
+// lives in LDS during the sweeps (dynamic shared memory of n*c elements).
+template <typename T, bool XL>
+__global__ void __launch_bounds__(1024)
+trsv_onewg_kernel(int n_, int nlevL, const int *__restrict__ lpL, const int *__restrict__ rowsL,
+                  const int *__restrict__ rpL, const int *__restrict__ ciL,
+                  const T *__restrict__ vL, int nlevU, const int *__restrict__ lpU,
+                  const int *__restrict__ rowsU, const int *__restrict__ rpU,
+                  const int *__restrict__ ciU, const T *__restrict__ vU,
+                  const T *__restrict__ dinv, int c, const T *__restrict__ b, T *xg) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *x = XL ? reinterpret_cast<T *>(smem_raw) : xg;
+    for (int l = 0; l < nlevL; ++l) {
+        const int r0 = lpL[l], n = (lpL[l + 1] - r0) * c;
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            const int k = idx / c, col = idx - k * c;
+            const int row = rowsL[r0 + k];
+            const size_t o = (size_t)row * c + col;
+            T s = b[o];
+            const int e1 = rpL[row + 1];
+            for (int e = rpL[row]; e < e1; ++e) s -= vL[e] * x[(size_t)ciL[e] * c + col];
+            x[o] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < nlevU; ++l) {
+        const int r0 = lpU[l], n = (lpU[l + 1] - r0) * c;
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            const int k = idx / c, col = idx - k * c;
+            const int row = rowsU[r0 + k];
+            const size_t o = (size_t)row * c + col;
+            T s = x[o];
+            const int e1 = rpU[row + 1];
+            for (int e = rpU[row]; e < e1; ++e) s -= vU[e] * x[(size_t)ciU[e] * c + col];
+            x[o] = dinv[row] * s;
+        }
+        __syncthreads();
+    }
+    if (XL)
+        for (int i = threadIdx.x; i < n_ * c; i += blockDim.x) xg[i] = x[i];
+}
+
 // Level schedule of a triangular CSR: forward (lower) or backward (upper).
 static void build_levels(int n, const std::vector<int> &rp, const std::vector<int> &ci,
                          bool forward, std::vector<int> &lev_ptr, std::vector<int> &lev_rows) {
@@ -61,6 +105,7 @@ struct Tri {
     DevCSR m;                    // strict triangle, 0-based, reference entry order
     DevBuf lev_rows;             // rows sorted by level
     std::vector<int> lev_ptr;    // host copy of level boundaries
+    DevBuf lev_ptr_dev;          // device copy (single-workgroup sweeps)
     std::vector<long long> lev_nnz;  // entries per level (algorithmic bytes)
 };
 
@@ -90,6 +135,7 @@ static int upload_tri(int n, const std::vector<int> &rp, const std::vector<int> 
     XT_TRY(T.m.val32.upload(v32.data(), sizeof(float) * v32.size()));
     std::vector<int> lrows;
     build_levels(n, rp, ci, forward, T.lev_ptr, lrows);
+    XT_TRY(T.lev_ptr_dev.upload(T.lev_ptr.data(), sizeof(int) * T.lev_ptr.size()));
     T.lev_nnz.assign(T.lev_ptr.size() - 1, 0);
     for (size_t l = 0; l + 1 < T.lev_ptr.size(); ++l)
         for (int k = T.lev_ptr[l]; k < T.lev_ptr[l + 1]; ++k)
@@ -124,6 +170,12 @@ static int run_tri(const Tri &tri, const T *vals, const T *dinv, int c, const T 
     return XT_OK;
 }
 
+// Sweeps whose work (factor entries + diagonal, times columns) is at most this run in a
+// single workgroup.
+constexpr long long kOneWgWork = 4LL << 20;
+// ... and keep the solution vector in LDS when it fits this many bytes.
+constexpr size_t kOneWgLds = 128 * 1024;
+
 template <typename T>
 int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
                  hipStream_t s) {
@@ -138,6 +190,31 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     const T *lv = f32 ? (const T *)pr->L.m.val32.ptr : (const T *)pr->L.m.val64.ptr;
     const T *uv = f32 ? (const T *)pr->U.m.val32.ptr : (const T *)pr->U.m.val64.ptr;
     const T *dv = f32 ? (const T *)pr->dinv32.ptr : (const T *)pr->dinv64.ptr;
+    // Small factors: both sweeps in one workgroup (a barrier per level instead of a
+    // launch per level); large ones: one launch per level over the whole device.
+    const long long work = (pr->L.m.nnz + pr->U.m.nnz + 2LL * pr->p) * c;
+    if (work <= kOneWgWork) {
+        const bool prof = prof_on(XT_PROF_TRSV);
+        if (prof) prof_begin(XT_PROF_TRSV, s);
+        const size_t xl = sizeof(T) * (size_t)pr->p * c;
+#define XT_ONEWG(XLDS)                                                                      \
+    hipLaunchKernelGGL((trsv_onewg_kernel<T, XLDS>), dim3(1), dim3(1024), XLDS ? xl : 0, s,  \
+                       pr->p, (int)pr->L.lev_ptr.size() - 1, pr->L.lev_ptr_dev.as<int>(),     \
+                       pr->L.lev_rows.as<int>(), pr->L.m.rowptr.as<int>(),                    \
+                       pr->L.m.col.as<int>(), lv, (int)pr->U.lev_ptr.size() - 1,              \
+                       pr->U.lev_ptr_dev.as<int>(), pr->U.lev_rows.as<int>(),                 \
+                       pr->U.m.rowptr.as<int>(), pr->U.m.col.as<int>(), uv, dv, c, b, y)
+        if (xl <= kOneWgLds) XT_ONEWG(true);
+        else XT_ONEWG(false);
+#undef XT_ONEWG
+        if (prof)
+            prof_end(XT_PROF_TRSV, s,
+                     (double)(pr->L.m.nnz + pr->U.m.nnz) * (sizeof(T) + 4 + c * sizeof(T)) +
+                         (double)pr->p * c * sizeof(T) * 3,
+                     2.0 * (pr->L.m.nnz + pr->U.m.nnz) * c);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
     XT_TRY(run_tri<T>(pr->L, lv, nullptr, c, b, y, s));
     XT_TRY(run_tri<T>(pr->U, uv, dv, c, nullptr, y, s));
     return XT_OK;
@@ -153,6 +230,10 @@ extern "C" {
 
 int xt_prec_create(int p, const double *alu, const int *jlu, const int *ju, xt_prec *out) {
     XT_ARG(out && p > 0 && alu && jlu && ju, "prec arguments");
+    XT_HIP(hipFuncSetAttribute((const void *)trsv_onewg_kernel<double, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOneWgLds));
+    XT_HIP(hipFuncSetAttribute((const void *)trsv_onewg_kernel<float, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOneWgLds));
     XT_ARG(jlu[0] == p + 2, "MSR factors must come from ilut (jlu(1) = n+2)");
     std::vector<int> lrp(p + 1, 0), urp(p + 1, 0), lci, uci;
     std::vector<double> lv, uv, dinv(p);

@@ -122,6 +122,47 @@ void kokkos_host_(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int n
 int xt_chkelesta(int ne, int ngpe, double Dc, const double *D, int *elesta,
                  int *gpsta, int *nef, int *nefail);
 
+/* K := K - sum of the element stiffness matrices of the nef failed elements
+ * nefail (1-based ids), each from elekmtx (elesubs.f90:207-251) with the one
+ * 6x6 material matrix stiff for all its Gauss points.  Replaces delelement,
+ * delelement.f90:1-59 (same arguments; xconn ne x nnse and xcoord nn x 3
+ * column-major, dns/dnxyz/sd/wqs as setshapefunc_m and setsdstiff give them).
+ * K is 1-based CSR with knz entries, updated in place with the reference's
+ * cs_add semantics (sorted rows, exact zeros dropped).  kj and kv must have
+ * room for the result, at most knz + nef*(3*nnse)^2 entries (couplings that
+ * boundary conditions removed from K come back with -K_e); a K assembled from
+ * these elements never needs more than the assembly's own entry count, which
+ * the reference's driver allocates (main3d.f90:757-760).  Hexahedra with ndofn = 3,
+ * ncmp = 6, ngpe = nquads^3 only (XT_E_UNSUPPORTED otherwise); a singular
+ * element Jacobian gives XT_E_SINGULAR where the reference STOPs. */
+int xt_delelement(int nn, int ne, int ndofn, int nnse, int ngpe, int ncmp, int nquads,
+                  const int *xconn, const double *xcoord, const double *dns,
+                  const double *dnxyz, const double *sd, const double *stiff,
+                  const double *wqs, int nef, const int *nefail, int *ki, int *kj,
+                  double *kv, int ndof, int *knz);
+
+/* necnt[node] -= 1 for every node of every failed element.  Replaces chkcnsta,
+ * delelement.f90:139-161. */
+int xt_chkcnsta(int nn, int ne, int nnse, const int *xconn, int nef, const int *nefail,
+                int *necnt);
+
+/* Nodes of the failed elements whose surviving-element count is 0, each once,
+ * in the order the element list first reaches them; ndfail has nn entries
+ * (zero-filled past ndf).  Replaces chkndsta, delelement.f90:164-201. */
+int xt_chkndsta(int nn, int ne, int nnse, const int *xconn, int nef, const int *nefail,
+                const int *necnt, int *ndf, int *ndfail);
+
+/* K := K + I on the ndofn DOFs of each orphan node (cs_add semantics).  kj and
+ * kv must hold knz + ndf*ndofn entries.  Replaces adddiagonal,
+ * delelement.f90:204-247. */
+int xt_adddiagonal(int nnse, int ndofn, int ndf, const int *ndfail, int *ki, int *kj,
+                   double *kv, int ndof, int *knz);
+
+/* 1-0 method on the DOFs with bc != 0: their rows and columns are cleared, the
+ * diagonal set to 1, exact zeros dropped, in place.  Replaces csrapplybc,
+ * spmtx.f90:268-307. */
+int xt_csrapplybc(int *ia, int *ja, double *a, int n, int *nz, const int *bc);
+
 /* ---- 2. device-resident handle API ------------------------------------------- */
 
 typedef struct xt_op_s *xt_op;        /* Kronecker operator sum_k T_k (x) S_k   */
@@ -193,6 +234,17 @@ int xt_from_interleaved(int p, int q, int R, const double *src, double *dst, voi
 int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
                   double *x, int im, double eps, int maxits, int mode, int *its,
                   int *ierr, double *relres, void *stream);
+
+/* chkelesta (delelement.f90:109-136) on device-resident state: failed[e] = 1 iff
+ * some Gauss point of element e has D >= Dc, else 0; for a failed element
+ * elesta[e] and its gpsta entries are set to 0 (others untouched).  Device
+ * pointers, ne*ngpe Gauss points element-major. */
+int xt_elestatus_device(int ne, int ngpe, double Dc, const double *D, int *elesta,
+                        int *gpsta, int *failed, void *stream);
+
+/* D = ws = 0 at every Gauss point with gpsta == 0 (kokkos_damage.f90:102-107),
+ * device pointers. */
+int xt_damage_zero_failed(int ngp, const int *gpsta, double *D, double *ws, void *stream);
 
 /* Damage kernel on device-resident state (kokkos_kernel.h:44-344), one lane per
  * Gauss point, nip temporal points per launch.  All pointers are device

@@ -258,3 +258,79 @@ def chkelesta(ne, ngpe, Dc, D, elesta=None, gpsta=None):
     nefail = np.zeros(ne, dtype=np.int32)
     nef = lib().or_chkelesta(ne, ngpe, Dc, _f64(D), el, gp, nefail)
     return nefail[:nef].copy(), el, gp
+
+
+# ---- element deletion (delelement.f90, main3d.f90:650-686), numpy restatement --------
+
+_HEX_SIGNS = np.array([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1],
+                       [1, -1, 1], [1, -1, -1], [1, 1, -1], [1, 1, 1]], dtype=np.float64)
+
+
+def hex_stiffness(X, C6):
+    """Stiffness of one C3D8 hexahedron with 2x2x2 Gauss points (elekmtx,
+    elesubs.f90:207-251): sum over points of B^T C6 B det J.  X [8, 3] node
+    coordinates in the local order of shape3x (procs.f90:1320-1327); DOF 3a + c.
+    Written from the mathematics (numpy products), so it agrees with the reference
+    to rounding, not bit for bit."""
+    g = 0.577350269189626
+    ke = np.zeros((24, 24))
+    for xi in (-g, g):
+        for eta in (-g, g):
+            for zeta in (-g, g):
+                pt = np.array([xi, eta, zeta])
+                f = 1.0 + _HEX_SIGNS * pt                      # [8, 3]
+                dN = np.empty((3, 8))
+                for m in range(3):
+                    o = [k for k in range(3) if k != m]
+                    dN[m] = 0.125 * _HEX_SIGNS[:, m] * f[:, o[0]] * f[:, o[1]]
+                J = dN @ X
+                G = np.linalg.solve(J, dN)                     # dN/dx [3, 8]
+                B = np.zeros((6, 24))
+                B[0, 0::3] = G[0]; B[1, 1::3] = G[1]; B[2, 2::3] = G[2]
+                B[3, 0::3] = G[1]; B[3, 1::3] = G[0]
+                B[4, 1::3] = G[2]; B[4, 2::3] = G[1]
+                B[5, 0::3] = G[2]; B[5, 2::3] = G[0]
+                ke += B.T @ C6 @ B * np.linalg.det(J)
+    return ke
+
+
+def deletion_block(xconn, xcoord, K, M, fixed, elesta, nefail, C6):
+    """main3d.f90:650-686 restated on dense-free CSR: K minus the failed elements'
+    stiffness, survivor counts by brute force over the still-active elements,
+    orphan nodes (those of the failed elements with no active element left, in the
+    order the failed list reaches them), unit diagonal and fixed DOFs on them, the
+    1-0 method on K and M.  elesta: 1 for elements active before this event.
+    Returns (K', M', fixed', ndfail) with K', M' as scipy CSR."""
+    import scipy.sparse as sp
+    n = len(K[0]) - 1
+    csr = lambda T: sp.csr_matrix((np.asarray(T[2]), np.asarray(T[1]) - 1, np.asarray(T[0]) - 1),
+                                  shape=(n, n))
+    Kd = csr(K).tolil()
+    for e in nefail:
+        nodes = xconn[e - 1] - 1
+        dofs = (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+        Kd[np.ix_(dofs, dofs)] = Kd[np.ix_(dofs, dofs)].toarray() - hex_stiffness(xcoord[nodes], C6)
+    alive = np.asarray(elesta, dtype=bool).copy()
+    alive[np.asarray(nefail) - 1] = False
+    count = np.zeros(len(xcoord), dtype=np.int64)
+    for e in np.flatnonzero(alive):
+        count[xconn[e] - 1] += 1
+    ndfail = []
+    for e in nefail:
+        for node in xconn[e - 1]:
+            if count[node - 1] == 0 and node not in ndfail:
+                ndfail.append(int(node))
+    fixed = np.array(fixed, dtype=np.int32)
+    for node in ndfail:
+        for c in range(3):
+            Kd[3 * (node - 1) + c, 3 * (node - 1) + c] += 1.0
+        fixed[3 * (node - 1):3 * node] = 1
+
+    def bc(S):
+        S = sp.csr_matrix(S)
+        keep = sp.diags((fixed == 0).astype(np.float64))
+        out = keep @ S @ keep + sp.diags((fixed != 0).astype(np.float64))
+        out.eliminate_zeros()
+        return out.tocsr()
+
+    return bc(Kd.tocsr()), bc(csr(M)), fixed, np.array(ndfail, dtype=np.int32)

@@ -231,3 +231,100 @@ def csrapplybc(S, bc):
     bc = np.asarray(bc, dtype=np.int32)
     lib().csrapplybc_(_p(ia), _p(ja), _p(a), _ri(n), C.byref(nz), _p(bc))
     return ia, ja[:nz.value].copy(), a[:nz.value].copy()
+
+
+def shape_tables_m(nquads=2):
+    """setshapefunc_m (elesubs.f90:299-335): wqs [nquads], dns [3*ngpe, 8], dnxyz [9*ngpe, 24]."""
+    ngpe = nquads ** 3
+    wqs = np.zeros(nquads)
+    ns = np.zeros((3 * ngpe, 24), order="F")
+    dns = np.zeros((3 * ngpe, 8), order="F")
+    dnxyz = np.zeros((9 * ngpe, 24), order="F")
+    lib().setshapefunc_m_(_ri(3), _ri(2), _ri(8), _ri(nquads), _ri(ngpe), _p(wqs), _p(ns),
+                          _p(dns), _p(dnxyz))
+    return wqs, np.array(dns), np.array(dnxyz)
+
+
+def setsdstiff(ym, nu):
+    """elesubs.f90:374-401 -> (sd [6, 9], stiff [6, 6])."""
+    sd = np.zeros((6, 9), order="F")
+    stiff = np.zeros((6, 6), order="F")
+    lib().setsdstiff_(_rd(ym), _rd(nu), _p(sd), _p(stiff))
+    return np.array(sd), np.array(stiff)
+
+
+def _csr_room(K, extra):
+    ia = _F(K[0], np.int32).copy()
+    nz = int(ia[-1]) - 1
+    ja = np.zeros(nz + extra, dtype=np.int32)
+    a = np.zeros(nz + extra)
+    ja[:nz] = K[1][:nz]
+    a[:nz] = K[2][:nz]
+    return ia, ja, a, C.c_int(nz)
+
+
+def delelement(xconn, xcoord, dns, dnxyz, sd, stiff, wqs, nefail, K):
+    """delelement.f90:1-59 -> new K (1-based CSR)."""
+    xconn = _F(xconn, np.int32)
+    xcoord = _F(xcoord)
+    ne, nnse = xconn.shape
+    nn = xcoord.shape[0]
+    nquads = len(wqs)
+    nefail = _F(nefail, np.int32)
+    ki, kj, kv, knz = _csr_room(K, len(nefail) * (3 * nnse) ** 2)
+    ndof = len(ki) - 1
+    lib().delelement_(_ri(nn), _ri(ne), _ri(3), _ri(nnse), _ri(nquads ** 3), _ri(6),
+                      _ri(nquads), _p(xconn), _p(xcoord), _p(_F(dns)), _p(_F(dnxyz)),
+                      _p(_F(sd)), _p(_F(stiff)), _p(_F(wqs)), _ri(len(nefail)), _p(nefail),
+                      _p(ki), _p(kj), _p(kv), _ri(ndof), C.byref(knz))
+    return ki, kj[:knz.value].copy(), kv[:knz.value].copy()
+
+
+def chkcnsta(nn, xconn, nefail, necnt):
+    """delelement.f90:139-161 -> updated necnt."""
+    xconn = _F(xconn, np.int32)
+    necnt = _F(necnt, np.int32).copy()
+    nefail = _F(nefail, np.int32)
+    lib().chkcnsta_(_ri(nn), _ri(xconn.shape[0]), _ri(xconn.shape[1]), _p(xconn),
+                    _ri(len(nefail)), _p(nefail), _p(necnt))
+    return necnt
+
+
+def chkndsta(nn, xconn, nefail, necnt):
+    """delelement.f90:164-201 -> orphan node ids (1-based, discovery order)."""
+    xconn = _F(xconn, np.int32)
+    nefail = _F(nefail, np.int32)
+    necnt = _F(necnt, np.int32)
+    ndfail = np.zeros(nn, dtype=np.int32)
+    ndf = C.c_int(0)
+    lib().chkndsta_(_ri(nn), _ri(xconn.shape[0]), _ri(xconn.shape[1]), _p(xconn),
+                    _ri(len(nefail)), _p(nefail), _p(necnt), C.byref(ndf), _p(ndfail))
+    return ndfail[:ndf.value].copy()
+
+
+def adddiagonal(ndofn, ndfail, K):
+    """delelement.f90:204-247 -> K + I on the orphan DOFs.  (Its tricsr call passes
+    five of tricsr's six arguments; the sixth only sizes a dummy array.)"""
+    ndfail = _F(ndfail, np.int32)
+    ki, kj, kv, knz = _csr_room(K, len(ndfail) * ndofn)
+    ndof = len(ki) - 1
+    lib().adddiagonal_(_ri(8), _ri(ndofn), _ri(len(ndfail)), _p(ndfail), _p(ki), _p(kj), _p(kv),
+                       _ri(ndof), C.byref(knz))
+    return ki, kj[:knz.value].copy(), kv[:knz.value].copy()
+
+
+def deletion_block(xconn, xcoord, K, M, fixed, necnt, nefail, ym, nu):
+    """The reference's deletion block main3d.f90:650-686 on one rank, run through the
+    compiled routines: delelement, chkcnsta, chkndsta, adddiagonal, the orphan
+    constraints, csrapplybc on M and K.  K, M: current (constrained) matrices.
+    Returns (K', M', fixed', necnt', ndfail)."""
+    wqs, dns, dnxyz = shape_tables_m()
+    sd, stiff = setsdstiff(ym, nu)
+    K1 = delelement(xconn, xcoord, dns, dnxyz, sd, stiff, wqs, nefail, K)
+    necnt = chkcnsta(len(xcoord), xconn, nefail, necnt)
+    ndfail = chkndsta(len(xcoord), xconn, nefail, necnt)
+    K2 = adddiagonal(3, ndfail, K1)
+    fixed = np.array(fixed, dtype=np.int32)
+    for j in ndfail:
+        fixed[3 * (j - 1):3 * j] = 1
+    return csrapplybc(K2, fixed), csrapplybc(M, fixed), fixed, necnt, ndfail

@@ -11,7 +11,8 @@ Outputs (numpy .npz, no pickles):
   golden_ref.npz    input/output vectors of the compiled reference routines
                     (stcoeff_m, stcoeff_t, getfextc_m, mpiglbmtx_m+tricsr,
                     csrapplybc, ilut, lusol, kronspmv, kronspgemv, kronlusol,
-                    kronpgmres, getdfinal, chkelesta) on seeded inputs.
+                    kronpgmres, getdfinal, chkelesta, the element-deletion block) on
+                    seeded inputs.
 """
 from __future__ import annotations
 
@@ -164,6 +165,32 @@ def main():
     D = rng.uniform(0.0, 0.35, 8 * mesh.ne)
     nefail, elesta, gpsta = ref.chkelesta(mesh.nn, mesh.ne, 8, 0.3, D)
     g["chk_D"], g["chk_nefail"], g["chk_elesta"], g["chk_gpsta"] = D, nefail, elesta, gpsta
+    # element deletion block (main3d.f90:650-686), two successive events on the rod:
+    # a corner element plus five random ones, then every element around an interior
+    # node; "del" chains the edited matrices, "delra" starts each event from the
+    # re-assembled matrices with the current fixed set, as the driver does
+    # (main3d.f90:752-783)
+    rng2 = np.random.default_rng(SEED + 1)
+    deg = fem.node_degree(mesh).astype(np.int32)
+    corner = int(np.flatnonzero(deg == 1)[0]) + 1
+    e_corner = int(np.flatnonzero((mesh.xconn == corner).any(axis=1))[0]) + 1
+    inner = int(np.flatnonzero(deg == 8)[len(np.flatnonzero(deg == 8)) // 2]) + 1
+    ev1 = np.unique(np.r_[e_corner, rng2.choice(mesh.ne, 5, replace=False) + 1]).astype(np.int32)
+    ring = np.flatnonzero((mesh.xconn == inner).any(axis=1)) + 1
+    ev2 = np.setdiff1d(ring, ev1).astype(np.int32)
+    for tag, readd in (("del", False), ("delra", True)):
+        Kc, Mc = Kb, Mb
+        fixed, necnt = bc.astype(np.int32), deg.copy()
+        for i, nef in enumerate((ev1, ev2)):
+            if readd and i > 0:
+                Kc, Mc = ref.csrapplybc(K, fixed), ref.csrapplybc(M, fixed)
+            Kc, Mc, fixed, necnt, ndfail = ref.deletion_block(mesh.xconn, mesh.xcoord, Kc, Mc,
+                                                              fixed, necnt, nef, YM_REF, NU_REF)
+            g[f"{tag}{i}_nefail"], g[f"{tag}{i}_ndfail"] = nef, ndfail
+            g[f"{tag}{i}_fixed"], g[f"{tag}{i}_necnt"] = fixed, necnt
+            g[f"{tag}{i}_K_digest"] = np.array(digest(*Kc))
+            g[f"{tag}{i}_M_digest"] = np.array(digest(*Mc))
+            g[f"{tag}{i}_K_nnz"] = np.array(len(Kc[2]))
     np.savez_compressed(os.path.join(HERE, "golden_ref.npz"), **g)
     print("wrote", os.path.join(HERE, "golden_ref.npz"))
 

@@ -31,7 +31,8 @@ def test_header_declares_the_reference_path():
     names = declared()
     for n in ("xt_kronspmv", "xt_kronspgemv", "xt_amux", "xt_ilut", "xt_lusol", "xt_kronlusol",
               "xt_kronpgmres", "xt_kokkos_host", "xt_chkelesta", "xt_slab_solve",
-              "xt_damage_device"):
+              "xt_damage_device", "xt_delelement", "xt_chkcnsta", "xt_chkndsta",
+              "xt_adddiagonal", "xt_csrapplybc", "xt_elestatus_device"):
         assert n in names
 
 
@@ -181,3 +182,59 @@ def test_fortran_binding_links_and_runs_ilut(oracle_lib, tmp_path):
     m = int(ojlu[n]) - 1
     assert np.array_equal(ju, oju) and np.array_equal(jlu, ojlu[:m])
     assert np.array_equal(alu, oa[:m])
+
+
+def test_fortran_binding_runs_the_deletion_sequence(tmp_path):
+    """delelement / chkcnsta / chkndsta / adddiagonal called by their reference names
+    from Fortran (tests/fortran/delete_driver.f90) give the same CSR, counts and orphan
+    list as the Python front end, bit for bit (host work, no device)."""
+    if not os.path.exists(FLANG):
+        pytest.skip("flang not available")
+    if not os.path.exists(LIB):
+        pytest.skip("libxtfem_hip.so not built")
+    from conftest import structured_system
+    from xtfem_amd import api, fem
+    S = structured_system(4, 2, 2, 8.0, 4.0, 4.0)
+    mesh = S["mesh"]
+    wqs, dns, dnxyz = fem.shape_tables_m()
+    stiff = fem.elastic_dmat(2.0e5, 0.3)
+    deg = fem.node_degree(mesh).astype(np.int32)
+    corner = int(np.flatnonzero(deg == 1)[0]) + 1
+    nef = np.unique(np.r_[np.flatnonzero((mesh.xconn == corner).any(axis=1)) + 1, 6, 11])
+    nef = nef.astype(np.int32)
+    K = S["Kb"]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    ndof, knz = len(K[0]) - 1, len(K[2])
+    with open(fin, "wb") as f:
+        f.write(np.array([mesh.nn, mesh.ne, ndof, knz, len(nef)], dtype=np.int32).tobytes())
+        for a, dt in ((mesh.xconn, np.int32), (mesh.xcoord, np.float64), (dns, np.float64),
+                      (dnxyz, np.float64), (fem.SD, np.float64), (stiff, np.float64)):
+            f.write(np.asfortranarray(a, dtype=dt).tobytes(order="F"))
+        for a in (wqs, nef, K[0], K[1], K[2], deg):
+            f.write(np.ascontiguousarray(a).tobytes())
+    fdir = os.path.join(ROOT, "xtfem_amd", "fortran")
+    exe = str(tmp_path / "delete_driver")
+    libdir = os.path.dirname(LIB)
+    subprocess.run([FLANG, "-c", os.path.join(fdir, "xtfem_hip.f90"), "-module-dir",
+                    str(tmp_path), "-o", str(tmp_path / "xtfem_hip.o")], check=True)
+    subprocess.run([FLANG, os.path.join(ROOT, "tests", "fortran", "delete_driver.f90"),
+                    "-I", str(tmp_path), str(tmp_path / "xtfem_hip.o"), "-L", libdir,
+                    "-lxtfem_hip", f"-Wl,-rpath,{libdir}", "-Wl,-rpath-link,/opt/rocm/lib",
+                    "-o", exe], check=True)
+    subprocess.run([exe, str(fin), str(fout)], check=True)
+    raw = open(fout, "rb").read()
+    knz2, ndf = np.frombuffer(raw[:8], dtype=np.int32)
+    off = 8
+    take = lambda n, dt: np.frombuffer(raw, dtype=dt, count=n, offset=off)
+    ki = take(ndof + 1, np.int32); off += 4 * (ndof + 1)
+    kj = take(knz2, np.int32); off += 4 * knz2
+    kv = take(knz2, np.float64); off += 8 * knz2
+    necnt = take(mesh.nn, np.int32); off += 4 * mesh.nn
+    ndfail = take(ndf, np.int32)
+    K1 = api.delelement(mesh.xconn, mesh.xcoord, dns, dnxyz, fem.SD, stiff, wqs, nef, K)
+    cnt = api.chkcnsta(mesh.nn, mesh.xconn, nef, deg)
+    nd = api.chkndsta(mesh.nn, mesh.xconn, nef, cnt)
+    K2 = api.adddiagonal(3, nd, K1)
+    assert len(nd) >= 1 and np.array_equal(ndfail, nd) and np.array_equal(necnt, cnt)
+    for a, b in zip((ki, kj, kv), K2):
+        assert np.array_equal(a, b)

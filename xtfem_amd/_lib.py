@@ -38,6 +38,13 @@ _SIGS = {
                            ci, vp, ci, vp, vp, vp]),
     "xt_kokkos_host": (ci, [ci] * 9 + [vp] * 17),
     "xt_chkelesta": (ci, [ci, ci, cd, vp, vp, vp, vp, vp]),
+    "xt_delelement": (ci, [ci] * 7 + [vp] * 7 + [ci, vp, vp, vp, vp, ci, vp]),
+    "xt_chkcnsta": (ci, [ci, ci, ci, vp, ci, vp, vp]),
+    "xt_chkndsta": (ci, [ci, ci, ci, vp, ci, vp, vp, vp, vp]),
+    "xt_adddiagonal": (ci, [ci, ci, ci, vp, vp, vp, vp, ci, vp]),
+    "xt_csrapplybc": (ci, [vp, vp, vp, ci, vp, vp]),
+    "xt_elestatus_device": (ci, [ci, ci, cd, vp, vp, vp, vp, vp]),
+    "xt_damage_zero_failed": (ci, [ci, vp, vp, vp, vp]),
     "xt_op_create": (ci, [ci, ci, ci, vp, vp, vp, vp]),
     "xt_op_destroy": (None, [vp]),
     "xt_op_nnz": (C.c_longlong, [vp, ci]),

@@ -2,7 +2,8 @@
 
 Same names, argument meaning and error behaviour as the reference routines; every
 call runs on the GPU through libxtfem_hip.so (C ABI declared in include/xtfem.h),
-except ``ilut``, whose factorisation is host C++ in the same library.
+except ``ilut`` and the element-deletion edits of K (``delelement``, ``chkcnsta``,
+``chkndsta``, ``adddiagonal``, ``csrapplybc``), which are host C++ in the same library.
 
 Conventions (the reference's): CSR triples ``(ia, ja, a)`` 1-based int32 / float64,
 temporal matrices q x q numpy arrays, space-time vectors temporal-major
@@ -171,5 +172,84 @@ def chkelesta(ne, ngpe, Dc, D, elesta=None, gpsta=None):
     return nefail[:nef.value].copy(), el, gp
 
 
+def _csr_in(S, extra=0):
+    """Copy of a 1-based CSR triple with room for ``extra`` more entries."""
+    ia = _i32(S[0]).copy()
+    nz = int(ia[-1]) - 1
+    ja = np.zeros(nz + extra, dtype=np.int32)
+    a = np.zeros(nz + extra)
+    ja[:nz] = S[1][:nz]
+    a[:nz] = S[2][:nz]
+    return ia, ja, a, C.c_int(nz)
+
+
+def delelement(xconn, xcoord, dns, dnxyz, sd, stiff, wqs, nefail, K):
+    """K - sum of the failed elements' stiffness matrices (delelement.f90:1-59).
+
+    xconn [ne, nnse] 1-based, xcoord [nn, 3], dns [3*ngpe, nnse],
+    dnxyz [9*ngpe, 3*nnse], sd [6, 9], stiff [6, 6] (one material matrix for every
+    Gauss point), wqs [nquads] as setshapefunc_m / setsdstiff give them; nefail
+    1-based failed element ids.  Returns the new CSR triple."""
+    xconn = np.asfortranarray(xconn, dtype=np.int32)
+    xcoord = np.asfortranarray(xcoord, dtype=np.float64)
+    dns = np.asfortranarray(dns, dtype=np.float64)
+    dnxyz = np.asfortranarray(dnxyz, dtype=np.float64)
+    sd = np.asfortranarray(sd, dtype=np.float64)
+    stiff = np.asfortranarray(stiff, dtype=np.float64)
+    wqs = _f64(wqs)
+    nefail = _i32(nefail)
+    ne, nnse = xconn.shape
+    nn = xcoord.shape[0]
+    ngpe = dns.shape[0] // 3
+    ki, kj, kv, knz = _csr_in(K, extra=len(nefail) * (3 * nnse) ** 2)
+    ndof = len(ki) - 1
+    check(lib().xt_delelement(nn, ne, 3, nnse, ngpe, 6, len(wqs), _P(xconn), _P(xcoord),
+                              _P(dns), _P(dnxyz), _P(sd), _P(stiff), _P(wqs), len(nefail),
+                              _P(nefail), _P(ki), _P(kj), _P(kv), ndof, C.addressof(knz)),
+          "delelement")
+    return ki, kj[:knz.value].copy(), kv[:knz.value].copy()
+
+
+def chkcnsta(nn, xconn, nefail, necnt):
+    """necnt minus one per failed element at each of its nodes (delelement.f90:139-161)."""
+    xconn = np.asfortranarray(xconn, dtype=np.int32)
+    nefail = _i32(nefail)
+    necnt = _i32(necnt).copy()
+    check(lib().xt_chkcnsta(nn, xconn.shape[0], xconn.shape[1], _P(xconn), len(nefail),
+                            _P(nefail), _P(necnt)), "chkcnsta")
+    return necnt
+
+
+def chkndsta(nn, xconn, nefail, necnt):
+    """Orphan nodes of the failed elements, in discovery order (delelement.f90:164-201)."""
+    xconn = np.asfortranarray(xconn, dtype=np.int32)
+    nefail = _i32(nefail)
+    necnt = _i32(necnt)
+    ndfail = np.zeros(nn, dtype=np.int32)
+    ndf = C.c_int(0)
+    check(lib().xt_chkndsta(nn, xconn.shape[0], xconn.shape[1], _P(xconn), len(nefail),
+                            _P(nefail), _P(necnt), C.addressof(ndf), _P(ndfail)), "chkndsta")
+    return ndfail[:ndf.value].copy()
+
+
+def adddiagonal(ndofn, ndfail, K):
+    """K + I on the DOFs of the orphan nodes (delelement.f90:204-247)."""
+    ndfail = _i32(ndfail)
+    ki, kj, kv, knz = _csr_in(K, extra=len(ndfail) * ndofn)
+    check(lib().xt_adddiagonal(8, ndofn, len(ndfail), _P(ndfail), _P(ki), _P(kj), _P(kv),
+                               len(ki) - 1, C.addressof(knz)), "adddiagonal")
+    return ki, kj[:knz.value].copy(), kv[:knz.value].copy()
+
+
+def csrapplybc(S, bc):
+    """1-0 method on the DOFs with bc != 0 (spmtx.f90:268-307)."""
+    ia, ja, a, nz = _csr_in(S)
+    bc = _i32(bc)
+    check(lib().xt_csrapplybc(_P(ia), _P(ja), _P(a), len(ia) - 1, C.addressof(nz), _P(bc)),
+          "csrapplybc")
+    return ia, ja[:nz.value].copy(), a[:nz.value].copy()
+
+
 __all__ = ["kronspmv", "kronspgemv", "amux", "ilut", "lusol", "kronlusol", "kronpgmres",
-           "kokkos_host", "kokkos_damage", "chkelesta", "XtError"]
+           "kokkos_host", "kokkos_damage", "chkelesta", "delelement", "chkcnsta", "chkndsta",
+           "adddiagonal", "csrapplybc", "XtError"]

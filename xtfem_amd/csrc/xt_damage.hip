@@ -260,6 +260,16 @@ elestatus_kernel(int ne, int ngpe, double Dc, const double *__restrict__ D, int 
     }
 }
 
+// kokkos_damage.f90:102-107: a failed Gauss point carries D = ws = 0
+__global__ void __launch_bounds__(256)
+zero_failed_kernel(int ngp, const int *__restrict__ gpsta, double *D, double *ws) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngp && gpsta[g] == 0) {
+        D[g] = 0.0;
+        ws[g] = 0.0;
+    }
+}
+
 }  // namespace xt
 
 using namespace xt;
@@ -298,3 +308,22 @@ int elestatus_device(int ne, int ngpe, double Dc, const double *D, int *elesta, 
     return XT_OK;
 }
 }  // namespace xt
+
+extern "C" int xt_elestatus_device(int ne, int ngpe, double Dc, const double *D, int *elesta,
+                                   int *gpsta, int *failed, void *stream) {
+    XT_ARG(ne >= 0 && ngpe > 0, "elestatus sizes");
+    if (ne == 0) return XT_OK;
+    XT_ARG(D && elesta && gpsta && failed, "elestatus arrays");
+    return elestatus_device(ne, ngpe, Dc, D, elesta, gpsta, failed, (hipStream_t)stream);
+}
+
+extern "C" int xt_damage_zero_failed(int ngp, const int *gpsta, double *D, double *ws,
+                                     void *stream) {
+    XT_ARG(ngp >= 0, "zero_failed size");
+    if (ngp == 0) return XT_OK;
+    XT_ARG(gpsta && D && ws, "zero_failed arrays");
+    hipLaunchKernelGGL(zero_failed_kernel, dim3(ceil_div(ngp, 256)), dim3(256), 0,
+                       (hipStream_t)stream, ngp, gpsta, D, ws);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}

@@ -1,0 +1,112 @@
+"""Element deletion at the top of a slab (driver block main3d.f90:625-691).
+
+When some Gauss point reaches the critical damage Dc the reference, in order:
+
+1. marks the failed elements and their Gauss points (chkelesta, on the device
+   state here: ``DamageState.failed_elements``);
+2. subtracts their stiffness from K (delelement, delelement.f90:1-59);
+3. counts the surviving elements of each node and collects the nodes left with
+   none (chkcnsta / chkndsta, delelement.f90:139-201);
+4. puts a unit diagonal on the DOFs of those orphan nodes (adddiagonal,
+   delelement.f90:204-247) and fixes them in every temporal block
+   (main3d.f90:666-675);
+5. applies the fixed set to M and K (csrapplybc) and refactors the
+   preconditioner (main3d.f90:681-686);
+6. shortens the slab: dt = dt0*n2, nip = nipc*n2 (main3d.f90:689-690).
+
+The sparse edits are host C++ in libxtfem_hip.so (bit-identical to the
+reference's CSR); the caller rebuilds its device operator and preconditioner
+from the matrices returned (``KronOperator``, ``BandPreconditioner``), which is
+where the time goes.
+
+The reference re-assembles K and M from every element later in the same slab
+(main3d.f90:752-783: mpiglbmtx_m over all elements, then csrapplybc with the
+updated fixed set), so the matrices its slab solve uses hold the deleted
+elements again, with the orphan DOFs fixed.  ``readd=True`` (default) follows
+that; ``readd=False`` keeps the edited matrices, so deleted elements stay
+removed.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api, fem
+
+
+@dataclass
+class DeletionEvent:
+    nefail: np.ndarray            # failed element ids (1-based, increasing)
+    ndfail: np.ndarray            # orphan node ids (1-based, discovery order)
+    K_edited: tuple               # K after delelement/adddiagonal/csrapplybc (main3d.f90:682)
+    M_edited: tuple               # M after csrapplybc (main3d.f90:681)
+    K: tuple                      # matrices the slab's solve uses (see ``readd``)
+    M: tuple
+    fixed: np.ndarray             # per-DOF fixed mask after the orphan constraints
+    timings: dict = field(default_factory=dict)
+
+
+class ElementDeletion:
+    """Deletion state of one mesh: node survivor counts, fixed DOFs, the current K and M.
+
+    K, M: the assembled matrices before boundary conditions (1-based CSR), as
+    ``fem.assemble`` returns them; fixed: per-DOF mask of the clamped DOFs.
+    ym, nu: the material; delelement uses the first Gauss point's 6x6 matrix for
+    every failed element (the driver passes stiff(:,:,1), main3d.f90:650-652).
+    """
+
+    def __init__(self, mesh, K, M, fixed, ym, nu, Dc=0.3, readd=True):
+        self.mesh = mesh
+        self.Dc = float(Dc)
+        self.readd = bool(readd)
+        self.K_asm, self.M_asm = K, M
+        self.fixed = (np.asarray(fixed) != 0).astype(np.int32)
+        self.K = api.csrapplybc(K, self.fixed)
+        self.M = api.csrapplybc(M, self.fixed)
+        self.necnt = fem.node_degree(mesh).astype(np.int32)     # main3d.f90:273-280
+        self.wqs, self.dns, self.dnxyz = fem.shape_tables_m()
+        ym1 = float(np.ravel(ym)[0])
+        nu1 = float(np.ravel(nu)[0])
+        self.stiff = fem.elastic_dmat(ym1, nu1)
+        self.events = []
+
+    def apply(self, nefail):
+        """Delete the elements ``nefail`` (1-based ids): main3d.f90:650-686.
+        Returns a DeletionEvent, or None when the list is empty."""
+        nefail = np.asarray(nefail, dtype=np.int32)
+        if nefail.size == 0:
+            return None
+        t = {}
+        t0 = time.perf_counter()
+        xconn = self.mesh.xconn
+        K1 = api.delelement(xconn, self.mesh.xcoord, self.dns, self.dnxyz, fem.SD, self.stiff,
+                            self.wqs, nefail, self.K)
+        t["delelement"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        self.necnt = api.chkcnsta(self.mesh.nn, xconn, nefail, self.necnt)
+        ndfail = api.chkndsta(self.mesh.nn, xconn, nefail, self.necnt)
+        K2 = api.adddiagonal(3, ndfail, K1)
+        for node in ndfail:
+            self.fixed[3 * (node - 1):3 * node] = 1
+        M_ed = api.csrapplybc(self.M, self.fixed)
+        K_ed = api.csrapplybc(K2, self.fixed)
+        if self.readd:
+            self.K = api.csrapplybc(self.K_asm, self.fixed)
+            self.M = api.csrapplybc(self.M_asm, self.fixed)
+        else:
+            self.K, self.M = K_ed, M_ed
+        t["orphans_bc"] = time.perf_counter() - t0
+        ev = DeletionEvent(nefail, ndfail, K_ed, M_ed, self.K, self.M, self.fixed.copy(), t)
+        self.events.append(ev)
+        return ev
+
+    def step(self, damage):
+        """chkelesta on a device ``DamageState`` and, if any element failed, ``apply``."""
+        return self.apply(damage.failed_elements(self.Dc))
+
+
+def shortened_slab(dt0, nipc, n2):
+    """Slab length and temporal points per slab after a deletion (main3d.f90:689-690)."""
+    return dt0 * n2, nipc * n2

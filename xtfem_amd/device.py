@@ -239,6 +239,7 @@ class DamageState:
         self.s = {k: torch.zeros(6 * ngp, dtype=torch.float64, device=dev) for k in self.FIELDS6}
         self.s.update({k: torch.zeros(ngp, dtype=torch.float64, device=dev) for k in self.FIELDS1})
         self.gpsta = torch.ones(ngp, dtype=torch.int32, device=dev)
+        self.elesta = torch.ones(mesh.ne, dtype=torch.int32, device=dev)
 
     def step(self, sol_dev, Nt_dev, nip):
         """sol_dev: temporal-major FP64 device vector [6*3*nn]; Nt_dev: [nip*6]."""
@@ -250,9 +251,18 @@ class DamageState:
                                      *[s[k].data_ptr() for k in self.FIELDS1], _stream()),
               "damage_device")
         # kokkos_damage.f90:102-107: failed Gauss points carry D = ws = 0
-        dead = self.gpsta == 0
-        s["D"].masked_fill_(dead, 0.0)
-        s["ws"].masked_fill_(dead, 0.0)
+        check(lib().xt_damage_zero_failed(8 * self.ne, self.gpsta.data_ptr(), s["D"].data_ptr(),
+                                          s["ws"].data_ptr(), _stream()), "damage_zero_failed")
+
+    def failed_elements(self, Dc):
+        """chkelesta (delelement.f90:109-136) on the device state: marks the newly failed
+        elements and their Gauss points in elesta / gpsta and returns their 1-based ids
+        in increasing order."""
+        flags = torch.empty(self.ne, dtype=torch.int32, device=self.gpsta.device)
+        check(lib().xt_elestatus_device(self.ne, 8, float(Dc), self.s["D"].data_ptr(),
+                                        self.elesta.data_ptr(), self.gpsta.data_ptr(),
+                                        flags.data_ptr(), _stream()), "elestatus_device")
+        return (np.flatnonzero(flags.cpu().numpy()) + 1).astype(np.int32)
 
     def host_state(self):
         return {k: v.cpu().numpy() for k, v in self.s.items()}

@@ -118,6 +118,29 @@ def _shape_tables():
     return N, dN
 
 
+# Strain-displacement selector sd of setsdstiff (elesubs.f90:390-399): row = Voigt
+# component (xx, yy, zz, xy, yz, xz, engineering shear), column 3*c + p holds
+# d u_c / d x_p.
+SD = np.zeros((6, 9))
+for _r, _cols in enumerate(((0,), (4,), (8,), (1, 3), (5, 7), (2, 6))):
+    SD[_r, list(_cols)] = 1.0
+
+
+def shape_tables_m():
+    """setshapefunc_m (elesubs.f90:299-335, shape3dx procs.f90:1344-1414) for C3D8
+    with the 2-point Gauss rule: wqs [2]; dns [24, 8] with row 3l + m = dN_a/dxi_m at
+    Gauss point l (l = 4i + 2j + k); dnxyz [72, 24] with dnxyz[9l + 3c + m, 3a + c] =
+    dN_a/dxi_m (the derivative block of displacement component c)."""
+    _, dN = _shape_tables()
+    wqs = np.ones(2)
+    dns = dN.reshape(24, 8).copy()
+    dnxyz = np.zeros((72, 24))
+    for l in range(8):
+        for c in range(3):
+            dnxyz[9 * l + 3 * c:9 * l + 3 * c + 3, c::3] = dN[l]
+    return wqs, dns, dnxyz
+
+
 def _det3(A):
     """procs.f90:476-478 (explicit 3x3 determinant, same term order)."""
     a = lambda i, j: A[..., i - 1, j - 1]
@@ -303,5 +326,5 @@ def clamp_mask(mesh: Mesh, axis, values, tol=1e-9):
 
 
 def node_degree(mesh: Mesh):
-    """Number of elements sharing each node (main3d.f90:496-501)."""
+    """Number of elements sharing each node (main3d.f90:273-280)."""
     return np.bincount((mesh.xconn - 1).ravel(), minlength=mesh.nn)

@@ -3,10 +3,11 @@
 ! Module xtfem_hip declares the C ABI of include/xtfem.h with BIND(C) interfaces
 ! (scalars by value, arrays by reference).  The external subroutines below keep the
 ! reference's names and argument lists (kronecker.f90:19-365, ilut.f:3-44,
-! 152-495, 2383-2432), so a driver that calls them through implicit interfaces
-! (main3d.f90) links against this file and the library in place of kronecker.f90 and
-! ilut.f.  Library failures stop the program, as the reference's fatal paths do;
-! the reference's own ierr conventions are passed through unchanged.  kronpgmres
+! 152-495, 2383-2432, delelement.f90:1-59, 109-247), so a driver that calls them
+! through implicit interfaces (main3d.f90) links against this file and the library
+! in place of kronecker.f90, ilut.f and delelement.f90.  Library failures stop the
+! program, as the reference's fatal paths do; the reference's own ierr conventions
+! are passed through unchanged.  kronpgmres
 ! writes the residual history to unit iout in the reference's format
 ! (kronecker.f90:258, 311, 354).
 module xtfem_hip
@@ -86,6 +87,52 @@ module xtfem_hip
       integer(c_int), intent(in) :: ki(*), kj(*), mi(*), mj(*), jlu(*), ju(*)
       real(c_double), intent(inout) :: rhs(*), sol(*), hist(*)
       integer(c_int), intent(out) :: nhist, its, ierr
+    end function
+
+    integer(c_int) function xt_chkelesta(ne, ngpe, dc, d, elesta, gpsta, nef, nefail) &
+        bind(C, name="xt_chkelesta")
+      import :: c_int, c_double
+      integer(c_int), value :: ne, ngpe
+      real(c_double), value :: dc
+      real(c_double), intent(in) :: d(*)
+      integer(c_int), intent(inout) :: elesta(*), gpsta(*)
+      integer(c_int), intent(out) :: nef, nefail(*)
+    end function
+
+    integer(c_int) function xt_delelement(nn, ne, ndofn, nnse, ngpe, ncmp, nquads, xconn, &
+        xcoord, dns, dnxyz, sd, stiff, wqs, nef, nefail, ki, kj, kv, ndof, knz) &
+        bind(C, name="xt_delelement")
+      import :: c_int, c_double
+      integer(c_int), value :: nn, ne, ndofn, nnse, ngpe, ncmp, nquads, nef, ndof
+      integer(c_int), intent(in) :: xconn(*), nefail(*)
+      real(c_double), intent(in) :: xcoord(*), dns(*), dnxyz(*), sd(*), stiff(*), wqs(*)
+      integer(c_int), intent(inout) :: ki(*), kj(*), knz
+      real(c_double), intent(inout) :: kv(*)
+    end function
+
+    integer(c_int) function xt_chkcnsta(nn, ne, nnse, xconn, nef, nefail, necnt) &
+        bind(C, name="xt_chkcnsta")
+      import :: c_int
+      integer(c_int), value :: nn, ne, nnse, nef
+      integer(c_int), intent(in) :: xconn(*), nefail(*)
+      integer(c_int), intent(inout) :: necnt(*)
+    end function
+
+    integer(c_int) function xt_chkndsta(nn, ne, nnse, xconn, nef, nefail, necnt, ndf, ndfail) &
+        bind(C, name="xt_chkndsta")
+      import :: c_int
+      integer(c_int), value :: nn, ne, nnse, nef
+      integer(c_int), intent(in) :: xconn(*), nefail(*), necnt(*)
+      integer(c_int), intent(out) :: ndf, ndfail(*)
+    end function
+
+    integer(c_int) function xt_adddiagonal(nnse, ndofn, ndf, ndfail, ki, kj, kv, ndof, knz) &
+        bind(C, name="xt_adddiagonal")
+      import :: c_int, c_double
+      integer(c_int), value :: nnse, ndofn, ndf, ndof
+      integer(c_int), intent(in) :: ndfail(*)
+      integer(c_int), intent(inout) :: ki(*), kj(*), knz
+      real(c_double), intent(inout) :: kv(*)
     end function
   end interface
 
@@ -193,3 +240,60 @@ subroutine kronpgmres(a, b, q, ki, kj, kv, mi, mj, mv, p, alu, jlu, ju, rhs, sol
   deallocate(hist)
 199 format('   its =', i4, ' rel. res. norm =', d20.6)
 end subroutine kronpgmres
+
+! Element deletion (delelement.f90): reference names and argument lists, so
+! main3d.f90:632-691 links against these in place of delelement.o.  As in the
+! driver, kj/kv must have room for the edited matrix (main3d.f90:757-760
+! allocates ne*(nnse*ndofn)**2 entries).
+subroutine chkelesta(nn, ne, nnse, ngpe, Dc, D, elesta, gpsta, nef, nefail)
+  use xtfem_hip
+  implicit none
+  integer, intent(in) :: nn, ne, nnse, ngpe
+  real(8), intent(in) :: Dc, D(*)
+  integer, intent(inout) :: elesta(*), gpsta(*)
+  integer, intent(out) :: nef, nefail(*)
+  integer :: i
+  do i = 1, ne
+    nefail(i) = 0
+  end do
+  call xt_check(xt_chkelesta(ne, ngpe, Dc, D, elesta, gpsta, nef, nefail), 'chkelesta')
+end subroutine chkelesta
+
+subroutine delelement(nn, ne, ndofn, nnse, ngpe, ncmp, nquads, xconn, xcoord, dns, dnxyz, &
+    sd, stiff, wqs, nef, nefail, ki, kj, kv, ndof, knz)
+  use xtfem_hip
+  implicit none
+  integer, intent(in) :: nn, ne, ndofn, nnse, ngpe, ncmp, nquads, nef, ndof
+  integer, intent(in) :: xconn(*), nefail(*)
+  real(8), intent(in) :: xcoord(*), dns(*), dnxyz(*), sd(*), stiff(*), wqs(*)
+  integer, intent(inout) :: ki(*), kj(*), knz
+  real(8), intent(inout) :: kv(*)
+  call xt_check(xt_delelement(nn, ne, ndofn, nnse, ngpe, ncmp, nquads, xconn, xcoord, dns, &
+                              dnxyz, sd, stiff, wqs, nef, nefail, ki, kj, kv, ndof, knz), &
+                'delelement')
+end subroutine delelement
+
+subroutine chkcnsta(nn, ne, nnse, xconn, nef, nefail, necnt)
+  use xtfem_hip
+  implicit none
+  integer, intent(in) :: nn, ne, nnse, nef, xconn(*), nefail(*)
+  integer, intent(inout) :: necnt(*)
+  call xt_check(xt_chkcnsta(nn, ne, nnse, xconn, nef, nefail, necnt), 'chkcnsta')
+end subroutine chkcnsta
+
+subroutine chkndsta(nn, ne, nnse, xconn, nef, nefail, necnt, ndf, ndfail)
+  use xtfem_hip
+  implicit none
+  integer, intent(in) :: nn, ne, nnse, nef, xconn(*), nefail(*), necnt(*)
+  integer, intent(out) :: ndf, ndfail(*)
+  call xt_check(xt_chkndsta(nn, ne, nnse, xconn, nef, nefail, necnt, ndf, ndfail), 'chkndsta')
+end subroutine chkndsta
+
+subroutine adddiagonal(nnse, ndofn, ndf, ndfail, ki, kj, kv, ndof, knz)
+  use xtfem_hip
+  implicit none
+  integer, intent(in) :: nnse, ndofn, ndf, ndof, ndfail(*)
+  integer, intent(inout) :: ki(*), kj(*), knz
+  real(8), intent(inout) :: kv(*)
+  call xt_check(xt_adddiagonal(nnse, ndofn, ndf, ndfail, ki, kj, kv, ndof, knz), 'adddiagonal')
+end subroutine adddiagonal

@@ -73,7 +73,8 @@ class Problem:
         keep = np.ones((nx, ny, nz), bool)
         keep[0:notch, ny // 2, :] = False
         self.mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz), keep=keep)
-        K, M, *_ = fem.assemble(self.mesh, 2.0e5, 0.3, 7.85e-9)
+        K, M, *self.Nxyz = fem.assemble(self.mesh, 2.0e5, 0.3, 7.85e-9)
+        self.K, self.M = K, M
         self.bc = fem.clamp_mask(self.mesh, 1, [0.0])
         self.Kb, self.Mb = fem.applybc(K, self.bc), fem.applybc(M, self.bc)
         self.p = 3 * self.mesh.nn
@@ -312,6 +313,67 @@ def damage_leg(nip, slabs=3):
                 max_D=float(D.max()))
 
 
+def deletion_leg(prob, R=8):
+    """Element deletion on the plate (main3d.f90:625-691): damage planted at the notch
+    tip (the element column through the thickness) and at a free corner element (its
+    corner node becomes an orphan); device element status, host CSR edits
+    (delelement, chkcnsta/chkndsta, adddiagonal, 1-0 method), then the device
+    operator and banded factor rebuilt from the matrices the next slab uses
+    (the reference's re-assembled ones), and one batched slab solve on them."""
+    import torch
+    from xtfem_amd import device, fem
+    from xtfem_amd.deletion import ElementDeletion
+    mesh = prob.mesh
+    nx, ny, nz = (int(round(mesh.xcoord[:, k].max())) for k in range(3))
+    cen = mesh.xcoord[mesh.xconn - 1].mean(axis=1)
+    tip_x = max(cen[(np.abs(cen[:, 1] - (ny // 2 + 0.5)) < 0.01), 0].min(), 0.5)
+    tip = np.flatnonzero((np.abs(cen[:, 0] - tip_x) < 0.01) &
+                         (np.abs(cen[:, 1] - (ny // 2 + 0.5)) < 0.01)) + 1
+    deg = fem.node_degree(mesh)
+    ytop = mesh.xcoord[:, 1].max()
+    corner = [n for n in np.flatnonzero(deg == 1) if mesh.xcoord[n, 1] == ytop][0] + 1
+    e_corner = int(np.flatnonzero((mesh.xconn == corner).any(axis=1))[0]) + 1
+    failed = np.unique(np.r_[tip, e_corner]).astype(np.int32)
+    ds = device.DamageState(mesh, *prob.Nxyz, 2.0e5, 0.3)
+    D = np.zeros(8 * mesh.ne)
+    D[8 * (failed - 1)] = 0.31
+    ds.s["D"].copy_(torch.as_tensor(D, device="cuda"))
+    dl = ElementDeletion(mesh, prob.K, prob.M, prob.bc, 2.0e5, 0.3, Dc=0.3, readd=True)
+    torch.cuda.synchronize()
+    t = {}
+    t0 = time.perf_counter()
+    nef = ds.failed_elements(0.3)
+    t["element_status"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ev = dl.apply(nef)
+    t["host_edits"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    op = device.KronOperator([ev.K, ev.M], q=6)
+    torch.cuda.synchronize()
+    t["operator_upload"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pr = device.BandPreconditioner(ev.K)
+    torch.cuda.synchronize()
+    t["band_refactor"] = time.perf_counter() - t0
+    cases = np.arange(R)
+    saved = prob.bc
+    prob.bc = ev.fixed
+    run = Runner(prob, op, pr, cases, mode=1, nsteps=2)
+    prob.bc = saved
+    t0 = time.perf_counter()
+    run.step()
+    torch.cuda.synchronize()
+    t["first_slab_solve"] = time.perf_counter() - t0
+    ok = bool(np.all(run.ierr[0] == 0) and np.all(run.relres[0] <= 1e-8))
+    return dict(workload=f"plate, {len(nef)} failed elements (notch tip + free corner), "
+                         f"{len(ev.ndfail)} orphan node(s), next slab with {R} cases",
+                nefail=int(len(nef)), ndfail=int(len(ev.ndfail)),
+                ms={k: 1e3 * v for k, v in t.items()},
+                host_edit_ms={k: 1e3 * v for k, v in ev.timings.items()},
+                solve_converged=ok, its_max=int(np.max(run.its[0])),
+                relres_max=float(np.max(run.relres[0])))
+
+
 def cpu_baseline(prob, budget_s=15.0):
     """The oracle on the host: FP64 restatement of kronpgmres (kronecker.f90:181-365,
     MGS, one core) preconditioned by A^-1 (x) K^-1 with the exact band Cholesky factor
@@ -534,6 +596,10 @@ def main():
     res["rod_ilut"] = rod_leg(K)
     print(f"# rod ILUT: fp64 {res['rod_ilut']['fp64']['ms_per_slab']:.2f} ms/slab, fp32-ir "
           f"{res['rod_ilut']['fp32_ir']['ms_per_slab']:.2f} ms/slab", file=sys.stderr, flush=True)
+    res["deletion"] = deletion_leg(prob)
+    print(f"# deletion: {res['deletion']['nefail']} elements, "
+          + ", ".join(f"{k} {v:.1f} ms" for k, v in res["deletion"]["ms"].items())
+          + f", next slab its max {res['deletion']['its_max']}", file=sys.stderr, flush=True)
     if not args.no_damage:
         res["damage"] = damage_leg(args.damage_nip)
         print(f"# damage: {res['damage']['ms_per_slab']:.1f} ms per slab, "

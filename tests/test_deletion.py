@@ -171,7 +171,8 @@ def test_deletion_block_golden(rod, golden, readd):
     (tests/golden/make_golden.py)."""
     from xtfem_amd.deletion import ElementDeletion
     tag = "delra" if readd else "del"
-    dl = ElementDeletion(rod["mesh"], rod["K"], rod["M"], rod["bc"], YM_REF, NU_REF, readd=readd)
+    dl = ElementDeletion(rod["mesh"], rod["K"], rod["M"], rod["bc"], YM_REF, NU_REF, readd=readd,
+                         edited=True)
     for i in range(2):
         ev = dl.apply(golden[f"{tag}{i}_nefail"])
         assert np.array_equal(ev.ndfail, golden[f"{tag}{i}_ndfail"])

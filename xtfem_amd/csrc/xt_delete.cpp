@@ -52,8 +52,22 @@ void csr_add_rows(int n, const int *ia, const int *ja, const double *a, double a
     c.reserve((size_t)(ia[n] - 1) + jb.size());
     ic[0] = 1;
     for (int r = 0; r < n; ++r) {
+        const int k0 = ia[r] - 1, k1 = ia[r + 1] - 1;
+        if (ib[r] == ib[r + 1] && alpha == 1.0) {
+            // B has nothing in this row: a row with strictly increasing columns
+            // and no zeros comes out of the scatter unchanged (1*a == a)
+            bool clean = true;
+            for (int k = k0; k < k1 && clean; ++k)
+                clean = a[k] != 0.0 && (k == k0 || ja[k] > ja[k - 1]);
+            if (clean) {
+                jc.insert(jc.end(), ja + k0, ja + k1);
+                c.insert(c.end(), a + k0, a + k1);
+                ic[r + 1] = (int)jc.size() + 1;
+                continue;
+            }
+        }
         cols.clear();
-        for (int k = ia[r] - 1; k < ia[r + 1] - 1; ++k) {
+        for (int k = k0; k < k1; ++k) {
             const int j = ja[k] - 1;
             if (mark[j] != r) { mark[j] = r; cols.push_back(j); x[j] = alpha * a[k]; }
             else x[j] = x[j] + alpha * a[k];

@@ -24,7 +24,8 @@ The reference re-assembles K and M from every element later in the same slab
 updated fixed set), so the matrices its slab solve uses hold the deleted
 elements again, with the orphan DOFs fixed.  ``readd=True`` (default) follows
 that; ``readd=False`` keeps the edited matrices, so deleted elements stay
-removed.
+removed.  With ``readd=True`` the edited K (which the reference factors and then
+replaces) is computed only when ``edited=True``.
 """
 from __future__ import annotations
 
@@ -40,8 +41,8 @@ from . import api, fem
 class DeletionEvent:
     nefail: np.ndarray            # failed element ids (1-based, increasing)
     ndfail: np.ndarray            # orphan node ids (1-based, discovery order)
-    K_edited: tuple               # K after delelement/adddiagonal/csrapplybc (main3d.f90:682)
-    M_edited: tuple               # M after csrapplybc (main3d.f90:681)
+    K_edited: tuple               # K after delelement/adddiagonal/csrapplybc (main3d.f90:682),
+    M_edited: tuple               # M after csrapplybc (main3d.f90:681); None if not computed
     K: tuple                      # matrices the slab's solve uses (see ``readd``)
     M: tuple
     fixed: np.ndarray             # per-DOF fixed mask after the orphan constraints
@@ -57,10 +58,11 @@ class ElementDeletion:
     every failed element (the driver passes stiff(:,:,1), main3d.f90:650-652).
     """
 
-    def __init__(self, mesh, K, M, fixed, ym, nu, Dc=0.3, readd=True):
+    def __init__(self, mesh, K, M, fixed, ym, nu, Dc=0.3, readd=True, edited=False):
         self.mesh = mesh
         self.Dc = float(Dc)
         self.readd = bool(readd)
+        self.edited = bool(edited) or not self.readd
         self.K_asm, self.M_asm = K, M
         self.fixed = (np.asarray(fixed) != 0).astype(np.int32)
         self.K = api.csrapplybc(K, self.fixed)
@@ -79,25 +81,28 @@ class ElementDeletion:
         if nefail.size == 0:
             return None
         t = {}
-        t0 = time.perf_counter()
         xconn = self.mesh.xconn
-        K1 = api.delelement(xconn, self.mesh.xcoord, self.dns, self.dnxyz, fem.SD, self.stiff,
-                            self.wqs, nefail, self.K)
-        t["delelement"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         self.necnt = api.chkcnsta(self.mesh.nn, xconn, nefail, self.necnt)
         ndfail = api.chkndsta(self.mesh.nn, xconn, nefail, self.necnt)
-        K2 = api.adddiagonal(3, ndfail, K1)
         for node in ndfail:
             self.fixed[3 * (node - 1):3 * node] = 1
-        M_ed = api.csrapplybc(self.M, self.fixed)
-        K_ed = api.csrapplybc(K2, self.fixed)
+        t["node_status"] = time.perf_counter() - t0
+        K_ed = M_ed = None
+        if self.edited:
+            t0 = time.perf_counter()
+            K1 = api.delelement(xconn, self.mesh.xcoord, self.dns, self.dnxyz, fem.SD,
+                                self.stiff, self.wqs, nefail, self.K)
+            K_ed = api.csrapplybc(api.adddiagonal(3, ndfail, K1), self.fixed)
+            M_ed = api.csrapplybc(self.M, self.fixed)
+            t["edit"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
         if self.readd:
             self.K = api.csrapplybc(self.K_asm, self.fixed)
             self.M = api.csrapplybc(self.M_asm, self.fixed)
         else:
             self.K, self.M = K_ed, M_ed
-        t["orphans_bc"] = time.perf_counter() - t0
+        t["next_matrices"] = time.perf_counter() - t0
         ev = DeletionEvent(nefail, ndfail, K_ed, M_ed, self.K, self.M, self.fixed.copy(), t)
         self.events.append(ev)
         return ev

@@ -328,6 +328,30 @@ fill_zero_kernel(long long n, T *d) {
         d[i] = T(0);
 }
 
+// Lower triangle of the 1-based CSR K into the column-major b x b diagonal blocks kd
+// and sub-diagonal blocks ko (row block i+1, column block i), one lane per row in
+// entry order (a repeated entry: the last one stays).  The rows of the last block
+// beyond p get a unit diagonal.  The blocks are zero on entry.
+__global__ void __launch_bounds__(256)
+band_scatter_kernel(int p, int b, int nb, const int *__restrict__ ia, const int *__restrict__ ja,
+                    const double *__restrict__ a, double *kd, double *ko) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const long long bb = (long long)b * b;
+    if (i >= nb * b) return;
+    const int bi = i / b, li = i - bi * b;
+    if (i >= p) {
+        kd[bi * bb + (long long)li * b + li] = 1.0;
+        return;
+    }
+    for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+        const int j = ja[k] - 1;
+        if (j > i) continue;
+        const int bj = j / b, lj = j - bj * b;
+        double *dst = bi == bj ? kd + bi * bb : ko + bj * bb;   // bi == bj + 1 otherwise
+        dst[(long long)lj * b + li] = a[k];
+    }
+}
+
 #define XT_RB(...)                                                                      \
     do {                                                                                \
         rocblas_status st_ = (__VA_ARGS__);                                             \
@@ -343,13 +367,27 @@ struct RbHandle {
     ~RbHandle() { if (h) rocblas_destroy_handle(h); }
 };
 
-static int factor(BandPrec &bp, std::vector<double> &kd, std::vector<double> &ko,
-                  hipStream_t s) {
-    const int b = bp.b, nb = bp.nb;
+static int factor(BandPrec &bp, const int *ia, const int *ja, const double *a, hipStream_t s) {
+    const int b = bp.b, nb = bp.nb, p = bp.p;
     const long long bb = (long long)b * b;
-    DevBuf L, E, Dl, G, H, info;
-    XT_TRY(L.upload(kd.data(), sizeof(double) * kd.size(), s));
-    if (nb > 1) XT_TRY(E.upload(ko.data(), sizeof(double) * ko.size(), s));
+    DevBuf L, E, Dl, G, H, info, dia, dja, da;
+    // the blocks are assembled on the device from the CSR (a few % of their bytes)
+    const long long nnz = ia[p] - 1;
+    XT_TRY(dia.upload(ia, sizeof(int) * (p + 1), s));
+    XT_TRY(dja.upload(ja, sizeof(int) * nnz, s));
+    XT_TRY(da.upload(a, sizeof(double) * nnz, s));
+    XT_TRY(L.reserve(sizeof(double) * bb * nb));
+    hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, bb * nb,
+                       L.as<double>());
+    if (nb > 1) {
+        XT_TRY(E.reserve(sizeof(double) * bb * (nb - 1)));
+        hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, bb * (nb - 1),
+                           E.as<double>());
+    }
+    hipLaunchKernelGGL(band_scatter_kernel, dim3(ceil_div((long long)nb * b, 256)), dim3(256), 0,
+                       s, p, b, nb, dia.as<int>(), dja.as<int>(), da.as<double>(), L.as<double>(),
+                       nb > 1 ? E.as<double>() : nullptr);
+    XT_HIP(hipGetLastError());
     XT_TRY(info.reserve(sizeof(int) * (nb + 1)));
     XT_HIP(hipMemsetAsync(info.ptr, 0, sizeof(int) * (nb + 1), s));
     RbHandle rb;
@@ -434,6 +472,7 @@ int band_build(int p, const int *ia, const int *ja, const double *a, int block, 
 #undef XT_ROWS_ATTR
     XT_ARG(ia[0] == 1, "CSR must be 1-based");
     int bw = 0;
+    for (int i = 0; i < p; ++i) XT_ARG(ia[i + 1] >= ia[i], "CSR row pointers must not decrease");
     for (int i = 0; i < p; ++i)
         for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
             const int j = ja[k] - 1;
@@ -447,20 +486,8 @@ int band_build(int p, const int *ia, const int *ja, const double *a, int block, 
         return XT_E_UNSUPPORTED;
     }
     const int nb = (p + b - 1) / b;
-    const long long bb = (long long)b * b;
-    std::vector<double> kd((size_t)bb * nb, 0.0), ko((size_t)bb * std::max(nb - 1, 0), 0.0);
-    for (int i = 0; i < p; ++i)
-        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
-            const int j = ja[k] - 1;
-            if (j > i) continue;                     // lower triangle only (symmetric K)
-            const int bi = i / b, bj = j / b, li = i - bi * b, lj = j - bj * b;
-            if (bi == bj) kd[(size_t)bi * bb + (size_t)lj * b + li] = a[k];
-            else ko[(size_t)bj * bb + (size_t)lj * b + li] = a[k];   // bi == bj + 1
-        }
-    for (int i = p; i < nb * b; ++i) {               // identity on the padding rows
-        const int li = i - (nb - 1) * b;
-        kd[(size_t)(nb - 1) * bb + (size_t)li * b + li] = 1.0;
-    }
+    // only the lower triangle is read (K symmetric); b >= bandwidth puts every lower
+    // entry in a diagonal block or the sub-diagonal block below it
     BandPrec *bp = new (std::nothrow) BandPrec;
     if (!bp) return XT_E_NOMEM;
     bp->p = p;
@@ -468,7 +495,7 @@ int band_build(int p, const int *ia, const int *ja, const double *a, int block, 
     bp->nb = nb;
     bp->bw = bw;
     hipStream_t s = nullptr;
-    int rc = factor(*bp, kd, ko, s);
+    int rc = factor(*bp, ia, ja, a, s);
     if (rc != XT_OK) { delete bp; return rc; }
     *out = bp;
     return XT_OK;

@@ -78,6 +78,179 @@ trsv_onewg_kernel(int n_, int nlevL, const int *__restrict__ lpL, const int *__r
         for (int i = threadIdx.x; i < n_ * c; i += blockDim.x) xg[i] = x[i];
 }
 
+// Both sweeps in one workgroup without a barrier per level: each wave takes rows in
+// level order (tasks w, w + nwaves, ... of the level-ordered rows of L, then of U) and
+// starts a row as soon as the rows it reads are finished, told by per-row flags in LDS
+// (1: forward result, 2: backward result).  Every row depends only on rows earlier in
+// that order and each wave walks its rows in that order, so the earliest unfinished
+// row can always proceed.  A wave loads the factor entries of its next row into
+// registers while it waits for and computes the current one; the solution and the
+// inverse diagonal live in LDS.  Lanes: S slices of the row's entries times
+// 64/S column groups; the slices' partial sums are reduced by DPP lane moves in a fixed
+// order.  Waits are bounded: a wave that waits past kSpinMax polls raises an abort
+// flag that releases every wave, and the output is then NaN.
+constexpr int kSfEnt = 4;           // entries per lane held in registers
+constexpr int kSpinMax = 1 << 22;
+
+template <typename T>
+static size_t sf_lds_bytes(int n, int c) {
+    return sizeof(T) * ((size_t)n * c + n) + sizeof(int) * (5 * (size_t)n + 3 + 1);
+}
+
+// Sum over aligned groups of S lanes (S = 1, 2, 4 or 8) with DPP lane moves: swap
+// neighbours, swap pairs within a quad, then mirror within the half-row of 8.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ T group_sum(T s, int S) {
+    if (S >= 2) s += dpp_move<0xB1>(s);    // quad_perm [1,0,3,2]
+    if (S >= 4) s += dpp_move<0x4E>(s);    // quad_perm [2,3,0,1]
+    if (S >= 8) s += dpp_move<0x141>(s);   // row_half_mirror
+    return s;
+}
+
+__device__ __forceinline__ int lds_flag(const int *f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Entries e0 + slice + j*S (j < kSfEnt) of one row; a position past the row reads the
+// zero entry at index nnz (whose column is 0).  Unconditional loads with no use in
+// this function, so they stay in flight while the wave works on its current row.
+template <typename T>
+__device__ __forceinline__ void sf_load(int e0, int e1, int nnz, int slice, int S,
+                                        const int *__restrict__ ci, const T *__restrict__ v,
+                                        T *pv, int *pc) {
+#pragma unroll
+    for (int j = 0; j < kSfEnt; ++j) {
+        const int e = e0 + slice + j * S;
+        const int es = e < e1 ? e : nnz;
+        pv[j] = v[es];
+        pc[j] = ci[es];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+trsv_sf_kernel(int n, int c, int S, const int *__restrict__ rowsL, const int *__restrict__ rpL,
+               const int *__restrict__ ciL, const T *__restrict__ vL, int nnzL,
+               const int *__restrict__ rowsU, const int *__restrict__ rpU,
+               const int *__restrict__ ciU, const T *__restrict__ vU, int nnzU,
+               const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ xg) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *x = reinterpret_cast<T *>(smem_raw);
+    T *sdinv = x + (size_t)n * c;
+    int *flag = reinterpret_cast<int *>(sdinv + n);             // [n + 1], flag[n] = 3
+    int *srL = flag + n + 1, *srU = srL + n, *spL = srU + n, *spU = spL + n + 1;
+    int *abort_flag = spU + n + 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < n * c; i += nt) x[i] = b[i];
+    for (int i = tid; i < n; i += nt) {
+        sdinv[i] = dinv[i];
+        flag[i] = 0;
+        srL[i] = rowsL[i];
+        srU[i] = rowsU[i];
+    }
+    for (int i = tid; i <= n; i += nt) {
+        spL[i] = rpL[i];
+        spU[i] = rpU[i];
+    }
+    if (tid == 0) {
+        flag[n] = 3;
+        *abort_flag = 0;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6, ntask = 2 * n;
+    const int slice = lane % S, cg = lane / S, ncg = 64 / S;
+    // task t < n: forward row srL[t]; t >= n: backward row srU[t - n]
+    // (branch-free: t is clamped to the last task, so the loads are issued on every path
+    // and the next task's loads need no wait before the current task's work)
+    auto task_load = [&](int t, T *vv, int *cc) {
+        t = t < ntask ? t : ntask - 1;
+        const bool up = t >= n;
+        const int *sr = up ? srU : srL, *rp = up ? spU : spL;
+        const int row = sr[up ? t - n : t];
+        sf_load<T>(rp[row], rp[row + 1], up ? nnzU : nnzL, slice, S, up ? ciU : ciL,
+                   up ? vU : vL, vv, cc);
+    };
+    auto process = [&](int t, const T *pv, const int *pc) {
+        const bool up = t >= n;
+        const int row = up ? srU[t - n] : srL[t];
+        const int *rp = up ? spU : spL;
+        const int *ci = up ? ciU : ciL;
+        const T *v = up ? vU : vL;
+        const int need = up ? 2 : 1;
+        const int e0 = rp[row], e1 = rp[row + 1];
+        const bool longrow = e1 - e0 > kSfEnt * S;
+        // wait for the rows this one reads (all flag reads issued together)
+        for (int spins = 0;; ++spins) {
+            int f[kSfEnt];
+#pragma unroll
+            for (int j = 0; j < kSfEnt; ++j)
+                f[j] = lds_flag(flag + (e0 + slice + j * S < e1 ? pc[j] : n));
+            const int fo = up ? lds_flag(flag + row) : 1;
+            const int ab = lds_flag(abort_flag);
+            bool ok = fo >= 1;
+#pragma unroll
+            for (int j = 0; j < kSfEnt; ++j) ok = ok && f[j] >= need;
+            if (longrow)
+                for (int e = e0 + slice + kSfEnt * S; e < e1; e += S)
+                    ok = ok && lds_flag(flag + ci[e]) >= need;
+            if (__all(ok) || ab) break;
+            if (spins >= kSpinMax) {
+                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                break;
+            }
+        }
+        // a wave's LDS operations complete in order: a compiler fence keeps the x reads
+        // after the flag reads (a workgroup fence would also wait for the next row's
+        // global loads)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        for (int col = cg; col < c; col += ncg) {
+            T s = 0;
+#pragma unroll
+            for (int j = 0; j < kSfEnt; ++j) s += pv[j] * x[(size_t)pc[j] * c + col];
+            if (longrow)
+                for (int e = e0 + slice + kSfEnt * S; e < e1; e += S)
+                    s += v[e] * x[(size_t)ci[e] * c + col];
+            s = group_sum(s, S);
+            if (slice == 0) {
+                const size_t o = (size_t)row * c + col;
+                const T y = x[o] - s;
+                x[o] = up ? sdinv[row] * y : y;
+            }
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);   // x stores before the flag store
+        if (lane == 0)
+            __hip_atomic_store(flag + row, need, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    // two register sets in turn: the loads of task t + nw fly while task t is processed
+    T av[kSfEnt], bv[kSfEnt];
+    int ac[kSfEnt], bc[kSfEnt];
+    task_load(wave, av, ac);
+    // (compiler fences keep the next task's loads issued before the current task's work)
+    for (int t = wave; t < ntask; t += 2 * nw) {
+        task_load(t + nw, bv, bc);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        process(t, av, ac);
+        if (t + nw >= ntask) break;
+        task_load(t + 2 * nw, av, ac);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        process(t + nw, bv, bc);
+    }
+    __syncthreads();
+    const bool bad = lds_flag(abort_flag) != 0;
+    for (int i = tid; i < n * c; i += nt) xg[i] = bad ? T(__builtin_nan("")) : x[i];
+}
+
 // Level schedule of a triangular CSR: forward (lower) or backward (upper).
 static void build_levels(int n, const std::vector<int> &rp, const std::vector<int> &ci,
                          bool forward, std::vector<int> &lev_ptr, std::vector<int> &lev_rows) {
@@ -129,9 +302,15 @@ static int upload_tri(int n, const std::vector<int> &rp, const std::vector<int> 
     T.m.n = n;
     T.m.nnz = rp[n];
     XT_TRY(T.m.rowptr.upload(rp.data(), sizeof(int) * (n + 1)));
-    XT_TRY(T.m.col.upload(ci.data(), sizeof(int) * ci.size()));
-    XT_TRY(T.m.val64.upload(v.data(), sizeof(double) * v.size()));
-    std::vector<float> v32(v.begin(), v.end());
+    // one zero entry past the end: the single-workgroup sweeps read it, branch-free, in
+    // place of a missing entry
+    std::vector<int> cp(ci);
+    std::vector<double> vp(v);
+    cp.push_back(0);
+    vp.push_back(0.0);
+    XT_TRY(T.m.col.upload(cp.data(), sizeof(int) * cp.size()));
+    XT_TRY(T.m.val64.upload(vp.data(), sizeof(double) * vp.size()));
+    std::vector<float> v32(vp.begin(), vp.end());
     XT_TRY(T.m.val32.upload(v32.data(), sizeof(float) * v32.size()));
     std::vector<int> lrows;
     build_levels(n, rp, ci, forward, T.lev_ptr, lrows);
@@ -175,6 +354,9 @@ static int run_tri(const Tri &tri, const T *vals, const T *dinv, int c, const T 
 constexpr long long kOneWgWork = 4LL << 20;
 // ... and keep the solution vector in LDS when it fits this many bytes.
 constexpr size_t kOneWgLds = 128 * 1024;
+// LDS of one workgroup (gfx950: 160 KB); the staged single-workgroup sweeps are used
+// when the solution and the level schedule fit.
+constexpr size_t kWgLdsMax = 160 * 1024;
 
 template <typename T>
 int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
@@ -190,13 +372,24 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     const T *lv = f32 ? (const T *)pr->L.m.val32.ptr : (const T *)pr->L.m.val64.ptr;
     const T *uv = f32 ? (const T *)pr->U.m.val32.ptr : (const T *)pr->U.m.val64.ptr;
     const T *dv = f32 ? (const T *)pr->dinv32.ptr : (const T *)pr->dinv64.ptr;
-    // Small factors: both sweeps in one workgroup (a barrier per level instead of a
-    // launch per level); large ones: one launch per level over the whole device.
+    // Small factors: both sweeps in one workgroup (row flags in LDS when the solution
+    // fits there, else a barrier per level) instead of a launch per level; large ones:
+    // one launch per level over the whole device.
     const long long work = (pr->L.m.nnz + pr->U.m.nnz + 2LL * pr->p) * c;
     if (work <= kOneWgWork) {
         const bool prof = prof_on(XT_PROF_TRSV);
         if (prof) prof_begin(XT_PROF_TRSV, s);
         const size_t xl = sizeof(T) * (size_t)pr->p * c;
+        const size_t sl = sf_lds_bytes<T>(pr->p, c);
+        if (sl <= kWgLdsMax) {
+            int S = 1;
+            while (S < 8 && S * 2 * c <= 64) S *= 2;   // entry slices per column: 1, 2, 4, 8
+            hipLaunchKernelGGL(trsv_sf_kernel<T>, dim3(1), dim3(1024), sl, s, pr->p, c, S,
+                               pr->L.lev_rows.as<int>(), pr->L.m.rowptr.as<int>(),
+                               pr->L.m.col.as<int>(), lv, (int)pr->L.m.nnz,
+                               pr->U.lev_rows.as<int>(), pr->U.m.rowptr.as<int>(),
+                               pr->U.m.col.as<int>(), uv, (int)pr->U.m.nnz, dv, b, y);
+        } else {
 #define XT_ONEWG(XLDS)                                                                      \
     hipLaunchKernelGGL((trsv_onewg_kernel<T, XLDS>), dim3(1), dim3(1024), XLDS ? xl : 0, s,  \
                        pr->p, (int)pr->L.lev_ptr.size() - 1, pr->L.lev_ptr_dev.as<int>(),     \
@@ -204,9 +397,10 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
                        pr->L.m.col.as<int>(), lv, (int)pr->U.lev_ptr.size() - 1,              \
                        pr->U.lev_ptr_dev.as<int>(), pr->U.lev_rows.as<int>(),                 \
                        pr->U.m.rowptr.as<int>(), pr->U.m.col.as<int>(), uv, dv, c, b, y)
-        if (xl <= kOneWgLds) XT_ONEWG(true);
-        else XT_ONEWG(false);
+            if (xl <= kOneWgLds) XT_ONEWG(true);
+            else XT_ONEWG(false);
 #undef XT_ONEWG
+        }
         if (prof)
             prof_end(XT_PROF_TRSV, s,
                      (double)(pr->L.m.nnz + pr->U.m.nnz) * (sizeof(T) + 4 + c * sizeof(T)) +
@@ -234,6 +428,10 @@ int xt_prec_create(int p, const double *alu, const int *jlu, const int *ju, xt_p
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOneWgLds));
     XT_HIP(hipFuncSetAttribute((const void *)trsv_onewg_kernel<float, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOneWgLds));
+    XT_HIP(hipFuncSetAttribute((const void *)trsv_sf_kernel<double>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWgLdsMax));
+    XT_HIP(hipFuncSetAttribute((const void *)trsv_sf_kernel<float>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWgLdsMax));
     XT_ARG(jlu[0] == p + 2, "MSR factors must come from ilut (jlu(1) = n+2)");
     std::vector<int> lrp(p + 1, 0), urp(p + 1, 0), lci, uci;
     std::vector<double> lv, uv, dinv(p);

@@ -107,6 +107,9 @@ struct SolverWorkspace {
     DevBuf V, Z;                     // Arnoldi basis, preconditioned basis (flexible)
     DevBuf w32a, w32b, z64, t64, d;  // work vectors
     DevBuf partial, hdev, hdev2, ndev, coef, frozen;
+    // Givens state on the device: Hessenberg columns, rotations, rhs of the least
+    // squares problem, targets, counters and the residual history
+    DevBuf ghh, gcs, gsn, grs, geps, gro, gits, gicyc, ghist;
 };
 SolverWorkspace *op_workspace(xt_op op);
 

@@ -115,6 +115,49 @@ scale_by_norm(long long n, int c, int q, const double *__restrict__ nrm2,
     v[idx] = (frozen[r] || t == 0.0) ? 0.0 : v[idx] * (1.0 / t);
 }
 
+// One Arnoldi step's Givens update per right-hand side (kronecker.f90:281-316), on
+// the device so the host checks convergence only every few steps: column i of the
+// Hessenberg matrix from the two Gram-Schmidt passes and the new norm, the previous
+// rotations applied, a new rotation, the least-squares right-hand side rotated, the
+// residual estimate |rs(i+1)| recorded; the right-hand side freezes when its cycle
+// ends (i + 1 = K or |rs(i+1)| <= eps1).  Frozen right-hand sides are left alone, so
+// steps run past a convergence change nothing.  Same arithmetic as the reference,
+// products and sums rounded one by one.
+#pragma clang fp contract(off)
+__global__ void __launch_bounds__(64)
+givens_kernel(int R, int i, int K, const double *__restrict__ hcol,
+              const double *__restrict__ hcol2, const double *__restrict__ nrm2, double *hh,
+              double *cs, double *sn, double *rs, int *frozen, const double *__restrict__ eps1,
+              int *its, int *icyc, double *ro_last, double *hist, double hist_ro0,
+              int hist_cap) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R || frozen[r]) return;
+    const int it = ++its[r];
+    double *H = hh + (size_t)r * (K + 1) * K;
+    auto h = [&](int a, int b) -> double & { return H[a + (size_t)b * (K + 1)]; };
+    double *c_ = cs + (size_t)r * K, *s_ = sn + (size_t)r * K, *rs_ = rs + (size_t)r * (K + 1);
+    for (int j = 0; j <= i; ++j) h(j, i) = hcol[j * R + r] + hcol2[j * R + r];
+    h(i + 1, i) = sqrt(nrm2[r]);
+    for (int kk = 1; kk <= i; ++kk) {
+        const double t = h(kk - 1, i);
+        h(kk - 1, i) = c_[kk - 1] * t + s_[kk - 1] * h(kk, i);
+        h(kk, i) = -s_[kk - 1] * t + c_[kk - 1] * h(kk, i);
+    }
+    double gam = sqrt(h(i, i) * h(i, i) + h(i + 1, i) * h(i + 1, i));
+    if (gam == 0.0) gam = 1.0e-16;
+    c_[i] = h(i, i) / gam;
+    s_[i] = h(i + 1, i) / gam;
+    rs_[i + 1] = -s_[i] * rs_[i];
+    rs_[i] = c_[i] * rs_[i];
+    h(i, i) = c_[i] * h(i, i) + s_[i] * h(i + 1, i);
+    const double ro = fabs(rs_[i + 1]);
+    ro_last[r] = ro;
+    if (hist && it < hist_cap) hist[it] = ro / hist_ro0;
+    icyc[r] = i + 1;
+    if (!(i + 1 < K && ro > eps1[r])) frozen[r] = 1;
+}
+#pragma clang fp contract(on)
+
 __global__ void __launch_bounds__(256) cvt_f64_f32(long long n, const double *a, float *b) {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < n) b[idx] = (float)a[idx];
@@ -193,10 +236,15 @@ struct Solver {
     DevBuf &w32a, &w32b;                 // FP32 work vectors
     DevBuf &z64, &t64;                   // FP64 work vectors
     DevBuf &partial, &hdev, &hdev2, &ndev, &coef, &frozen;
+    DevBuf &ghh, &gcs, &gsn, &grs, &geps, &gro, &gits, &gicyc, &ghist;
     explicit Solver(SolverWorkspace &w)
         : tm64(w.tm64), tm32(w.tm32), mi64(w.mi64), mi32(w.mi32), V(w.V), Z(w.Z), w32a(w.w32a),
           w32b(w.w32b), z64(w.z64), t64(w.t64), partial(w.partial), hdev(w.hdev),
-          hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), frozen(w.frozen) {}
+          hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), frozen(w.frozen), ghh(w.ghh), gcs(w.gcs),
+          gsn(w.gsn), grs(w.grs), geps(w.geps), gro(w.gro), gits(w.gits), gicyc(w.gicyc),
+          ghist(w.ghist) {}
+    // Arnoldi steps between host checks of the frozen flags
+    static constexpr int kCheck = 4;
     int nblk = 0, chunk = 0;
     std::vector<double> *hist = nullptr;  // R == 1: ro/ro0 per iteration (kronpgmres log)
     double hist_ro0 = 1.0;
@@ -223,6 +271,14 @@ struct Solver {
         XT_TRY(ndev.reserve(sizeof(double) * R));
         XT_TRY(coef.reserve(sizeof(double) * (im + 1) * R));
         XT_TRY(frozen.reserve(sizeof(int) * R));
+        XT_TRY(ghh.reserve(sizeof(double) * (size_t)R * (im + 1) * im));
+        XT_TRY(gcs.reserve(sizeof(double) * (size_t)R * im));
+        XT_TRY(gsn.reserve(sizeof(double) * (size_t)R * im));
+        XT_TRY(grs.reserve(sizeof(double) * (size_t)R * (im + 1)));
+        XT_TRY(geps.reserve(sizeof(double) * R));
+        XT_TRY(gro.reserve(sizeof(double) * R));
+        XT_TRY(gits.reserve(sizeof(int) * R));
+        XT_TRY(gicyc.reserve(sizeof(int) * R));
         return XT_OK;
     }
 
@@ -321,11 +377,15 @@ struct Solver {
         const int K = im;
         std::vector<int> ncyc(R, 0);
         std::vector<double> hh((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K),
-            rs((size_t)R * (K + 1)), hcol((size_t)(K + 1) * R), hcol2((size_t)(K + 1) * R);
+            rs((size_t)R * (K + 1));
         std::vector<int> icyc(R, 0), fz(R, 0);
         std::vector<double> nrm;
         auto HH = [&](int r, int a, int b) -> double & { return hh[(size_t)r * (K + 1) * K + a + (size_t)b * (K + 1)]; };
         for (int r = 0; r < R; ++r) conv[r] = 0;
+        const int hist_cap = maxits + K + 2;
+        if (hist) XT_TRY(ghist.reserve(sizeof(double) * hist_cap));
+        XT_HIP(hipMemcpyAsync(geps.ptr, eps1.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
+        std::vector<int> its_before(R);
         for (;;) {
             // start of a cycle: normalise V_0
             XT_TRY(norms2(Vj(0), nrm));
@@ -342,7 +402,14 @@ struct Solver {
                 icyc[r] = 0;
             }
             if (!any) break;
+            // the cycle's Givens state lives on the device; the host reads the frozen
+            // flags every kCheck steps and the whole state at the end of the cycle
+            its_before = its;
             XT_HIP(hipMemcpyAsync(frozen.ptr, fz.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            XT_HIP(hipMemcpyAsync(grs.ptr, rs.data(), sizeof(double) * rs.size(), hipMemcpyHostToDevice, s));
+            XT_HIP(hipMemcpyAsync(gits.ptr, its.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            XT_HIP(hipMemcpyAsync(gicyc.ptr, icyc.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            XT_HIP(hipMemcpyAsync(gro.ptr, ro_last.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
                                ndev.as<double>(), frozen.as<int>(), Vj(0));
             int i = 0;
@@ -359,40 +426,35 @@ struct Solver {
                 // (the restart recombination reads it); zero it for the others
                 hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
                                    ndev.as<double>(), frozen.as<int>(), Vj(i + 1));
-                XT_HIP(hipMemcpyAsync(hcol.data(), hdev.ptr, sizeof(double) * (i + 1) * R, hipMemcpyDeviceToHost, s));
-                XT_HIP(hipMemcpyAsync(hcol2.data(), hdev2.ptr, sizeof(double) * (i + 1) * R, hipMemcpyDeviceToHost, s));
-                nrm.resize(R);
-                XT_HIP(hipMemcpyAsync(nrm.data(), ndev.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
-                XT_HIP(hipStreamSynchronize(s));
-                bool cont = false;
-                for (int r = 0; r < R; ++r) {
-                    if (fz[r]) continue;
-                    its[r]++;
-                    for (int j = 0; j <= i; ++j) HH(r, j, i) = hcol[j * R + r] + hcol2[j * R + r];
-                    HH(r, i + 1, i) = std::sqrt(nrm[r]);
-                    double *c_ = &cs[(size_t)r * K], *s_ = &sn[(size_t)r * K], *rs_ = &rs[(size_t)r * (K + 1)];
-                    for (int kk = 1; kk <= i; ++kk) {
-                        const double t = HH(r, kk - 1, i);
-                        HH(r, kk - 1, i) = c_[kk - 1] * t + s_[kk - 1] * HH(r, kk, i);
-                        HH(r, kk, i) = -s_[kk - 1] * t + c_[kk - 1] * HH(r, kk, i);
-                    }
-                    double gam = std::sqrt(HH(r, i, i) * HH(r, i, i) + HH(r, i + 1, i) * HH(r, i + 1, i));
-                    if (gam == 0.0) gam = 1.0e-16;
-                    c_[i] = HH(r, i, i) / gam;
-                    s_[i] = HH(r, i + 1, i) / gam;
-                    rs_[i + 1] = -s_[i] * rs_[i];
-                    rs_[i] = c_[i] * rs_[i];
-                    HH(r, i, i) = c_[i] * HH(r, i, i) + s_[i] * HH(r, i + 1, i);
-                    const double ro = std::fabs(rs_[i + 1]);
-                    ro_last[r] = ro;
-                    if (hist) hist->push_back(ro / hist_ro0);
-                    icyc[r] = i + 1;
-                    if (i + 1 < K && ro > eps1[r]) cont = true;
-                    else fz[r] = 1;  // this RHS ends its cycle here
+                hipLaunchKernelGGL(givens_kernel, dim3(ceil_div(R, 64)), dim3(64), 0, s, R, i, K,
+                                   hdev.as<double>(), hdev2.as<double>(), ndev.as<double>(),
+                                   ghh.as<double>(), gcs.as<double>(), gsn.as<double>(),
+                                   grs.as<double>(), frozen.as<int>(), geps.as<double>(),
+                                   gits.as<int>(), gicyc.as<int>(), gro.as<double>(),
+                                   hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
+                XT_HIP(hipGetLastError());
+                if ((i + 1) % kCheck == 0 || i + 1 == K) {
+                    XT_HIP(hipMemcpyAsync(fz.data(), frozen.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+                    XT_HIP(hipStreamSynchronize(s));
+                    bool cont = false;
+                    for (int r = 0; r < R; ++r) cont = cont || !fz[r];
+                    if (!cont) break;
                 }
-                if (!cont) break;
-                XT_HIP(hipMemcpyAsync(frozen.ptr, fz.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
                 ++i;
+            }
+            XT_HIP(hipMemcpyAsync(hh.data(), ghh.ptr, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(cs.data(), gcs.ptr, sizeof(double) * cs.size(), hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(sn.data(), gsn.ptr, sizeof(double) * sn.size(), hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(rs.data(), grs.ptr, sizeof(double) * rs.size(), hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(icyc.data(), gicyc.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(its.data(), gits.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(ro_last.data(), gro.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+            XT_HIP(hipStreamSynchronize(s));
+            if (hist && its[0] > its_before[0]) {
+                std::vector<double> hv(its[0] - its_before[0]);
+                XT_HIP(hipMemcpy(hv.data(), ghist.as<double>() + its_before[0] + 1,
+                                 sizeof(double) * hv.size(), hipMemcpyDeviceToHost));
+                hist->insert(hist->end(), hv.begin(), hv.end());
             }
             // back substitution and solution update, per RHS with its own cycle length
             const int kmax = i + 1;

@@ -83,21 +83,24 @@ trsv_onewg_kernel(int n_, int nlevL, const int *__restrict__ lpL, const int *__r
 // starts a row as soon as the rows it reads are finished, told by per-row flags in LDS
 // (1: forward result, 2: backward result).  Every row depends only on rows earlier in
 // that order and each wave walks its rows in that order, so the earliest unfinished
-// row can always proceed.  A wave loads the factor entries of its next row into
-// registers while it waits for and computes the current one; the solution and the
-// inverse diagonal live in LDS.  Lanes: S slices of the row's entries times
-// 64/S column groups; the slices' partial sums are reduced by DPP lane moves in a fixed
-// order.  Waits are bounded: a wave that waits past kSpinMax polls raises an abort
+// row can always proceed.  Each row's leading entries sit in a fixed-size slot in
+// level order, so a wave loads its next two rows' entries with no dependent address
+// while it waits for and computes the current one; the solution, the inverse diagonal
+// and the flags live in LDS.  Lanes: S slices of the row's entries times 64/S column
+// groups; the slices' partial sums are reduced by DPP lane moves in a fixed order.  Waits are bounded: a wave that waits past kSpinMax polls raises an abort
 // flag that releases every wave, and the output is then NaN.
-constexpr int kSfEnt = 4;           // entries per lane held in registers
+constexpr int kSlot = 32;           // leading entries of a row kept in its fixed-size slot
+constexpr int kSfMaxRows = 4096;    // slots are built for factors up to this order
 constexpr int kSpinMax = 1 << 22;
 
+// LDS: x [(n + 1) c] (row n stays zero), inverse diagonal [n], flags [n + 1] (flag[n] = 3,
+// always satisfied), row pointers of L and U [n + 1] each, abort flag.
 template <typename T>
 static size_t sf_lds_bytes(int n, int c) {
-    return sizeof(T) * ((size_t)n * c + n) + sizeof(int) * (5 * (size_t)n + 3 + 1);
+    return sizeof(T) * ((size_t)(n + 1) * c + n) + sizeof(int) * (3 * (size_t)n + 4);
 }
 
-// Sum over aligned groups of S lanes (S = 1, 2, 4 or 8) with DPP lane moves: swap
+// Sum over aligned groups of S lanes (S = 4 or 8) with DPP lane moves: swap
 // neighbours, swap pairs within a quad, then mirror within the half-row of 8.
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(float v) {
@@ -110,8 +113,8 @@ __device__ __forceinline__ double dpp_move(double v) {
     const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-template <typename T>
-__device__ __forceinline__ T group_sum(T s, int S) {
+template <int S, typename T>
+__device__ __forceinline__ T group_sum(T s) {
     if (S >= 2) s += dpp_move<0xB1>(s);    // quad_perm [1,0,3,2]
     if (S >= 4) s += dpp_move<0x4E>(s);    // quad_perm [2,3,0,1]
     if (S >= 8) s += dpp_move<0x141>(s);   // row_half_mirror
@@ -122,46 +125,36 @@ __device__ __forceinline__ int lds_flag(const int *f) {
     return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Entries e0 + slice + j*S (j < kSfEnt) of one row; a position past the row reads the
-// zero entry at index nnz (whose column is 0).  Unconditional loads with no use in
-// this function, so they stay in flight while the wave works on its current row.
-template <typename T>
-__device__ __forceinline__ void sf_load(int e0, int e1, int nnz, int slice, int S,
-                                        const int *__restrict__ ci, const T *__restrict__ v,
-                                        T *pv, int *pc) {
-#pragma unroll
-    for (int j = 0; j < kSfEnt; ++j) {
-        const int e = e0 + slice + j * S;
-        const int es = e < e1 ? e : nnz;
-        pv[j] = v[es];
-        pc[j] = ci[es];
-    }
-}
+// One triangle as the single-workgroup sweep reads it (device pointers): rows in level
+// order (bit 30 set when the row has more than kSlot entries), the first kSlot entries
+// of each of those rows in its slot (columns past the row are n, values 0), and the
+// CSR for the entries past the slot.
+struct SfTri {
+    const int *trow, *sci, *rp, *ci;
+    const void *sv, *v;
+};
 
-template <typename T>
+template <typename T, int S>
 __global__ void __launch_bounds__(1024)
-trsv_sf_kernel(int n, int c, int S, const int *__restrict__ rowsL, const int *__restrict__ rpL,
-               const int *__restrict__ ciL, const T *__restrict__ vL, int nnzL,
-               const int *__restrict__ rowsU, const int *__restrict__ rpU,
-               const int *__restrict__ ciU, const T *__restrict__ vU, int nnzU,
-               const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ xg) {
+trsv_sf_kernel(int n, int c, SfTri L, SfTri U, const T *__restrict__ dinv,
+               const T *__restrict__ b, T *__restrict__ xg) {
+    constexpr int E = kSlot / S;    // slot entries per lane
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *x = reinterpret_cast<T *>(smem_raw);
-    T *sdinv = x + (size_t)n * c;
-    int *flag = reinterpret_cast<int *>(sdinv + n);             // [n + 1], flag[n] = 3
-    int *srL = flag + n + 1, *srU = srL + n, *spL = srU + n, *spU = spL + n + 1;
+    T *sdinv = x + (size_t)(n + 1) * c;
+    int *flag = reinterpret_cast<int *>(sdinv + n);
+    int *spL = flag + n + 1, *spU = spL + n + 1;
     int *abort_flag = spU + n + 1;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int i = tid; i < n * c; i += nt) x[i] = b[i];
+    for (int i = tid; i < c; i += nt) x[(size_t)n * c + i] = T(0);
     for (int i = tid; i < n; i += nt) {
         sdinv[i] = dinv[i];
         flag[i] = 0;
-        srL[i] = rowsL[i];
-        srU[i] = rowsU[i];
     }
     for (int i = tid; i <= n; i += nt) {
-        spL[i] = rpL[i];
-        spU[i] = rpU[i];
+        spL[i] = L.rp[i];
+        spU[i] = U.rp[i];
     }
     if (tid == 0) {
         flag[n] = 3;
@@ -170,39 +163,43 @@ trsv_sf_kernel(int n, int c, int S, const int *__restrict__ rowsL, const int *__
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6, ntask = 2 * n;
     const int slice = lane % S, cg = lane / S, ncg = 64 / S;
-    // task t < n: forward row srL[t]; t >= n: backward row srU[t - n]
-    // (branch-free: t is clamped to the last task, so the loads are issued on every path
-    // and the next task's loads need no wait before the current task's work)
-    auto task_load = [&](int t, T *vv, int *cc) {
+    // task t < n: forward row of level-ordered position t; t >= n: backward row at t - n.
+    // Branch-free loads (t clamped to the last task) with no use here, so they stay in
+    // flight while the wave works on earlier tasks.
+    auto task_load = [&](int t, int &row, T *pv, int *pc) {
         t = t < ntask ? t : ntask - 1;
         const bool up = t >= n;
-        const int *sr = up ? srU : srL, *rp = up ? spU : spL;
-        const int row = sr[up ? t - n : t];
-        sf_load<T>(rp[row], rp[row + 1], up ? nnzU : nnzL, slice, S, up ? ciU : ciL,
-                   up ? vU : vL, vv, cc);
+        const int k = up ? t - n : t;
+        const int *trow = up ? U.trow : L.trow, *sci = up ? U.sci : L.sci;
+        const T *sv = static_cast<const T *>(up ? U.sv : L.sv);
+        row = trow[k];
+        const size_t base = (size_t)k * kSlot + slice;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pv[j] = sv[base + j * S];
+            pc[j] = sci[base + j * S];
+        }
     };
-    auto process = [&](int t, const T *pv, const int *pc) {
+    auto process = [&](int t, int rowf, const T *pv, const int *pc) {
         const bool up = t >= n;
-        const int row = up ? srU[t - n] : srL[t];
-        const int *rp = up ? spU : spL;
-        const int *ci = up ? ciU : ciL;
-        const T *v = up ? vU : vL;
+        const int row = rowf & 0x3fffffff;
+        const bool longrow = (rowf >> 30) != 0;
         const int need = up ? 2 : 1;
-        const int e0 = rp[row], e1 = rp[row + 1];
-        const bool longrow = e1 - e0 > kSfEnt * S;
+        const int *rp = up ? spU : spL;
+        const int *ci = up ? U.ci : L.ci;
+        const T *v = static_cast<const T *>(up ? U.v : L.v);
         // wait for the rows this one reads (all flag reads issued together)
         for (int spins = 0;; ++spins) {
-            int f[kSfEnt];
+            int f[E];
 #pragma unroll
-            for (int j = 0; j < kSfEnt; ++j)
-                f[j] = lds_flag(flag + (e0 + slice + j * S < e1 ? pc[j] : n));
+            for (int j = 0; j < E; ++j) f[j] = lds_flag(flag + pc[j]);
             const int fo = up ? lds_flag(flag + row) : 1;
             const int ab = lds_flag(abort_flag);
             bool ok = fo >= 1;
 #pragma unroll
-            for (int j = 0; j < kSfEnt; ++j) ok = ok && f[j] >= need;
+            for (int j = 0; j < E; ++j) ok = ok && f[j] >= need;
             if (longrow)
-                for (int e = e0 + slice + kSfEnt * S; e < e1; e += S)
+                for (int e = rp[row] + kSlot + slice; e < rp[row + 1]; e += S)
                     ok = ok && lds_flag(flag + ci[e]) >= need;
             if (__all(ok) || ab) break;
             if (spins >= kSpinMax) {
@@ -211,40 +208,42 @@ trsv_sf_kernel(int n, int c, int S, const int *__restrict__ rowsL, const int *__
             }
         }
         // a wave's LDS operations complete in order: a compiler fence keeps the x reads
-        // after the flag reads (a workgroup fence would also wait for the next row's
-        // global loads)
+        // after the flag reads (a workgroup fence would also wait for the prefetches)
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         for (int col = cg; col < c; col += ncg) {
+            const size_t o = (size_t)row * c + col;
+            const T xo = x[o];
             T s = 0;
 #pragma unroll
-            for (int j = 0; j < kSfEnt; ++j) s += pv[j] * x[(size_t)pc[j] * c + col];
+            for (int j = 0; j < E; ++j) s += pv[j] * x[(size_t)pc[j] * c + col];
             if (longrow)
-                for (int e = e0 + slice + kSfEnt * S; e < e1; e += S)
+                for (int e = rp[row] + kSlot + slice; e < rp[row + 1]; e += S)
                     s += v[e] * x[(size_t)ci[e] * c + col];
-            s = group_sum(s, S);
-            if (slice == 0) {
-                const size_t o = (size_t)row * c + col;
-                const T y = x[o] - s;
-                x[o] = up ? sdinv[row] * y : y;
-            }
+            s = group_sum<S>(s);
+            if (slice == 0) x[o] = up ? sdinv[row] * (xo - s) : xo - s;
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);   // x stores before the flag store
         if (lane == 0)
             __hip_atomic_store(flag + row, need, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
-    // two register sets in turn: the loads of task t + nw fly while task t is processed
-    T av[kSfEnt], bv[kSfEnt];
-    int ac[kSfEnt], bc[kSfEnt];
-    task_load(wave, av, ac);
-    // (compiler fences keep the next task's loads issued before the current task's work)
-    for (int t = wave; t < ntask; t += 2 * nw) {
-        task_load(t + nw, bv, bc);
+    // three register sets in turn: the loads of tasks t + nw and t + 2 nw fly while task
+    // t is processed (compiler fences keep them issued before the current task's work)
+    T av[E], bv[E], cv[E];
+    int ac[E], bc[E], cc[E], ar, br, cr;
+    task_load(wave, ar, av, ac);
+    task_load(wave + nw, br, bv, bc);
+    for (int t = wave; t < ntask; t += 3 * nw) {
+        task_load(t + 2 * nw, cr, cv, cc);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        process(t, av, ac);
+        process(t, ar, av, ac);
         if (t + nw >= ntask) break;
-        task_load(t + 2 * nw, av, ac);
+        task_load(t + 3 * nw, ar, av, ac);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        process(t + nw, bv, bc);
+        process(t + nw, br, bv, bc);
+        if (t + 2 * nw >= ntask) break;
+        task_load(t + 4 * nw, br, bv, bc);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        process(t + 2 * nw, cr, cv, cc);
     }
     __syncthreads();
     const bool bad = lds_flag(abort_flag) != 0;
@@ -280,6 +279,9 @@ struct Tri {
     std::vector<int> lev_ptr;    // host copy of level boundaries
     DevBuf lev_ptr_dev;          // device copy (single-workgroup sweeps)
     std::vector<long long> lev_nnz;  // entries per level (algorithmic bytes)
+    // single-workgroup sweeps (small factors only): level-ordered rows with the long-row
+    // bit, fixed-size entry slots
+    DevBuf task_row, slot_ci, slot_v64, slot_v32;
 };
 
 }  // namespace xt
@@ -320,6 +322,23 @@ static int upload_tri(int n, const std::vector<int> &rp, const std::vector<int> 
         for (int k = T.lev_ptr[l]; k < T.lev_ptr[l + 1]; ++k)
             T.lev_nnz[l] += rp[lrows[k] + 1] - rp[lrows[k]];
     XT_TRY(T.lev_rows.upload(lrows.data(), sizeof(int) * n));
+    if (n <= kSfMaxRows) {
+        std::vector<int> trow(n), sci((size_t)n * kSlot, n);
+        std::vector<double> sv((size_t)n * kSlot, 0.0);
+        for (int k = 0; k < n; ++k) {
+            const int r = lrows[k], len = rp[r + 1] - rp[r];
+            trow[k] = r | (len > kSlot ? (1 << 30) : 0);
+            for (int q = 0; q < std::min(len, kSlot); ++q) {
+                sci[(size_t)k * kSlot + q] = ci[rp[r] + q];
+                sv[(size_t)k * kSlot + q] = v[rp[r] + q];
+            }
+        }
+        std::vector<float> sv32(sv.begin(), sv.end());
+        XT_TRY(T.task_row.upload(trow.data(), sizeof(int) * n));
+        XT_TRY(T.slot_ci.upload(sci.data(), sizeof(int) * sci.size()));
+        XT_TRY(T.slot_v64.upload(sv.data(), sizeof(double) * sv.size()));
+        XT_TRY(T.slot_v32.upload(sv32.data(), sizeof(float) * sv32.size()));
+    }
     XT_HIP(hipDeviceSynchronize());
     return XT_OK;
 }
@@ -381,14 +400,24 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
         if (prof) prof_begin(XT_PROF_TRSV, s);
         const size_t xl = sizeof(T) * (size_t)pr->p * c;
         const size_t sl = sf_lds_bytes<T>(pr->p, c);
-        if (sl <= kWgLdsMax) {
-            int S = 1;
-            while (S < 8 && S * 2 * c <= 64) S *= 2;   // entry slices per column: 1, 2, 4, 8
-            hipLaunchKernelGGL(trsv_sf_kernel<T>, dim3(1), dim3(1024), sl, s, pr->p, c, S,
-                               pr->L.lev_rows.as<int>(), pr->L.m.rowptr.as<int>(),
-                               pr->L.m.col.as<int>(), lv, (int)pr->L.m.nnz,
-                               pr->U.lev_rows.as<int>(), pr->U.m.rowptr.as<int>(),
-                               pr->U.m.col.as<int>(), uv, (int)pr->U.m.nnz, dv, b, y);
+        const int S = c <= 8 ? 8 : c <= 16 ? 4 : 0;   // entry slices per column
+        if (S && sl <= kWgLdsMax && pr->L.slot_ci.ptr && pr->U.slot_ci.ptr) {
+            auto tri = [&](const Tri &t, const T *vals) {
+                SfTri w;
+                w.trow = t.task_row.as<int>();
+                w.sci = t.slot_ci.as<int>();
+                w.sv = f32 ? t.slot_v32.ptr : t.slot_v64.ptr;
+                w.rp = t.m.rowptr.as<int>();
+                w.ci = t.m.col.as<int>();
+                w.v = vals;
+                return w;
+            };
+#define XT_SF(SS)                                                                           \
+    hipLaunchKernelGGL((trsv_sf_kernel<T, SS>), dim3(1), dim3(1024), sl, s, pr->p, c,       \
+                       tri(pr->L, lv), tri(pr->U, uv), dv, b, y)
+            if (S == 8) XT_SF(8);
+            else XT_SF(4);
+#undef XT_SF
         } else {
 #define XT_ONEWG(XLDS)                                                                      \
     hipLaunchKernelGGL((trsv_onewg_kernel<T, XLDS>), dim3(1), dim3(1024), XLDS ? xl : 0, s,  \
@@ -428,10 +457,12 @@ int xt_prec_create(int p, const double *alu, const int *jlu, const int *ju, xt_p
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOneWgLds));
     XT_HIP(hipFuncSetAttribute((const void *)trsv_onewg_kernel<float, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOneWgLds));
-    XT_HIP(hipFuncSetAttribute((const void *)trsv_sf_kernel<double>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWgLdsMax));
-    XT_HIP(hipFuncSetAttribute((const void *)trsv_sf_kernel<float>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWgLdsMax));
+#define XT_SF_ATTR(T, SS)                                                                   \
+    XT_HIP(hipFuncSetAttribute((const void *)trsv_sf_kernel<T, SS>,                          \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWgLdsMax))
+    XT_SF_ATTR(double, 8); XT_SF_ATTR(double, 4);
+    XT_SF_ATTR(float, 8); XT_SF_ATTR(float, 4);
+#undef XT_SF_ATTR
     XT_ARG(jlu[0] == p + 2, "MSR factors must come from ilut (jlu(1) = n+2)");
     std::vector<int> lrp(p + 1, 0), urp(p + 1, 0), lci, uci;
     std::vector<double> lv, uv, dinv(p);

@@ -243,8 +243,10 @@ struct Solver {
           hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), frozen(w.frozen), ghh(w.ghh), gcs(w.gcs),
           gsn(w.gsn), grs(w.grs), geps(w.geps), gro(w.gro), gits(w.gits), gicyc(w.gicyc),
           ghist(w.ghist) {}
-    // Arnoldi steps between host checks of the frozen flags
-    static constexpr int kCheck = 4;
+    // Arnoldi steps between host checks of the frozen flags: a few for small systems,
+    // where a step costs less than the host round trip; every step for large ones, where
+    // a step past convergence would cost more than the check
+    int check_every() const { return n <= (1LL << 16) ? 4 : 1; }
     int nblk = 0, chunk = 0;
     std::vector<double> *hist = nullptr;  // R == 1: ro/ro0 per iteration (kronpgmres log)
     double hist_ro0 = 1.0;
@@ -433,7 +435,7 @@ struct Solver {
                                    gits.as<int>(), gicyc.as<int>(), gro.as<double>(),
                                    hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
                 XT_HIP(hipGetLastError());
-                if ((i + 1) % kCheck == 0 || i + 1 == K) {
+                if ((i + 1) % check_every() == 0 || i + 1 == K) {
                     XT_HIP(hipMemcpyAsync(fz.data(), frozen.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
                     XT_HIP(hipStreamSynchronize(s));
                     bool cont = false;

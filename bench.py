@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-damage", action="store_true", help="skip the damage-kernel leg")
     ap.add_argument("--damage-nip", type=int, default=2560,
                     help="temporal points per slab of the damage leg")
+    ap.add_argument("--rod-only", action="store_true",
+                    help="only run the prismatic-rod ILUT leg (K timed slabs) and print it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -514,6 +516,9 @@ def main():
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.rod_only:
+        print(json.dumps(rod_leg(args.steps)))
+        return
     R, K, W = args.cases, args.steps, args.warmup
     nx, ny, nz = args.grid
     t0 = time.perf_counter()

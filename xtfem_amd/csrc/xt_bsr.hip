@@ -305,6 +305,140 @@ kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
     }
 }
 
+// Sum over aligned groups of 4 or 8 lanes with DPP moves (swap neighbours, swap pairs
+// in a quad, mirror in the half-row): every lane of the group ends with the group's sum.
+template <int CTRL>
+__device__ __forceinline__ float quad_move(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double quad_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int SPL, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    v += quad_move<0xB1>(v);                     // quad_perm [1,0,3,2]
+    v += quad_move<0x4E>(v);                     // quad_perm [2,3,0,1]
+    if (SPL == 8) v += quad_move<0x141>(v);      // row_half_mirror
+    return v;
+}
+
+// One load case (R = 1): SPL lanes per node row, lane `sub` taking the slot's entries
+// sub, sub + SPL, ...; the partial K X and M X sums are added across the SPL lanes, and
+// lane i < 3 applies the temporal matrices to row i of the node.  SPL times the waves
+// of the lane-per-node kernel, and 1/SPL of its dependent load chain per lane, for the
+// small launches where that chain is the bound.
+template <typename T, int SPL>
+__global__ void __launch_bounds__(256)
+kron_bsr3_r1_kernel(int nslices, const int *__restrict__ slot_node,
+                    const int *__restrict__ slice_base, const int *__restrict__ slice_len,
+                    const int *__restrict__ bcol, const T *__restrict__ bval,
+                    const T *__restrict__ tmat, T alpha, T beta, const T *__restrict__ x,
+                    T *__restrict__ y) {
+    using V2 = typename Vec2<T>::type;
+    constexpr int NPW = 64 / SPL, c = 6;
+    const int lane = threadIdx.x & 63, sub = lane % SPL;
+    const long long blk = xcd_block(blockIdx.x, gridDim.x);
+    const long long wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long slot = wave * NPW + lane / SPL;
+    const int s = (int)(slot / kSlice);
+    if (s >= nslices) return;   // whole quads leave together
+    const int k = (int)(slot - (long long)s * kSlice);
+    const int node = slot_node[slot];
+    const int L = slice_len[s];
+    const long long b0 = slice_base[s];
+    T aK[3][6], aM[3][6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) { aK[i][j] = T(0); aM[i][j] = T(0); }
+    auto accumulate = [&](const T (&kv)[12], const T (&xv)[3][6]) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                aK[i][j] += kv[3 * i + 0] * xv[0][j] + kv[3 * i + 1] * xv[1][j] + kv[3 * i + 2] * xv[2][j];
+                aM[i][j] += kv[9 + i] * xv[i][j];
+            }
+    };
+    auto gather = [&](int m, T (&xv)[3][6]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T *xr = x + (size_t)(3 * m + d) * c;
+#pragma unroll
+            for (int j = 0; j < 6; j += 2) {
+                const V2 v = *reinterpret_cast<const V2 *>(xr + j);
+                xv[d][j] = v.x;
+                xv[d][j + 1] = v.y;
+            }
+        }
+    };
+    constexpr int UNR = sizeof(T) == 4 ? 4 : 2;
+    int e = sub;
+    for (; e + (UNR - 1) * SPL < L; e += UNR * SPL) {
+        int m[UNR];
+        T kv[UNR][12], xv[UNR][3][6];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const long long idx = b0 + (long long)(e + u * SPL) * kSlice + k;
+            m[u] = bcol[idx];
+            load_block(bval + idx * 12, kv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) gather(m[u], xv[u]);
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) accumulate(kv[u], xv[u]);
+    }
+    for (; e < L; e += SPL) {
+        const long long idx = b0 + (long long)e * kSlice + k;
+        T kv[12], xv[3][6];
+        load_block(bval + idx * 12, kv);
+        gather(bcol[idx], xv);
+        accumulate(kv, xv);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            aK[i][j] = group_sum<SPL>(aK[i][j]);
+            aM[i][j] = group_sum<SPL>(aM[i][j]);
+        }
+    if (node < 0 || sub >= 3) return;
+    const T *TA = tmat, *TB = tmat + 36;
+    T out[6];
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+        T sacc = T(0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            // row sub of the node: select without dynamic register indexing
+            const T kk = sub == 0 ? aK[0][j] : sub == 1 ? aK[1][j] : aK[2][j];
+            const T mm = sub == 0 ? aM[0][j] : sub == 1 ? aM[1][j] : aM[2][j];
+            sacc += TA[o * 6 + j] * kk + TB[o * 6 + j] * mm;
+        }
+        out[o] = sacc;
+    }
+    T *yr = y + (size_t)(3 * node + sub) * c;
+#pragma unroll
+    for (int o = 0; o < 6; o += 2) {
+        V2 v;
+        if (beta == T(0)) {
+            v.x = alpha * out[o];
+            v.y = alpha * out[o + 1];
+        } else {
+            const V2 old = *reinterpret_cast<const V2 *>(yr + o);
+            v.x = alpha * out[o] + beta * old.x;
+            v.y = alpha * out[o + 1] + beta * old.y;
+        }
+        *reinterpret_cast<V2 *>(yr + o) = v;
+    }
+}
+
+static constexpr int kR1Split = 4;   // lanes per node row for one load case (8: slower)
+
 template <typename T>
 int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta, const T *x,
               T *y, bool f32, hipStream_t s) {
@@ -324,7 +458,14 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
                        P.slot_node.as<int>(), P.slice_base.as<int>(), P.slice_len.as<int>(), \
                        P.bcol.as<int>(), bv, R, r0, c, tmat, (T)alpha, (T)beta, x, y)
         if (prof_on(XT_PROF_SPMM)) prof_begin(XT_PROF_SPMM, s);
-        switch (lpn) {
+        if (R == 1) {   // kR1Split lanes per node
+            const int grid1 = (int)(((slots * kR1Split / 64) + 7) / 8 * 8);
+            hipLaunchKernelGGL((kron_bsr3_r1_kernel<T, kR1Split>), dim3(grid1), dim3(64), 0, s,
+                               P.nslices,
+                               P.slot_node.as<int>(), P.slice_base.as<int>(),
+                               P.slice_len.as<int>(), P.bcol.as<int>(), bv, tmat, (T)alpha,
+                               (T)beta, x, y);
+        } else switch (lpn) {
             case 1: XT_BSR_LAUNCH(1); break;
             case 2: XT_BSR_LAUNCH(2); break;
             case 4: XT_BSR_LAUNCH(4); break;

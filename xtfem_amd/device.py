@@ -59,7 +59,7 @@ class KronOperator:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and lib is not None:   # (module teardown clears lib)
             lib().xt_op_destroy(h)
             self.h = None
 
@@ -144,7 +144,7 @@ class IlutPreconditioner:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and lib is not None:   # (module teardown clears lib)
             lib().xt_prec_destroy(h)
             self.h = None
 

@@ -180,6 +180,11 @@ long long xt_op_nnz(xt_op op, int k);
 int xt_op_set_format(xt_op op, int format);
 /* Format used by xt_op_apply: 0 node blocks, 1 CSR. */
 int xt_op_format(xt_op op);
+/* New values of every term, in the CSR pattern the operator was created with (device
+ * arrays of nnz(k) doubles, e.g. from xt_asm_values): both copies and the node blocks
+ * are updated on the device.  No counterpart in the reference, which rebuilds its
+ * matrices every slab (main3d.f90:752-783). */
+int xt_op_set_values(xt_op op, const double *const *vals, void *stream);
 /* Stored node blocks (real != 0: without slice padding); 0 without node-block form. */
 long long xt_op_blocks(xt_op op, int real);
 
@@ -202,10 +207,48 @@ void xt_prec_destroy(xt_prec pr);
  * device.  XT_E_SINGULAR if K is not positive definite. */
 int xt_prec_create_band(int p, const int *rowptr, const int *colidx, const double *vals,
                         int block, xt_prec *out);
+/* Refactor a banded-factor preconditioner with new values of K (1-based CSR in device
+ * memory, within the band it was created for: XT_E_ARG otherwise).  Replaces the
+ * per-slab refactorization mumps_solver(job = 1), main3d.f90:783. */
+int xt_prec_refactor_band(xt_prec pr, const int *rowptr, const int *colidx,
+                          const double *vals, void *stream);
 /* Block size, number of blocks and bandwidth of a banded-factor preconditioner. */
 int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth);
 /* Sequential steps of the two sweeps: ILUT levels, or blocks of a banded factor. */
 int xt_prec_levels(xt_prec pr, int *nlev_lower, int *nlev_upper);
+
+/* ---- spatial assembly on the device ------------------------------------------ */
+
+typedef struct xt_asm_s *xt_asm;      /* mesh, global patterns, element buffers  */
+
+/* Symbolic assembly of a C3D8 mesh, once per mesh: xcoord [nn][3] and xconn [ne][8]
+ * (1-based node ids, shape3x node order) row-major host arrays; fixed [3*nn] nonzero
+ * for the constrained DOFs (DOF 3*(node-1) + component).  Builds the global patterns
+ * of K (all couplings of each element's DOFs) and M (equal components only) and, for
+ * each global entry, its element contributions in element order.  XT_E_ARG for an
+ * element with a non-positive Jacobian determinant. */
+int xt_asm_create(int nn, int ne, const double *xcoord, const int *xconn, const int *fixed,
+                  xt_asm *out);
+void xt_asm_destroy(xt_asm h);
+/* Host copy of a pattern (which 0: K, 1: M): nnz, then 1-based rowptr [3*nn+1] and
+ * colidx [nnz] when non-NULL. */
+int xt_asm_pattern(xt_asm h, int which, long long *nnz, int *rowptr, int *colidx);
+/* Numeric assembly on the device for per-Gauss-point E (ym) and nu (device [8*ne],
+ * element-major) and density rho: the element matrices of elemtx_m
+ * (elesubs.f90:2-58, stiffness per point as setsdstiff builds it), summed per entry in
+ * element order as tricsr does (spmtx.f90:359-407), the 1-0 method of csrapplybc
+ * (spmtx.f90:268-307) on the fixed DOFs, and the shape-derivative tables Nx, Ny, Nz
+ * [8*ne][8].  Replaces mpiglbmtx_m + tricsr + csrapplybc (mpisubs.f90:69-139,
+ * main3d.f90:752-781).  Bit-identical values; entries that sum to zero stay in the
+ * pattern (the reference drops them). */
+int xt_asm_run(xt_asm h, const double *ym, const double *nu, double rho, void *stream);
+/* Device arrays of the last xt_asm_run: which 0 K values, 1 M values (in the patterns'
+ * order), 2/3/4 Nx/Ny/Nz. */
+int xt_asm_values(xt_asm h, int which, const double **vals);
+/* Copy array `which` (as xt_asm_values) into device memory dst (nnz or 64*ne doubles). */
+int xt_asm_copy(xt_asm h, int which, double *dst, void *stream);
+/* Device copies of a pattern's 1-based column indices and 0-based rows of its entries. */
+int xt_asm_device_pattern(xt_asm h, int which, const int **colidx, const int **row);
 
 /* Y = (LU)^-1 X Minv_r^T on R right-hand sides (Minv: device [R][q][q],
  * row-major, the inverse temporal matrix; NULL = identity).  x and y must not

@@ -52,6 +52,15 @@ _SIGS = {
     "xt_op_format": (ci, [vp]),
     "xt_op_blocks": (C.c_longlong, [vp, ci]),
     "xt_op_apply": (ci, [vp, ci, vp, cd, cd, vp, vp, ci, vp]),
+    "xt_op_set_values": (ci, [vp, vp, vp]),
+    "xt_prec_refactor_band": (ci, [vp, vp, vp, vp, vp]),
+    "xt_asm_create": (ci, [ci, ci, vp, vp, vp, vp]),
+    "xt_asm_destroy": (None, [vp]),
+    "xt_asm_pattern": (ci, [vp, ci, vp, vp, vp]),
+    "xt_asm_run": (ci, [vp, vp, vp, cd, vp]),
+    "xt_asm_values": (ci, [vp, ci, vp]),
+    "xt_asm_copy": (ci, [vp, ci, vp, vp]),
+    "xt_asm_device_pattern": (ci, [vp, ci, vp, vp]),
     "xt_prec_create": (ci, [ci, vp, vp, vp, vp]),
     "xt_prec_create_band": (ci, [ci, vp, vp, vp, ci, vp]),
     "xt_prec_band_info": (ci, [vp, vp, vp, vp]),

@@ -21,7 +21,7 @@ ARCH = os.environ.get("XTFEM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["xt_kron.hip", "xt_bsr.hip", "xt_prec.hip", "xt_band.hip", "xt_gmres.hip",
            "xt_damage.hip", "xt_capi.hip", "xt_prof.hip", "xt_ilut.cpp",
-           "xt_delete.cpp"]
+           "xt_delete.cpp", "xt_asm.hip"]
 # rocSOLVER / rocBLAS: dense block factorization of the banded preconditioner (setup only).
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 

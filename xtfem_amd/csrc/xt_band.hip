@@ -334,7 +334,7 @@ fill_zero_kernel(long long n, T *d) {
 // beyond p get a unit diagonal.  The blocks are zero on entry.
 __global__ void __launch_bounds__(256)
 band_scatter_kernel(int p, int b, int nb, const int *__restrict__ ia, const int *__restrict__ ja,
-                    const double *__restrict__ a, double *kd, double *ko) {
+                    const double *__restrict__ a, double *kd, double *ko, int *outside) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const long long bb = (long long)b * b;
     if (i >= nb * b) return;
@@ -346,6 +346,10 @@ band_scatter_kernel(int p, int b, int nb, const int *__restrict__ ia, const int 
     for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
         const int j = ja[k] - 1;
         if (j > i) continue;
+        if (j < 0 || i - j > b) {   // not in the band this factor was sized for
+            outside[0] = 1;
+            continue;
+        }
         const int bj = j / b, lj = j - bj * b;
         double *dst = bi == bj ? kd + bi * bb : ko + bj * bb;   // bi == bj + 1 otherwise
         dst[(long long)lj * b + li] = a[k];
@@ -367,15 +371,12 @@ struct RbHandle {
     ~RbHandle() { if (h) rocblas_destroy_handle(h); }
 };
 
-static int factor(BandPrec &bp, const int *ia, const int *ja, const double *a, hipStream_t s) {
+// Block Cholesky of K given as a 1-based CSR in device memory (lower triangle read).
+static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double *da,
+                      hipStream_t s) {
     const int b = bp.b, nb = bp.nb, p = bp.p;
     const long long bb = (long long)b * b;
-    DevBuf L, E, Dl, G, H, info, dia, dja, da;
-    // the blocks are assembled on the device from the CSR (a few % of their bytes)
-    const long long nnz = ia[p] - 1;
-    XT_TRY(dia.upload(ia, sizeof(int) * (p + 1), s));
-    XT_TRY(dja.upload(ja, sizeof(int) * nnz, s));
-    XT_TRY(da.upload(a, sizeof(double) * nnz, s));
+    DevBuf L, E, Dl, G, H, info;
     XT_TRY(L.reserve(sizeof(double) * bb * nb));
     hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, bb * nb,
                        L.as<double>());
@@ -384,11 +385,21 @@ static int factor(BandPrec &bp, const int *ia, const int *ja, const double *a, h
         hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, bb * (nb - 1),
                            E.as<double>());
     }
-    hipLaunchKernelGGL(band_scatter_kernel, dim3(ceil_div((long long)nb * b, 256)), dim3(256), 0,
-                       s, p, b, nb, dia.as<int>(), dja.as<int>(), da.as<double>(), L.as<double>(),
-                       nb > 1 ? E.as<double>() : nullptr);
-    XT_HIP(hipGetLastError());
     XT_TRY(info.reserve(sizeof(int) * (nb + 1)));
+    XT_HIP(hipMemsetAsync(info.ptr, 0, sizeof(int) * (nb + 1), s));
+    hipLaunchKernelGGL(band_scatter_kernel, dim3(ceil_div((long long)nb * b, 256)), dim3(256), 0,
+                       s, p, b, nb, dia, dja, da, L.as<double>(), nb > 1 ? E.as<double>() : nullptr,
+                       info.as<int>() + nb);
+    XT_HIP(hipGetLastError());
+    {
+        int outside = 0;
+        XT_HIP(hipMemcpyAsync(&outside, info.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+        XT_HIP(hipStreamSynchronize(s));
+        if (outside) {
+            set_error("K has entries outside the band of width %d the factor was built for", b);
+            return XT_E_ARG;
+        }
+    }
     XT_HIP(hipMemsetAsync(info.ptr, 0, sizeof(int) * (nb + 1), s));
     RbHandle rb;
     XT_RB(rocblas_create_handle(&rb.h));
@@ -460,6 +471,23 @@ static int factor(BandPrec &bp, const int *ia, const int *ja, const double *a, h
     }
     XT_HIP(hipStreamSynchronize(s));
     return XT_OK;
+}
+
+// Host CSR: uploaded, then assembled into the blocks on the device (a few % of their bytes).
+static int factor(BandPrec &bp, const int *ia, const int *ja, const double *a, hipStream_t s) {
+    const long long nnz = ia[bp.p] - 1;
+    DevBuf dia, dja, da;
+    XT_TRY(dia.upload(ia, sizeof(int) * (bp.p + 1), s));
+    XT_TRY(dja.upload(ja, sizeof(int) * nnz, s));
+    XT_TRY(da.upload(a, sizeof(double) * nnz, s));
+    return factor_dev(bp, dia.as<int>(), dja.as<int>(), da.as<double>(), s);
+}
+
+// Refactor with new values of a K whose lower-triangle pattern fits the existing band
+// (the same mesh, e.g. new material or boundary conditions), CSR in device memory.
+int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da,
+                  hipStream_t s) {
+    return factor_dev(*bp, dia, dja, da, s);
 }
 
 int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out) {

@@ -42,6 +42,8 @@ struct BsrPlan {
     DevBuf slice_len;       // [nslices] blocks per slot
     DevBuf bcol;            // [nblocks] column node
     DevBuf bval32, bval64;  // [nblocks][12]
+    DevBuf kmap, mmap;      // value position in bval of every CSR entry of K and of M
+    long long nk = 0, nm = 0;
 };
 
 static constexpr int kSlice = 64;
@@ -96,10 +98,11 @@ int bsr_build(int p, const int *krp, const int *kci, const double *kv, const int
         base[s] = (int)nb;
         nb += (long long)len[s] * kSlice;
     }
-    if (nb > 0x7fffffffLL) return XT_E_UNSUPPORTED;
+    if (nb * 12 > 0x7fffffffLL) return XT_E_UNSUPPORTED;
     std::vector<int> bcol(nb);
     std::vector<double> bval((size_t)nb * 12, 0.0);
     std::vector<int> pos(nn, -1);  // position of column node b in the current row
+    std::vector<int> kmap(krp[p]), mmap(mrp[p]);
     long long nreal = 0;
     for (int s = 0; s < nslices; ++s)
         for (int k = 0; k < kSlice; ++k) {
@@ -137,11 +140,13 @@ int bsr_build(int p, const int *krp, const int *kci, const double *kv, const int
                     const int b = kci[e] / 3, cd = kci[e] % 3;
                     const long long idx = (long long)base[s] + (long long)pos[b] * kSlice + k;
                     bval[idx * 12 + d * 3 + cd] = kv[e];
+                    kmap[e] = (int)(idx * 12 + d * 3 + cd);
                 }
                 for (int e = mrp[i]; e < mrp[i + 1]; ++e) {
                     const int b = mci[e] / 3;
                     const long long idx = (long long)base[s] + (long long)pos[b] * kSlice + k;
                     bval[idx * 12 + 9 + d] = mv[e];
+                    mmap[e] = (int)(idx * 12 + 9 + d);
                 }
             }
             for (int b : c) pos[b] = -1;
@@ -154,6 +159,10 @@ int bsr_build(int p, const int *krp, const int *kci, const double *kv, const int
     XT_TRY(P.slice_base.upload(base.data(), sizeof(int) * nslices));
     XT_TRY(P.slice_len.upload(len.data(), sizeof(int) * nslices));
     XT_TRY(P.bcol.upload(bcol.data(), sizeof(int) * nb));
+    P.nk = krp[p];
+    P.nm = mrp[p];
+    if (P.nk) XT_TRY(P.kmap.upload(kmap.data(), sizeof(int) * P.nk));
+    if (P.nm) XT_TRY(P.mmap.upload(mmap.data(), sizeof(int) * P.nm));
     XT_TRY(P.bval64.upload(bval.data(), sizeof(double) * bval.size()));
     std::vector<float> b32(bval.begin(), bval.end());
     XT_TRY(P.bval32.upload(b32.data(), sizeof(float) * b32.size()));
@@ -435,6 +444,29 @@ kron_bsr3_r1_kernel(int nslices, const int *__restrict__ slot_node,
         }
         *reinterpret_cast<V2 *>(yr + o) = v;
     }
+}
+
+// New K and M values (CSR entries in the order the plan was built from) into the node
+// blocks, FP64 and FP32.
+__global__ void __launch_bounds__(256)
+bsr_scatter_kernel(long long n, const int *__restrict__ map, const double *__restrict__ v,
+                   double *__restrict__ b64, float *__restrict__ b32) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = map[i];
+    b64[t] = v[i];
+    b32[t] = (float)v[i];
+}
+
+int bsr_set_values(BsrPlan &P, const double *kv, const double *mv, hipStream_t s) {
+    if (P.nk)
+        hipLaunchKernelGGL(bsr_scatter_kernel, dim3(ceil_div(P.nk, 256)), dim3(256), 0, s, P.nk,
+                           P.kmap.as<int>(), kv, P.bval64.as<double>(), P.bval32.as<float>());
+    if (P.nm)
+        hipLaunchKernelGGL(bsr_scatter_kernel, dim3(ceil_div(P.nm, 256)), dim3(256), 0, s, P.nm,
+                           P.mmap.as<int>(), mv, P.bval64.as<double>(), P.bval32.as<float>());
+    XT_HIP(hipGetLastError());
+    return XT_OK;
 }
 
 static constexpr int kR1Split = 4;   // lanes per node row for one load case (8: slower)

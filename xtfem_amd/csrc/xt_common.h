@@ -118,6 +118,8 @@ struct BandPrec;
 int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out);
 void band_delete(BandPrec *bp);
 void band_dims(const BandPrec *bp, int *b, int *nb, int *bw);
+int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da,
+                  hipStream_t s);
 template <typename T>
 int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s);
 

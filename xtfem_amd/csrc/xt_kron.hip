@@ -242,6 +242,17 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
 BsrPlan *bsr_new();
 void bsr_delete(BsrPlan *);
 long long bsr_blocks(const BsrPlan *, int real);
+int bsr_set_values(BsrPlan &P, const double *kv, const double *mv, hipStream_t s);
+
+__global__ void __launch_bounds__(256)
+copy_values_kernel(long long n, const double *__restrict__ v, double *__restrict__ d64,
+                   float *__restrict__ d32) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        d64[i] = v[i];
+        if (d32) d32[i] = (float)v[i];
+    }
+}
 }  // namespace xt
 
 struct xt_op_s {
@@ -311,6 +322,21 @@ int xt_op_create(int p, int q, int nterms, const int *const *rowptr, const int *
         }
     }
     *out = op;
+    return XT_OK;
+}
+
+int xt_op_set_values(xt_op op, const double *const *vals, void *stream) {
+    XT_ARG(op && vals, "op_set_values arguments");
+    hipStream_t s = (hipStream_t)stream;
+    for (int t = 0; t < op->nterms; ++t) {
+        XT_ARG(vals[t], "op_set_values: null value array");
+        const long long n = op->S[t].nnz;
+        if (n)
+            hipLaunchKernelGGL(copy_values_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                               vals[t], op->S[t].val64.as<double>(), op->S[t].val32.as<float>());
+    }
+    if (op->bsr) XT_TRY(bsr_set_values(*op->bsr, vals[0], vals[1], s));
+    XT_HIP(hipGetLastError());
     return XT_OK;
 }
 

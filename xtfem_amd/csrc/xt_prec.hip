@@ -515,6 +515,12 @@ int xt_prec_create_band(int p, const int *ia, const int *ja, const double *a, in
     return XT_OK;
 }
 
+int xt_prec_refactor_band(xt_prec pr, const int *ia, const int *ja, const double *a,
+                          void *stream) {
+    XT_ARG(pr && pr->band && ia && ja && a, "prec_refactor_band arguments");
+    return band_refactor(pr->band, ia, ja, a, (hipStream_t)stream);
+}
+
 int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth) {
     XT_ARG(pr && pr->band, "not a banded-factor preconditioner");
     band_dims(pr->band, block, nblocks, bandwidth);

@@ -63,6 +63,11 @@ class KronOperator:
             lib().xt_op_destroy(h)
             self.h = None
 
+    def set_values(self, *vals):
+        """New values (device FP64 tensors, one per term, in the creation pattern)."""
+        ptrs = (C.c_void_p * self.nterms)(*[v.data_ptr() for v in vals])
+        check(lib().xt_op_set_values(self.h, ptrs, _stream()), "op_set_values")
+
     def algorithmic_bytes(self, R, fp32, beta=False, fmt=None, padded=True):
         """Bytes one application moves in the stored format, each read once and written
         once: matrix (values + indices + row/slice pointers), X read, Y written.
@@ -181,12 +186,77 @@ class BandPreconditioner(IlutPreconditioner):
               "prec_band_info")
         self.block, self.nblocks, self.bandwidth = b.value, nb.value, bw.value
 
+    def refactor(self, rowptr, colidx, vals):
+        """Refactor from K on the device (1-based CSR int32/FP64 tensors, same band)."""
+        check(lib().xt_prec_refactor_band(self.h, rowptr.data_ptr(), colidx.data_ptr(),
+                                          vals.data_ptr(), _stream()), "prec_refactor_band")
+
     def matrix_bytes(self, fp32):
         """Bytes of the four block-matrix sets one apply streams (D, D^T halves counted
         as stored: whole b x b blocks)."""
         sv = 4 if fp32 else 8
         bb = self.block * self.block
         return sv * bb * (2 * self.nblocks + 2 * (self.nblocks - 1))
+
+
+class Assembly:
+    """Spatial assembly on the device (mpiglbmtx_m + tricsr + csrapplybc,
+    mpisubs.f90:69-139, spmtx.f90:268-407): the global patterns and each entry's element
+    contributions are built once per mesh; ``run`` recomputes every element matrix for
+    per-Gauss-point E and nu and sums them in the reference's order, with the fixed DOFs'
+    rows and columns set by the 1-0 method.  ``K``/``M`` are device tensors of values
+    in the host patterns ``K_pattern``/``M_pattern`` (1-based CSR)."""
+
+    def __init__(self, mesh, fixed):
+        dev = _dev()
+        self.nn, self.ne = mesh.nn, mesh.ne
+        self.p = 3 * mesh.nn
+        xcoord = np.ascontiguousarray(mesh.xcoord, dtype=np.float64)
+        xconn = np.ascontiguousarray(mesh.xconn, dtype=np.int32)
+        fx = np.ascontiguousarray(np.asarray(fixed) != 0, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().xt_asm_create(self.nn, self.ne, xcoord.ctypes.data, xconn.ctypes.data,
+                                  fx.ctypes.data, C.addressof(h)), "asm_create")
+        self.h = h
+        self.K_pattern = self._pattern(0)
+        self.M_pattern = self._pattern(1)
+        t = lambda a: torch.as_tensor(a, device=dev)
+        self.K_rowptr, self.K_col = t(self.K_pattern[0]), t(self.K_pattern[1])
+        self.K = torch.zeros(len(self.K_pattern[1]), dtype=torch.float64, device=dev)
+        self.M = torch.zeros(len(self.M_pattern[1]), dtype=torch.float64, device=dev)
+        self.Nxyz = [torch.zeros(64 * self.ne, dtype=torch.float64, device=dev) for _ in range(3)]
+
+    def _pattern(self, which):
+        nnz = C.c_longlong(0)
+        check(lib().xt_asm_pattern(self.h, which, C.addressof(nnz), None, None), "asm_pattern")
+        ia = np.zeros(self.p + 1, dtype=np.int32)
+        ja = np.zeros(nnz.value, dtype=np.int32)
+        check(lib().xt_asm_pattern(self.h, which, C.addressof(nnz), ia.ctypes.data,
+                                   ja.ctypes.data), "asm_pattern")
+        return ia, ja
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and lib is not None:   # (module teardown clears lib)
+            lib().xt_asm_destroy(h)
+            self.h = None
+
+    def run(self, ym, nu, rho):
+        """Assemble for E, nu (scalars, or per Gauss point [8*ne], host or device)."""
+        dev = self.K.device
+        g = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).expand(8 * self.ne).contiguous()
+        ymd, nud = g(ym), g(nu)
+        s = _stream()
+        check(lib().xt_asm_run(self.h, ymd.data_ptr(), nud.data_ptr(), float(rho), s), "asm_run")
+        for w, dst in ((0, self.K), (1, self.M), (2, self.Nxyz[0]), (3, self.Nxyz[1]),
+                       (4, self.Nxyz[2])):
+            check(lib().xt_asm_copy(self.h, w, dst.data_ptr(), s), "asm_copy")
+        return self.K, self.M
+
+    def host(self):
+        """(K, M) as host CSR triples (1-based), values of the last ``run``."""
+        return ((self.K_pattern[0], self.K_pattern[1], self.K.cpu().numpy()),
+                (self.M_pattern[0], self.M_pattern[1], self.M.cpu().numpy()))
 
 
 def slab_solve(op, pr, A, B, rhs, x0=None, im=30, eps=1e-8, maxits=500, mode=1):

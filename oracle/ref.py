@@ -174,9 +174,13 @@ def chkelesta(nn, ne, ngpe, Dc, D):
     return nefail[:nef.value].copy(), elesta, gpsta
 
 
-def assemble_m(xcoord, xconn, ym0, nu0, rho):
+def assemble_m(xcoord, xconn, ym0, nu0, rho, ym=None, nu=None, temp=None, alpha=0.0,
+               Tref=0.0, with_vv=False):
     """mpiglbmtx_m (mpisubs.f90:69-139) over all elements, then tricsr
-    (spmtx.f90:359-407).  Returns 1-based CSR triples K, M and Nx, Ny, Nz [8, 8*ne]."""
+    (spmtx.f90:359-407).  Returns 1-based CSR triples K, M and Nx, Ny, Nz [8, 8*ne];
+    with ``with_vv`` also the thermal load vv [3*nn] that the same call accumulates
+    from the nodal temperatures ``temp`` (calc_thermo_stress + elevec_m).  ym, nu:
+    per-Gauss-point arrays [8*ne] (default ym0, nu0 everywhere)."""
     L = lib()
     xcoord = _F(xcoord)
     xconn = _F(xconn, np.int32)
@@ -195,8 +199,8 @@ def assemble_m(xcoord, xconn, ym0, nu0, rho):
     L.setshapefunc_t_(_ri(nnsd), _ri(nnse), _ri(nquads), _ri(ngpe), _p(wqs_t), _p(ns_t),
                       _p(dns_t))
     ele = np.arange(1, ne + 1, dtype=np.int32)
-    ym = np.full(ngpe * ne, float(ym0))
-    nu = np.full(ngpe * ne, float(nu0))
+    ym = np.full(ngpe * ne, float(ym0)) if ym is None else _F(ym)
+    nu = np.full(ngpe * ne, float(nu0)) if nu is None else _F(nu)
     sd = np.zeros((6, 9), order="F")
     stiff = np.zeros((6, 6, ne * ngpe), order="F")
     Nx = np.zeros((nnse, ngpe * ne), order="F")
@@ -208,10 +212,10 @@ def assemble_m(xcoord, xconn, ym0, nu0, rho):
     ki = np.zeros(nti, dtype=np.int32); kj = np.zeros(knz, dtype=np.int32); kv = np.zeros(knz)
     mi = np.zeros(nti, dtype=np.int32); mj = np.zeros(knz, dtype=np.int32); mv = np.zeros(knz)
     vv = np.zeros(ndof)
-    tempst = np.zeros(nn)
+    tempst = np.zeros(nn) if temp is None else _F(temp)
     L.mpiglbmtx_m_(_ri(ne), _p(ele), _ri(ne), _ri(nn), _ri(ndofn), _ri(nnse), _ri(nquads),
                    _ri(ngpe), _ri(ncmp), _p(xconn), _p(xcoord), _p(ym), _p(nu), _rd(ym0),
-                   _rd(nu0), _rd(rho), _rd(0.0), _rd(0.0), _p(wqs), _p(ns_t), _p(ns), _p(dns_t),
+                   _rd(nu0), _rd(rho), _rd(alpha), _rd(Tref), _p(wqs), _p(ns_t), _p(ns), _p(dns_t),
                    _p(dns), _p(dnxyz), _p(sd), _p(tempst), _p(stiff), _p(Nx), _p(Ny), _p(Nz),
                    _p(ki), _p(kj), _p(kv), _ri(knz), _p(mi), _p(mj), _p(mv), _ri(knz), _p(vv))
     out = []
@@ -219,7 +223,66 @@ def assemble_m(xcoord, xconn, ym0, nu0, rho):
         nz = C.c_int(knz)
         L.tricsr_(_p(ti), _p(tj), _p(tv), _ri(ndof), C.byref(nz), _ri(nti))
         out.append((ti[:ndof + 1].copy(), tj[:nz.value].copy(), tv[:nz.value].copy()))
+    if with_vv:
+        return out[0], out[1], np.array(Nx), np.array(Ny), np.array(Nz), vv.copy()
     return out[0], out[1], np.array(Nx), np.array(Ny), np.array(Nz)
+
+
+def shape_tables_t(nquads=2):
+    """setshapefunc_t (elesubs.f90:337-371): wqs [nquads], ns [ngpe, 8], dns [ngpe, 8, 3]."""
+    ngpe = nquads ** 3
+    wqs = np.zeros(nquads)
+    ns = np.zeros((ngpe, 8), order="F")
+    dns = np.zeros((ngpe, 8, 3), order="F")
+    lib().setshapefunc_t_(_ri(2), _ri(8), _ri(nquads), _ri(ngpe), _p(wqs), _p(ns), _p(dns))
+    return wqs, np.array(ns), np.array(dns)
+
+
+def assemble_t(xcoord, xconn, kappa, cp, rho):
+    """mpiglbmtx_t (mpisubs.f90:141-188, elemtx_t elesubs.f90:60-116) over all
+    elements, then tricsr: thermal conductivity K_t and capacity M_t (1 DOF a node)."""
+    L = lib()
+    xcoord = _F(xcoord)
+    xconn = _F(xconn, np.int32)
+    nn, ne = xcoord.shape[0], xconn.shape[0]
+    wqs, ns, dns = (_F(a) for a in shape_tables_t())
+    ele = np.arange(1, ne + 1, dtype=np.int32)
+    knz = ne * 64
+    nti = max(knz, nn + 1)
+    ki = np.zeros(nti, dtype=np.int32); kj = np.zeros(knz, dtype=np.int32); kv = np.zeros(knz)
+    mi = np.zeros(nti, dtype=np.int32); mj = np.zeros(knz, dtype=np.int32); mv = np.zeros(knz)
+    L.mpiglbmtx_t_(_ri(ne), _p(ele), _ri(ne), _ri(nn), _ri(1), _ri(8), _ri(2), _ri(8),
+                   _p(xconn), _p(xcoord), _rd(kappa), _rd(cp), _rd(rho), _p(wqs), _p(ns),
+                   _p(dns), _p(ki), _p(kj), _p(kv), _ri(knz), _p(mi), _p(mj), _p(mv), _ri(knz))
+    out = []
+    for (ti, tj, tv) in ((ki, kj, kv), (mi, mj, mv)):
+        nz = C.c_int(knz)
+        L.tricsr_(_p(ti), _p(tj), _p(tv), _ri(nn), C.byref(nz), _ri(nti))
+        out.append((ti[:nn + 1].copy(), tj[:nz.value].copy(), tv[:nz.value].copy()))
+    return out[0], out[1]
+
+
+def material_update(xconn, nn, tempst, Tref, ym0, nu0):
+    """thermoelastic.f90:1-46: E and nu at every Gauss point from the temperature of
+    the slab's third temporal node, tempst [3*nn] (temporal-major)."""
+    xconn = _F(xconn, np.int32)
+    ne = xconn.shape[0]
+    _, ns, _ = shape_tables_t()
+    ele = np.arange(1, ne + 1, dtype=np.int32)
+    tempst = _F(tempst)
+    ym = np.zeros(8 * ne)
+    nu = np.zeros(8 * ne)
+    lib().material_update_(_ri(8), _ri(2), _ri(len(tempst)), _ri(nn), _ri(ne), _ri(8),
+                           _ri(ne), _p(ele), _p(xconn), _p(_F(ns)), _p(tempst), _rd(Tref),
+                           _rd(ym0), _rd(nu0), _p(ym), _p(nu))
+    return ym, nu
+
+
+def getfextc_t(t1, dt):
+    """getfextc.f90:1-20 -> fextc [3]."""
+    f = np.zeros(3)
+    lib().getfextc_t_(_rd(t1), _rd(dt), _p(f))
+    return f
 
 
 def csrapplybc(S, bc):

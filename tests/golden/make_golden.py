@@ -191,6 +191,31 @@ def main():
             g[f"{tag}{i}_K_digest"] = np.array(digest(*Kc))
             g[f"{tag}{i}_M_digest"] = np.array(digest(*Mc))
             g[f"{tag}{i}_K_nnz"] = np.array(len(Kc[2]))
+    # thermal side: mpiglbmtx_t + tricsr (conductivity, capacity), material_update and
+    # the thermal load vv of mpiglbmtx_m for a temperature field on the rod, and the
+    # heat-flux coefficients getfextc_t
+    rho = RHO_REF
+    Kt, Mt = ref.assemble_t(mesh.xcoord, mesh.xconn, 0.1, 452.0, rho)
+    g["th_Kt_digest"], g["th_Mt_digest"] = np.array(digest(*Kt)), np.array(digest(*Mt))
+    g["th_Kt_nnz"] = np.array(len(Kt[2]))
+    rng3 = np.random.default_rng(SEED + 2)
+    T = np.concatenate([rng3.uniform(-5.0, 60.0, 2 * mesh.nn),
+                        20.0 + 40.0 * np.sin(mesh.xcoord[:, 0] / 40.0)
+                        - 8.0 * (mesh.xcoord[:, 1] > 6.0)])
+    g["th_T"] = T
+    for tag, Tref in (("", 0.0), ("r", 4.0)):
+        ym, nu = ref.material_update(mesh.xconn, mesh.nn, T, Tref, YM_REF, NU_REF)
+        g[f"th{tag}_ym"], g[f"th{tag}_nu"] = ym, nu
+        Kc, Mc, _, _, _, vv = ref.assemble_m(mesh.xcoord, mesh.xconn, YM_REF, NU_REF, rho,
+                                             ym=ym, nu=nu, temp=T[2 * mesh.nn:],
+                                             alpha=float(np.float32(1.2e-5)), Tref=Tref,
+                                             with_vv=True)
+        g[f"th{tag}_vv"] = vv
+        g[f"th{tag}_K_digest"] = np.array(digest(*Kc))
+        g[f"th{tag}_M_digest"] = np.array(digest(*Mc))
+    tpts = np.array([[0.0, 0.5], [0.3, 0.05], [12.0, 1.0], [7.5, 0.05]])
+    g["fxt_pts"] = tpts
+    g["fxt"] = np.array([ref.getfextc_t(t1, dt) for t1, dt in tpts])
     np.savez_compressed(os.path.join(HERE, "golden_ref.npz"), **g)
     print("wrote", os.path.join(HERE, "golden_ref.npz"))
 

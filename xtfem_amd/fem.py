@@ -15,6 +15,12 @@ the reference would build:
   duplicates summed in element order, exact zeros removed.
 * 1-0 boundary conditions ``csrapplybc`` (spmtx.f90:268-307).
 * Traction lumping per element-node incidence (main3d.f90:508-531).
+* Thermal side (1 DOF a node): ``setshapefunc_t`` tables (elesubs.f90:337-371),
+  conductivity/capacity matrices ``elemtx_t`` (elesubs.f90:60-116), the
+  temperature-dependent material ``material_update`` (thermoelastic.f90:1-46), the
+  thermal load ``vv`` that ``mpiglbmtx_m`` accumulates from ``calc_thermo_stress`` and
+  ``elevec_m`` (mpisubs.f90:124-130, thermoelastic.f90:48-78, elesubs.f90:119-167), and
+  the heat-flux lumping and fixed-temperature mask (main3d.f90:429-478).
 
 All CSR output is 1-based int32 (the reference's convention); DOF numbering is
 ``3*(node-1) + comp`` (mpisubs.f90:111-116).
@@ -328,3 +334,160 @@ def clamp_mask(mesh: Mesh, axis, values, tol=1e-9):
 def node_degree(mesh: Mesh):
     """Number of elements sharing each node (main3d.f90:273-280)."""
     return np.bincount((mesh.xconn - 1).ravel(), minlength=mesh.nn)
+
+
+# ---------------------------------------------------------------------------------
+# Thermal side
+# ---------------------------------------------------------------------------------
+
+# Default-real (FP32) literals of material_update (thermoelastic.f90:34-35) and of the
+# driver's thermal expansion coefficient (main3d.f90:157), widened to FP64 as the
+# compiled reference widens them.
+YM_DT1 = float(np.float32(1.6))
+YM_DT2 = float(np.float32(0.1))
+NU_DT1 = float(np.float32(5.0e-5))
+ALPHA_REF = float(np.float32(1.2e-5))
+
+
+def shape_tables_t():
+    """setshapefunc_t (elesubs.f90:337-371; shapex procs.f90:1416-1448, shapedx
+    procs.f90:1452-1501): wqs [2], ns [8 GP, 8 nodes], dns [8 GP, 8 nodes, 3]."""
+    N, dN = _shape_tables()
+    return np.ones(2), N.copy(), np.ascontiguousarray(np.transpose(dN, (0, 2, 1)))
+
+
+def _thermal_geometry(X, dN_l):
+    """Jacobian, determinant and shape-function gradients at one Gauss point the way
+    elemtx_t and elevec_m form them (elesubs.f90:86-95, 143-152): jac(m, p) =
+    sum_a dns(a, p) X(a, m) -- the transpose of elemtx_m's jac, so the determinant and
+    cofactors round differently -- then Niix(m) = sum_p gam(p, m) dns(a, p).
+    X [E, 8a, 3], dN_l [3, 8a]; returns det [E] and G [E, 3m, 8a]."""
+    E = X.shape[0]
+    jac = np.zeros((E, 3, 3))
+    for a in range(8):
+        jac = jac + X[:, a, :, None] * dN_l[None, None, :, a]
+    det = _det3(jac)
+    if np.any(det <= 0.0):
+        raise ValueError("non-positive Jacobian: element node order does not match C3D8")
+    gam = _inv3(jac, det)
+    G = np.zeros((E, 3, 8))
+    for p in range(3):
+        G = G + gam[:, p, :, None] * dN_l[p][None, None, :]
+    return det, G
+
+
+def thermal_element_matrices(mesh: Mesh, kappa, cp, rho, chunk=8192):
+    """elemtx_t (elesubs.f90:60-116) for every element: ke [ne, 8, 8] =
+    sum_l sum_m kappa Niix(m) Njjx(m) det, me = sum_l rho cp N_i N_j det, accumulated
+    Gauss point by Gauss point and, inside ke, direction by direction, with the
+    reference's left-to-right products (the weights are 1 for the 2-point rule)."""
+    ne = mesh.ne
+    N, dN = _shape_tables()
+    ke = np.zeros((ne, 8, 8))
+    me = np.zeros((ne, 8, 8))
+    rc = rho * cp
+    for e0 in range(0, ne, chunk):
+        e1 = min(ne, e0 + chunk)
+        X = mesh.xcoord[mesh.xconn[e0:e1] - 1]
+        k = np.zeros((e1 - e0, 8, 8))
+        m = np.zeros((e1 - e0, 8, 8))
+        for l in range(8):
+            det, G = _thermal_geometry(X, dN[l])
+            mm = (rc * N[l])[:, None] * N[l][None, :]
+            m = m + mm[None] * det[:, None, None]
+            for d in range(3):
+                k = k + ((kappa * G[:, d, :, None]) * G[:, d, None, :]) * det[:, None, None]
+        ke[e0:e1] = k
+        me[e0:e1] = m
+    return ke, me
+
+
+def assemble_t(mesh: Mesh, kappa, cp, rho):
+    """Thermal K_t, M_t (1-based CSR over nodes) as mpiglbmtx_t (mpisubs.f90:141-188)
+    + tricsr build them."""
+    ke, me = thermal_element_matrices(mesh, kappa, cp, rho)
+    ne, n = mesh.ne, mesh.nn
+    dofs = (mesh.xconn.astype(np.int64) - 1)
+    rows = np.repeat(dofs[:, :, None], 8, axis=2).ravel()
+    cols = np.repeat(dofs[:, None, :], 8, axis=1).ravel()
+    return _triplet_to_csr(rows, cols, ke.ravel(), n), _triplet_to_csr(rows, cols, me.ravel(), n)
+
+
+def gauss_temperature(mesh: Mesh, T):
+    """Temperature at every Gauss point, sum_a N_a T(node_a) in local node order
+    (thermoelastic.f90:26-30 and 65-69 form it identically).  T [nn] -> [ne, 8]."""
+    N, _ = _shape_tables()
+    Te = np.asarray(T, dtype=np.float64)[mesh.xconn - 1]          # [ne, 8a]
+    t = np.zeros((mesh.ne, 8))
+    for a in range(8):
+        t = t + N[None, :, a] * Te[:, a, None]
+    return t
+
+
+def material_update(mesh: Mesh, T, Tref, ym0, nu0):
+    """thermoelastic.f90:1-46: E = E0 - 1.6 (T - Tref) - 0.1 (T - Tref)^2 and
+    nu = nu0 + 5e-5 (T - Tref) at Gauss points whose temperature is >= 0, E0 and nu0
+    elsewhere.  T: the nodal temperatures of the slab's third temporal node [nn].
+    Returns ym, nu [8*ne]."""
+    t = gauss_temperature(mesh, T)
+    d = t - Tref
+    ok = t >= 0.0
+    ym = np.where(ok, (ym0 - YM_DT1 * d) - (YM_DT2 * d) * d, ym0)
+    nu = np.where(ok, nu0 + NU_DT1 * d, nu0)
+    return ym.ravel(), nu.ravel()
+
+
+def thermal_load(mesh: Mesh, T, Tref, alpha, ym, nu, chunk=8192):
+    """The thermal load vv [3*nn] that mpiglbmtx_m accumulates (mpisubs.f90:124-130):
+    per Gauss point the thermal stress sigma_m = (D(m,1) + D(m,2) + D(m,3)) alpha (T -
+    Tref) (calc_thermo_stress, thermoelastic.f90:48-78; D from setsdstiff with that
+    point's E and nu), then ve(3a + m) += dN_a/dx_m sigma_m det (elevec_m,
+    elesubs.f90:119-167, geometry as elemtx_t), summed into vv element by element."""
+    ne = mesh.ne
+    t = gauss_temperature(mesh, T)
+    strain = alpha * (t - Tref)                                    # [ne, 8]
+    D = elastic_dmat(np.asarray(ym, dtype=np.float64).reshape(ne, 8),
+                     np.asarray(nu, dtype=np.float64).reshape(ne, 8))
+    rs = (D[..., :3, 0] + D[..., :3, 1]) + D[..., :3, 2]           # [ne, 8, 3m]
+    stress = rs * strain[..., None]
+    _, dN = _shape_tables()
+    ve = np.zeros((ne, 8, 3))
+    for e0 in range(0, ne, chunk):
+        e1 = min(ne, e0 + chunk)
+        X = mesh.xcoord[mesh.xconn[e0:e1] - 1]
+        v = np.zeros((e1 - e0, 8, 3))
+        for l in range(8):
+            det, G = _thermal_geometry(X, dN[l])
+            v = v + (np.transpose(G, (0, 2, 1)) * stress[e0:e1, l, None, :]) * det[:, None, None]
+        ve[e0:e1] = v
+    vv = np.zeros(3 * mesh.nn)
+    dofs = 3 * (mesh.xconn.astype(np.int64) - 1)[:, :, None] + np.arange(3)[None, None, :]
+    np.add.at(vv, dofs.ravel(), ve.ravel())       # sequential: element order per DOF
+    return vv
+
+
+def fixed_temperature_mask(mesh: Mesh, axis, values, tol=0.0):
+    """Spatial thermal DOF mask [nn] (one DOF a node) fixing the temperature on the
+    planes x_axis == v (main3d.f90:432-440; the reference compares exactly)."""
+    m = np.zeros(mesh.nn, dtype=bool)
+    for v in np.atleast_1d(values):
+        m |= np.abs(mesh.xcoord[:, axis] - v) <= tol
+    return m.astype(np.int32)
+
+
+def heat_flux_load(mesh: Mesh, axis, count_value, values, total, tol=0.0):
+    """Lumped heat flux (main3d.f90:455-478): fnode = total / (element-node incidences
+    on the plane x_axis == count_value); every incidence on a plane in ``values`` then
+    receives fnode (the reference counts one face and loads two)."""
+    xa = mesh.xcoord[:, axis]
+    inc_count = (np.abs(xa - count_value) <= tol)[mesh.xconn - 1].sum()
+    r = np.zeros(mesh.nn)
+    if inc_count == 0:
+        return r
+    fnode = total / float(inc_count)
+    on = np.zeros(mesh.nn, dtype=bool)
+    for v in np.atleast_1d(values):
+        on |= np.abs(xa - v) <= tol
+    nodes = (mesh.xconn - 1)[on[mesh.xconn - 1]]
+    np.add.at(r, nodes, fnode)
+    return r

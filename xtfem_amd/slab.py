@@ -12,6 +12,8 @@
 * ``damage_nt`` -- temporal interpolation table of the damage update
   (kokkos_damage.f90:36-47).
 * ``slab_rhs`` -- the right-hand side of one mechanical slab (main3d.f90:803-823).
+* ``getfextc_t`` -- temporal coefficients of the heat flux (getfextc.f90:1-20).
+* ``thermal_rhs`` -- the right-hand side of one thermal slab (main3d.f90:703-723).
 """
 from __future__ import annotations
 
@@ -257,6 +259,44 @@ def getfextc_m(t1, w, dt):
     g[..., 2] = 1.
     g[..., 5] = -S1 + sin(w * (t1 + dt))
     return f, g
+
+
+# Angular frequency of the thermal load, w = 2*3.14159/24 evaluated in default real
+# (FP32) arithmetic and widened (getfextc.f90:9).
+W_THERMAL = float(np.float32(np.float32(2.0 * np.float32(3.14159)) / np.float32(24.0)))
+
+
+def getfextc_t(t1, dt):
+    """Temporal load coefficients of the heat flux (getfextc.f90:1-20) -> fextc[..., 3],
+    with the reference's operand order ((3 dt) w cos(...) etc.)."""
+    t1, dt = np.broadcast_arrays(np.asarray(t1, dtype=np.float64),
+                                 np.asarray(dt, dtype=np.float64))
+    w = W_THERMAL
+    C1, C2 = np.cos(t1 * w), np.cos((dt + t1) * w)
+    S1, S2 = np.sin(t1 * w), np.sin((dt + t1) * w)
+    den = (dt * dt) * (w * w)
+    f = np.zeros(t1.shape + (3,))
+    f[..., 0] = -((3. * dt * w * C1 + dt * w * C2 + 4. * S1 - 4. * S2) / den)
+    f[..., 1] = (4. * (dt * w * C1 + dt * w * C2 + 2. * S1 - 2. * S2)) / den
+    f[..., 2] = -((dt * w * C1 + 3. * dt * w * C2 + 4. * S1 - 4. * S2) / den)
+    return f
+
+
+def thermal_rhs(p, fextc, rext, jump=None, init=None, bound=None, fixed=None):
+    """Right-hand side of one thermal slab, temporal-major [3*p] (main3d.f90:703-723):
+    ``init`` (fInit, first slab only) plus block j = fextc(j) rext, plus the jump
+    (ktn1 (x) K_t + mtn1 (x) M_t) T_prev, plus ``bound`` (fBound), then zero at the fixed
+    space-time DOFs."""
+    f = np.zeros(3 * p) if init is None else np.array(init, dtype=np.float64)
+    for j in range(3):
+        f[j * p:(j + 1) * p] = f[j * p:(j + 1) * p] + fextc[j] * rext
+    if jump is not None:
+        f = f + jump
+    if bound is not None:
+        f = f + bound
+    if fixed is not None:
+        f[np.asarray(fixed) != 0] = 0.0
+    return f
 
 
 def damage_nt(tcoorde, w, nip):

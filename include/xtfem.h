@@ -230,8 +230,9 @@ typedef struct xt_asm_s *xt_asm;      /* mesh, global patterns, element buffers 
 int xt_asm_create(int nn, int ne, const double *xcoord, const int *xconn, const int *fixed,
                   xt_asm *out);
 void xt_asm_destroy(xt_asm h);
-/* Host copy of a pattern (which 0: K, 1: M): nnz, then 1-based rowptr [3*nn+1] and
- * colidx [nnz] when non-NULL. */
+/* Host copy of a pattern (which 0: K, 1: M, 2: thermal K_t and M_t, after
+ * xt_asm_thermal): nnz, then 1-based rowptr [3*nn+1, or nn+1 for which 2] and colidx
+ * [nnz] when non-NULL. */
 int xt_asm_pattern(xt_asm h, int which, long long *nnz, int *rowptr, int *colidx);
 /* Numeric assembly on the device for per-Gauss-point E (ym) and nu (device [8*ne],
  * element-major) and density rho: the element matrices of elemtx_m
@@ -243,12 +244,33 @@ int xt_asm_pattern(xt_asm h, int which, long long *nnz, int *rowptr, int *colidx
  * pattern (the reference drops them). */
 int xt_asm_run(xt_asm h, const double *ym, const double *nu, double rho, void *stream);
 /* Device arrays of the last xt_asm_run: which 0 K values, 1 M values (in the patterns'
- * order), 2/3/4 Nx/Ny/Nz. */
+ * order), 2/3/4 Nx/Ny/Nz; of xt_asm_thermal: 5 K_t and 6 M_t with the 1-0 method on the
+ * fixed temperatures, 7 K_t and 8 M_t without it (the ki0/kv0 copies of
+ * main3d.f90:444-449). */
 int xt_asm_values(xt_asm h, int which, const double **vals);
 /* Copy array `which` (as xt_asm_values) into device memory dst (nnz or 64*ne doubles). */
 int xt_asm_copy(xt_asm h, int which, double *dst, void *stream);
 /* Device copies of a pattern's 1-based column indices and 0-based rows of its entries. */
 int xt_asm_device_pattern(xt_asm h, int which, const int **colidx, const int **row);
+/* Thermal conductivity K_t and capacity M_t (one DOF a node) on the device: elemtx_t
+ * for every element (elesubs.f90:60-116) summed per entry in element order as
+ * mpiglbmtx_t + tricsr build them (mpisubs.f90:141-188, main3d.f90:365-375), then the
+ * 1-0 method of csrapplybc on the nodes with tfixed [nn] (host) nonzero
+ * (main3d.f90:452-453).  Replaces that setup block.  The first call builds the
+ * node-level pattern; later calls recompute the values.  Bit-identical values. */
+int xt_asm_thermal(xt_asm h, const int *tfixed, double kappa, double cp, double rho,
+                   void *stream);
+/* One slab's temperature-dependent material and thermal load on the device
+ * (main3d.f90:758-768): material_update (thermoelastic.f90:1-46) writes E, nu [8*ne]
+ * to ym, nu; calc_thermo_stress (thermoelastic.f90:48-78) and elevec_m
+ * (elesubs.f90:119-167) per element, summed per DOF in element order as mpiglbmtx_m
+ * sums vv (mpisubs.f90:124-131), write vv [3*nn].  T: device FP64 with the temperature
+ * of node n (0-based) at T[n*tstride + toff] (0 <= toff < tstride), e.g. the third
+ * temporal node of an interleaved thermal solution (tstride 3R, toff 3r + 2).
+ * alpha, Tref, ym0, nu0 as the driver's (main3d.f90:145-157).  Bit-identical. */
+int xt_asm_thermo_load(xt_asm h, const double *T, long long tstride, long long toff, double Tref,
+                       double alpha, double ym0, double nu0, double *ym, double *nu, double *vv,
+                       void *stream);
 
 /* Y = (LU)^-1 X Minv_r^T on R right-hand sides (Minv: device [R][q][q],
  * row-major, the inverse temporal matrix; NULL = identity).  x and y must not

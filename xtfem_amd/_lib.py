@@ -61,6 +61,9 @@ _SIGS = {
     "xt_asm_values": (ci, [vp, ci, vp]),
     "xt_asm_copy": (ci, [vp, ci, vp, vp]),
     "xt_asm_device_pattern": (ci, [vp, ci, vp, vp]),
+    "xt_asm_thermal": (ci, [vp, vp, cd, cd, cd, vp]),
+    "xt_asm_thermo_load": (ci, [vp, vp, C.c_longlong, C.c_longlong, cd, cd, cd, cd, vp, vp, vp,
+                                vp]),
     "xt_prec_create": (ci, [ci, vp, vp, vp, vp]),
     "xt_prec_create_band": (ci, [ci, vp, vp, vp, ci, vp]),
     "xt_prec_band_info": (ci, [vp, vp, vp, vp]),

@@ -229,7 +229,7 @@ class Assembly:
     def _pattern(self, which):
         nnz = C.c_longlong(0)
         check(lib().xt_asm_pattern(self.h, which, C.addressof(nnz), None, None), "asm_pattern")
-        ia = np.zeros(self.p + 1, dtype=np.int32)
+        ia = np.zeros((self.nn if which == 2 else self.p) + 1, dtype=np.int32)
         ja = np.zeros(nnz.value, dtype=np.int32)
         check(lib().xt_asm_pattern(self.h, which, C.addressof(nnz), ia.ctypes.data,
                                    ja.ctypes.data), "asm_pattern")
@@ -257,6 +257,48 @@ class Assembly:
         """(K, M) as host CSR triples (1-based), values of the last ``run``."""
         return ((self.K_pattern[0], self.K_pattern[1], self.K.cpu().numpy()),
                 (self.M_pattern[0], self.M_pattern[1], self.M.cpu().numpy()))
+
+    def thermal(self, tfixed, kappa, cp, rho):
+        """Thermal conductivity and capacity (elemtx_t + mpiglbmtx_t, elesubs.f90:60-116,
+        mpisubs.f90:141-188) on the device, with the 1-0 method on the nodes where
+        ``tfixed`` [nn] is nonzero.  Sets ``T_pattern`` (1-based node CSR shared by both)
+        and the device value tensors ``Kt``, ``Mt`` (with BCs) and ``Kt0``, ``Mt0``
+        (without).  Returns (Kt, Mt)."""
+        tf = np.ascontiguousarray(np.asarray(tfixed) != 0, dtype=np.int32)
+        if tf.shape != (self.nn,):
+            raise api.XtError(f"tfixed must have one entry per node ({self.nn})")
+        s = _stream()
+        check(lib().xt_asm_thermal(self.h, tf.ctypes.data, float(kappa), float(cp), float(rho), s),
+              "asm_thermal")
+        if not hasattr(self, "T_pattern"):
+            self.T_pattern = self._pattern(2)
+            dev = self.K.device
+            self.T_rowptr = torch.as_tensor(self.T_pattern[0], device=dev)
+            self.T_col = torch.as_tensor(self.T_pattern[1], device=dev)
+            z = lambda: torch.zeros(len(self.T_pattern[1]), dtype=torch.float64, device=dev)
+            self.Kt, self.Mt, self.Kt0, self.Mt0 = z(), z(), z(), z()
+        for w, dst in ((5, self.Kt), (6, self.Mt), (7, self.Kt0), (8, self.Mt0)):
+            check(lib().xt_asm_copy(self.h, w, dst.data_ptr(), s), "asm_copy")
+        return self.Kt, self.Mt
+
+    def thermo_load(self, T, tstride, toff, Tref, alpha, ym0, nu0):
+        """material_update and the thermal load vv of one slab (thermoelastic.f90:1-78,
+        elesubs.f90:119-167) from the device temperatures T[n*tstride + toff].
+        Returns device tensors (ym [8*ne], nu [8*ne], vv [3*nn]), reused across calls."""
+        if T.dtype != torch.float64 or not T.is_contiguous() or T.device != self.K.device:
+            raise api.XtError("thermo_load: T must be a contiguous FP64 tensor on the device")
+        if not (0 <= toff < tstride) or T.numel() < self.nn * tstride:
+            raise api.XtError("thermo_load: T too short for nn * tstride")
+        if not hasattr(self, "ym_t"):
+            dev = self.K.device
+            self.ym_t = torch.zeros(8 * self.ne, dtype=torch.float64, device=dev)
+            self.nu_t = torch.zeros(8 * self.ne, dtype=torch.float64, device=dev)
+            self.vv = torch.zeros(self.p, dtype=torch.float64, device=dev)
+        check(lib().xt_asm_thermo_load(self.h, T.data_ptr(), int(tstride), int(toff), float(Tref),
+                                       float(alpha), float(ym0), float(nu0), self.ym_t.data_ptr(),
+                                       self.nu_t.data_ptr(), self.vv.data_ptr(), _stream()),
+              "asm_thermo_load")
+        return self.ym_t, self.nu_t, self.vv
 
 
 def slab_solve(op, pr, A, B, rhs, x0=None, im=30, eps=1e-8, maxits=500, mode=1):

@@ -60,6 +60,8 @@ def parse():
                     help="temporal points per slab of the damage leg")
     ap.add_argument("--rod-only", action="store_true",
                     help="only run the prismatic-rod ILUT leg (K timed slabs) and print it")
+    ap.add_argument("--thermo-only", action="store_true",
+                    help="only run the thermo-mechanical plate leg and print it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -423,6 +425,55 @@ def reassembly_leg(prob, reps=3):
                 host_assembly_ms=1e3 * t_host)
 
 
+def thermo_leg(prob, nslab=4):
+    """The thermo-mechanical slab loop on the plate (main3d.f90:699-840): per slab a
+    thermal slab solve (q = 3, exact banded factor of K_t), material update and thermal
+    load, re-assembly of K and M, operator update and band refactor, and the mechanical
+    slab solve with the thermal load on the right-hand side.  Temperatures fixed on
+    y = 0 and y = ny (the reference's z = 0 and z = le), heat flux 2 W/mm^2 lumped on the
+    side faces x = 0 and x = nx (its y = +-35), traction on y = ny, slabs of 0.5 s at
+    20 Hz.  One warm-up slab, then ``nslab`` timed slabs per solver mode; the host
+    restatement of the per-slab assembly work (material update, thermal load, K and M)
+    is timed beside it."""
+    import torch
+    from xtfem_amd import fem, thermo
+    mesh = prob.mesh
+    nx, ny, nz = (int(round(mesh.xcoord[:, k].max())) for k in range(3))
+    tfixed = fem.fixed_temperature_mask(mesh, 1, [0.0, float(ny)])
+    rext_t = fem.heat_flux_load(mesh, 0, 0.0, [0.0, float(nx)], 2.0 * ny * nz)
+    rext_m = prob.amp[0] * fem.traction_load(mesh, 1, float(ny), 1, float(nx * nz))
+    out = {}
+    for mode, name in ((0, "fp64"), (1, "fp32_ir")):
+        run = thermo.ThermoMechanicalSlabs(mesh, prob.bc, tfixed, rext_m, rext_t, 0.5,
+                                           2 * np.pi * 20.0, mode=mode, profile=True)
+        run.step()
+        torch.cuda.synchronize()
+        wall, its, phases = [], [], []
+        for _ in range(nslab):
+            t0 = time.perf_counter()
+            its.append(run.step())
+            torch.cuda.synchronize()
+            wall.append(time.perf_counter() - t0)
+            phases.append(run.phase_ms())
+        ph = {k: float(np.median([p[k] for p in phases])) for k in run.PHASES}
+        T = run.temperature()
+        out[name] = dict(ms_per_slab=1e3 * float(np.median(wall)), phases_ms=ph,
+                         its_thermal=[i[0] for i in its], its_mech=[i[1] for i in its],
+                         T_max=float(T.max()), T_min=float(T.min()),
+                         relres_t=run.last["relres_t"], relres_m=run.last["relres_m"])
+        del run
+    T = np.zeros(mesh.nn)   # host restatement of the per-slab assembly work
+    T[:] = 20.0 + 10.0 * np.sin(mesh.xcoord[:, 0] / nx * np.pi)
+    t0 = time.perf_counter()
+    ym, nu = fem.material_update(mesh, T, 0.0, thermo.YM0, thermo.NU0)
+    fem.thermal_load(mesh, T, 0.0, fem.ALPHA_REF, ym, nu)
+    fem.assemble(mesh, ym, nu, thermo.RHO)
+    out["host_assembly_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["workload"] = (f"plate, {mesh.ne} C3D8 elements, thermal {mesh.nn} DOFs x 3, "
+                       f"mechanical {prob.p} DOFs x 6, one load case")
+    return out
+
+
 def cpu_baseline(prob, budget_s=15.0):
     """The oracle on the host: FP64 restatement of kronpgmres (kronecker.f90:181-365,
     MGS, one core) preconditioned by A^-1 (x) K^-1 with the exact band Cholesky factor
@@ -568,6 +619,9 @@ def main():
         return
     R, K, W = args.cases, args.steps, args.warmup
     nx, ny, nz = args.grid
+    if args.thermo_only:
+        print(json.dumps(thermo_leg(Problem(nx, ny, nz, args.notch, 1), nslab=args.steps)))
+        return
     t0 = time.perf_counter()
     prob = Problem(nx, ny, nz, args.notch, R, args.cases_kind)
     t_asm = time.perf_counter() - t0
@@ -653,6 +707,13 @@ def main():
           + ", ".join(f"{k} {v:.1f} ms" for k, v in res["reassembly"]["ms"].items())
           + f"; host numpy assembly {res['reassembly']['host_assembly_ms']:.0f} ms",
           file=sys.stderr, flush=True)
+    res["thermo"] = thermo_leg(prob)
+    for k in ("fp64", "fp32_ir"):
+        r = res["thermo"][k]
+        print(f"# thermo-mechanical slab ({k}): {r['ms_per_slab']:.1f} ms/slab, its "
+              f"{r['its_thermal']}/{r['its_mech']}; "
+              + ", ".join(f"{a} {b:.1f}" for a, b in r["phases_ms"].items()) + " ms",
+              file=sys.stderr, flush=True)
     res["deletion"] = deletion_leg(prob)
     print(f"# deletion: {res['deletion']['nefail']} elements, "
           + ", ".join(f"{k} {v:.1f} ms" for k, v in res["deletion"]["ms"].items())

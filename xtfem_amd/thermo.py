@@ -46,7 +46,7 @@ class ThermoMechanicalSlabs:
     def __init__(self, mesh, fixed_m, tfixed, rext_m, rext_t, dt, w, *, ym0=YM0, nu0=NU0,
                  rho=RHO, kappa=KAPPA, cp=CP, alpha=fem.ALPHA_REF, Tref=0.0, T0=0.0,
                  im_t=100, eps_t=1e-10, maxits_t=500, im_m=100, eps_m=1e-10, maxits_m=500,
-                 mode=0):
+                 mode=0, profile=False):
         dev = _dev()
         self.mesh, self.nn, self.p = mesh, mesh.nn, 3 * mesh.nn
         self.dt, self.w = float(dt), float(w)
@@ -103,6 +103,26 @@ class ThermoMechanicalSlabs:
         self.tmask = self.tfixed.bool().repeat_interleave(3)
         self.mmask = self.fixed_m.bool().repeat_interleave(6)
         self.last = {}
+        self.profile = bool(profile)
+        self._marks = []
+
+    PHASES = ("thermal_rhs", "thermal_solve", "thermo_load", "assembly", "operator_update",
+              "band_refactor", "mech_rhs", "mech_solve")
+
+    def _mark(self):
+        if self.profile:
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            self._marks.append(e)
+
+    def phase_ms(self):
+        """Per-phase device time (ms) of the last ``step`` when built with profile=True,
+        from HIP events on the stream the work runs on."""
+        if len(self._marks) != len(self.PHASES) + 1:
+            return {}
+        self._marks[-1].synchronize()
+        return {k: a.elapsed_time(b) for k, a, b in zip(self.PHASES, self._marks[:-1],
+                                                        self._marks[1:])}
 
     def _rhs(self, p, q, coef, rext, fixed, b, slot=0):
         c = self.coef[6 * slot:6 * slot + q]
@@ -118,6 +138,8 @@ class ThermoMechanicalSlabs:
         dt = self.dt
         self.tcoorde = self.tcoorde[2] + np.arange(3) * dt / 2
         t1 = float(self.tcoorde[0])
+        self._marks = []
+        self._mark()
         # thermal right-hand side (main3d.f90:703-723)
         bt = self.bt
         bt.copy_(self.fbound)
@@ -126,6 +148,7 @@ class ThermoMechanicalSlabs:
         self.op_t.apply(self.jump_t, self.tempst, bt, 1, beta=1.0)
         self._rhs(self.nn, 3, slab.getfextc_t(t1, dt), self.rext_t, self.tfixed, bt)
         self.tempst.zero_()
+        self._mark()
         im, eps, maxits = self.solver_t
         _, its_t, ierr_t, rel_t = slab_solve_device(self.op_t, self.pr_t, self.At[None],
                                                     self.Bt[None], bt, self.tempst, 1, im, eps,
@@ -133,12 +156,17 @@ class ThermoMechanicalSlabs:
         if ierr_t[0] != 0:
             raise api.XtError(f"thermal slab {self.slab}: GMRES ierr={int(ierr_t[0])}")
         self.tempst[self.tmask] = self.T0
+        self._mark()
         # material, thermal load and re-assembly (main3d.f90:752-783)
         ym, nu, vv = self.asm.thermo_load(self.tempst, 3, 2, self.Tref, self.alpha, self.ym0,
                                           self.nu0)
+        self._mark()
         K, M = self.asm.run(ym, nu, self.rho)
+        self._mark()
         self.op_m.set_values(K, M)
+        self._mark()
         self.pr_m.refactor(self.asm.K_rowptr, self.asm.K_col, K)
+        self._mark()
         # mechanical right-hand side (main3d.f90:803-823)
         fe, fi = slab.getfextc_m(t1, self.w, dt)
         bm = self.bm
@@ -146,6 +174,7 @@ class ThermoMechanicalSlabs:
         self._rhs(self.p, 6, fe, self.rext_m, None, bm)
         self._rhs(self.p, 6, fi, vv, self.fixed_m, bm, slot=1)
         self.dispst.zero_()
+        self._mark()
         im, eps, maxits = self.solver_m
         _, its_m, ierr_m, rel_m = slab_solve_device(self.op_m, self.pr_m, self.Am[None],
                                                     self.Bm[None], bm, self.dispst, 1, im, eps,
@@ -153,6 +182,7 @@ class ThermoMechanicalSlabs:
         if ierr_m[0] != 0:
             raise api.XtError(f"mechanical slab {self.slab}: GMRES ierr={int(ierr_m[0])}")
         self.dispst[self.mmask] = 0.0
+        self._mark()
         self.last = dict(relres_t=float(rel_t[0]), relres_m=float(rel_m[0]))
         return int(its_t[0]), int(its_m[0])
 

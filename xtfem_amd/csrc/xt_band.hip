@@ -301,6 +301,241 @@ zero_upper_kernel(int b, double *L) {   // strict upper triangle of each col-maj
     }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Cholesky of the band, 64-column panels (replaces potrf + trsm + syrk per block)
+//
+// Block i's column of the band is the tall panel [L_i; E_i] (diagonal block K_ii, then
+// the sub-diagonal block K_{i+1,i}), both column-major b x b with leading dimension b.
+// Panel k (columns 64k..64k+63) takes two launches:
+//   chol_panel_kernel: every workgroup factors the 64 x 64 diagonal tile (identical
+//     arithmetic in each workgroup, so identical results) and workgroup t > 0 solves
+//     row tile k + t of the tall panel, P = A_t L_kk^-T, alongside.  Results go to the
+//     scratch W; the matrix is only read.
+//   chol_update_kernel: copies W back into the panel's tiles and applies the rank-64
+//     update to every tile to the right: the lower tiles of L_i, all tiles of E_i, and
+//     the lower tiles of the next diagonal block (K_{i+1,i+1} -= E_i E_i^T, a piece per
+//     panel).
+// ---------------------------------------------------------------------------
+constexpr int kCT = 64;   // panel width and tile size
+
+__device__ inline const double *tall_tile(const double *L, const double *E, int b, int nt,
+                                          int rt, int ct) {
+    const double *base = rt < nt ? L + (size_t)rt * kCT : E + (size_t)(rt - nt) * kCT;
+    return base + (size_t)ct * kCT * b;
+}
+
+// value of v in lane `src` of the wave (every lane gets it)
+__device__ inline double lane_bcast(double v, int src) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+typedef double xt_d4c __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+chol_panel_kernel(int b, int nt, int k, const double *__restrict__ L,
+                  const double *__restrict__ E, double *__restrict__ W, int *__restrict__ info) {
+    // The 64 x 64 diagonal tile (sA) and this workgroup's row tile (sX) in LDS, factored
+    // and solved in four 16-column steps: the 16 x 16 diagonal block by one wave in
+    // registers, the rows below it (and the row tile's) by forward substitution, one
+    // row per thread, then the rank-16 update of the columns to the right on the matrix
+    // cores (v_mfma_f64_16x16x4f64, one 16 x 16 block per wave at a time).
+    __shared__ double sA[kCT][kCT + 1], sX[kCT][kCT + 1];
+    __shared__ double rdiag[16];
+    __shared__ int sbad;
+    const int tid = threadIdx.x, t = blockIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const double *D = tall_tile(L, E, b, nt, k, k);
+    for (int e = tid; e < kCT * kCT; e += 256) {
+        const int c = e >> 6, r = e & 63;
+        sA[r][c] = D[(size_t)c * b + r];
+    }
+    if (t > 0) {
+        const double *At = tall_tile(L, E, b, nt, k + t, k);
+        for (int e = tid; e < kCT * kCT; e += 256) {
+            const int c = e >> 6, r = e & 63;
+            sX[r][c] = At[(size_t)c * b + r];
+        }
+    }
+    if (tid == 0) sbad = 0;
+    __syncthreads();
+    for (int s = 0; s < 4; ++s) {
+        const int c0 = 16 * s;
+        if (tid < 16) {   // (1) 16 x 16 Cholesky, lane r holds row r
+            const int r = tid;
+            double a[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) a[c] = sA[c0 + r][c0 + c];
+            int bad = 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const double pj = lane_bcast(a[j], j);
+                bad = (!(pj > 0.0) && !bad) ? j + 1 : bad;
+                const double d = sqrt(pj), rd = 1.0 / d;
+                const double col = r > j ? a[j] * rd : (r == j ? d : 0.0);
+                a[j] = r >= j ? col : a[j];
+                if (r == j) rdiag[j] = rd;
+#pragma unroll
+                for (int l = j + 1; l < 16; ++l) {
+                    const double cl = lane_bcast(col, l);
+                    a[l] = l <= r ? a[l] - col * cl : a[l];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) sA[c0 + r][c0 + c] = a[c];
+            if (r == 0 && bad && !sbad) sbad = c0 + bad;
+        }
+        __syncthreads();
+        // (2) rows below the block in the tile, then the row tile's rows: x L_s^T = a
+        const int nd = kCT - c0 - 16;
+        double *row = nullptr;
+        if (tid < nd) row = &sA[c0 + 16 + tid][0];
+        else if (t > 0 && tid < nd + kCT) row = &sX[tid - nd][0];
+        if (row) {
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = row[c0 + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                double v = x[j];
+#pragma unroll
+                for (int m = 0; m < j; ++m) v = v - x[m] * sA[c0 + j][c0 + m];
+                x[j] = v * rdiag[j];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) row[c0 + c] = x[c];
+        }
+        __syncthreads();
+        // (3) C -= P Q^T over the 16-column step: lower blocks (I, J), s < J <= I, of the
+        // tile, and blocks (R, J), J > s, of the row tile; P from the block row, Q from
+        // block row J of the tile, both at columns c0..c0+15
+        const int nrem = 3 - s;
+        const int n1 = nrem * (nrem + 1) / 2, n2 = t > 0 ? 4 * nrem : 0;
+        const int g = lane >> 4, q = lane & 15;
+        for (int blk = wave; blk < n1 + n2; blk += 4) {
+            int I, J;
+            double (*Cm)[kCT + 1];
+            if (blk < n1) {
+                int jj = 0, rem = blk;
+                while (rem >= nrem - jj) { rem -= nrem - jj; ++jj; }
+                J = s + 1 + jj;
+                I = J + rem;
+                Cm = sA;
+            } else {
+                const int e = blk - n1;
+                I = e / nrem;
+                J = s + 1 + (e - I * nrem);
+                Cm = sX;
+            }
+            xt_d4c acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k4 = 0; k4 < 4; ++k4) {
+                const double pa = Cm[16 * I + q][c0 + 4 * k4 + g];
+                const double qb = sA[16 * J + q][c0 + 4 * k4 + g];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa, qb, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double &cv = Cm[16 * I + g + 4 * i][16 * J + q];
+                cv = cv - acc[i];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (tid == 0 && sbad) info[0] = k * kCT + sbad;
+        for (int e = tid; e < kCT * kCT; e += 256) {
+            const int c = e >> 6, r = e & 63;
+            W[e] = c <= r ? sA[r][c] : 0.0;
+        }
+        return;
+    }
+    double *wt = W + (size_t)t * kCT * kCT;
+    for (int e = tid; e < kCT * kCT; e += 256) {
+        const int c = e >> 6, r = e & 63;
+        wt[e] = sX[r][c];
+    }
+}
+
+// C (64 x 64 at C, leading dimension b) -= P Q^T with P, Q 64 x 64 column-major in W, on
+// the matrix cores: wave w takes the 16-row band 16w of C, four 16 x 16 blocks, k = 64.
+__device__ inline void tile_update(double *C, int b, const double *P, const double *Q,
+                                   double (*sP)[kCT], double (*sQ)[kCT]) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kCT * kCT; e += 256) {
+        (&sP[0][0])[e] = P[e];   // sP[m][r] = P(r, m)
+        (&sQ[0][0])[e] = Q[e];
+    }
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6, g = lane >> 4, q = lane & 15;
+    xt_d4c acc[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[n] = xt_d4c{0.0, 0.0, 0.0, 0.0};
+    for (int k4 = 0; k4 < kCT / 4; ++k4) {
+        const int m = 4 * k4 + g;
+        const double pa = sP[m][16 * w + q];
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+            acc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa, sQ[m][16 * n + q], acc[n], 0, 0, 0);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double &c = C[(size_t)(16 * n + q) * b + 16 * w + g + 4 * i];
+            c = c - acc[n][i];
+        }
+}
+
+// Work items: [0, ncp) copy W tile t to tall-panel tile (k + t, k); then the lower tiles
+// (i, j), k < j <= i < nt, of L_i; then (if E) the nt x (nt - 1 - k) tiles right of the
+// panel in E_i; then the nt (nt + 1) / 2 lower tiles of Knext.
+__global__ void __launch_bounds__(256)
+chol_update_kernel(int b, int nt, int k, double *L, double *E, double *Knext,
+                   const double *__restrict__ W) {
+    __shared__ double sP[kCT][kCT], sQ[kCT][kCT];
+    const int ntot = E ? 2 * nt : nt;
+    const int ncp = ntot - k, T = nt - 1 - k;
+    int id = blockIdx.x;
+    if (id < ncp) {
+        const int rt = k + id;
+        double *dst = (double *)tall_tile(L, E, b, nt, rt, k);
+        const double *src = W + (size_t)id * kCT * kCT;
+        for (int e = threadIdx.x; e < kCT * kCT; e += 256) {
+            const int c = e >> 6, r = e & 63;
+            dst[(size_t)c * b + r] = src[e];
+        }
+        return;
+    }
+    id -= ncp;
+    const int nl = T * (T + 1) / 2;
+    if (id < nl) {   // lower tile (i, j) of L_i right of the panel: j = k+1+jj, i = j + ii
+        int jj = 0, rem = id;
+        while (rem >= T - jj) { rem -= T - jj; ++jj; }
+        const int j = k + 1 + jj, i = j + rem;
+        tile_update((double *)tall_tile(L, E, b, nt, i, j), b, W + (size_t)(i - k) * kCT * kCT,
+                    W + (size_t)(j - k) * kCT * kCT, sP, sQ);
+        return;
+    }
+    id -= nl;
+    if (!E) return;
+    if (id < nt * T) {   // E_i tile (i, j), j > k
+        const int i = id / T, j = k + 1 + (id - (id / T) * T);
+        tile_update((double *)tall_tile(L, E, b, nt, nt + i, j), b,
+                    W + (size_t)(nt + i - k) * kCT * kCT, W + (size_t)(j - k) * kCT * kCT, sP, sQ);
+        return;
+    }
+    id -= nt * T;
+    // Knext lower tile (i, j), j <= i: -= E_i(row tile i, panel) E_i(row tile j, panel)^T
+    int j = 0, rem = id;
+    while (rem >= nt - j) { rem -= nt - j; ++j; }
+    const int i = j + rem;
+    tile_update(Knext + (size_t)j * kCT * b + (size_t)i * kCT, b,
+                W + (size_t)(nt + i - k) * kCT * kCT, W + (size_t)(nt + j - k) * kCT * kCT, sP, sQ);
+}
+
 // dst_blk = src_blk^T (b x b blocks), 32 x 32 tiles through LDS.
 __global__ void __launch_bounds__(256)
 transpose_kernel(int b, const double *__restrict__ src, double *__restrict__ dst) {
@@ -405,20 +640,28 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
     XT_RB(rocblas_create_handle(&rb.h));
     XT_RB(rocblas_set_stream(rb.h, s));
     XT_RB(rocblas_set_pointer_mode(rb.h, rocblas_pointer_mode_host));
-    const double one = 1.0, mone = -1.0, zero = 0.0;
+    const double one = 1.0, zero = 0.0;
     double *Lp = L.as<double>(), *Ep = E.as<double>();
     int *ip = info.as<int>();
+    // L_ii, E_i = K_{i+1,i} L_ii^-T and K_{i+1,i+1} -= E_i E_i^T, panel by panel
+    const int nt = b / kCT;
+    DevBuf Wb;
+    XT_TRY(Wb.reserve(sizeof(double) * 2 * nt * kCT * kCT));
     for (int i = 0; i < nb; ++i) {
         double *Li = Lp + i * bb;
-        if (i > 0)   // K_ii - E_{i-1} E_{i-1}^T
-            XT_RB(rocblas_dsyrk(rb.h, rocblas_fill_lower, rocblas_operation_none, b, b, &mone,
-                                Ep + (i - 1) * bb, b, &one, Li, b));
-        XT_RB(rocsolver_dpotrf(rb.h, rocblas_fill_lower, b, Li, b, ip + i));
-        if (i + 1 < nb)  // E_i = K_{i+1,i} L_ii^-T
-            XT_RB(rocblas_dtrsm(rb.h, rocblas_side_right, rocblas_fill_lower,
-                                rocblas_operation_transpose, rocblas_diagonal_non_unit, b, b,
-                                &one, Li, b, Ep + i * bb, b));
+        double *Ei = i + 1 < nb ? Ep + i * bb : nullptr;
+        double *Kn = i + 1 < nb ? Lp + (i + 1) * bb : nullptr;
+        const int ntot = Ei ? 2 * nt : nt;
+        for (int k = 0; k < nt; ++k) {
+            const int T = nt - 1 - k;
+            hipLaunchKernelGGL(chol_panel_kernel, dim3(ntot - k), dim3(256), 0, s, b, nt, k, Li,
+                               Ei, Wb.as<double>(), ip + i);
+            const int nwork = (ntot - k) + T * (T + 1) / 2 + (Ei ? nt * T + nt * (nt + 1) / 2 : 0);
+            hipLaunchKernelGGL(chol_update_kernel, dim3(nwork), dim3(256), 0, s, b, nt, k, Li, Ei,
+                               Kn, Wb.as<const double>());
+        }
     }
+    XT_HIP(hipGetLastError());
     std::vector<int> hinfo(nb + 1);
     XT_HIP(hipMemcpyAsync(hinfo.data(), ip, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
     XT_HIP(hipStreamSynchronize(s));

@@ -85,15 +85,16 @@ def coupled_case():
     return mesh, fixed_m, tfixed, rext_m, rext_t
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_coupled_slabs_match_oracle(dev, oracle_lib, mode):
+@pytest.mark.parametrize("mode,shift", [(0, 0.0), (0, "auto"), (1, "auto")])
+def test_coupled_slabs_match_oracle(dev, oracle_lib, mode, shift):
     from xtfem_amd import fem, thermo
     mesh, fixed_m, tfixed, rext_m, rext_t = coupled_case()
     dt, w, nslab = 0.5, 2 * np.pi * 20.0, 3
     ref = oracle_lib.thermo_slabs(mesh, fixed_m, tfixed, rext_m, rext_t, nslab, dt, w,
                                   thermo.YM0, thermo.NU0, thermo.RHO, thermo.KAPPA, thermo.CP,
                                   fem.ALPHA_REF, 0.0, 0.0)
-    run = thermo.ThermoMechanicalSlabs(mesh, fixed_m, tfixed, rext_m, rext_t, dt, w, mode=mode)
+    run = thermo.ThermoMechanicalSlabs(mesh, fixed_m, tfixed, rext_m, rext_t, dt, w, mode=mode,
+                                       thermal_shift=shift)
     for s in range(nslab):
         its_t, its_m = run.step()
         T_ref, u_ref, ym_ref, nu_ref, vv_ref = ref[s]

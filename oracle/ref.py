@@ -137,7 +137,26 @@ def kronlusol(A, alu, jlu, ju, x):
     return y
 
 
-def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
+def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500, log=False):
+    """kronecker.f90:181-365.  With ``log`` the residual history the reference writes to
+    its log unit (gmres.log, FORMAT 199) is returned too, as text."""
+    if log:
+        import tempfile
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as d:
+            os.chdir(d)
+            try:
+                sol, ierr = _kronpgmres(A, B, K, M, prec, rhs, sol0, im, eps, maxits, 40)
+                lib().xt_ref_close_(_ri(40))
+            finally:
+                os.chdir(cwd)
+            name = os.path.join(d, "fort.40")
+            text = open(name).read() if os.path.exists(name) else ""
+        return sol, ierr, text
+    return _kronpgmres(A, B, K, M, prec, rhs, sol0, im, eps, maxits, -1)
+
+
+def _kronpgmres(A, B, K, M, prec, rhs, sol0, im, eps, maxits, iout):
     q = A.shape[0]
     ki, kj, kv = _csr(K)
     mi, mj, mv = _csr(M)
@@ -148,7 +167,7 @@ def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
     ierr = C.c_int(0)
     lib().kronpgmres_(_p(_F(A)), _p(_F(B)), _ri(q), _p(ki), _p(kj), _p(kv), _p(mi), _p(mj),
                       _p(mv), _ri(p), _p(alu), _p(jlu), _p(ju), _p(rhs), _p(sol), _ri(im),
-                      _rd(eps), _ri(maxits), _ri(-1), C.byref(ierr))
+                      _rd(eps), _ri(maxits), _ri(iout), C.byref(ierr))
     return sol, ierr.value
 
 
@@ -391,3 +410,40 @@ def deletion_block(xconn, xcoord, K, M, fixed, necnt, nefail, ym, nu):
     for j in ndfail:
         fixed[3 * (j - 1):3 * j] = 1
     return csrapplybc(K2, fixed), csrapplybc(M, fixed), fixed, necnt, ndfail
+
+
+def calstrain(disp, Nx, Ny, Nz, dmat):
+    """postprocess.f90:79-104 -> (strain [6], stress [6]) at one Gauss point."""
+    strain = np.zeros(6)
+    stress = np.zeros(6)
+    lib().calstrain_(_ri(8), _ri(3), _ri(6), _p(_F(disp)), _p(_F(Nx)), _p(_F(Ny)), _p(_F(Nz)),
+                     _p(_F(dmat)), _p(strain), _p(stress))
+    return strain, stress
+
+
+def mpipostprocess(job, xconn, Nx, Ny, Nz, ym, nu, w, tcoorde, sol, tempst, gpsta, elsta, D,
+                   ws, nip, nipc):
+    """postprocess.f90:106-210 run in a scratch directory (rank 0): returns the text of
+    every file it appends to, keyed by suffix (time, disp, temp, strain, stress, damage,
+    energy, status).  Nx/Ny/Nz: [8, 8*ne] (column = Gauss point)."""
+    import tempfile
+    xconn = _F(xconn, np.int32)
+    ne = xconn.shape[0]
+    nn = len(tempst) // 3
+    name = job.ljust(80).encode()
+    cwd = os.getcwd()
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        os.chdir(d)
+        try:
+            lib().mpipostprocess_(C.c_char_p(name), _ri(ne), _ri(nn), _ri(3), _ri(8), _ri(3),
+                                  _ri(8), _ri(6), _ri(nip), _ri(nipc), _p(xconn), _p(_F(Nx)),
+                                  _p(_F(Ny)), _p(_F(Nz)), _p(_F(ym)), _p(_F(nu)), _rd(w),
+                                  _p(_F(tcoorde)), _p(_F(sol)), _p(_F(tempst)),
+                                  _p(_F(gpsta, np.int32)), _p(_F(elsta, np.int32)), _p(_F(D)),
+                                  _p(_F(ws)), _ri(0), C.c_size_t(80))
+        finally:
+            os.chdir(cwd)
+        for f in os.listdir(d):
+            out[f.rsplit(".", 1)[1]] = open(os.path.join(d, f)).read()
+    return out

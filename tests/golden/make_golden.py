@@ -114,9 +114,11 @@ def main():
     rhs[np.tile(bc, 6) != 0] = 0.0
     g["gmres_rhs"] = rhs
     for eps, tag in ((1e-8, "e8"), (1e-12, "e12")):
-        sol, ierr = ref.kronpgmres(A, B, Kb, Mb, (alu, jlu, ju), rhs, im=30, eps=eps, maxits=3000)
+        sol, ierr, log = ref.kronpgmres(A, B, Kb, Mb, (alu, jlu, ju), rhs, im=30, eps=eps,
+                                        maxits=3000, log=True)
         assert ierr == 0, ierr
         g[f"gmres_{tag}_sol"] = sol
+        g[f"gmres_{tag}_log"] = np.array(log)
     # two-scale damage return map (getdfinal.f) on seeded strain paths
     ngp, nsteps = 24, 1200
     E, nu = YM_REF, NU_REF
@@ -216,6 +218,31 @@ def main():
     tpts = np.array([[0.0, 0.5], [0.3, 0.05], [12.0, 1.0], [7.5, 0.05]])
     g["fxt_pts"] = tpts
     g["fxt"] = np.array([ref.getfextc_t(t1, dt) for t1, dt in tpts])
+    # result files of one output step (mpipostprocess, postprocess.f90:106-210) on a
+    # distorted 3 x 2 x 2 block with per-point material, a failed point and element
+    rng4 = np.random.default_rng(SEED + 3)
+    pm = fem.structured_hex(3, 2, 2, 3.0, 2.0, 2.0)
+    pm = fem.Mesh(pm.xcoord + rng4.uniform(-0.05, 0.05, pm.xcoord.shape), pm.xconn)
+    pym = rng4.uniform(1.9e5, 2.0e5, 8 * pm.ne)
+    pnu = rng4.uniform(0.28, 0.31, 8 * pm.ne)
+    _, _, pNx, pNy, pNz = ref.assemble_m(pm.xcoord, pm.xconn, YM_REF, NU_REF, RHO_REF,
+                                         ym=pym, nu=pnu)
+    psol = rng4.standard_normal(18 * pm.nn) * 10.0 ** rng4.integers(-6, 2, 18 * pm.nn)
+    ptemp = rng4.uniform(-10.0, 50.0, 3 * pm.nn)
+    pgp = np.ones(8 * pm.ne, np.int32)
+    pgp[3] = 0
+    pel = np.ones(pm.ne, np.int32)
+    pel[1] = 0
+    pD = rng4.uniform(0.0, 0.3, 8 * pm.ne)
+    pD[:4] = [0.0, 1e-300, 0.5, 12345.678]
+    pws = rng4.uniform(0.0, 1.0, 8 * pm.ne) * 10.0 ** rng4.integers(-300, 300, 8 * pm.ne)
+    ptc = np.array([0.5, 0.75, 1.0])
+    files = ref.mpipostprocess("job", pm.xconn, pNx, pNy, pNz, pym, pnu, 2 * np.pi * 20, ptc,
+                               psol, ptemp, pgp, pel, pD, pws, 2560, 256)
+    g.update(pp_xcoord=pm.xcoord, pp_xconn=pm.xconn, pp_ym=pym, pp_nu=pnu, pp_sol=psol,
+             pp_temp=ptemp, pp_gpsta=pgp, pp_elsta=pel, pp_D=pD, pp_ws=pws, pp_tc=ptc)
+    for k, v in files.items():
+        g[f"pp_file_{k}"] = np.array(v)
     np.savez_compressed(os.path.join(HERE, "golden_ref.npz"), **g)
     print("wrote", os.path.join(HERE, "golden_ref.npz"))
 

@@ -53,18 +53,33 @@ class Mesh:
 
 
 def read_inp(path) -> Mesh:
-    """Mesh file reader (main3d.f90:220-229): nn, nn x (id x y z), ne, ne x (id n1..n8)."""
+    """Mesh file reader (main3d.f90:216-226): nn, then nn records (id x y z), ne, then
+    ne records (id n1..n8), each read as the reference's list-directed READ reads it: a
+    READ starts at a new record, takes its values across records if one is short, and
+    drops what is left of its last record; blanks and commas separate values."""
     with open(path) as f:
-        toks = f.read().split()
-    pos = 0
-    nn = int(toks[pos]); pos += 1
+        recs = [ln.replace(",", " ").split() for ln in f]
+    pos = [0]
+
+    def read(n):
+        vals = []
+        while len(vals) < n:
+            if pos[0] >= len(recs):
+                raise ValueError(f"{path}: end of file inside a record")
+            vals.extend(recs[pos[0]])
+            pos[0] += 1
+        return vals[:n]
+
+    nn = int(read(1)[0])
     xc = np.zeros((nn, 3))
     for _ in range(nn):
-        i = int(toks[pos]); xc[i - 1] = [float(t) for t in toks[pos + 1:pos + 4]]; pos += 4
-    ne = int(toks[pos]); pos += 1
+        v = read(4)
+        xc[int(v[0]) - 1] = [float(t.replace("d", "e").replace("D", "E")) for t in v[1:]]
+    ne = int(read(1)[0])
     xe = np.zeros((ne, 8), dtype=np.int32)
     for _ in range(ne):
-        i = int(toks[pos]); xe[i - 1] = [int(t) for t in toks[pos + 1:pos + 9]]; pos += 9
+        v = read(9)
+        xe[int(v[0]) - 1] = [int(t) for t in v[1:]]
     return Mesh(xc, xe)
 
 

@@ -109,3 +109,35 @@ def test_coupled_slabs_match_oracle(dev, oracle_lib, mode, shift):
         assert rel(run.asm.vv.cpu().numpy(), vv_ref) < 1e-8
     # the material did change with the temperature
     assert np.ptp(run.asm.ym_t.cpu().numpy()) > 1.0
+
+
+def test_coupled_result_files(dev, oracle_lib, tmp_path):
+    """Result files of a coupled slab: the same records as the host restatement writes
+    from the oracle's loop (numbers to the solver tolerance)."""
+    from xtfem_amd import fem, postprocess as pp, thermo
+    mesh, fixed_m, tfixed, rext_m, rext_t = coupled_case()
+    dt, w = 0.5, 2 * np.pi * 20.0
+    ref = oracle_lib.thermo_slabs(mesh, fixed_m, tfixed, rext_m, rext_t, 2, dt, w,
+                                  thermo.YM0, thermo.NU0, thermo.RHO, thermo.KAPPA, thermo.CP,
+                                  fem.ALPHA_REF, 0.0, 0.0)
+    run = thermo.ThermoMechanicalSlabs(mesh, fixed_m, tfixed, rext_m, rext_t, dt, w)
+    run.step()
+    run.step()
+    (tmp_path / "d").mkdir()
+    (tmp_path / "h").mkdir()
+    t = run.write_results(pp.ResultFiles("job", str(tmp_path / "d")), 2560, 256)
+    T_ref, u_ref, ym_ref, nu_ref, _ = ref[1]
+    _, _, Nx, Ny, Nz = fem.assemble(mesh, ym_ref, nu_ref, thermo.RHO)
+    ne = mesh.ne
+    t2 = pp.ResultFiles("job", str(tmp_path / "h")).write_step(
+        mesh, Nx, Ny, Nz, ym_ref, nu_ref, w, [0.5, 0.75, 1.0], u_ref, T_ref,
+        np.ones(8 * ne, np.int32), np.ones(ne, np.int32), np.zeros(8 * ne), np.zeros(8 * ne),
+        2560, 256)
+    assert t == t2
+    for suf in ("time", "disp", "temp", "strain", "stress", "damage", "energy", "status"):
+        name = f"job.{suf}" if suf in ("time", "disp", "temp") else f"job_0.{suf}"
+        a = (tmp_path / "d" / name).read_text().split()
+        b = (tmp_path / "h" / name).read_text().split()
+        assert len(a) == len(b), suf
+        fa, fb = np.array([float(x) for x in a]), np.array([float(x) for x in b])
+        assert rel(fa, fb) < 1e-6, suf

@@ -205,6 +205,26 @@ class ThermoMechanicalSlabs:
         self.last = dict(relres_t=float(rel_t[0]), relres_m=float(rel_m[0]))
         return int(its_t[0]), int(its_m[0])
 
+    def write_results(self, files, nip, nipc, damage=None):
+        """One output step of the last slab into ``files`` (postprocess.ResultFiles),
+        as the driver writes it every nout slabs (main3d.f90:913-917): displacement,
+        temperature, strain and stress at the output point, and D, ws and the element
+        status from ``damage`` (a device.DamageState; zeros and all active without)."""
+        ne = self.mesh.ne
+        Nx, Ny, Nz = (t.cpu().numpy().reshape(8 * ne, 8) for t in self.asm.Nxyz)
+        if damage is not None:
+            st = damage.host_state()
+            D, ws = st["D"], st["ws"]
+            gpsta = damage.gpsta.cpu().numpy()
+            elsta = damage.elesta.cpu().numpy()
+        else:
+            D = ws = np.zeros(8 * ne)
+            gpsta, elsta = np.ones(8 * ne, np.int32), np.ones(ne, np.int32)
+        return files.write_step(self.mesh, Nx, Ny, Nz, self.asm.ym_t.cpu().numpy(),
+                                self.asm.nu_t.cpu().numpy(), self.w, self.tcoorde,
+                                self.displacement(), self.temperature(), gpsta, elsta, D, ws,
+                                nip, nipc)
+
     def temperature(self):
         """tempst, temporal-major [3*nn] on the host."""
         return self.tempst.view(self.nn, 3).t().reshape(-1).cpu().numpy()

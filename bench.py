@@ -229,31 +229,49 @@ def kernel_profile(run, K, mask):
     return out
 
 
-def spmm_microbench(op, p, reps=20, warm=3):
-    """Kron-SpMM alone, R = 1, FP32: median per-launch time over reps after warm-ups."""
+def spmm_microbench(op, p, reps=20, warm=3, R=1):
+    """Kron-SpMM alone, R load cases, FP32: median per-launch time over reps after
+    warm-ups."""
     import ctypes as C
     import torch
     from xtfem_amd import slab
     from xtfem_amd._lib import check, lib
     c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * 20.0)
-    T = op.tmat(np.stack([c["k"] + c["ktn"], c["m"] + c["mtn"]])[None], True)
+    T = op.tmat(np.tile(np.stack([c["k"] + c["ktn"], c["m"] + c["mtn"]])[None], (R, 1, 1, 1)),
+                True)
     g = torch.Generator(device="cuda").manual_seed(SEED)
-    x = torch.rand(p * 6, device="cuda", generator=g, dtype=torch.float32) * 2 - 1
+    x = torch.rand(p * 6 * R, device="cuda", generator=g, dtype=torch.float32) * 2 - 1
     y = torch.empty_like(x)
     for _ in range(warm):
-        op.apply(T, x, y, 1, fp32=True)
+        op.apply(T, x, y, R, fp32=True)
     times = []
     for _ in range(reps):
         check(lib().xt_prof_enable(1), "prof_enable")
-        op.apply(T, x, y, 1, fp32=True)
+        op.apply(T, x, y, R, fp32=True)
         ms, n, by, fl = C.c_double(), C.c_longlong(), C.c_double(), C.c_double()
         check(lib().xt_prof_read(0, C.addressof(ms), C.addressof(n), C.addressof(by),
                                  C.addressof(fl)), "prof_read")
         times.append(ms.value)
     check(lib().xt_prof_enable(0), "prof_enable")
     med = float(np.median(times))
-    return dict(R=1, dtype="f32", median_us=1e3 * med, bytes=by.value,
+    return dict(R=R, dtype="f32", median_us=1e3 * med, bytes=by.value,
                 gbs=by.value / (med * 1e-3) / 1e9, frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS)
+
+
+def spmm_panel_leg():
+    """Kron-SpMM on the stiffened-panel grid 240 x 200 x 2 C3D8 (145,323 nodes,
+    p = 435,969), clamped at y = 0, K and M assembled on the device: one load case
+    (c = 6, HBM-bound) and 32 (c = 192)."""
+    from xtfem_amd import device, fem
+    mesh = fem.structured_hex(240, 200, 2, 240.0, 200.0, 2.0)
+    asm = device.Assembly(mesh, fem.clamp_mask(mesh, 1, [0.0]))
+    K, M = asm.run(2.0e5, 0.3, 7.85e-9)
+    op = device.KronOperator([(asm.K_pattern[0], asm.K_pattern[1], K.cpu().numpy()),
+                              (asm.M_pattern[0], asm.M_pattern[1], M.cpu().numpy())], q=6)
+    del asm
+    out = {f"R{R}": spmm_microbench(op, 3 * mesh.nn, R=R) for R in (1, 32)}
+    out["workload"] = "panel 240x200x2 C3D8, p=435969, node-block format, FP32"
+    return out
 
 
 # Algorithmic FP64 operations of one Gauss point and time point of the damage kernel
@@ -702,6 +720,12 @@ def main():
               f"{its64.mean():.1f} max {its64.max()}, relres max {rr64.max():.2e}, "
               f"rel err fp32-ir vs fp64 max {err.max():.2e}", file=sys.stderr, flush=True)
     spmm = spmm_microbench(op, prob.p)
+    spmm["R32"] = spmm_microbench(op, prob.p, R=32)
+    spmm["panel"] = spmm_panel_leg()
+    print(f"# kron-spmm fp32: plate R=1 {spmm['gbs']:.0f} GB/s ({spmm['frac']:.2f}), R=32 "
+          f"{spmm['R32']['gbs']:.0f} GB/s; panel R=1 {spmm['panel']['R1']['gbs']:.0f} GB/s "
+          f"({spmm['panel']['R1']['frac']:.2f}), R=32 {spmm['panel']['R32']['gbs']:.0f} GB/s",
+          file=sys.stderr, flush=True)
     res["rod_ilut"] = rod_leg(K)
     print(f"# rod ILUT: fp64 {res['rod_ilut']['fp64']['ms_per_slab']:.2f} ms/slab, fp32-ir "
           f"{res['rod_ilut']['fp32_ir']['ms_per_slab']:.2f} ms/slab", file=sys.stderr, flush=True)

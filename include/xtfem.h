@@ -344,6 +344,12 @@ int xt_prof_enable(unsigned mask);
  * algorithmic bytes and flops since xt_prof_enable.  Synchronises on the last event. */
 int xt_prof_read(int kind, double *ms, long long *launches, double *bytes, double *flops);
 
+/* Replay the banded factor's sweeps (2 + 2 (nb - 1) launches of fixed buffers) from a
+ * HIP graph captured on first use per column count and precision (on = 1, the
+ * default), or launch them one by one (0).  No counterpart in the reference (launch
+ * overhead control).  Process-wide. */
+int xt_graphs_enable(int on);
+
 /* Synchronise the device (all streams of this process). */
 int xt_device_sync(void);
 

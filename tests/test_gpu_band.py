@@ -127,3 +127,46 @@ def test_slab_solve_with_band_factor(dev, oracle_lib, mode):
                                               eps=1e-12, maxits=100)
         assert ie == 0
         assert rel(x[r], ref) < 1e-6
+
+
+@pytest.mark.parametrize("R,fp32", [(1, False), (1, True), (8, True), (32, False)])
+def test_band_sweeps_graph_replay(dev, R, fp32):
+    """The sweeps replayed from a captured HIP graph equal the launch-by-launch sweeps
+    bit for bit, on every replay with eager work and synchronisations between them,
+    and after a refactorization (new factor buffers: a new capture)."""
+    import torch
+    from xtfem_amd._lib import lib
+    S = notched_plate(12, 9, 2, 4)
+    p, K = S["p"], S["Kb"]
+    pr = dev.BandPreconditioner(K)
+    assert pr.nblocks > 2
+    dt = torch.float32 if fp32 else torch.float64
+    g = torch.Generator(device="cuda").manual_seed(5 + R)
+    x = (torch.rand(p * 6 * R, device="cuda", generator=g, dtype=torch.float64) - 0.5).to(dt)
+
+    def apply(on):
+        lib().xt_graphs_enable(int(on))
+        y = torch.empty_like(x)
+        pr.apply(None, x, y, R, q=6, fp32=fp32)
+        torch.cuda.synchronize()
+        return y
+
+    try:
+        want = apply(False)
+        for rep in range(4):
+            got = apply(True)
+            assert torch.equal(got, want), rep
+            z = torch.sin(x) * 3.0        # eager work between replays
+            torch.cuda.synchronize()
+            assert z.numel() == x.numel()
+        # refactor from the device CSR with the same values: new buffers, same result
+        rp = torch.as_tensor(np.asarray(K[0], dtype=np.int32), device="cuda")
+        ci = torch.as_tensor(np.asarray(K[1], dtype=np.int32), device="cuda")
+        va = torch.as_tensor(np.asarray(K[2], dtype=np.float64), device="cuda")
+        pr.refactor(rp, ci, va)
+        want2 = apply(False)
+        for rep in range(3):
+            assert torch.equal(apply(True), want2), rep
+        assert rel(want2.double().cpu().numpy(), want.double().cpu().numpy()) < 1e-12
+    finally:
+        lib().xt_graphs_enable(1)

@@ -76,6 +76,7 @@ _SIGS = {
     "xt_damage_device": (ci, [ci, ci, ci] + [vp] * 18),
     "xt_slab_rhs": (ci, [ci, ci, ci, vp, vp, C.c_longlong, vp, vp, vp]),
     "xt_prof_enable": (ci, [C.c_uint]),
+    "xt_graphs_enable": (ci, [ci]),
     "xt_prof_read": (ci, [ci, vp, vp, vp, vp]),
     "xt_device_sync": (ci, []),
 }

@@ -30,12 +30,33 @@
 
 namespace xt {
 
+// The sweeps of one apply are a fixed sequence of 2 + 2 (nb - 1) launches on fixed
+// buffers for a given column count and precision: captured once into a HIP graph and
+// replayed, they skip the per-launch host overhead.  Keyed on every pointer the
+// launches use, so a refactorization or a grown workspace captures anew.
+struct SweepGraph {
+    int c = 0, pi = -1;
+    const void *key[6] = {};
+    hipGraphExec_t exec = nullptr;
+};
+
 struct BandPrec {
     int p = 0, b = 0, nb = 0, bw = 0;
     // [0] FP64, [1] FP32.  Dl: D_i row-major (lower), Du: D_i^T row-major (upper),
     // G, H: recurrence matrices (nb - 1 of each).
     DevBuf Dl[2], Du[2], G[2], H[2];
     DevBuf w1, w2;
+    SweepGraph graph[4];
+    int next_graph = 0;
+    hipStream_t cap = nullptr;   // capture stream (non-blocking, default priority)
+    BandPrec() = default;
+    BandPrec(const BandPrec &) = delete;
+    BandPrec &operator=(const BandPrec &) = delete;
+    ~BandPrec() {
+        for (SweepGraph &g : graph)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -796,16 +817,56 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     const long long bb = (long long)b * b, bc = (long long)b * c;
     const T *Dl = bp->Dl[pi].as<T>(), *Du = bp->Du[pi].as<T>();
     const T *G = bp->G[pi].as<T>(), *H = bp->H[pi].as<T>();
-    // forward: w2 = D w1 (all blocks), w2_{i+1} -= G_i w2_i
-    XT_TRY(launch_gemm<T>(*bp, c, 1, Dl, bb, w1, bc, nullptr, 0, w2, bc, nb, s));
-    for (int i = 0; i + 1 < nb; ++i)
-        XT_TRY(launch_gemm<T>(*bp, c, 0, G + i * bb, 0, w2 + i * bc, 0, w2 + (i + 1) * bc, 0,
-                              w2 + (i + 1) * bc, 0, 1, s));
-    // backward: w1 = D^T w2 (all blocks), w1_i -= H_i w1_{i+1}
-    XT_TRY(launch_gemm<T>(*bp, c, 2, Du, bb, w2, bc, nullptr, 0, w1, bc, nb, s));
-    for (int i = nb - 2; i >= 0; --i)
-        XT_TRY(launch_gemm<T>(*bp, c, 0, H + i * bb, 0, w1 + (i + 1) * bc, 0, w1 + i * bc, 0,
-                              w1 + i * bc, 0, 1, s));
+    auto sweeps = [&](hipStream_t st) -> int {
+        // forward: w2 = D w1 (all blocks), w2_{i+1} -= G_i w2_i
+        XT_TRY(launch_gemm<T>(*bp, c, 1, Dl, bb, w1, bc, nullptr, 0, w2, bc, nb, st));
+        for (int i = 0; i + 1 < nb; ++i)
+            XT_TRY(launch_gemm<T>(*bp, c, 0, G + i * bb, 0, w2 + i * bc, 0, w2 + (i + 1) * bc,
+                                  0, w2 + (i + 1) * bc, 0, 1, st));
+        // backward: w1 = D^T w2 (all blocks), w1_i -= H_i w1_{i+1}
+        XT_TRY(launch_gemm<T>(*bp, c, 2, Du, bb, w2, bc, nullptr, 0, w1, bc, nb, st));
+        for (int i = nb - 2; i >= 0; --i)
+            XT_TRY(launch_gemm<T>(*bp, c, 0, H + i * bb, 0, w1 + (i + 1) * bc, 0, w1 + i * bc,
+                                  0, w1 + i * bc, 0, 1, st));
+        return XT_OK;
+    };
+    if (graphs_on() && !prof_on(XT_PROF_BAND)) {
+        const void *key[6] = {w1, w2, Dl, Du, G, H};
+        SweepGraph *g = nullptr;
+        for (SweepGraph &cand : bp->graph)
+            if (cand.exec && cand.c == c && cand.pi == pi && std::equal(key, key + 6, cand.key)) {
+                g = &cand;
+                break;
+            }
+        if (!g) {
+            if (!bp->cap) XT_HIP(hipStreamCreateWithFlags(&bp->cap, hipStreamNonBlocking));
+            g = &bp->graph[bp->next_graph];
+            bp->next_graph = (bp->next_graph + 1) % 4;
+            if (g->exec) {
+                (void)hipGraphExecDestroy(g->exec);
+                g->exec = nullptr;
+            }
+            XT_HIP(hipStreamBeginCapture(bp->cap, hipStreamCaptureModeThreadLocal));
+            const int rc = sweeps(bp->cap);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(bp->cap, &graph);
+            if (rc != XT_OK || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                if (rc != XT_OK) return rc;
+                XT_HIP(e);
+            }
+            const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) g->exec = nullptr;
+            XT_HIP(ei);
+            g->c = c;
+            g->pi = pi;
+            std::copy(key, key + 6, g->key);
+        }
+        XT_HIP(hipGraphLaunch(g->exec, s));
+    } else {
+        XT_TRY(sweeps(s));
+    }
     XT_HIP(hipGetLastError());
     XT_HIP(hipMemcpyAsync(y, w1, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
     return XT_OK;

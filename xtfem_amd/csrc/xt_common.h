@@ -97,6 +97,8 @@ int small_inverse_rowmajor(int q, const double *A, double *Ainv);
 
 // Kernel timing (xt_prof.hip): kinds are the XT_PROF_* codes of xtfem.h.
 bool prof_on(int kind);
+// Replay fixed launch sequences (the band sweeps) from captured HIP graphs.
+bool graphs_on();
 void prof_begin(int kind, hipStream_t s);
 void prof_end(int kind, hipStream_t s, double bytes, double flops);
 

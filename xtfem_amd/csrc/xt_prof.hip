@@ -31,6 +31,11 @@ hipEvent_t next_event(Kind &k) {
 
 bool prof_on(int kind) { return (g_mask >> kind) & 1u; }
 
+namespace {
+bool g_graphs = true;
+}
+bool graphs_on() { return g_graphs; }
+
 void prof_begin(int kind, hipStream_t s) {
     if (!prof_on(kind)) return;
     Kind &k = g_kind[kind];
@@ -54,6 +59,11 @@ void prof_end(int kind, hipStream_t s, double bytes, double flops) {
 using namespace xt;
 
 extern "C" {
+
+int xt_graphs_enable(int on) {
+    g_graphs = on != 0;
+    return XT_OK;
+}
 
 int xt_prof_enable(unsigned mask) {
     g_mask = mask;

@@ -587,12 +587,13 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
     } else {
         // GMRES-IR: one flexible GMRES(im) cycle per refinement step on the current
         // residual with the FP32 operator and FP32 preconditioner (FP64 basis, stored
-        // preconditioned vectors), FP64 outer residual.  The inner target starts at 1e-4
-        // of the residual and is relaxed to the reduction the last step achieved when
-        // FP32 rounding is the floor (ill-conditioned temporal matrices); a right-hand
-        // side whose step reduced its residual by less than 2x continues with inner
-        // cycles on the FP64 operator (FP32 preconditioner).
-        std::vector<double> rn(R), irel(R, 1.0e-4), prev(R, 0.0);
+        // preconditioned vectors), FP64 outer residual.  The inner target starts at 1e-3
+        // of the residual (one inner iteration a step with an exact factor) and is
+        // relaxed to the reduction the last step achieved when FP32 rounding is the floor
+        // (ill-conditioned temporal matrices); a right-hand side whose step reduced its
+        // residual by less than 2x continues with inner cycles in FP64 (operator and
+        // preconditioner), where FP32 does not pay.
+        std::vector<double> rn(R), irel(R, 1.0e-3), prev(R, 0.0);
         std::vector<int> fb(R, 0);
         DevBuf &d = ws->d;
         XT_TRY(d.reserve(sizeof(double) * S.n));
@@ -623,7 +624,8 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             if (any32)
                 XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act32, itv, cin, ro_last, 1, true, true));
             if (any64)
-                XT_TRY(S.gmres(d.as<double>(), false, e1, maxits, act64, itv, cin, ro_last, 1, true, true));
+                XT_TRY(S.gmres(d.as<double>(), false, e1, maxits, act64, itv, cin, ro_last, 1, true,
+                               false));
             hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n,
                                d.as<double>(), x);
             XT_HIP(hipGetLastError());

@@ -203,7 +203,8 @@ void xt_prec_destroy(xt_prec pr);
  * (mumps_solver, mumps.f90:22-145; mpilusol, kronecker_mumps.f90:120-183), and is
  * ilut's lfil >= n, droptol = 0 limit (ilut.f:2169-2170).  K is cut into diagonal
  * blocks of max(block, bandwidth) rows rounded up to a multiple of 64 (at most
- * 2048: number the mesh along its longest axis last); factorization in FP64 on the
+ * 4096: number the mesh along its longest axis last, or renumber it by reverse
+ * Cuthill-McKee); factorization in FP64 on the
  * device.  XT_E_SINGULAR if K is not positive definite. */
 int xt_prec_create_band(int p, const int *rowptr, const int *colidx, const double *vals,
                         int block, xt_prec *out);

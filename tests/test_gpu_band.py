@@ -170,3 +170,32 @@ def test_band_sweeps_graph_replay(dev, R, fp32):
         assert rel(want2.double().cpu().numpy(), want.double().cpu().numpy()) < 1e-12
     finally:
         lib().xt_graphs_enable(1)
+
+
+def test_band_factor_after_renumbering_and_wide_blocks(dev):
+    """A scrambled mesh renumbered by reverse Cuthill-McKee takes the band factor; and a
+    cross-section of 31 x 31 nodes (half-bandwidth 2885 DOFs, blocks of 2944 rows, above
+    the former 2048 limit) factors and solves K y = x to FP64 accuracy."""
+    import scipy.sparse as sp
+    import torch
+    from xtfem_amd import fem
+    base = fem.structured_hex(10, 6, 3, 10.0, 6.0, 3.0)
+    scr = fem.renumber(base, np.random.default_rng(3).permutation(base.nn))
+    cases = [fem.renumber(scr, fem.rcm_order(scr)),
+             fem.structured_hex(6, 30, 30, 6.0, 30.0, 30.0)]
+    for mesh in cases:
+        K, *_ = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+        bc = fem.clamp_mask(mesh, 0, [0.0])
+        Kb = fem.applybc(K, bc)
+        p = 3 * mesh.nn
+        pr = dev.BandPreconditioner(Kb)
+        assert pr.block >= pr.bandwidth
+        rng = np.random.default_rng(p)
+        X = rng.standard_normal((6, p))
+        x = torch.as_tensor(np.ascontiguousarray(X.T).ravel(), device="cuda")
+        y = torch.empty_like(x)
+        pr.apply(None, x, y, 1, q=6, fp32=False)
+        Y = y.cpu().numpy().reshape(p, 6).T
+        A = sp.csr_matrix((Kb[2], Kb[1] - 1, Kb[0] - 1), shape=(p, p))
+        for j in range(6):
+            assert np.linalg.norm(A @ Y[j] - X[j]) / np.linalg.norm(X[j]) < 1e-8, (p, j)

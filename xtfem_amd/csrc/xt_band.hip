@@ -773,8 +773,8 @@ int band_build(int p, const int *ia, const int *ja, const double *a, int block, 
         }
     int b = std::max(std::max(block, bw), 64);
     b = (b + 63) / 64 * 64;
-    if (b > 2048) {
-        set_error("bandwidth %d exceeds the supported block size 2048 (renumber the mesh)", bw);
+    if (b > 4096) {
+        set_error("bandwidth %d exceeds the supported block size 4096 (renumber the mesh)", bw);
         return XT_E_UNSUPPORTED;
     }
     const int nb = (p + b - 1) / b;

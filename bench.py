@@ -495,6 +495,38 @@ def thermo_leg(prob, nslab=4):
     return out
 
 
+def driver_panel_leg(nslab=3):
+    """The coupled loop on the reference's own panel mesh (panel_7634_3749: 7634 nodes,
+    3749 C3D8, -50..50 x -35..35 x 0..130 mm, renumbered by reverse Cuthill-McKee) with
+    main3d.f90's boundary conditions, loads and material, slabs of 0.5 s at 20 Hz."""
+    import torch
+    from xtfem_amd import fem, thermo
+    with np.load(os.path.join(ROOT, "tests", "golden", "panel_7634_3749.npz"),
+                 allow_pickle=False) as z:
+        mesh = fem.Mesh(z["xcoord"], z["xconn"])
+    bw0 = fem.node_bandwidth(mesh)
+    mesh = fem.renumber(mesh, fem.rcm_order(mesh))
+    run = thermo.ThermoMechanicalSlabs(mesh, *thermo.driver_conditions(mesh), 0.5,
+                                       2 * np.pi * 20.0, profile=True)
+    run.step()
+    torch.cuda.synchronize()
+    wall, its, phases = [], [], []
+    for _ in range(nslab):
+        t0 = time.perf_counter()
+        its.append(run.step())
+        torch.cuda.synchronize()
+        wall.append(time.perf_counter() - t0)
+        phases.append(run.phase_ms())
+    T = run.temperature()
+    return dict(workload=f"reference panel mesh, {mesh.nn} nodes, {mesh.ne} C3D8, node "
+                         f"half-bandwidth {bw0} -> {fem.node_bandwidth(mesh)} after RCM, "
+                         f"band blocks {run.pr_m.block} (mech) / {run.pr_t.block} (thermal)",
+                ms_per_slab=1e3 * float(np.median(wall)),
+                phases_ms={k: float(np.median([p[k] for p in phases])) for k in run.PHASES},
+                its_thermal=[i[0] for i in its], its_mech=[i[1] for i in its],
+                T_max=float(T.max()), relres_m=run.last["relres_m"])
+
+
 def cpu_baseline(prob, budget_s=15.0):
     """The oracle on the host: FP64 restatement of kronpgmres (kronecker.f90:181-365,
     MGS, one core) preconditioned by A^-1 (x) K^-1 with the exact band Cholesky factor
@@ -741,6 +773,12 @@ def main():
               f"{r['its_thermal']}/{r['its_mech']}; "
               + ", ".join(f"{a} {b:.1f}" for a, b in r["phases_ms"].items()) + " ms",
               file=sys.stderr, flush=True)
+    res["driver_panel"] = driver_panel_leg()
+    print(f"# reference panel mesh, coupled slab: {res['driver_panel']['ms_per_slab']:.1f} ms, its "
+          f"{res['driver_panel']['its_thermal']}/{res['driver_panel']['its_mech']}, T max "
+          f"{res['driver_panel']['T_max']:.1f}; "
+          + ", ".join(f"{a} {b:.1f}" for a, b in res["driver_panel"]["phases_ms"].items()),
+          file=sys.stderr, flush=True)
     res["deletion"] = deletion_leg(prob)
     print(f"# deletion: {res['deletion']['nefail']} elements, "
           + ", ".join(f"{k} {v:.1f} ms" for k, v in res["deletion"]["ms"].items())

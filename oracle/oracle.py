@@ -337,7 +337,7 @@ def deletion_block(xconn, xcoord, K, M, fixed, elesta, nefail, C6):
 
 
 def thermo_slabs(mesh, fixed_m, tfixed, rext_m, rext_t, nslab, dt, w, ym0, nu0, rho, kappa,
-                 cp, alpha, Tref, T0):
+                 cp, alpha, Tref, T0, mechanical=True):
     """The thermo-mechanical slab loop of main3d.f90:579-834 (damage and deletion left
     out) with exact sparse direct solves in place of the preconditioned GMRES.
 
@@ -356,7 +356,7 @@ def thermo_slabs(mesh, fixed_m, tfixed, rext_m, rext_t, nslab, dt, w, ym0, nu0, 
     in xtfem_amd.fem, which the CPU tests pin bit for bit against the compiled
     reference; the temporal data, the Kronecker products and the solves are this
     module's.  Returns a list of (tempst [3*nn], dispst [18*nn], ym, nu, vv) per slab,
-    temporal-major."""
+    temporal-major (only tempst with ``mechanical=False``)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
     from xtfem_amd import fem, slab
@@ -393,6 +393,9 @@ def thermo_slabs(mesh, fixed_m, tfixed, rext_m, rext_t, nslab, dt, w, ym0, nu0, 
         tempst = lu_t.solve(f)
         tempst[fx_t] = fx0_t[fx_t]
         T = tempst[2 * nn:]
+        if not mechanical:
+            out.append((tempst.copy(), None, None, None, None))
+            continue
         ym, nu = fem.material_update(mesh, T, Tref, ym0, nu0)
         vv = fem.thermal_load(mesh, T, Tref, alpha, ym, nu)
         K, M, *_ = fem.assemble(mesh, ym, nu, rho)

@@ -55,6 +55,11 @@ def main():
     mesh = fem.read_inp(INP)
     np.savez_compressed(os.path.join(HERE, "rod_325_192.npz"), xcoord=mesh.xcoord,
                         xconn=mesh.xconn)
+    # the driver's own panel mesh (the geometry main3d.f90's boundary conditions are
+    # written for: -50..50 x -35..35 x 0..130 mm), kept as the reader returns it
+    pan = fem.read_inp(os.path.join(os.path.dirname(INP), "panel_7634_3749.inp"))
+    np.savez_compressed(os.path.join(HERE, "panel_7634_3749.npz"), xcoord=pan.xcoord,
+                        xconn=pan.xconn)
     rng = np.random.default_rng(SEED)
     g = {}
     # temporal matrices and load coefficients (stcoeff.f90, getfextc.f90)

@@ -141,3 +141,37 @@ def test_coupled_result_files(dev, oracle_lib, tmp_path):
         assert len(a) == len(b), suf
         fa, fb = np.array([float(x) for x in a]), np.array([float(x) for x in b])
         assert rel(fa, fb) < 1e-6, suf
+
+
+def test_driver_panel_mesh(dev, oracle_lib):
+    """The coupled loop on the reference's own panel mesh (inp/panel_7634_3749.inp, kept
+    in tests/golden) with the driver's boundary conditions and loads, renumbered by
+    reverse Cuthill-McKee for the band factor: temperatures against the oracle's thermal
+    slabs with direct solves, mechanical slabs to the solver tolerance, constraints
+    held."""
+    import os
+    from conftest import GOLDEN
+    from xtfem_amd import fem, thermo
+    with np.load(os.path.join(GOLDEN, "panel_7634_3749.npz"), allow_pickle=False) as z:
+        mesh = fem.Mesh(z["xcoord"], z["xconn"])
+    mesh = fem.renumber(mesh, fem.rcm_order(mesh))
+    fixed_m, tfixed, rext_m, rext_t = thermo.driver_conditions(mesh)
+    assert tfixed.sum() > 0 and fixed_m.sum() > 0 and rext_m.sum() > 0 and rext_t.sum() > 0
+    dt, w = 0.5, 2 * np.pi * 20.0
+    run = thermo.ThermoMechanicalSlabs(mesh, fixed_m, tfixed, rext_m, rext_t, dt, w)
+    assert run.pr_m.bandwidth <= 3 * fem.node_bandwidth(mesh) + 2
+    ref = oracle_lib.thermo_slabs(mesh, fixed_m, tfixed, rext_m, rext_t, 2, dt, w, thermo.YM0,
+                                  thermo.NU0, thermo.RHO, thermo.KAPPA, thermo.CP,
+                                  fem.ALPHA_REF, 0.0, 0.0, mechanical=False)
+    for s in range(2):
+        its_t, its_m = run.step()
+        T = run.temperature()
+        print(f"panel slab {s + 1}: its {its_t}/{its_m}, T {T.min():.3f}..{T.max():.3f}, "
+              f"rel T {rel(T, ref[s][0]):.2e}, relres m {run.last['relres_m']:.2e}", flush=True)
+        assert rel(T, ref[s][0]) < 1e-8
+        assert run.last["relres_m"] <= 1e-10 and run.last["relres_t"] <= 1e-10
+        u = run.displacement().reshape(6, mesh.nn, 3)
+        assert np.all(u[:, fixed_m.reshape(-1, 3)[:, 0] != 0, :] == 0.0)
+        assert np.abs(u).max() > 0.0
+        Tn = T.reshape(3, mesh.nn)
+        assert np.all(Tn[:, tfixed != 0] == 0.0) and Tn.max() > 0.0

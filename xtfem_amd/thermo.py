@@ -35,6 +35,21 @@ KAPPA = 0.1
 CP = 452.0
 
 
+def driver_conditions(mesh, le=130.0, half_width=35.0, p0=70.0, area_m=664.0, q0=2.0,
+                      area_t=13000.0):
+    """The boundary conditions and loads main3d.f90 sets up for its panel meshes:
+    displacements fixed on z = 0 (and z = 10 le, main3d.f90:482-488), temperatures fixed
+    on z = 0 and z = le (429-440), the traction p0 area_m in z lumped per element-node
+    incidence on z = le (504-526), the heat flux q0 area_t counted on y = +half_width and
+    applied on y = +-half_width (454-478).  Returns (fixed_m [3nn], tfixed [nn],
+    rext_m [3nn], rext_t [nn])."""
+    fixed_m = fem.clamp_mask(mesh, 2, [0.0, 10.0 * le], tol=0.0)
+    tfixed = fem.fixed_temperature_mask(mesh, 2, [0.0, le])
+    rext_m = fem.traction_load(mesh, 2, le, 2, p0 * area_m, tol=0.0)
+    rext_t = fem.heat_flux_load(mesh, 1, half_width, [half_width, -half_width], q0 * area_t)
+    return fixed_m, tfixed, rext_m, rext_t
+
+
 class ThermoMechanicalSlabs:
     """Device state of the coupled loop for one mesh.
 

@@ -60,6 +60,8 @@ def parse():
                     help="temporal points per slab of the damage leg")
     ap.add_argument("--rod-only", action="store_true",
                     help="only run the prismatic-rod ILUT leg (K timed slabs) and print it")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the batched plate solves, SpMM microbenchmarks and kernel profile")
     ap.add_argument("--thermo-only", action="store_true",
                     help="only run the thermo-mechanical plate leg and print it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -255,7 +257,8 @@ def spmm_microbench(op, p, reps=20, warm=3, R=1):
     check(lib().xt_prof_enable(0), "prof_enable")
     med = float(np.median(times))
     return dict(R=R, dtype="f32", median_us=1e3 * med, bytes=by.value,
-                gbs=by.value / (med * 1e-3) / 1e9, frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS)
+                gbs=by.value / (med * 1e-3) / 1e9,
+                frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS)
 
 
 def spmm_panel_leg():
@@ -758,35 +761,36 @@ def main():
           f"{spmm['R32']['gbs']:.0f} GB/s; panel R=1 {spmm['panel']['R1']['gbs']:.0f} GB/s "
           f"({spmm['panel']['R1']['frac']:.2f}), R=32 {spmm['panel']['R32']['gbs']:.0f} GB/s",
           file=sys.stderr, flush=True)
-    res["rod_ilut"] = rod_leg(K)
-    print(f"# rod ILUT: fp64 {res['rod_ilut']['fp64']['ms_per_slab']:.2f} ms/slab, fp32-ir "
-          f"{res['rod_ilut']['fp32_ir']['ms_per_slab']:.2f} ms/slab", file=sys.stderr, flush=True)
-    res["reassembly"] = reassembly_leg(prob)
-    print(f"# re-assembly per slab: "
-          + ", ".join(f"{k} {v:.1f} ms" for k, v in res["reassembly"]["ms"].items())
-          + f"; host numpy assembly {res['reassembly']['host_assembly_ms']:.0f} ms",
-          file=sys.stderr, flush=True)
-    res["thermo"] = thermo_leg(prob)
-    for k in ("fp64_reference_prec", "fp64", "fp32_ir"):
-        r = res["thermo"][k]
-        print(f"# thermo-mechanical slab ({k}): {r['ms_per_slab']:.1f} ms/slab, its "
-              f"{r['its_thermal']}/{r['its_mech']}; "
-              + ", ".join(f"{a} {b:.1f}" for a, b in r["phases_ms"].items()) + " ms",
+    if not args.headline_only:
+        res["rod_ilut"] = rod_leg(K)
+        print(f"# rod ILUT: fp64 {res['rod_ilut']['fp64']['ms_per_slab']:.2f} ms/slab, fp32-ir "
+              f"{res['rod_ilut']['fp32_ir']['ms_per_slab']:.2f} ms/slab", file=sys.stderr, flush=True)
+        res["reassembly"] = reassembly_leg(prob)
+        print(f"# re-assembly per slab: "
+              + ", ".join(f"{k} {v:.1f} ms" for k, v in res["reassembly"]["ms"].items())
+              + f"; host numpy assembly {res['reassembly']['host_assembly_ms']:.0f} ms",
               file=sys.stderr, flush=True)
-    res["driver_panel"] = driver_panel_leg()
-    print(f"# reference panel mesh, coupled slab: {res['driver_panel']['ms_per_slab']:.1f} ms, its "
-          f"{res['driver_panel']['its_thermal']}/{res['driver_panel']['its_mech']}, T max "
-          f"{res['driver_panel']['T_max']:.1f}; "
-          + ", ".join(f"{a} {b:.1f}" for a, b in res["driver_panel"]["phases_ms"].items()),
-          file=sys.stderr, flush=True)
-    res["deletion"] = deletion_leg(prob)
-    print(f"# deletion: {res['deletion']['nefail']} elements, "
-          + ", ".join(f"{k} {v:.1f} ms" for k, v in res["deletion"]["ms"].items())
-          + f", next slab its max {res['deletion']['its_max']}", file=sys.stderr, flush=True)
-    if not args.no_damage:
-        res["damage"] = damage_leg(args.damage_nip)
-        print(f"# damage: {res['damage']['ms_per_slab']:.1f} ms per slab, "
-              f"{res['damage']['gp_steps_per_s']:.3e} GP-steps/s", file=sys.stderr, flush=True)
+        res["thermo"] = thermo_leg(prob)
+        for k in ("fp64_reference_prec", "fp64", "fp32_ir"):
+            r = res["thermo"][k]
+            print(f"# thermo-mechanical slab ({k}): {r['ms_per_slab']:.1f} ms/slab, its "
+                  f"{r['its_thermal']}/{r['its_mech']}; "
+                  + ", ".join(f"{a} {b:.1f}" for a, b in r["phases_ms"].items()) + " ms",
+                  file=sys.stderr, flush=True)
+        res["driver_panel"] = driver_panel_leg()
+        print(f"# reference panel mesh, coupled slab: {res['driver_panel']['ms_per_slab']:.1f} ms, its "
+              f"{res['driver_panel']['its_thermal']}/{res['driver_panel']['its_mech']}, T max "
+              f"{res['driver_panel']['T_max']:.1f}; "
+              + ", ".join(f"{a} {b:.1f}" for a, b in res["driver_panel"]["phases_ms"].items()),
+              file=sys.stderr, flush=True)
+        res["deletion"] = deletion_leg(prob)
+        print(f"# deletion: {res['deletion']['nefail']} elements, "
+              + ", ".join(f"{k} {v:.1f} ms" for k, v in res["deletion"]["ms"].items())
+              + f", next slab its max {res['deletion']['its_max']}", file=sys.stderr, flush=True)
+        if not args.no_damage:
+            res["damage"] = damage_leg(args.damage_nip)
+            print(f"# damage: {res['damage']['ms_per_slab']:.1f} ms per slab, "
+                  f"{res['damage']['gp_steps_per_s']:.3e} GP-steps/s", file=sys.stderr, flush=True)
     # per-kernel durations: the batched FP32-IR steps once more, every launch of the
     # operator and sweep kernels bracketed by HIP events (after the timed legs)
     prof = kernel_profile(run32, K, mask=0b11)

@@ -611,7 +611,7 @@ def cpu_damage_sample(ne_s=1000, nip=2560):
                        f"{t:.1f} s on one host core")
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "band_mfma_kernel_notched_plate_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "band_sweep_kernels_notched_plate_pmc.md")
 
 
 def pmc_traffic():

@@ -371,11 +371,13 @@ struct Solver {
     // max_cycles bounds the restart cycles of this call (GMRES-IR inner solves run one).
     // Precision: f32 = FP32 operator and preconditioner; flexible (FGMRES) stores
     // Z_i = P^-1 V_i and updates x += Z y, with the preconditioner in FP32 when f32prec
-    // and the operator in FP32 when f32.
+    // and the operator in FP32 when f32.  nrm0: the squared column norms of V_0 when the
+    // caller has just computed them with norms2 (ndev then still holds them), so the
+    // first cycle does not compute them again.
     int gmres(double *x, bool f32, const std::vector<double> &eps1, int maxits,
               std::vector<int> &active, std::vector<int> &its, std::vector<int> &conv,
               std::vector<double> &ro_last, int max_cycles = 1 << 30, bool flexible = false,
-              bool f32prec = false) {
+              bool f32prec = false, const std::vector<double> *nrm0 = nullptr) {
         const int K = im;
         std::vector<int> ncyc(R, 0);
         std::vector<double> hh((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K),
@@ -390,7 +392,12 @@ struct Solver {
         std::vector<int> its_before(R);
         for (;;) {
             // start of a cycle: normalise V_0
-            XT_TRY(norms2(Vj(0), nrm));
+            if (nrm0) {
+                nrm = *nrm0;
+                nrm0 = nullptr;
+            } else {
+                XT_TRY(norms2(Vj(0), nrm));
+            }
             bool any = false;
             for (int r = 0; r < R; ++r) {
                 if (!active[r]) { fz[r] = 1; continue; }
@@ -597,8 +604,10 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         std::vector<int> fb(R, 0);
         DevBuf &d = ws->d;
         XT_TRY(d.reserve(sizeof(double) * S.n));
-        for (int outer = 0; outer < 64; ++outer) {
+        for (int outer = 0;; ++outer) {
+            // V_0 holds b - A x for the current x; its norms give the true residual
             XT_TRY(S.norms2(S.Vj(0), rn));
+            if (outer == 64) break;
             bool any32 = false, any64 = false;
             std::vector<double> e1(R, 0.0);
             std::vector<int> act32(R, 0), act64(R, 0);
@@ -621,18 +630,24 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             // inner solves for the correction d: x0 = 0, V_0 = residual (already there)
             XT_HIP(hipMemsetAsync(d.ptr, 0, sizeof(double) * S.n, S.s));
             std::vector<int> cin(R, 0);
+            // the first inner call starts from the norms just computed (V_0 unchanged)
             if (any32)
-                XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act32, itv, cin, ro_last, 1, true, true));
+                XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act32, itv, cin, ro_last, 1, true,
+                               true, &rn));
             if (any64)
                 XT_TRY(S.gmres(d.as<double>(), false, e1, maxits, act64, itv, cin, ro_last, 1, true,
-                               false));
+                               false, any32 ? nullptr : &rn));
             hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n,
                                d.as<double>(), x);
             XT_HIP(hipGetLastError());
             XT_TRY(S.residual(b, x, S.Vj(0)));
-            XT_HIP(hipStreamSynchronize(S.s));
         }
         for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
+        for (int r = 0; r < R; ++r) {   // true residual: the loop's last b - A x
+            its[r] = itv[r];
+            relres[r] = bn[r] > 0 ? std::sqrt(rn[r]) / std::sqrt(bn[r]) : 0.0;
+        }
+        return XT_OK;
     }
     // true residual
     std::vector<double> rn;

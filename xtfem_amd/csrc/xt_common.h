@@ -75,6 +75,36 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Owning page-locked host allocation (hipHostMalloc / hipHostFree): staging for small
+// host <-> device transfers, which then need no pageable bounce copy.  The caller keeps
+// the contents until the copies that read them have completed.
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    void release() { if (ptr) (void)hipHostFree(ptr); ptr = nullptr; bytes = 0; }
+    // Grow to at least n bytes (contents not preserved); no copy may be in flight.
+    int reserve(size_t n) {
+        if (n <= bytes) return XT_OK;
+        release();
+        if (n == 0) return XT_OK;
+        hipError_t e = hipHostMalloc(&ptr, n, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            set_error("hipHostMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+            return XT_E_HIP;
+        }
+        bytes = n;
+        return XT_OK;
+    }
+    template <class T> T *at(size_t off) const {
+        return reinterpret_cast<T *>(static_cast<char *>(ptr) + off);
+    }
+};
+
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace xt
@@ -108,10 +138,12 @@ struct SolverWorkspace {
     DevBuf tm64, tm32, mi64, mi32;   // temporal matrices and inverses
     DevBuf V, Z;                     // Arnoldi basis, preconditioned basis (flexible)
     DevBuf w32a, w32b, z64, t64, d;  // work vectors
-    DevBuf partial, hdev, hdev2, ndev, coef, frozen;
-    // Givens state on the device: Hessenberg columns, rotations, rhs of the least
-    // squares problem, targets, counters and the residual history
-    DevBuf ghh, gcs, gsn, grs, geps, gro, gits, gicyc, ghist;
+    DevBuf partial, hdev, hdev2, ndev, coef;
+    // Givens state of a cycle in one device region (Hessenberg columns, rotations, rhs
+    // of the least squares problem, last residual norms, counters, frozen flags), the
+    // targets and the residual history; pinned host staging mirrors the state region
+    DevBuf gstate, geps, ghist;
+    PinnedBuf hstage;
 };
 SolverWorkspace *op_workspace(xt_op op);
 

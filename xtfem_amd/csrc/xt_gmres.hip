@@ -235,14 +235,27 @@ struct Solver {
     DevBuf &Z;                           // im x n preconditioned basis (flexible variant)
     DevBuf &w32a, &w32b;                 // FP32 work vectors
     DevBuf &z64, &t64;                   // FP64 work vectors
-    DevBuf &partial, &hdev, &hdev2, &ndev, &coef, &frozen;
-    DevBuf &ghh, &gcs, &gsn, &grs, &geps, &gro, &gits, &gicyc, &ghist;
+    DevBuf &partial, &hdev, &hdev2, &ndev, &coef;
+    DevBuf &gstate, &geps, &ghist;
+    PinnedBuf &hstage;
+    // Packed cycle state, the same byte offsets in gstate (device) and hstage (pinned
+    // host): doubles hh | cs | sn | rs | ro, then ints its | icyc | frozen.  A cycle
+    // uploads rs..frozen in one copy and reads the whole state back in one.  hstage
+    // continues with staging areas for norms, coefficients, targets and the temporal
+    // matrices.
+    size_t o_hh = 0, o_cs = 0, o_sn = 0, o_rs = 0, o_ro = 0, o_its = 0, o_icyc = 0, o_fz = 0;
+    size_t st_bytes = 0, o_nrm = 0, o_coef = 0, o_eps = 0, o_tm = 0, o_mi = 0, o_tm32 = 0,
+           o_mi32 = 0;
+    double *ghh = nullptr, *gcs = nullptr, *gsn = nullptr, *grs = nullptr, *gro = nullptr;
+    int *gits = nullptr, *gicyc = nullptr, *frozen = nullptr;
     explicit Solver(SolverWorkspace &w)
         : tm64(w.tm64), tm32(w.tm32), mi64(w.mi64), mi32(w.mi32), V(w.V), Z(w.Z), w32a(w.w32a),
           w32b(w.w32b), z64(w.z64), t64(w.t64), partial(w.partial), hdev(w.hdev),
-          hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), frozen(w.frozen), ghh(w.ghh), gcs(w.gcs),
-          gsn(w.gsn), grs(w.grs), geps(w.geps), gro(w.gro), gits(w.gits), gicyc(w.gicyc),
-          ghist(w.ghist) {}
+          hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), gstate(w.gstate), geps(w.geps),
+          ghist(w.ghist), hstage(w.hstage) {}
+    template <class T> T *dst(size_t off) {
+        return reinterpret_cast<T *>(static_cast<char *>(gstate.ptr) + off);
+    }
     // Arnoldi steps between host checks of the frozen flags: a few for small systems,
     // where a step costs less than the host round trip; every step for large ones, where
     // a step past convergence would cost more than the check
@@ -272,15 +285,39 @@ struct Solver {
         XT_TRY(hdev2.reserve(sizeof(double) * (im + 1) * R));
         XT_TRY(ndev.reserve(sizeof(double) * R));
         XT_TRY(coef.reserve(sizeof(double) * (im + 1) * R));
-        XT_TRY(frozen.reserve(sizeof(int) * R));
-        XT_TRY(ghh.reserve(sizeof(double) * (size_t)R * (im + 1) * im));
-        XT_TRY(gcs.reserve(sizeof(double) * (size_t)R * im));
-        XT_TRY(gsn.reserve(sizeof(double) * (size_t)R * im));
-        XT_TRY(grs.reserve(sizeof(double) * (size_t)R * (im + 1)));
         XT_TRY(geps.reserve(sizeof(double) * R));
-        XT_TRY(gro.reserve(sizeof(double) * R));
-        XT_TRY(gits.reserve(sizeof(int) * R));
-        XT_TRY(gicyc.reserve(sizeof(int) * R));
+        size_t o = 0;
+        auto take = [&o](size_t bytes) {
+            const size_t at = o;
+            o += (bytes + 255) / 256 * 256;
+            return at;
+        };
+        o_hh = take(sizeof(double) * (size_t)R * (im + 1) * im);
+        o_cs = take(sizeof(double) * (size_t)R * im);
+        o_sn = take(sizeof(double) * (size_t)R * im);
+        o_rs = take(sizeof(double) * (size_t)R * (im + 1));
+        o_ro = take(sizeof(double) * R);
+        o_its = take(sizeof(int) * R);
+        o_icyc = take(sizeof(int) * R);
+        o_fz = take(sizeof(int) * R);
+        st_bytes = o;
+        o_nrm = take(sizeof(double) * R);
+        o_coef = take(sizeof(double) * (im + 1) * R);
+        o_eps = take(sizeof(double) * R);
+        o_tm = take(sizeof(double) * (size_t)R * 2 * q * q);
+        o_mi = take(sizeof(double) * (size_t)R * q * q);
+        o_tm32 = take(sizeof(float) * (size_t)R * 2 * q * q);
+        o_mi32 = take(sizeof(float) * (size_t)R * q * q);
+        XT_TRY(gstate.reserve(st_bytes));
+        XT_TRY(hstage.reserve(o));
+        ghh = dst<double>(o_hh);
+        gcs = dst<double>(o_cs);
+        gsn = dst<double>(o_sn);
+        grs = dst<double>(o_rs);
+        gro = dst<double>(o_ro);
+        gits = dst<int>(o_its);
+        gicyc = dst<int>(o_icyc);
+        frozen = dst<int>(o_fz);
         return XT_OK;
     }
 
@@ -359,9 +396,10 @@ struct Solver {
     }
     int norms2(const double *v, std::vector<double> &out) {
         XT_TRY(dots(1, v, v, ndev.as<double>()));
-        out.resize(R);
-        XT_HIP(hipMemcpyAsync(out.data(), ndev.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+        double *h = hstage.at<double>(o_nrm);
+        XT_HIP(hipMemcpyAsync(h, ndev.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
         XT_HIP(hipStreamSynchronize(s));
+        out.assign(h, h + R);
         return XT_OK;
     }
 
@@ -388,7 +426,9 @@ struct Solver {
         for (int r = 0; r < R; ++r) conv[r] = 0;
         const int hist_cap = maxits + K + 2;
         if (hist) XT_TRY(ghist.reserve(sizeof(double) * hist_cap));
-        XT_HIP(hipMemcpyAsync(geps.ptr, eps1.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
+        std::copy(eps1.begin(), eps1.begin() + R, hstage.at<double>(o_eps));
+        XT_HIP(hipMemcpyAsync(geps.ptr, hstage.at<double>(o_eps), sizeof(double) * R,
+                              hipMemcpyHostToDevice, s));
         std::vector<int> its_before(R);
         for (;;) {
             // start of a cycle: normalise V_0
@@ -414,13 +454,17 @@ struct Solver {
             // the cycle's Givens state lives on the device; the host reads the frozen
             // flags every kCheck steps and the whole state at the end of the cycle
             its_before = its;
-            XT_HIP(hipMemcpyAsync(frozen.ptr, fz.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
-            XT_HIP(hipMemcpyAsync(grs.ptr, rs.data(), sizeof(double) * rs.size(), hipMemcpyHostToDevice, s));
-            XT_HIP(hipMemcpyAsync(gits.ptr, its.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
-            XT_HIP(hipMemcpyAsync(gicyc.ptr, icyc.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
-            XT_HIP(hipMemcpyAsync(gro.ptr, ro_last.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
+            // rs | ro | its | icyc | frozen in one copy (the previous copies that read this
+            // staging area finished at the last synchronisation)
+            std::copy(rs.begin(), rs.end(), hstage.at<double>(o_rs));
+            std::copy(ro_last.begin(), ro_last.begin() + R, hstage.at<double>(o_ro));
+            std::copy(its.begin(), its.begin() + R, hstage.at<int>(o_its));
+            std::copy(icyc.begin(), icyc.end(), hstage.at<int>(o_icyc));
+            std::copy(fz.begin(), fz.end(), hstage.at<int>(o_fz));
+            XT_HIP(hipMemcpyAsync(grs, hstage.at<char>(o_rs), st_bytes - o_rs,
+                                  hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
-                               ndev.as<double>(), frozen.as<int>(), Vj(0));
+                               ndev.as<double>(), frozen, Vj(0));
             int i = 0;
             for (;;) {
                 // Arnoldi step i -> i+1
@@ -434,31 +478,41 @@ struct Solver {
                 // normalise V_{i+1} for every RHS still in its cycle at this step
                 // (the restart recombination reads it); zero it for the others
                 hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
-                                   ndev.as<double>(), frozen.as<int>(), Vj(i + 1));
+                                   ndev.as<double>(), frozen, Vj(i + 1));
                 hipLaunchKernelGGL(givens_kernel, dim3(ceil_div(R, 64)), dim3(64), 0, s, R, i, K,
-                                   hdev.as<double>(), hdev2.as<double>(), ndev.as<double>(),
-                                   ghh.as<double>(), gcs.as<double>(), gsn.as<double>(),
-                                   grs.as<double>(), frozen.as<int>(), geps.as<double>(),
-                                   gits.as<int>(), gicyc.as<int>(), gro.as<double>(),
+                                   hdev.as<double>(), hdev2.as<double>(), ndev.as<double>(), ghh,
+                                   gcs, gsn, grs, frozen, geps.as<double>(), gits, gicyc, gro,
                                    hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
                 XT_HIP(hipGetLastError());
                 if ((i + 1) % check_every() == 0 || i + 1 == K) {
-                    XT_HIP(hipMemcpyAsync(fz.data(), frozen.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+                    XT_HIP(hipMemcpyAsync(hstage.at<int>(o_fz), frozen, sizeof(int) * R,
+                                          hipMemcpyDeviceToHost, s));
                     XT_HIP(hipStreamSynchronize(s));
+                    std::copy(hstage.at<int>(o_fz), hstage.at<int>(o_fz) + R, fz.begin());
                     bool cont = false;
                     for (int r = 0; r < R; ++r) cont = cont || !fz[r];
                     if (!cont) break;
                 }
                 ++i;
             }
-            XT_HIP(hipMemcpyAsync(hh.data(), ghh.ptr, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, s));
-            XT_HIP(hipMemcpyAsync(cs.data(), gcs.ptr, sizeof(double) * cs.size(), hipMemcpyDeviceToHost, s));
-            XT_HIP(hipMemcpyAsync(sn.data(), gsn.ptr, sizeof(double) * sn.size(), hipMemcpyDeviceToHost, s));
-            XT_HIP(hipMemcpyAsync(rs.data(), grs.ptr, sizeof(double) * rs.size(), hipMemcpyDeviceToHost, s));
-            XT_HIP(hipMemcpyAsync(icyc.data(), gicyc.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
-            XT_HIP(hipMemcpyAsync(its.data(), gits.ptr, sizeof(int) * R, hipMemcpyDeviceToHost, s));
-            XT_HIP(hipMemcpyAsync(ro_last.data(), gro.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+            XT_HIP(hipMemcpyAsync(hstage.ptr, gstate.ptr, st_bytes, hipMemcpyDeviceToHost, s));
             XT_HIP(hipStreamSynchronize(s));
+            {
+                const double *h = hstage.at<double>(o_hh);
+                std::copy(h, h + hh.size(), hh.begin());
+                h = hstage.at<double>(o_cs);
+                std::copy(h, h + cs.size(), cs.begin());
+                h = hstage.at<double>(o_sn);
+                std::copy(h, h + sn.size(), sn.begin());
+                h = hstage.at<double>(o_rs);
+                std::copy(h, h + rs.size(), rs.begin());
+                h = hstage.at<double>(o_ro);
+                std::copy(h, h + R, ro_last.begin());
+                const int *hi = hstage.at<int>(o_icyc);
+                std::copy(hi, hi + R, icyc.begin());
+                hi = hstage.at<int>(o_its);
+                std::copy(hi, hi + R, its.begin());
+            }
             if (hist && its[0] > its_before[0]) {
                 std::vector<double> hv(its[0] - its_before[0]);
                 XT_HIP(hipMemcpy(hv.data(), ghist.as<double>() + its_before[0] + 1,
@@ -481,7 +535,9 @@ struct Solver {
                 }
                 for (int j = 0; j < ir; ++j) y[(size_t)j * R + r] = yy[j];
             }
-            XT_HIP(hipMemcpyAsync(coef.ptr, y.data(), sizeof(double) * kmax * R, hipMemcpyHostToDevice, s));
+            std::copy(y.begin(), y.end(), hstage.at<double>(o_coef));
+            XT_HIP(hipMemcpyAsync(coef.ptr, hstage.at<double>(o_coef), sizeof(double) * kmax * R,
+                                  hipMemcpyHostToDevice, s));
             if (flexible) {
                 XT_TRY(maxpy(kmax, 1.0, coef.as<double>(), Zj(0), x, x));
             } else {
@@ -504,9 +560,11 @@ struct Solver {
                 for (int j = 0; j <= ir; ++j) rr[(size_t)j * R + r] = rs_[j];
                 again = true;
             }
-            XT_HIP(hipStreamSynchronize(s));
             if (!again) break;
-            XT_HIP(hipMemcpyAsync(coef.ptr, rr.data(), sizeof(double) * (kmax + 1) * R, hipMemcpyHostToDevice, s));
+            XT_HIP(hipStreamSynchronize(s));   // y's copy read the staging area rr reuses
+            std::copy(rr.begin(), rr.end(), hstage.at<double>(o_coef));
+            XT_HIP(hipMemcpyAsync(coef.ptr, hstage.at<double>(o_coef),
+                                  sizeof(double) * (kmax + 1) * R, hipMemcpyHostToDevice, s));
             XT_TRY(maxpy(kmax + 1, 1.0, coef.as<double>(), Vj(0), nullptr, t64.as<double>()));
             XT_HIP(hipMemcpyAsync(Vj(0), t64.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             XT_HIP(hipStreamSynchronize(s));
@@ -567,11 +625,14 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         }
     }
     std::vector<float> tm32(tm.begin(), tm.end()), mi32(mi.begin(), mi.end());
-    XT_TRY(S.tm64.upload(tm.data(), sizeof(double) * tm.size(), S.s));
-    XT_TRY(S.mi64.upload(mi.data(), sizeof(double) * mi.size(), S.s));
-    XT_TRY(S.tm32.upload(tm32.data(), sizeof(float) * tm32.size(), S.s));
-    XT_TRY(S.mi32.upload(mi32.data(), sizeof(float) * mi32.size(), S.s));
-    XT_HIP(hipStreamSynchronize(S.s));
+    std::copy(tm.begin(), tm.end(), S.hstage.at<double>(S.o_tm));
+    std::copy(mi.begin(), mi.end(), S.hstage.at<double>(S.o_mi));
+    std::copy(tm32.begin(), tm32.end(), S.hstage.at<float>(S.o_tm32));
+    std::copy(mi32.begin(), mi32.end(), S.hstage.at<float>(S.o_mi32));
+    XT_TRY(S.tm64.upload(S.hstage.at<double>(S.o_tm), sizeof(double) * tm.size(), S.s));
+    XT_TRY(S.mi64.upload(S.hstage.at<double>(S.o_mi), sizeof(double) * mi.size(), S.s));
+    XT_TRY(S.tm32.upload(S.hstage.at<float>(S.o_tm32), sizeof(float) * tm32.size(), S.s));
+    XT_TRY(S.mi32.upload(S.hstage.at<float>(S.o_mi32), sizeof(float) * mi32.size(), S.s));
 
     std::vector<int> active(R, 1), itv(R, 0), conv(R, 0);
     std::vector<double> ro_last(R, 0.0), eps1(R, 0.0), r0(R), bn(R);
@@ -580,6 +641,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
     // V_0 = b - A x0
     XT_TRY(S.residual(b, x, S.Vj(0)));
     XT_TRY(S.norms2(S.Vj(0), r0));
+    const std::vector<double> r0sq = r0;   // squared norms of V_0 (ndev still holds them)
     for (int r = 0; r < R; ++r) {
         r0[r] = std::sqrt(r0[r]);
         eps1[r] = eps * r0[r];
@@ -605,8 +667,10 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         DevBuf &d = ws->d;
         XT_TRY(d.reserve(sizeof(double) * S.n));
         for (int outer = 0;; ++outer) {
-            // V_0 holds b - A x for the current x; its norms give the true residual
-            XT_TRY(S.norms2(S.Vj(0), rn));
+            // V_0 holds b - A x for the current x; its norms give the true residual (the
+            // first pass has them from the setup)
+            if (outer == 0) rn = r0sq;
+            else XT_TRY(S.norms2(S.Vj(0), rn));
             if (outer == 64) break;
             bool any32 = false, any64 = false;
             std::vector<double> e1(R, 0.0);

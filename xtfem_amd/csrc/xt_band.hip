@@ -182,7 +182,7 @@ struct Mfma;
 template <>
 struct Mfma<float> {
     typedef xt_f4m acc_t;
-    static constexpr int KG = 4, UNR = 2;   // 2 chunks a pass: fastest measured (of 1-12)
+    static constexpr int KG = 4, UNR = 4;
     __device__ static acc_t step(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }

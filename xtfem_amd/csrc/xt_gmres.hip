@@ -485,8 +485,10 @@ struct Solver {
                                    hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
                 XT_HIP(hipGetLastError());
                 if ((i + 1) % check_every() == 0 || i + 1 == K) {
-                    XT_HIP(hipMemcpyAsync(hstage.at<int>(o_fz), frozen, sizeof(int) * R,
-                                          hipMemcpyDeviceToHost, s));
+                    // the whole state in one copy: the cycle ends at a check where every
+                    // case is frozen (all are at i + 1 == K), so this is then its final state
+                    XT_HIP(hipMemcpyAsync(hstage.ptr, gstate.ptr, st_bytes, hipMemcpyDeviceToHost,
+                                          s));
                     XT_HIP(hipStreamSynchronize(s));
                     std::copy(hstage.at<int>(o_fz), hstage.at<int>(o_fz) + R, fz.begin());
                     bool cont = false;
@@ -495,9 +497,7 @@ struct Solver {
                 }
                 ++i;
             }
-            XT_HIP(hipMemcpyAsync(hstage.ptr, gstate.ptr, st_bytes, hipMemcpyDeviceToHost, s));
-            XT_HIP(hipStreamSynchronize(s));
-            {
+            {   // the state read at the last check
                 const double *h = hstage.at<double>(o_hh);
                 std::copy(h, h + hh.size(), hh.begin());
                 h = hstage.at<double>(o_cs);

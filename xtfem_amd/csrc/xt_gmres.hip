@@ -364,7 +364,10 @@ struct Solver {
         } else {
             XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), v, z, false, s));
         }
-        if (f32op) {
+        if (f32op && f32prec) {   // w32b holds Z_i in FP32 exactly (Z_i is its widening)
+            XT_TRY(op_apply_internal(op, R, tm32.ptr, 1.0, 0.0, w32b.ptr, w32a.ptr, 1, s));
+            hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32a.as<float>(), w);
+        } else if (f32op) {
             hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, z, w32a.as<float>());
             XT_TRY(op_apply_internal(op, R, tm32.ptr, 1.0, 0.0, w32a.ptr, w32b.ptr, 1, s));
             hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), w);

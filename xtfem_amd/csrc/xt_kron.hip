@@ -171,12 +171,50 @@ mix_kernel(long long n, int c, int q, const TI *__restrict__ M, const TI *__rest
     y[idx] = (TO)s;
 }
 
+// The same mix for the temporal orders of the slabs (Q = 6 mechanical, 3 thermal): one
+// thread per (DOF, case) group g = dof * R + r, whose Q inputs x[g Q .. g Q + Q) and Q
+// outputs are contiguous; the R matrices sit in LDS.  The sums run in the same order as
+// mix_kernel's (bit-identical), without its per-element 64-bit index division.
+template <typename TI, typename TO, int Q>
+__global__ void __launch_bounds__(256)
+mix_group_kernel(int ngroups, int R, const TI *__restrict__ M, const TI *__restrict__ x,
+                 TO *__restrict__ y) {
+    __shared__ TI Ms[256 * Q];   // R Q <= 256
+    for (int e = threadIdx.x; e < R * Q * Q; e += 256) Ms[e] = M[e];
+    __syncthreads();
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ngroups) return;
+    const int r = g % R;
+    const TI *xr = x + (size_t)g * Q;
+    TI v[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) v[j] = xr[j];
+    const TI *m = Ms + r * Q * Q;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        TI sum = TI(0);
+#pragma unroll
+        for (int j = 0; j < Q; ++j) sum += m[i * Q + j] * v[j];
+        y[(size_t)g * Q + i] = (TO)sum;
+    }
+}
+
 template <typename TI, typename TO>
 int temporal_mix(int p, int q, int R, const TI *M, const TI *x, TO *y, hipStream_t s) {
     long long n = (long long)p * q * R;
     if (n == 0) return XT_OK;
-    hipLaunchKernelGGL((mix_kernel<TI, TO>), dim3(ceil_div(n, 256)), dim3(256), 0, s, n, q * R,
-                       q, M, x, y);
+    const long long ng = (long long)p * R;
+    if ((q == 6 || q == 3) && q * R <= 256 && ng < (1LL << 31)) {
+        if (q == 6)
+            hipLaunchKernelGGL((mix_group_kernel<TI, TO, 6>), dim3(ceil_div(ng, 256)), dim3(256), 0,
+                               s, (int)ng, R, M, x, y);
+        else
+            hipLaunchKernelGGL((mix_group_kernel<TI, TO, 3>), dim3(ceil_div(ng, 256)), dim3(256), 0,
+                               s, (int)ng, R, M, x, y);
+    } else {
+        hipLaunchKernelGGL((mix_kernel<TI, TO>), dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                           q * R, q, M, x, y);
+    }
     XT_HIP(hipGetLastError());
     return XT_OK;
 }

@@ -13,13 +13,15 @@
 //   forward   y_0 = D_0 x_0,      y_{i+1} = D_{i+1} x_{i+1} - G_i y_i,   G_i = D_{i+1} E_i
 //   backward  z_nb = D^T y,       z_i     = D_i^T y_i     - H_i z_{i+1}, H_i = (E_i D_i)^T
 //
-// The D x and D^T y products of all blocks run in one batched launch.  Each recurrence
-// advances two blocks a step: with c = D x,
-//   y_j = (c_j - G_{j-1} c_{j-1}) - P_j y_{j-2},   P_j = -G_{j-1} G_{j-2}   (j = 2, 4, ..)
-// so one batched launch forms the bracketed right-hand sides, the dependent chain runs
-// over every second block only (one launch a step), and one more batched launch fills
-// in the blocks between (y_k = c_k - G_{k-1} y_{k-1}); the backward sweep pairs
-// z_j = (c_j - H_j c_{j+1}) - Q_j z_{j+2}, Q_j = -H_j H_{j+1}, from the last block down.
+// The D x and D^T y products of all blocks run in one batched launch.  A recurrence
+// w_v -= M_v w_{v-1} (v = 1..m, m = nb - 1; M_v = G_{v-1} forward, and H in reverse
+// block order backward) advances 2^L blocks per dependent step: pairing two steps,
+//   w_v -= M_v w_{v-1}  for even v            (one batched launch: the new right-hand sides)
+//   w_v -= M'_v w_{v-2} for v = 2, 4, .. in order,   M'_v = -M_v M_{v-1}
+//   w_v -= M_v w_{v-1}  for odd v             (one batched launch: the blocks between)
+// and the middle line is again such a recurrence over every second block, paired in
+// turn for L levels.  The level matrices are formed once per factorization; the
+// dependent chain shrinks from m launches to m / 2^L plus 2 L batched launches.
 // All matrix sets are stored row-major (the row of the operator a tile needs is
 // contiguous) in FP64 and FP32.
 //
@@ -41,19 +43,23 @@ namespace xt {
 // buffers for a given column count and precision: captured once into a HIP graph and
 // replayed, they skip the per-launch host overhead.  Keyed on every pointer the
 // launches use, so a refactorization or a grown workspace captures anew.
+constexpr int kLevels = 4;     // pairing levels of the block recurrences, at most
+constexpr int kMinChain = 6;   // a level is added while its chain keeps this many steps
+constexpr int kKey = 6 + 2 * kLevels;
 struct SweepGraph {
     int c = 0, pi = -1;
-    const void *key[8] = {};
+    const void *key[kKey] = {};
     hipGraphExec_t exec = nullptr;
 };
 
 struct BandPrec {
     int p = 0, b = 0, nb = 0, bw = 0;
-    // [0] FP64, [1] FP32.  Dl: D_i row-major (lower), Du: D_i^T row-major (upper),
-    // G, H: recurrence matrices (nb - 1 of each); P, Q: the paired-step products
-    // (floor((nb - 1) / 2) of each; P[t] for block j = 2 t + 2, Q[t] for the t-th chain
-    // block of the backward sweep counted upward).
-    DevBuf Dl[2], Du[2], G[2], H[2], P[2], Q[2];
+    // [.][0] FP64, [.][1] FP32.  Dl: D_i row-major (lower), Du: D_i^T row-major (upper),
+    // G, H: recurrence matrices (nb - 1 of each); Lv[dir][L - 1]: the level-L matrices of
+    // the forward (0) and backward (1) recurrence, (nb - 1) >> L of them in chain order.
+    DevBuf Dl[2], Du[2], G[2], H[2];
+    DevBuf Lv[2][kLevels][2];
+    int nlev = 0;
     DevBuf w1, w2;
     SweepGraph graph[4];
     int next_graph = 0;
@@ -833,36 +839,46 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
                                             rocblas_operation_none, b, b, b, &one, Ep, b, bb,
                                             Lp, b, bb, &zero, H.as<double>(), b, bb, nb - 1));
     }
-    DevBuf P, Q;
-    const int m = nb - 1, npair = m / 2;
-    if (npair > 0) {
-        // Row-major X is column-major X^T: P_t = -(G_{2t+1} G_{2t}) row-major is
-        // -(G_{2t}^T G_{2t+1}^T) column-major, the product of the stored blocks in order
-        const double minus = -1.0;
-        XT_TRY(P.reserve(sizeof(double) * bb * npair));
-        XT_TRY(Q.reserve(sizeof(double) * bb * npair));
-        XT_RB(rocblas_dgemm_strided_batched(rb.h, rocblas_operation_none,
-                                            rocblas_operation_none, b, b, b, &minus,
-                                            G.as<double>(), b, 2 * bb, G.as<double>() + bb, b,
-                                            2 * bb, &zero, P.as<double>(), b, bb, npair));
-        // Q for chain blocks j = jmin, jmin + 2, .., m - 2 (jmin = m - 2 npair):
-        // -(H_j H_{j+1}) row-major = -(H_{j+1}^T H_j^T) column-major
-        const int jmin = m - 2 * npair;
-        XT_RB(rocblas_dgemm_strided_batched(rb.h, rocblas_operation_none,
-                                            rocblas_operation_none, b, b, b, &minus,
-                                            H.as<double>() + (jmin + 1) * bb, b, 2 * bb,
-                                            H.as<double>() + jmin * bb, b, 2 * bb, &zero,
-                                            Q.as<double>(), b, bb, npair));
-    }
+    // Level matrices: M^L_t = -(M^{L-1}_{2t+1} M^{L-1}_{2t}), t < m >> L, where
+    // M^0_t = G_t forward and H_{m-1-t} backward.  Row-major X is column-major X^T, so
+    // the row-major product is the column-major product of the stored blocks in the
+    // order M_{2t}, M_{2t+1}.
+    const int m = nb - 1;
+    int nlev = 0;
+    while (nlev < kLevels && (m >> (nlev + 1)) >= kMinChain) ++nlev;
+    DevBuf lv[2][kLevels];
+    const double minus = -1.0;
+    for (int dir = 0; dir < 2 && nlev > 0; ++dir)
+        for (int L = 1; L <= nlev; ++L) {
+            const int cnt = m >> L;
+            XT_TRY(lv[dir][L - 1].reserve(sizeof(double) * bb * cnt));
+            for (int t = 0; t < cnt; ++t) {
+                const double *A, *B;
+                if (L == 1 && dir == 0) {
+                    A = G.as<double>() + 2LL * t * bb;
+                    B = A + bb;
+                } else if (L == 1) {
+                    A = H.as<double>() + (long long)(m - 1 - 2 * t) * bb;
+                    B = A - bb;
+                } else {
+                    A = lv[dir][L - 2].as<double>() + 2LL * t * bb;
+                    B = A + bb;
+                }
+                XT_RB(rocblas_dgemm(rb.h, rocblas_operation_none, rocblas_operation_none, b, b, b,
+                                    &minus, A, b, B, b, &zero,
+                                    lv[dir][L - 1].as<double>() + (long long)t * bb, b));
+            }
+        }
     bp.Du[0] = std::move(L);
     bp.Dl[0] = std::move(Dl);
     bp.G[0] = std::move(G);
     bp.H[0] = std::move(H);
-    bp.P[0] = std::move(P);
-    bp.Q[0] = std::move(Q);
-    for (int k = 0; k < 6; ++k) {
+    for (int dir = 0; dir < 2; ++dir)
+        for (int L = 0; L < kLevels; ++L) bp.Lv[dir][L][0] = std::move(lv[dir][L]);
+    bp.nlev = nlev;
+    for (int k = 0; k < 4 + 2 * kLevels; ++k) {
         DevBuf *src = k == 0 ? bp.Dl : k == 1 ? bp.Du : k == 2 ? bp.G : k == 3 ? bp.H
-                    : k == 4 ? bp.P : bp.Q;
+                    : bp.Lv[(k - 4) / kLevels][(k - 4) % kLevels];
         if (!src[0].ptr) continue;
         const long long n = (long long)(src[0].bytes / sizeof(double));
         XT_TRY(src[1].reserve(sizeof(float) * n));
@@ -954,54 +970,54 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     const long long bb = (long long)b * b, bc = (long long)b * c;
     const T *Dl = bp->Dl[pi].as<T>(), *Du = bp->Du[pi].as<T>();
     const T *G = bp->G[pi].as<T>(), *H = bp->H[pi].as<T>();
-    const T *P = bp->P[pi].as<T>(), *Q = bp->Q[pi].as<T>();
-    const int m = nb - 1, npair = m / 2, nodd = (m + 1) / 2;
-    auto sweeps = [&](hipStream_t st) -> int {
-        // forward: w2 = D w1 (all blocks)
-        XT_TRY(launch_gemm<T>(*bp, c, 1, Dl, bb, w1, bc, nullptr, 0, w2, bc, nb, st));
-        if (npair > 0) {
-            // blocks j = 2, 4, ..: w2_j -= G_{j-1} w2_{j-1}; then w2_j -= P_j w2_{j-2} in
-            // order; then blocks k = 1, 3, ..: w2_k -= G_{k-1} w2_{k-1}
-            XT_TRY(launch_gemm<T>(*bp, c, 0, G + bb, 2 * bb, w2 + bc, 2 * bc, w2 + 2 * bc,
-                                  2 * bc, w2 + 2 * bc, 2 * bc, npair, st));
-            for (int t = 0; t < npair; ++t)
-                XT_TRY(launch_gemm<T>(*bp, c, 0, P + t * bb, 0, w2 + 2 * t * bc, 0,
-                                      w2 + (2 * t + 2) * bc, 0, w2 + (2 * t + 2) * bc, 0, 1, st));
-            XT_TRY(launch_gemm<T>(*bp, c, 0, G, 2 * bb, w2, 2 * bc, w2 + bc, 2 * bc, w2 + bc,
-                                  2 * bc, nodd, st));
-        } else {
-            for (int i = 0; i + 1 < nb; ++i)
-                XT_TRY(launch_gemm<T>(*bp, c, 0, G + i * bb, 0, w2 + i * bc, 0,
-                                      w2 + (i + 1) * bc, 0, w2 + (i + 1) * bc, 0, 1, st));
+    const int m = nb - 1, nlev = bp->nlev;
+    // One recurrence in chain coordinates: block v (0..m) at W + v S, level-L matrix of
+    // step v at Mb[L] + (v - 1) MS[L] (level 0: G forward, H in reverse backward).
+    auto chain = [&](int dir, T *w, const T *M0, hipStream_t st) -> int {
+        if (m < 1) return XT_OK;
+        T *W = dir == 0 ? w : w + (long long)m * bc;
+        const long long S = dir == 0 ? bc : -bc;
+        const T *Mb[kLevels + 1];
+        long long MS[kLevels + 1];
+        Mb[0] = dir == 0 ? M0 : M0 + (long long)(m - 1) * bb;
+        MS[0] = dir == 0 ? bb : -bb;
+        for (int L = 1; L <= nlev; ++L) {
+            Mb[L] = bp->Lv[dir][L - 1][pi].as<T>();
+            MS[L] = bb;
         }
-        // backward: w1 = D^T w2 (all blocks)
-        XT_TRY(launch_gemm<T>(*bp, c, 2, Du, bb, w2, bc, nullptr, 0, w1, bc, nb, st));
-        if (npair > 0) {
-            // chain blocks j = m - 2, m - 4, .. >= jmin: w1_j -= H_j w1_{j+1}; then
-            // w1_j -= Q_j w1_{j+2} from the top down; then the blocks between,
-            // k = m - 1, m - 3, .. >= k0: w1_k -= H_k w1_{k+1}
-            const int jmin = m - 2 * npair, k0 = (m + 1) % 2;
-            XT_TRY(launch_gemm<T>(*bp, c, 0, H + jmin * bb, 2 * bb, w1 + (jmin + 1) * bc, 2 * bc,
-                                  w1 + jmin * bc, 2 * bc, w1 + jmin * bc, 2 * bc, npair, st));
-            for (int t = npair - 1; t >= 0; --t) {
-                const int j = jmin + 2 * t;
-                XT_TRY(launch_gemm<T>(*bp, c, 0, Q + t * bb, 0, w1 + (j + 2) * bc, 0,
-                                      w1 + j * bc, 0, w1 + j * bc, 0, 1, st));
-            }
-            XT_TRY(launch_gemm<T>(*bp, c, 0, H + k0 * bb, 2 * bb, w1 + (k0 + 1) * bc, 2 * bc,
-                                  w1 + k0 * bc, 2 * bc, w1 + k0 * bc, 2 * bc, nodd, st));
-        } else {
-            for (int i = nb - 2; i >= 0; --i)
-                XT_TRY(launch_gemm<T>(*bp, c, 0, H + i * bb, 0, w1 + (i + 1) * bc, 0,
-                                      w1 + i * bc, 0, w1 + i * bc, 0, 1, st));
+        for (int L = 0; L < nlev; ++L) {   // even steps of level L: the next level's rhs
+            const long long d = 1LL << L;
+            XT_TRY(launch_gemm<T>(*bp, c, 0, Mb[L] + MS[L], 2 * MS[L], W + d * S, 2 * d * S,
+                                  W + 2 * d * S, 2 * d * S, W + 2 * d * S, 2 * d * S,
+                                  (m >> L) / 2, st));
+        }
+        const long long D = 1LL << nlev;   // the dependent chain of the top level
+        for (int v = 1; v <= (m >> nlev); ++v)
+            XT_TRY(launch_gemm<T>(*bp, c, 0, Mb[nlev] + (v - 1) * MS[nlev], 0, W + (v - 1) * D * S,
+                                  0, W + v * D * S, 0, W + v * D * S, 0, 1, st));
+        for (int L = nlev - 1; L >= 0; --L) {   // odd steps of level L, from the finished ones
+            const long long d = 1LL << L;
+            XT_TRY(launch_gemm<T>(*bp, c, 0, Mb[L], 2 * MS[L], W, 2 * d * S, W + d * S,
+                                  2 * d * S, W + d * S, 2 * d * S, ((m >> L) + 1) / 2, st));
         }
         return XT_OK;
     };
+    auto sweeps = [&](hipStream_t st) -> int {
+        // forward: w2 = D w1 (all blocks), then w2_{i+1} -= G_i w2_i
+        XT_TRY(launch_gemm<T>(*bp, c, 1, Dl, bb, w1, bc, nullptr, 0, w2, bc, nb, st));
+        XT_TRY(chain(0, w2, G, st));
+        // backward: w1 = D^T w2 (all blocks), then w1_i -= H_i w1_{i+1}
+        XT_TRY(launch_gemm<T>(*bp, c, 2, Du, bb, w2, bc, nullptr, 0, w1, bc, nb, st));
+        XT_TRY(chain(1, w1, H, st));
+        return XT_OK;
+    };
     if (graphs_on() && !prof_on(XT_PROF_BAND)) {
-        const void *key[8] = {w1, w2, Dl, Du, G, H, P, Q};
+        const void *key[kKey] = {w1, w2, Dl, Du, G, H};
+        for (int dir = 0; dir < 2; ++dir)
+            for (int L = 0; L < kLevels; ++L) key[6 + dir * kLevels + L] = bp->Lv[dir][L][pi].ptr;
         SweepGraph *g = nullptr;
         for (SweepGraph &cand : bp->graph)
-            if (cand.exec && cand.c == c && cand.pi == pi && std::equal(key, key + 8, cand.key)) {
+            if (cand.exec && cand.c == c && cand.pi == pi && std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
                 break;
             }
@@ -1028,7 +1044,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             XT_HIP(ei);
             g->c = c;
             g->pi = pi;
-            std::copy(key, key + 8, g->key);
+            std::copy(key, key + kKey, g->key);
         }
         XT_HIP(hipGraphLaunch(g->exec, s));
     } else {

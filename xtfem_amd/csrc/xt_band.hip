@@ -852,21 +852,18 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
         for (int L = 1; L <= nlev; ++L) {
             const int cnt = m >> L;
             XT_TRY(lv[dir][L - 1].reserve(sizeof(double) * bb * cnt));
-            for (int t = 0; t < cnt; ++t) {
-                const double *A, *B;
-                if (L == 1 && dir == 0) {
-                    A = G.as<double>() + 2LL * t * bb;
-                    B = A + bb;
-                } else if (L == 1) {
-                    A = H.as<double>() + (long long)(m - 1 - 2 * t) * bb;
-                    B = A - bb;
-                } else {
-                    A = lv[dir][L - 2].as<double>() + 2LL * t * bb;
-                    B = A + bb;
+            if (L == 1 && dir == 1) {   // level 0 backward runs through H in reverse
+                for (int t = 0; t < cnt; ++t) {
+                    const double *A = H.as<double>() + (long long)(m - 1 - 2 * t) * bb;
+                    XT_RB(rocblas_dgemm(rb.h, rocblas_operation_none, rocblas_operation_none, b,
+                                        b, b, &minus, A, b, A - bb, b, &zero,
+                                        lv[dir][L - 1].as<double>() + (long long)t * bb, b));
                 }
-                XT_RB(rocblas_dgemm(rb.h, rocblas_operation_none, rocblas_operation_none, b, b, b,
-                                    &minus, A, b, B, b, &zero,
-                                    lv[dir][L - 1].as<double>() + (long long)t * bb, b));
+            } else {   // consecutive pairs of a contiguous set: one batched call
+                const double *A = L == 1 ? G.as<double>() : lv[dir][L - 2].as<double>();
+                XT_RB(rocblas_dgemm_strided_batched(
+                    rb.h, rocblas_operation_none, rocblas_operation_none, b, b, b, &minus, A, b,
+                    2 * bb, A + bb, b, 2 * bb, &zero, lv[dir][L - 1].as<double>(), b, bb, cnt));
             }
         }
     bp.Du[0] = std::move(L);

@@ -639,11 +639,22 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
 
     std::vector<int> active(R, 1), itv(R, 0), conv(R, 0);
     std::vector<double> ro_last(R, 0.0), eps1(R, 0.0), r0(R), bn(R);
+    // V_0 = b - A x0; when every x0 is zero that is b itself (A 0 = 0 exactly), so the
+    // operator apply and a second norm pass are skipped (ndev then holds b's norms,
+    // which are V_0's)
+    std::vector<double> xn;
+    XT_TRY(S.norms2(x, xn));
+    bool x0zero = true;
+    for (int r = 0; r < R; ++r) x0zero = x0zero && xn[r] == 0.0;
     XT_TRY(S.norms2(b, bn));
     S.hist_ro0 = std::sqrt(bn[0]);
-    // V_0 = b - A x0
-    XT_TRY(S.residual(b, x, S.Vj(0)));
-    XT_TRY(S.norms2(S.Vj(0), r0));
+    if (x0zero) {
+        XT_HIP(hipMemcpyAsync(S.Vj(0), b, sizeof(double) * S.n, hipMemcpyDeviceToDevice, S.s));
+        r0 = bn;
+    } else {
+        XT_TRY(S.residual(b, x, S.Vj(0)));
+        XT_TRY(S.norms2(S.Vj(0), r0));
+    }
     const std::vector<double> r0sq = r0;   // squared norms of V_0 (ndev still holds them)
     for (int r = 0; r < R; ++r) {
         r0[r] = std::sqrt(r0[r]);

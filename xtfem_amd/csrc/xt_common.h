@@ -137,7 +137,7 @@ void prof_end(int kind, hipStream_t s, double bytes, double flops);
 struct SolverWorkspace {
     DevBuf tm64, tm32, mi64, mi32;   // temporal matrices and inverses
     DevBuf V, Z;                     // Arnoldi basis, preconditioned basis (flexible)
-    DevBuf w32a, w32b, z64, t64, d;  // work vectors
+    DevBuf w32a, w32b, z64, t64;     // work vectors
     DevBuf partial, hdev, hdev2, ndev, coef;
     // Givens state of a cycle in one device region (Hessenberg columns, rotations, rhs
     // of the least squares problem, last residual norms, counters, frozen flags), the

@@ -678,8 +678,6 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         // preconditioner), where FP32 does not pay.
         std::vector<double> rn(R), irel(R, 1.0e-3), prev(R, 0.0);
         std::vector<int> fb(R, 0);
-        DevBuf &d = ws->d;
-        XT_TRY(d.reserve(sizeof(double) * S.n));
         for (int outer = 0;; ++outer) {
             // V_0 holds b - A x for the current x; its norms give the true residual (the
             // first pass has them from the setup)
@@ -705,19 +703,15 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                 else { act32[r] = 1; any32 = true; }
             }
             if (!any32 && !any64) break;
-            // inner solves for the correction d: x0 = 0, V_0 = residual (already there)
-            XT_HIP(hipMemsetAsync(d.ptr, 0, sizeof(double) * S.n, S.s));
+            // inner solves for the correction on V_0 = the residual (already there): each
+            // flexible cycle adds its correction Z y to x itself
             std::vector<int> cin(R, 0);
             // the first inner call starts from the norms just computed (V_0 unchanged)
             if (any32)
-                XT_TRY(S.gmres(d.as<double>(), true, e1, maxits, act32, itv, cin, ro_last, 1, true,
-                               true, &rn));
+                XT_TRY(S.gmres(x, true, e1, maxits, act32, itv, cin, ro_last, 1, true, true, &rn));
             if (any64)
-                XT_TRY(S.gmres(d.as<double>(), false, e1, maxits, act64, itv, cin, ro_last, 1, true,
-                               false, any32 ? nullptr : &rn));
-            hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n,
-                               d.as<double>(), x);
-            XT_HIP(hipGetLastError());
+                XT_TRY(S.gmres(x, false, e1, maxits, act64, itv, cin, ro_last, 1, true, false,
+                               any32 ? nullptr : &rn));
             XT_TRY(S.residual(b, x, S.Vj(0)));
         }
         for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;

@@ -274,36 +274,40 @@ band_mfma_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
     }
 }
 
-// Batched products of many blocks (the D x and D^T y passes, where parallelism is
-// plentiful and L2 bandwidth is the limit of the tile kernel above): a workgroup computes
-// a 64 x 64 output tile of one block, each of its 4 waves a 32 x 32 quadrant as 2 x 2
-// matrix-core tiles.  k advances in chunks of 16 staged through LDS, double-buffered
-// (the next chunk's global loads are in flight while the current chunk is multiplied),
-// so every operand element fetched feeds 64 outputs instead of 16.  In MFMA step s of
-// a chunk, lane group g takes k = 4 g + s for both operands.  Columns past c read a
-// clamped column and are not stored.  One-dimensional grid, a multiple of 8 covering
-// the nblk (b / 64) ceil(c / 64) tiles: workgroups are dealt to the 8 XCDs round-robin
+// Batched products of many blocks (the block-diagonal passes and the pairing levels'
+// passes, where parallelism is plentiful and L2 bandwidth is the limit of the tile
+// kernel above): a workgroup computes a TM x 64 output tile of one block.  TM = 64: each
+// of its 4 waves a 32 x 32 quadrant as 2 x 2 matrix-core tiles; TM = 128 (large
+// batches): each wave a 32 x 64 strip as 2 x 4 tiles, fewer operand reads per product.
+// k advances in chunks of 16 staged through LDS, double-buffered (the next chunk's
+// global loads are in flight while the current chunk is multiplied), so every operand
+// element fetched feeds TM or 64 outputs instead of 16.  In MFMA step s of a chunk,
+// lane group g takes k = 4 g + s for both operands.  Columns past c read a clamped
+// column and are not stored.  One-dimensional grid, a multiple of 8 covering the
+// nblk (b / TM) ceil(c / 64) tiles: workgroups are dealt to the 8 XCDs round-robin
 // (workgroup w runs on XCD w % 8), so tile t = (w % 8) (grid / 8) + w / 8 gives each XCD
 // a contiguous run of whole blocks and one L2 serves all tiles of a block (the A row
 // tiles and V column tiles each tile shares with its neighbours); surplus workgroups
 // return at once.
-template <typename T>
+template <typename T, int TM>
 __global__ void __launch_bounds__(256)
-band_tile64_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A,
-                   long long a_stride, const T *__restrict__ V, long long v_stride,
-                   const T *add, long long add_stride, T *out, long long out_stride) {
+band_tile_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A, long long a_stride,
+                 const T *__restrict__ V, long long v_stride, const T *add,
+                 long long add_stride, T *out, long long out_stride) {
     using M = Mfma<T>;
     constexpr int KC = 16, AP = KC + 4, VP = 64 + 4;
+    constexpr int AV = TM / 64;                    // A vec4 loads per thread per chunk
+    constexpr int RT = 2, CT = TM == 64 ? 2 : 4;   // MFMA tiles per wave (rows, columns)
     typedef T vec4_t __attribute__((ext_vector_type(4)));
-    __shared__ T As[2][64][AP];
+    __shared__ T As[2][TM][AP];
     __shared__ T Vs[2][KC][VP];
-    const int ntx = b / 64;
+    const int ntx = b / TM;
     const long long per_blk = (long long)ntx * ((c + 63) / 64);
     const long long tile = (long long)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
     if (tile >= per_blk * nblk) return;
     const long long z = tile / per_blk;
     const int rem = (int)(tile - z * per_blk);
-    const int R0 = (rem % ntx) * 64, N0 = (rem / ntx) * 64;
+    const int R0 = (rem % ntx) * TM, N0 = (rem / ntx) * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, t = lane & 15;
     A += z * a_stride;
@@ -311,21 +315,25 @@ band_tile64_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A,
     out += z * out_stride;
     if (add) add += z * add_stride;
     int k0 = 0, k1 = b;
-    if (tri == 1) k1 = R0 + 64;
+    if (tri == 1) k1 = R0 + TM;
     if (tri == 2) k0 = R0;
-    const int ar = tid >> 2, ac = (tid & 3) * 4;   // A chunk: row ar, k ac..ac+3
+    const int ar = tid >> 2, ac = (tid & 3) * 4;   // A chunk: rows ar + 64 u, k ac..ac+3
     const int vr = tid >> 4, vc = (tid & 15) * 4;  // V chunk: k row vr, columns vc..vc+3
     const T *ap = A + (size_t)(R0 + ar) * b + ac;  // 4-element aligned: b % 64 == 0
     int vcol[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) vcol[e] = min(N0 + vc + e, c - 1);
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    typename M::acc_t acc[2][2];
+    const int wr = TM == 64 ? (wave >> 1) * 32 : wave * 32;
+    const int wc = TM == 64 ? (wave & 1) * 32 : 0;
+    typename M::acc_t acc[RT][CT];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < RT; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = typename M::acc_t{T(0), T(0), T(0), T(0)};
-    vec4_t ga = *reinterpret_cast<const vec4_t *>(ap + k0);
+        for (int j = 0; j < CT; ++j) acc[i][j] = typename M::acc_t{T(0), T(0), T(0), T(0)};
+    vec4_t ga[AV];
+#pragma unroll
+    for (int u = 0; u < AV; ++u)
+        ga[u] = *reinterpret_cast<const vec4_t *>(ap + (size_t)64 * u * b + k0);
     T gv[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) gv[e] = V[(size_t)(k0 + vr) * c + vcol[e]];
@@ -333,37 +341,40 @@ band_tile64_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A,
     for (int kc = k0; kc < k1; kc += KC) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            As[buf][ar][ac + e] = ga[e];
+#pragma unroll
+            for (int u = 0; u < AV; ++u) As[buf][ar + 64 * u][ac + e] = ga[u][e];
             Vs[buf][vr][vc + e] = gv[e];
         }
         __syncthreads();
         if (kc + KC < k1) {
-            ga = *reinterpret_cast<const vec4_t *>(ap + kc + KC);
+#pragma unroll
+            for (int u = 0; u < AV; ++u)
+                ga[u] = *reinterpret_cast<const vec4_t *>(ap + (size_t)64 * u * b + kc + KC);
 #pragma unroll
             for (int e = 0; e < 4; ++e) gv[e] = V[(size_t)(kc + KC + vr) * c + vcol[e]];
         }
-        T a[2][4], bv[2][4];
+        T a[RT][4], bv[CT][4];
 #pragma unroll
         for (int s2 = 0; s2 < 4; ++s2) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i][s2] = As[buf][wr + 16 * i + t][4 * g + s2];
+            for (int i = 0; i < RT; ++i) a[i][s2] = As[buf][wr + 16 * i + t][4 * g + s2];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) bv[j][s2] = Vs[buf][4 * g + s2][wc + 16 * j + t];
+            for (int j = 0; j < CT; ++j) bv[j][s2] = Vs[buf][4 * g + s2][wc + 16 * j + t];
         }
 #pragma unroll
         for (int s2 = 0; s2 < 4; ++s2)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < RT; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = M::step(a[i][s2], bv[j][s2], acc[i][j]);
+                for (int j = 0; j < CT; ++j) acc[i][j] = M::step(a[i][s2], bv[j][s2], acc[i][j]);
         buf ^= 1;
     }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < CT; ++j) {
         const int col = N0 + wc + 16 * j + t;
         if (col >= c) continue;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < RT; ++i)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const size_t idx = (size_t)(R0 + wr + 16 * i + M::row(g, e)) * c + col;
@@ -390,10 +401,17 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
     const int b = bp.b;
     const bool prof = prof_on(XT_PROF_BAND);
     if (prof) prof_begin(XT_PROF_BAND, s);
-    if (nblk > 1) {   // batched pass: enough workgroups for 64 x 64 tiles
-        const long long tiles = (long long)(b / 64) * ceil_div(c, 64) * nblk;
-        hipLaunchKernelGGL(band_tile64_kernel<T>, dim3((unsigned)(ceil_div(tiles, 8) * 8)),
-                           dim3(256), 0, s, b, c, tri, nblk, A, as, V, vs, add, ads, out, os);
+    if (nblk > 1) {   // batched pass: 128-row tiles for large FP32 batches, else 64 (the
+                      // FP64 128-row tile holds 2 waves a SIMD and measured slower)
+        const int tm = (sizeof(T) == 4 && nblk >= 23 && b % 128 == 0) ? 128 : 64;
+        const long long tiles = (long long)(b / tm) * ceil_div(c, 64) * nblk;
+        const dim3 grid((unsigned)(ceil_div(tiles, 8) * 8));
+        if (tm == 128)
+            hipLaunchKernelGGL((band_tile_kernel<T, 128>), grid, dim3(256), 0, s, b, c, tri, nblk,
+                               A, as, V, vs, add, ads, out, os);
+        else
+            hipLaunchKernelGGL((band_tile_kernel<T, 64>), grid, dim3(256), 0, s, b, c, tri, nblk,
+                               A, as, V, vs, add, ads, out, os);
     }
     else
         hipLaunchKernelGGL(band_mfma_kernel<T>, dim3(b / 16, ceil_div(c, 16), nblk), dim3(256), 0,

@@ -412,6 +412,15 @@ def deletion_block(xconn, xcoord, K, M, fixed, necnt, nefail, ym, nu):
     return csrapplybc(K2, fixed), csrapplybc(M, fixed), fixed, necnt, ndfail
 
 
+def dispinterp(nodes, sol, Nt, nn):
+    """postprocess.f90:54-76 -> disp [8*3] of one element at one temporal point:
+    nodes (1-based, 8), sol temporal-major [6*3*nn], Nt [6]."""
+    disp = np.zeros(24)
+    lib().dispinterp_(_ri(3), _ri(8), _ri(nn), _ri(3), _p(_F(nodes, np.int32)), _p(_F(sol)),
+                      _p(_F(Nt)), _p(disp))
+    return disp
+
+
 def calstrain(disp, Nx, Ny, Nz, dmat):
     """postprocess.f90:79-104 -> (strain [6], stress [6]) at one Gauss point."""
     strain = np.zeros(6)

@@ -52,7 +52,8 @@ def exact_ilut(oracle_lib, K):
                                            ((12, 9, 2, 4), 0, 32),    # c = 192
                                            ((12, 9, 2, 4), 256, 1),   # one padded block, c = 6
                                            ((260, 1, 1, 0), 64, 32),  # 49 blocks: 3 pairing levels
-                                           ((260, 1, 1, 0), 64, 2)])  # the same, rows kernel
+                                           ((260, 1, 1, 0), 64, 2),   # the same, rows kernel
+                                           ((520, 1, 1, 0), 128, 32)])  # FP32 128-row tiles
 def test_band_sweeps_match_exact_lu(dev, oracle_lib, shape, block, R):
     from xtfem_amd import slab
     S = notched_plate(*shape)
@@ -64,6 +65,8 @@ def test_band_sweeps_match_exact_lu(dev, oracle_lib, shape, block, R):
         assert pr.nblocks > 4 and p % pr.block != 0
     if shape[0] == 260:   # recurrences of 48 steps: paired over three levels (chain of 6)
         assert pr.nblocks >= 49
+    if shape[0] == 520:   # 128-row blocks, >= 23 blocks in a batched pass: the FP32
+        assert pr.block == 128 and pr.nblocks >= 49   # sweeps take the 128 x 64 tiles
     alu, jlu, ju = exact_ilut(oracle_lib, K)
     rng = np.random.default_rng(7 + R)
     c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))

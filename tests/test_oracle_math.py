@@ -324,3 +324,48 @@ def test_chkelesta_brute_force(oracle_lib):
     assert np.array_equal(nefail, np.flatnonzero(failed) + 1)
     assert np.array_equal(elesta, (~failed).astype(np.int32))
     assert np.array_equal(gpsta, np.repeat(~failed, 8).astype(np.int32))
+
+
+def _distorted_block(seed=3):
+    """A 3 x 2 x 2 C3D8 block whose interior and surface nodes are moved at random
+    (every element keeps a positive Jacobian)."""
+    from xtfem_amd import fem
+    mesh = fem.structured_hex(3, 2, 2, 3.0, 2.0, 2.0)
+    rng = np.random.default_rng(seed)
+    return fem.Mesh(mesh.xcoord + rng.uniform(-0.15, 0.15, mesh.xcoord.shape), mesh.xconn)
+
+
+def test_kokkos_host_strain_of_linear_field(oracle_lib):
+    """The interpolation and strain stage of the damage functor (kokkos_kernel.h:58-84).
+    A linear displacement field u(x, t) = G(t) x with G(t) = sum_k Nt_k G_k is
+    reproduced exactly by trilinear hexahedra, so at every Gauss point of a distorted
+    mesh the strain is (G11, G22, G33, G12 + G21, G23 + G32, G13 + G31) of G(t)
+    (engineering shear, Voigt order xx, yy, zz, xy, yz, xz) and, below yield, the
+    stress is D_el(E_g, nu_g) times it.  The G_k are not symmetric and differ per
+    temporal function, so a swapped shear pair, a transposed gradient, a mis-indexed
+    DOF (node, component or temporal block of the temporal-major solution) or a
+    wrong temporal weight changes the result."""
+    from xtfem_amd import fem
+    mesh = _distorted_block()
+    nn, ne = mesh.nn, mesh.ne
+    ngp = 8 * ne
+    _, _, Nx, Ny, Nz = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+    rng = np.random.default_rng(11)
+    G = rng.uniform(-1e-4, 1e-4, (6, 3, 3))
+    sol = np.einsum("kij,nj->kni", G, mesh.xcoord).ravel()     # sol[k][node][comp]
+    ym = rng.uniform(1.9e5, 2.1e5, ngp)
+    nu = rng.uniform(0.25, 0.33, ngp)
+    nodes = np.repeat(mesh.xconn, 8, axis=0).ravel()
+    st0 = {k: np.zeros(6 * ngp) for k in ("eps_old", "eps_pl", "sig_eff", "sig_back")}
+    st0.update({k: np.zeros(ngp) for k in ("p", "D", "ws", "del_po", "sig_d")})
+    for Nt in rng.uniform(-1.0, 1.0, (4, 6)):
+        st = oracle_lib.kokkos_host(ne, nn, 1, nodes, Nx.ravel(), Ny.ravel(), Nz.ravel(), Nt,
+                                    sol, ym, nu, st0)
+        Gt = np.einsum("k,kij->ij", Nt, G)
+        eps = np.array([Gt[0, 0], Gt[1, 1], Gt[2, 2], Gt[0, 1] + Gt[1, 0],
+                        Gt[1, 2] + Gt[2, 1], Gt[0, 2] + Gt[2, 0]])
+        got = st["eps_old"].reshape(ngp, 6)
+        assert np.max(np.abs(got - eps)) < 1e-11 * np.max(np.abs(eps))
+        assert np.all(st["p"] == 0) and np.all(st["D"] == 0)          # elastic branch
+        sig = np.stack([fem.elastic_dmat(ym[g], nu[g]) @ got[g] for g in range(ngp)])
+        assert rel(st["sig_eff"].reshape(ngp, 6), sig) < 1e-13

@@ -25,9 +25,10 @@
 // All matrix sets are stored row-major (the row of the operator a tile needs is
 // contiguous) in FP64 and FP32.
 //
-// The factorization runs once per K on the device: this file's panel kernels for the
-// block Cholesky, rocSOLVER trtri and rocBLAS gemm for D_i, G_i and H_i (setup, not on
-// the per-iteration path).
+// The factorization runs on the device once per K, and again at every refactorization
+// (xt_prec_refactor_band: per slab when K follows the temperature): this file's panel
+// kernels for the block Cholesky, rocSOLVER trtri and rocBLAS gemm for D_i, G_i, H_i and
+// the level matrices.  None of it is on the per-iteration path.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 

@@ -279,6 +279,18 @@ int xt_asm_thermo_load(xt_asm h, const double *T, long long tstride, long long t
 int xt_prec_apply(xt_prec pr, int R, int q, const void *minv, const void *x, void *y,
                   int fp32, void *stream);
 
+/* xt_prec_apply for ILUT factors without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The temporal
+ * mix is staged in a fixed per-device array of XT_NOALLOC_SCRATCH_BYTES held until the
+ * process ends, so p*q*R*sizeof(value) must not exceed it when minv != NULL (XT_E_ARG
+ * otherwise); no kernel timing.  Banded factors: XT_E_UNSUPPORTED.  Every call on one
+ * device shares that array: calls that overlap on one device (other streams or threads)
+ * race.  Same results as xt_prec_apply (the same kernels). */
+#define XT_NOALLOC_SCRATCH_BYTES (16u << 20)
+int xt_prec_apply_noalloc(xt_prec pr, int R, int q, const void *minv, const void *x, void *y,
+                          int fp32, void *stream);
+
 /* Layout conversion between temporal-major x[r][j*p + dof] (R stacked
  * reference vectors) and interleaved X[dof][r*q + j], FP64. */
 int xt_to_interleaved(int p, int q, int R, const double *src, double *dst, void *stream);

@@ -70,6 +70,7 @@ _SIGS = {
     "xt_prec_destroy": (None, [vp]),
     "xt_prec_levels": (ci, [vp, vp, vp]),
     "xt_prec_apply": (ci, [vp, ci, ci, vp, vp, vp, ci, vp]),
+    "xt_prec_apply_noalloc": (ci, [vp, ci, ci, vp, vp, vp, ci, vp]),
     "xt_to_interleaved": (ci, [ci, ci, ci, vp, vp, vp]),
     "xt_from_interleaved": (ci, [ci, ci, ci, vp, vp, vp]),
     "xt_slab_solve": (ci, [vp, vp, ci, vp, vp, vp, ci, cd, ci, ci, vp, vp, vp, vp]),

@@ -10,6 +10,7 @@
 // row update x_i = y_i - sum_k alu(k) x(jlu(k)) (backward: times the stored
 // inverse diagonal) in the reference's entry order.
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 
 #include "xt_common.h"
@@ -345,14 +346,14 @@ static int upload_tri(int n, const std::vector<int> &rp, const std::vector<int> 
 
 template <typename T>
 static int run_tri(const Tri &tri, const T *vals, const T *dinv, int c, const T *b, T *x,
-                   hipStream_t s) {
+                   bool timed, hipStream_t s) {
     const int nlev = (int)tri.lev_ptr.size() - 1;
     const int *rows = tri.lev_rows.as<int>();
     for (int l = 0; l < nlev; ++l) {
         const int r0 = tri.lev_ptr[l], nr = tri.lev_ptr[l + 1] - r0;
         if (!nr) continue;
         long long n = (long long)nr * c;
-        const bool prof = prof_on(XT_PROF_TRSV);
+        const bool prof = timed && prof_on(XT_PROF_TRSV);
         if (prof) prof_begin(XT_PROF_TRSV, s);
         hipLaunchKernelGGL(trsv_level_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, nr,
                            rows + r0, tri.m.rowptr.as<int>(), tri.m.col.as<int>(), vals, dinv,
@@ -377,16 +378,17 @@ constexpr size_t kOneWgLds = 128 * 1024;
 // when the solution and the level schedule fit.
 constexpr size_t kWgLdsMax = 160 * 1024;
 
+// ILUT sweeps Y = (LU)^-1 X Minv_r^T.  stage: p*q*R elements of device memory for the
+// temporal mix (used only when minv is given); timed: per-launch HIP-event timing when
+// enabled (it creates events, so the allocation-free entry point passes false).
 template <typename T>
-int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
-                 hipStream_t s) {
-    if (pr->band) return band_apply<T>(pr->band, R, q, minv, x, y, s);
+static int ilut_apply(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
+                      T *stage, bool timed, hipStream_t s) {
     const int c = q * R;
     const T *b = x;
     if (minv) {
-        XT_TRY(pr->work.reserve(sizeof(T) * (size_t)pr->p * c));
-        XT_TRY(temporal_mix<T, T>(pr->p, q, R, minv, x, pr->work.as<T>(), s));
-        b = pr->work.as<T>();
+        XT_TRY(temporal_mix<T, T>(pr->p, q, R, minv, x, stage, s));
+        b = stage;
     }
     const T *lv = f32 ? (const T *)pr->L.m.val32.ptr : (const T *)pr->L.m.val64.ptr;
     const T *uv = f32 ? (const T *)pr->U.m.val32.ptr : (const T *)pr->U.m.val64.ptr;
@@ -396,7 +398,7 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     // one launch per level over the whole device.
     const long long work = (pr->L.m.nnz + pr->U.m.nnz + 2LL * pr->p) * c;
     if (work <= kOneWgWork) {
-        const bool prof = prof_on(XT_PROF_TRSV);
+        const bool prof = timed && prof_on(XT_PROF_TRSV);
         if (prof) prof_begin(XT_PROF_TRSV, s);
         const size_t xl = sizeof(T) * (size_t)pr->p * c;
         const size_t sl = sf_lds_bytes<T>(pr->p, c);
@@ -438,14 +440,50 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
-    XT_TRY(run_tri<T>(pr->L, lv, nullptr, c, b, y, s));
-    XT_TRY(run_tri<T>(pr->U, uv, dv, c, nullptr, y, s));
+    XT_TRY(run_tri<T>(pr->L, lv, nullptr, c, b, y, timed, s));
+    XT_TRY(run_tri<T>(pr->U, uv, dv, c, nullptr, y, timed, s));
     return XT_OK;
+}
+
+template <typename T>
+int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
+                 hipStream_t s) {
+    if (pr->band) return band_apply<T>(pr->band, R, q, minv, x, y, s);
+    T *stage = nullptr;
+    if (minv) {
+        XT_TRY(pr->work.reserve(sizeof(T) * (size_t)pr->p * q * R));
+        stage = pr->work.as<T>();
+    }
+    return ilut_apply<T>(pr, R, q, minv, x, y, f32, stage, true, s);
 }
 template int prec_apply_t<double>(xt_prec, int, int, const double *, const double *, double *,
                                   bool, hipStream_t);
 template int prec_apply_t<float>(xt_prec, int, int, const float *, const float *, float *, bool,
                                  hipStream_t);
+
+// Staging of the allocation-free apply (xt_prec_apply_noalloc): one array per device,
+// held until the process ends, shared by every call on that device.  Host code never
+// passes the array's name to a kernel or a copy (on the host it is a stand-in, not a
+// device address): it resolves the device address once per device with
+// hipGetSymbolAddress and uses that.
+static __device__ __align__(256) unsigned char scratch[XT_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::atomic<void *> g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (resolved on the first call there).
+static int scratch_ptr(void **out) {
+    int dev = 0;
+    XT_HIP(hipGetDevice(&dev));
+    XT_ARG(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the scratch table");
+    void *ptr = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!ptr) {
+        XT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        XT_ARG(ptr != nullptr, "hipGetSymbolAddress returned no address for scratch");
+        g_scratch_ptr[dev].store(ptr, std::memory_order_release);
+    }
+    *out = ptr;
+    return XT_OK;
+}
 
 }  // namespace xt
 
@@ -550,6 +588,31 @@ int xt_prec_apply(xt_prec pr, int R, int q, const void *minv, const void *x, voi
                                    true, s);
     return prec_apply_t<double>(pr, R, q, (const double *)minv, (const double *)x, (double *)y,
                                 false, s);
+}
+
+int xt_prec_apply_noalloc(xt_prec pr, int R, int q, const void *minv, const void *x, void *y,
+                          int fp32, void *stream) {
+    XT_ARG(pr && R > 0 && q > 0 && x && y && x != y, "prec_apply_noalloc arguments");
+    if (pr->band) {
+        set_error("xt_prec_apply_noalloc: ILUT factors only (the banded factor keeps its own "
+                  "workspace)");
+        return XT_E_UNSUPPORTED;
+    }
+    const size_t need = (size_t)pr->p * q * R * (fp32 ? sizeof(float) : sizeof(double));
+    if (minv && need > XT_NOALLOC_SCRATCH_BYTES) {
+        set_error("xt_prec_apply_noalloc: p*q*R = %lld values need %zu bytes of staging, more "
+                  "than the %zu-byte scratch; use xt_prec_apply",
+                  (long long)pr->p * q * R, need, (size_t)XT_NOALLOC_SCRATCH_BYTES);
+        return XT_E_ARG;
+    }
+    void *stage = nullptr;
+    if (minv) XT_TRY(scratch_ptr(&stage));
+    hipStream_t s = (hipStream_t)stream;
+    if (fp32)
+        return ilut_apply<float>(pr, R, q, (const float *)minv, (const float *)x, (float *)y,
+                                 true, (float *)stage, false, s);
+    return ilut_apply<double>(pr, R, q, (const double *)minv, (const double *)x, (double *)y,
+                              false, (double *)stage, false, s);
 }
 
 }  // extern "C"

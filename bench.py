@@ -1,30 +1,35 @@
 #!/usr/bin/env python3
 """Space-time slab solves on one MI355X.
 
-Workload: the single-edge-notched plate, C3D8 grid 100 x 46 x 4 elements over
-100 x 46 x 4 mm with a 20 x 1 element strip removed at mid-height (x < 20 mm,
-23 mm < y < 24 mm), 3 displacement DOF per node (p = 71,205 before the notch),
-clamped at y = 0, tensile traction on y = 46, linear elastic steel (E = 2e5 MPa,
-nu = 0.3, rho = 7.85e-9 t/mm^3), n_t = 6, slabs of dt = 0.5 s (10 load periods).
-R load cases on the specimen, all at the plate's 20 Hz: case 0 is the uniform
-80 MPa traction; the others draw an amplitude U[20, 120] MPa and a traction profile
-1 + a sin(2 pi m x / 100 + phi) along the loaded edge (a U[0, 0.5], m in {1, 2, 3},
-phi U[0, 2 pi]) from a fixed seed.
---cases-kind sweep instead draws frequencies U[10, 100] Hz (uniform traction), each
-case's slab spanning the whole number of periods closest to 0.5 s
-(dt_r = round(f_r / 2) / f_r), the regime in which the reference's t1 = 0 temporal
-matrices (main3d.f90:816) describe every slab.
+Headline workload (the stiffened-panel configuration): the panel 240 x 200 x 2 mm as a
+C3D8 grid of 240 x 200 x 2 elements (241 x 201 x 3 = 145,323 nodes, p = 435,969),
+clamped on three edges (x = 0, x = 240, y = 0), in-plane tensile traction along y on
+the free edge y = 200, steel (E = 2e5 MPa, nu = 0.3, rho = 7.85e-9 t/mm^3), n_t = 6.
+32 load cases solved together: traction amplitudes U[20, 120] MPa and load
+frequencies U[10, 100] Hz from a fixed seed, each case's slab spanning the whole number
+of load periods nearest 0.5 s (dt_r = round(f_r / 2) / f_r), the regime in which the
+reference's t1 = 0 temporal matrices (main3d.f90:816) describe every slab.  Every case
+has its own temporal matrices A_r, B_r.
 
 One step = one space-time slab for every load case, all on the device:
   jump term   b = (ktn1_r (x) K + mtn1_r (x) M) x_prev_r          (kronspgemv)
   load        b += fextc_r (x) amp_r rext, fixed DOFs zeroed       (main3d.f90:803-823)
   solve       (A_r (x) K + B_r (x) M) x_r = b_r, x0 = 0, GMRES(30) to 1e-8 relative
-              residual, right preconditioner A_r^-1 (x) K^-1 (exact banded factor).
+              residual.
 value: slab solves per second of the batched FP32-IR solve (FP32 operator and factor
-sweeps inside FP64 GMRES iterative refinement), every solve checked at true FP64
-relative residual <= 1e-8.  The plain FP64 GPU GMRES (one load case per solve) runs
-the same slabs for the comparison.  Setup (assembly, upload, factorization) is
-outside the timed region and reported separately.
+sweeps inside FP64 GMRES iterative refinement, every case also held to a forward-error
+test: last correction <= 1e-7 of the solution), preconditioned by A_r^-1 (x) K^-1 with
+each case's dominant temporal mode split onto a shifted factor (K + s M)^-1 (exact
+banded factors).  Beside it, on the same slabs: plain FP64 GPU GMRES with the
+reference's preconditioner A_r^-1 (x) K^-1 on the whole batch (precision ratio) and
+one load case per solve (batching ratio); the displacement of each against an FP64
+solve to 1e-12; the Kron-SpMM; the CPU baseline (the oracle's FP64 kronpgmres, one
+process per host core).  Setup (assembly, upload, factorizations) is outside the timed
+region and reported separately.
+
+--workload plate runs the single-edge-notched plate instead (100 x 46 x 4 C3D8, notch
+20 x 1 at mid-height, clamped at y = 0, traction on y = 46, p = 71,205), with
+--cases-kind plate (all cases at 20 Hz, traction profiles) or sweep (10-100 Hz).
 """
 from __future__ import annotations
 
@@ -48,27 +53,102 @@ SEED = 20261202
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cases", type=int, default=32, help="load cases per batched solve")
-    ap.add_argument("--grid", type=int, nargs=3, default=[100, 46, 4])
-    ap.add_argument("--notch", type=int, default=20)
+    ap.add_argument("--workload", choices=("panel", "plate"), default="panel")
+    ap.add_argument("--grid", type=int, nargs=3, default=None,
+                    help="elements per axis (panel: 240 200 2, plate: 100 46 4)")
+    ap.add_argument("--notch", type=int, default=20, help="plate: notch length in elements")
     ap.add_argument("--block", type=int, default=0, help="factor block rows (0: bandwidth)")
-    ap.add_argument("--cases-kind", choices=("plate", "sweep"), default="plate")
-    ap.add_argument("--no-damage", action="store_true", help="skip the damage-kernel leg")
+    ap.add_argument("--cases-kind", choices=("plate", "sweep"), default="sweep",
+                    help="plate workload: all cases at 20 Hz (plate) or 10-100 Hz (sweep)")
+    ap.add_argument("--xtol", type=float, default=1e-7,
+                    help="forward-error test of the FP32-IR refinement (0: residual only)")
+    ap.add_argument("--legs", default="",
+                    help="extra legs, comma-separated: rod, reassembly, thermo, driver_panel, "
+                         "deletion, damage, fatigue (or 'all')")
+    ap.add_argument("--fatigue-blocks", type=int, default=40,
+                    help="timed cycle blocks of the fatigue leg")
     ap.add_argument("--damage-nip", type=int, default=2560,
                     help="temporal points per slab of the damage leg")
     ap.add_argument("--rod-only", action="store_true",
                     help="only run the prismatic-rod ILUT leg (K timed slabs) and print it")
-    ap.add_argument("--headline-only", action="store_true",
-                    help="only the batched plate solves, SpMM microbenchmarks and kernel profile")
     ap.add_argument("--thermo-only", action="store_true",
                     help="only run the thermo-mechanical plate leg and print it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only the CPU baseline (no GPU), printed as one JSON line")
+    ap.add_argument("--cpu-budget", type=float, default=45.0,
+                    help="seconds of solving per CPU worker in the CPU baseline")
+    ap.add_argument("--cpu-matrices", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     return ap.parse_args()
+
+
+def panel_mesh(nx=240, ny=200, nz=2):
+    """The stiffened-panel grid (1 mm C3D8 elements, z fastest, then y, then x: node
+    half-bandwidth (ny + 1)(nz + 1) + nz + 2) and its clamp: x = 0, x = nx, y = 0."""
+    from xtfem_amd import fem
+    mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz))
+    x = mesh.xcoord
+    clamp = (x[:, 0] == 0.0) | (x[:, 0] == float(nx)) | (x[:, 1] == 0.0)
+    return mesh, np.repeat(clamp, 3).astype(np.int32)
+
+
+def load_cases(R, seed, kind="sweep"):
+    """Amplitudes U[20, 120] MPa and frequencies U[10, 100] Hz (kind sweep; case 0 at
+    80 MPa for the plate) or all 20 Hz (kind plate), with each case's slab length, the
+    whole number of load periods nearest 0.5 s."""
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(20.0, 120.0, R)
+    freq = rng.uniform(10.0, 100.0, R) if kind == "sweep" else np.full(R, 20.0)
+    dt = np.round(0.5 * freq) / freq
+    return amp, freq, dt
+
+
+class PanelProblem:
+    """Mesh, spatial matrices (assembled on the device) and per-case slab data of the
+    stiffened panel."""
+
+    def __init__(self, R, grid=(240, 200, 2), host_assembly=False):
+        from xtfem_amd import fem, slab
+        nx, ny, nz = grid
+        self.grid = grid
+        self.mesh, self.bc = panel_mesh(nx, ny, nz)
+        self.p = 3 * self.mesh.nn
+        if host_assembly:     # CPU-only process (the CPU baseline): numpy restatement
+            K, M, *_ = fem.assemble(self.mesh, 2.0e5, 0.3, 7.85e-9)
+            self.Kb, self.Mb = fem.applybc(K, self.bc), fem.applybc(M, self.bc)
+        else:                 # device assembly with the 1-0 method (xt_asm_run)
+            from xtfem_amd import device
+            asm = device.Assembly(self.mesh, self.bc)
+            asm.run(2.0e5, 0.3, 7.85e-9)
+            self.Kb, self.Mb = asm.host()
+            del asm
+        self.amp, self.freq, self.dt = load_cases(R, SEED + 3)
+        self.w = 2 * np.pi * self.freq
+        c = slab.stcoeff_m(0.0, self.dt, self.w)     # t1 = 0 as main3d.f90:816 passes it
+        self.A, self.B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+        self.J1, self.J2 = c["ktn1"], c["mtn1"]
+        unit = fem.traction_load(self.mesh, 1, float(ny), 1, float(nx * nz))
+        self.rext = np.tile(unit, (R, 1))
+
+    def coef(self, step, cases):
+        """fextc_r(j) * amp_r for slab `step` (tcoorde(1) = step * dt_r, main3d.f90:806)."""
+        from xtfem_amd import slab
+        fx, _ = slab.getfextc_m(step * self.dt[cases], self.w[cases], self.dt[cases])
+        return (np.atleast_2d(fx) * self.amp[cases, None]).ravel()
+
+    def describe(self):
+        nx, ny, nz = self.grid
+        return (f"stiffened panel {nx}x{ny}x{nz} mm, C3D8 {nx}x{ny}x{nz} elements, "
+                f"{self.mesh.nn} nodes, p={self.p}, 3 DOF/node, n_t=6, clamped on x=0, "
+                f"x={nx}, y=0, in-plane traction on y={ny}; {len(self.amp)} load cases, "
+                f"amplitudes U[20,120] MPa, frequencies U[10,100] Hz (seed {SEED + 3}), "
+                f"slabs of whole load periods nearest 0.5 s")
 
 
 class Problem:
@@ -92,6 +172,7 @@ class Problem:
             self.freq = np.full(R, 20.0)
         self.w = 2 * np.pi * self.freq
         self.dt = np.round(0.5 * self.freq) / self.freq
+        self.kind, self.grid, self.notch = kind, (nx, ny, nz), notch
         # t1 = 0 in stcoeff_m, as the reference's call passes it (main3d.f90:816)
         c = slab.stcoeff_m(0.0, self.dt, self.w)
         self.A, self.B = c["k"] + c["ktn"], c["m"] + c["mtn"]
@@ -114,6 +195,11 @@ class Problem:
         from xtfem_amd import slab
         fx, _ = slab.getfextc_m(step * self.dt[cases], self.w[cases], self.dt[cases])
         return (np.atleast_2d(fx) * self.amp[cases, None]).ravel()
+
+    def describe(self):
+        nx, ny, nz = self.grid
+        return (f"notched plate {nx}x{ny}x{nz} C3D8, notch {self.notch}x1, 3 DOF/node, "
+                f"p={self.p}, n_t=6, {len(self.amp)} load cases ({self.kind})")
 
 
 class RodProblem(Problem):
@@ -159,10 +245,10 @@ def rod_leg(K):
 class Runner:
     """Device state of one slab sequence (R load cases) and its step."""
 
-    def __init__(self, prob, op, pr, cases, mode, nsteps):
+    def __init__(self, prob, op, pr, cases, mode, nsteps, xtol=0.0):
         import torch
         from xtfem_amd import device
-        self.prob, self.op, self.pr, self.mode = prob, op, pr, mode
+        self.prob, self.op, self.pr, self.mode, self.xtol = prob, op, pr, mode, xtol
         self.cases = np.asarray(cases)
         self.R = len(self.cases)
         p, R = prob.p, self.R
@@ -177,6 +263,7 @@ class Runner:
         self.xprev = torch.zeros_like(self.b)
         self.A, self.B = prob.A[self.cases], prob.B[self.cases]
         self.device = device
+        self.eps = 1e-8
         self.n = 0
         self.its, self.relres, self.ierr = [], [], []
 
@@ -191,8 +278,8 @@ class Runner:
                                 self.b.data_ptr(), s), "slab_rhs")
         self.x.zero_()
         _, its, ierr, relres = self.device.slab_solve_device(
-            self.op, self.pr, self.A, self.B, self.b, self.x, self.R, im=30, eps=1e-8,
-            maxits=300, mode=self.mode)
+            self.op, self.pr, self.A, self.B, self.b, self.x, self.R, im=30, eps=self.eps,
+            maxits=500, mode=self.mode, xtol=self.xtol)
         self.its.append(its)
         self.ierr.append(ierr)
         self.relres.append(relres)
@@ -223,10 +310,16 @@ def kernel_profile(run, K, mask):
         ms, n, by, fl = C.c_double(), C.c_longlong(), C.c_double(), C.c_double()
         check(lib().xt_prof_read(kind, C.addressof(ms), C.addressof(n), C.addressof(by),
                                  C.addressof(fl)), "prof_read")
+        ab, af = C.c_double(), C.c_double()
+        check(lib().xt_prof_read_alg(kind, C.addressof(ab), C.addressof(af)), "prof_read_alg")
         if n.value:
-            out[name] = dict(ms=ms.value, launches=n.value, bytes=by.value, flops=fl.value,
+            # bytes / flops: algorithmic (the unpaired computation, each operand once);
+            # performed_*: what the launches do (the paired recurrences add products)
+            out[name] = dict(ms=ms.value, launches=n.value, bytes=ab.value, flops=af.value,
+                             performed_bytes=by.value, performed_flops=fl.value,
                              avg_us=1e3 * ms.value / n.value,
-                             gbs=by.value / (ms.value * 1e-3) / 1e9)
+                             gbs=ab.value / (ms.value * 1e-3) / 1e9,
+                             tflops=af.value / (ms.value * 1e-3) / 1e12)
     check(lib().xt_prof_enable(0), "prof_enable")
     return out
 
@@ -256,25 +349,18 @@ def spmm_microbench(op, p, reps=20, warm=3, R=1):
         times.append(ms.value)
     check(lib().xt_prof_enable(0), "prof_enable")
     med = float(np.median(times))
-    return dict(R=R, dtype="f32", median_us=1e3 * med, bytes=by.value,
+    csr = op.algorithmic_bytes(R, True, fmt="csr")
+    return dict(R=R, dtype="f32", format=op.format, median_us=1e3 * med, bytes=by.value,
                 gbs=by.value / (med * 1e-3) / 1e9,
-                frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS)
+                frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                csr_bytes=csr, csr_gbs=csr / (med * 1e-3) / 1e9,
+                tflops=op.flops(R) / (med * 1e-3) / 1e12)
 
 
-def spmm_panel_leg():
-    """Kron-SpMM on the stiffened-panel grid 240 x 200 x 2 C3D8 (145,323 nodes,
-    p = 435,969), clamped at y = 0, K and M assembled on the device: one load case
-    (c = 6, HBM-bound) and 32 (c = 192)."""
-    from xtfem_amd import device, fem
-    mesh = fem.structured_hex(240, 200, 2, 240.0, 200.0, 2.0)
-    asm = device.Assembly(mesh, fem.clamp_mask(mesh, 1, [0.0]))
-    K, M = asm.run(2.0e5, 0.3, 7.85e-9)
-    op = device.KronOperator([(asm.K_pattern[0], asm.K_pattern[1], K.cpu().numpy()),
-                              (asm.M_pattern[0], asm.M_pattern[1], M.cpu().numpy())], q=6)
-    del asm
-    out = {f"R{R}": spmm_microbench(op, 3 * mesh.nn, R=R) for R in (1, 32)}
-    out["workload"] = "panel 240x200x2 C3D8, p=435969, node-block format, FP32"
-    return out
+def spmm_legs(op, p):
+    """Kron-SpMM alone on the workload's operator: one load case (c = 6, HBM-bound) and
+    32 (c = 192); bytes of the stored (node-block) format and of canonical CSR."""
+    return {f"R{R}": spmm_microbench(op, p, R=R) for R in (1, 32)}
 
 
 # Algorithmic FP64 operations of one Gauss point and time point of the damage kernel
@@ -336,6 +422,55 @@ def damage_leg(nip, slabs=3):
                 gp_steps_per_s=gps, tflops_elastic_count=gps * DAMAGE_FLOPS_ELASTIC / 1e12,
                 frac_fp64_peak=gps * DAMAGE_FLOPS_ELASTIC / 78.6e12, yielded_fraction=yielded,
                 max_D=float(D.max()))
+
+
+FATIGUE_PARAMS = [350.0, 577.0, 0.08, 1740.0, 0.2, 0.5, 0.5]   # sig_f = 350 MPa yield
+
+
+def fatigue_leg(blocks=40, nip=2560, amp=370.0):
+    """The fatigue-damage configuration: notched block 60 x 60 x 12 mm (C3D8 60 x 60 x 12
+    minus a 15 x 1 x 12 notch at mid-height, 43,020 elements), clamped at y = 0, 20 Hz
+    tensile traction on y = 60, damage law with a 350 MPa fatigue limit and critical
+    damage 0.95.  One cycle block = one slab of 10 load periods: FP32-IR slab solve, the
+    damage kernel over nip temporal points of the slab, element status, and deletion with
+    the device rebuild when elements fail (xtfem_amd.fatigue)."""
+    import torch
+    from xtfem_amd import fem
+    from xtfem_amd.fatigue import FatigueSlabs
+    nx, ny, nz = 60, 60, 12
+    keep = np.ones((nx, ny, nz), bool)
+    keep[0:15, ny // 2, :] = False
+    mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz), keep=keep)
+    K, M, Nx, Ny, Nz = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+    bc = fem.clamp_mask(mesh, 1, [0.0])
+    unit = fem.traction_load(mesh, 1, float(ny), 1, float(nx * nz))
+    t0 = time.perf_counter()
+    fs = FatigueSlabs(mesh, K, M, (Nx, Ny, Nz), bc, unit, amp, 20.0, 0.5, nip, 0.95, 2.0e5,
+                      0.3, params=FATIGUE_PARAMS, mode=1, readd=False)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    fs.step()                                          # warm-up block
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(blocks):
+        recs.append(fs.step())
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    dels = [r for r in recs if r["failed"]]
+    plain = [r["solve_damage_ms"] for r in recs if not r["failed"]]
+    st = fs.damage.host_state()
+    return dict(workload=f"notched block {nx}x{ny}x{nz} C3D8 ({mesh.ne} elements, "
+                         f"{8 * mesh.ne} Gauss points, p={3 * mesh.nn}), traction {amp} MPa at "
+                         f"20 Hz, fatigue limit 350 MPa, Dc 0.95, nip {nip} per block, FP32-IR",
+                blocks=blocks, ms_per_block=1e3 * t / blocks,
+                ms_solve_and_damage_median=float(np.median(plain)) if plain else None,
+                deletion_events=len(dels), deleted_elements=int(len(fs.deleted())),
+                deletion_ms=[r["deletion_ms"] for r in dels][:8],
+                its_max=int(max(r["its"] for r in recs)),
+                relres_max=float(max(r["relres"] for r in recs)),
+                all_converged=bool(all(r["ierr"] == 0 for r in recs)),
+                max_D=float(st["D"].max()), yielded_fraction=float(np.mean(st["p"] > 0)),
+                setup_s=t_setup)
 
 
 def deletion_leg(prob, R=8):
@@ -530,42 +665,124 @@ def driver_panel_leg(nslab=3):
                 T_max=float(T.max()), relres_m=run.last["relres_m"])
 
 
-def cpu_baseline(prob, budget_s=15.0):
-    """The oracle on the host: FP64 restatement of kronpgmres (kronecker.f90:181-365,
-    MGS, one core) preconditioned by A^-1 (x) K^-1 with the exact band Cholesky factor
-    of K (the factor the GPU path applies), on consecutive slabs of load case 0.  The
-    factorization is timed separately and excluded, as on the GPU."""
+_CPU = {}
+
+
+def _cpu_worker(widx):
+    """One host core of the CPU baseline: the oracle's FP64 kronpgmres
+    (kronecker.f90:181-365) on consecutive slabs of load case widx mod R (jump term, RHS,
+    solve to 1e-8, preconditioner A_r^-1 (x) K^-1 with the exact factor), until the time
+    budget is spent (at least one slab).  BLAS runs single-threaded in the worker."""
+    from threadpoolctl import threadpool_limits
     from oracle import oracle
     from xtfem_amd import slab
+    prob, lf, budget = _CPU["prob"], _CPU["factor"], _CPU["budget"]
+    r = widx % len(prob.amp)
     p = prob.p
-    t0 = time.perf_counter()
-    bf = oracle.BandFactor(prob.Kb)
-    t_fact = time.perf_counter() - t0
-    if bf.info != 0:
-        return None
     fixed = np.tile(prob.bc, 6)
     x = np.zeros(6 * p)
-    n, t_solve, its_all = 0, 0.0, []
-    while n == 0 or (t_solve < budget_s and n < 8):
+    n, its_all = 0, []
+    with threadpool_limits(1):
         t0 = time.perf_counter()
-        jump = oracle.kronspmv(prob.J1[0], prob.J2[0], prob.Kb, prob.Mb, x)
-        rhs = slab.slab_rhs(p, prob.coef(n, np.array([0])), None, prob.rext[0], jump=jump,
-                            fixed=fixed)
-        x, ie, its, _ = oracle.kronpgmres(prob.A[0], prob.B[0], prob.Kb, prob.Mb, bf, rhs,
-                                          im=30, eps=1e-8, maxits=300)
-        t_solve += time.perf_counter() - t0
-        its_all.append(its)
-        n += 1
-    out = dict(value=n / t_solve, unit="slab solves/s", cores=1, kind="port",
-                sample=f"{n} consecutive slab(s) of load case 0 (80 MPa, 20 Hz): jump term, "
-                       f"RHS and oracle FP64 GMRES(30) to 1e-8 with the exact band Cholesky "
-                       f"factor (half-bandwidth {bf.w}; factorization {t_fact:.1f} s excluded), "
-                       f"its {its_all}, {t_solve:.1f} s on one host core")
-    # the damage functor on the host: the oracle's kokkos_host over the first 1000
-    # elements of the damage leg's block, same field and temporal points
+        while n == 0 or time.perf_counter() - t0 < budget:
+            jump = oracle.kronspmv(prob.J1[r], prob.J2[r], prob.Kb, prob.Mb, x)
+            rhs = slab.slab_rhs(p, prob.coef(n, np.array([r])), None, prob.rext[r], jump=jump,
+                                fixed=fixed)
+            x, ie, its, _ = oracle.kronpgmres(prob.A[r], prob.B[r], prob.Kb, prob.Mb, lf, rhs,
+                                              im=30, eps=1e-8, maxits=300)
+            its_all.append(int(its))
+            n += 1
+        t = time.perf_counter() - t0
+    return n, t, its_all
+
+
+def cpu_baseline(prob, budget_s=45.0):
+    """The oracle on the host, one process per host core (the reference runs one
+    single-threaded Kokkos instance per MPI rank, kokkos_host.cpp:17): each worker solves
+    consecutive slabs of its own load case with the oracle's FP64 kronpgmres and the exact
+    factor of K (LAPACK band Cholesky, computed once before the workers fork and shared;
+    its time, on all cores, is reported and excluded, as the GPU factorization is).
+    value = slabs finished by all workers / wall time of the parallel section."""
+    import multiprocessing as mp
+    from oracle import oracle
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    nw = max(1, min(ncores, 16))
+    t0 = time.perf_counter()
+    lf = oracle.LapackBandFactor(prob.Kb)
+    t_fact = time.perf_counter() - t0
+    _CPU.update(prob=prob, factor=lf, budget=budget_s)
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(nw) as pool:
+        res = pool.map(_cpu_worker, range(nw))
+    wall = time.perf_counter() - t0
+    nsl = sum(r[0] for r in res)
+    its = [i for r in res for i in r[2]]
+    return dict(value=nsl / wall, unit="slab solves/s", cores=nw, kind="port",
+                sample=f"{nw} worker processes (one per host core, single-threaded BLAS), each "
+                       f"on consecutive slabs of its own load case for about {budget_s:.0f} s: "
+                       f"jump term, RHS and the oracle's FP64 GMRES(30) to 1e-8 with "
+                       f"A_r^-1 (x) K^-1, K^-1 the exact band Cholesky factor (half-bandwidth "
+                       f"{lf.w}; LAPACK factorization {t_fact:.1f} s on {ncores} cores, "
+                       f"excluded); {nsl} slabs in {wall:.1f} s, GMRES its min {min(its)} max "
+                       f"{max(its)}", nproc=ncores, factor_s=t_fact, slabs=nsl, wall_s=wall)
+
+
+def cpu_only(args):
+    """--cpu-only: the CPU baseline in a process that never touches the GPU (its workers
+    fork from it).  The parent passes the workload's matrices in a directory."""
+    R = args.cases
+    if args.workload == "panel":
+        prob = PanelProblem.__new__(PanelProblem)
+        grid = tuple(args.grid) if args.grid else (240, 200, 2)
+        prob.grid = grid
+        prob.mesh, prob.bc = panel_mesh(*grid)
+        prob.p = 3 * prob.mesh.nn
+        from xtfem_amd import fem, slab
+        prob.amp, prob.freq, prob.dt = load_cases(R, SEED + 3)
+        prob.w = 2 * np.pi * prob.freq
+        c = slab.stcoeff_m(0.0, prob.dt, prob.w)
+        prob.A, prob.B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+        prob.J1, prob.J2 = c["ktn1"], c["mtn1"]
+        unit = fem.traction_load(prob.mesh, 1, float(grid[1]), 1, float(grid[0] * grid[2]))
+        prob.rext = np.tile(unit, (R, 1))
+    else:
+        grid = args.grid or [100, 46, 4]
+        prob = Problem(*grid, args.notch, R, args.cases_kind)
+    if args.cpu_matrices:
+        d = args.cpu_matrices
+        ld = lambda n: np.load(os.path.join(d, n + ".npy"), allow_pickle=False)
+        prob.Kb = (ld("kia"), ld("kja"), ld("kv"))
+        prob.Mb = (ld("mia"), ld("mja"), ld("mv"))
+    elif not hasattr(prob, "Kb"):
+        from xtfem_amd import fem
+        K, M, *_ = fem.assemble(prob.mesh, 2.0e5, 0.3, 7.85e-9)
+        prob.Kb, prob.Mb = fem.applybc(K, prob.bc), fem.applybc(M, prob.bc)
+    out = cpu_baseline(prob, args.cpu_budget)
     out["damage"] = cpu_damage_sample()
-    out["rod_ilut"] = cpu_rod_sample()
-    return out
+    print(json.dumps(out))
+
+
+def run_cpu_baseline(args, prob):
+    """The CPU baseline in a child process (no GPU in it, so its workers may fork), with
+    the workload's matrices handed over through a temporary directory."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        for name, arr in zip(("kia", "kja", "kv"), prob.Kb):
+            np.save(os.path.join(d, name + ".npy"), np.asarray(arr))
+        for name, arr in zip(("mia", "mja", "mv"), prob.Mb):
+            np.save(os.path.join(d, name + ".npy"), np.asarray(arr))
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-only", "--workload",
+               args.workload, "--cases", str(args.cases), "--cpu-budget", str(args.cpu_budget),
+               "--cpu-matrices", d, "--notch", str(args.notch), "--cases-kind", args.cases_kind]
+        if args.grid:
+            cmd += ["--grid", *map(str, args.grid)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            print(r.stderr[-2000:], file=sys.stderr)
+            return None
+        return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def cpu_rod_sample(slabs=5):
@@ -624,19 +841,26 @@ def pmc_traffic():
     return float(m.group(1)) if m else None
 
 
-def roofline(name, k):
-    """Roofline entry of the dominant kernel: matrix-core bound when its algorithmic
-    intensity (flops per byte) is above the FP32 ridge (157.3 TFLOP/s / 8 TB/s), else
-    HBM bound.  The sweeps of the batched FP32-IR solve run in FP32."""
+def roofline(name, k, prob, pr):
+    """Roofline entry of the dominant kernel, on its algorithmic work (xt_prof_read_alg):
+    matrix-core bound when the algorithmic intensity is above the FP32 ridge (157.3
+    TFLOP/s / 8 TB/s), else HBM bound.  For the factor sweeps the algorithmic flops are
+    those of the unpaired block recurrences (the pairing's extra products are reported as
+    performed); the banded minimum 4 p w c flops per apply (two triangular sweeps over
+    half-bandwidth w) is given beside it."""
     intensity = k["flops"] / k["bytes"]
     out = {"kernel": name, "traffic": pmc_traffic(),
            "algorithmic_bytes_per_launch": k["bytes"] / k["launches"],
            "algorithmic_flops_per_launch": k["flops"] / k["launches"],
+           "performed_flops_per_launch": k["performed_flops"] / k["launches"],
            "avg_launch_us": k["avg_us"], "intensity_flop_per_byte": intensity}
     if intensity > PEAK_MFMA_TFLOPS["f32"] * 1e12 / (PEAK_HBM_GBS * 1e9):
         tf = k["flops"] / (k["ms"] * 1e-3) / 1e12
         out.update(bound="mfma", achieved=tf, peak=PEAK_MFMA_TFLOPS["f32"], unit="TFLOP/s",
-                   frac=tf / PEAK_MFMA_TFLOPS["f32"])
+                   frac=tf / PEAK_MFMA_TFLOPS["f32"],
+                   performed_tflops=k["performed_flops"] / (k["ms"] * 1e-3) / 1e12)
+        if name == "band_sweep":
+            out["banded_minimum_flops_per_apply_c192"] = 4.0 * prob.p * pr.bandwidth * 192
     else:
         out.update(bound="hbm", achieved=k["gbs"], peak=PEAK_HBM_GBS, unit="GB/s",
                    frac=k["gbs"] / PEAK_HBM_GBS)
@@ -655,6 +879,55 @@ def copy_probe():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
 
+def setup_solver(prob, args, fp32_only):
+    """Operator and preconditioner of the workload: the exact banded factor of K and the
+    shifted factors (K + s M) covering the cases' dominant temporal eigenvalues above
+    0.05 of the structure's lowest K^-1 M eigenvalue (device power iteration)."""
+    import torch
+    from xtfem_amd import device
+    t0 = time.perf_counter()
+    op = device.KronOperator([prob.Kb, prob.Mb], q=6)
+    pr0 = device.BandPreconditioner(prob.Kb, block=args.block)
+    mu = device.lowest_mode(pr0, prob.Mb)
+    del pr0
+    torch.cuda.synchronize()
+    lam = device.dominant_temporal_mode(prob.A, prob.B)
+    shifts = device.shifts_for_cases(lam, mu)
+    t1 = time.perf_counter()
+    pr = device.BandPreconditioner(prob.Kb, block=args.block, M=prob.Mb, shifts=shifts,
+                                   shifted_fp32_only=fp32_only)
+    torch.cuda.synchronize()
+    info = dict(mu_min=mu, lambda_min=float(np.nanmin(lam)), lambda_max=float(np.nanmax(lam)),
+                shifts=[float(v) for v in shifts], shifted_fp32_only=fp32_only,
+                bandwidth=pr.bandwidth, blocks=pr.nblocks, block=pr.block,
+                setup_s=dict(operator_and_mu=t1 - t0, factors=time.perf_counter() - t1))
+    return op, pr, info
+
+
+def accuracy_check(prob, op, pr, xtol):
+    """Slab 0 of every case (x_prev = 0): FP32-IR, FP64 GMRES at 1e-8 and an FP64 solve to
+    1e-12; relative 2-norm displacement error of the first two against the third."""
+    import torch
+    from xtfem_amd import device
+    R = len(prob.amp)
+    runs = {}
+    for name, mode, eps, xt in (("ref", 0, 1e-12, 0.0), ("fp32_ir", 1, 1e-8, xtol),
+                                ("fp64", 0, 1e-8, 0.0)):
+        run = Runner(prob, op, pr, np.arange(R), mode=mode, nsteps=1, xtol=xt)
+        run.eps = eps
+        run.step()
+        runs[name] = run
+    x = {k: device.from_interleaved(v.xprev, prob.p, 6, R) for k, v in runs.items()}
+    err = {k: np.linalg.norm(x[k] - x["ref"], axis=1) / np.linalg.norm(x["ref"], axis=1)
+           for k in ("fp32_ir", "fp64")}
+    return dict(rel_err_fp32_ir_max=float(err["fp32_ir"].max()),
+                rel_err_fp64_max=float(err["fp64"].max()),
+                ref_relres_max=float(np.max(runs["ref"].relres[0])),
+                ref_its_max=int(np.max(runs["ref"].its[0])),
+                fp32_ir_its=runs["fp32_ir"].its[0].tolist(),
+                fp64_its=runs["fp64"].its[0].tolist())
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -665,8 +938,10 @@ def main():
               "(independent load-case groups or meshes would shard across GPUs).",
               file=sys.stderr)
         sys.exit(2)
+    if args.cpu_only:
+        cpu_only(args)
+        return
     import torch
-    from xtfem_amd import device
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
@@ -674,24 +949,24 @@ def main():
         print(json.dumps(rod_leg(args.steps)))
         return
     R, K, W = args.cases, args.steps, args.warmup
-    nx, ny, nz = args.grid
     if args.thermo_only:
-        print(json.dumps(thermo_leg(Problem(nx, ny, nz, args.notch, 1), nslab=args.steps)))
+        grid = args.grid or [100, 46, 4]
+        print(json.dumps(thermo_leg(Problem(*grid, args.notch, 1), nslab=args.steps)))
         return
     t0 = time.perf_counter()
-    prob = Problem(nx, ny, nz, args.notch, R, args.cases_kind)
+    if args.workload == "panel":
+        prob = PanelProblem(R, tuple(args.grid) if args.grid else (240, 200, 2))
+        fp32_only = True      # FP64 shifted factors of the panel would not fit beside the rest
+    else:
+        prob = Problem(*(args.grid or [100, 46, 4]), args.notch, R, args.cases_kind)
+        fp32_only = False
     t_asm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    op = device.KronOperator([prob.Kb, prob.Mb], q=6)
-    pr = device.BandPreconditioner(prob.Kb, block=args.block)
-    torch.cuda.synchronize()
-    t_fac = time.perf_counter() - t0
-    print(f"# setup: p={prob.p} assembly {t_asm:.1f}s, upload+factor {t_fac:.1f}s "
-          f"(bandwidth {pr.bandwidth}, {pr.nblocks} blocks of {pr.block})", file=sys.stderr,
-          flush=True)
+    op, pr, sinfo = setup_solver(prob, args, fp32_only)
+    print(f"# setup: p={prob.p} assembly {t_asm:.1f}s, {sinfo}", file=sys.stderr, flush=True)
 
     nsteps = W + K
-    run32 = Runner(prob, op, pr, np.arange(R), mode=1, nsteps=2 * nsteps)
+    res = {}
+    run32 = Runner(prob, op, pr, np.arange(R), mode=1, nsteps=2 * nsteps, xtol=args.xtol)
     for _ in range(W):
         run32.step()
     t32 = timed(run32, K)
@@ -699,36 +974,23 @@ def main():
     rr32 = np.array(run32.relres[W:])
     ok32 = bool(np.all(np.array(run32.ierr[W:]) == 0) and np.all(rr32 <= 1e-8))
     print(f"# fp32-ir batched: {K} steps x {R} cases in {t32:.3f}s, its max {its32.max()}, "
-          f"relres max {rr32.max():.2e}", file=sys.stderr, flush=True)
-    from xtfem_amd.device import from_interleaved
-    print(f"# fp32-ir its per case (last slab): {run32.its[-1].tolist()}", file=sys.stderr)
-    x32 = from_interleaved(run32.xprev, prob.p, 6, R)          # every case after W + K slabs
-
-    res = {}
-    # FP64 flexible GMRES with the FP32 factor sweeps (FP64 operator) on the same batch
-    runf = Runner(prob, op, pr, np.arange(R), mode=2, nsteps=nsteps)
-    for _ in range(W):
-        runf.step()
-    tf = timed(runf, K)
-    rrf = np.array(runf.relres[W:])
-    res["fgmres_fp32prec_batched_slabs_per_s"] = K * R / tf
-    res["fgmres_fp32prec_its_max"] = int(np.max(runf.its[W:]))
-    res["fgmres_fp32prec_relres_max"] = float(rrf.max())
-    print(f"# fp64 fgmres + fp32 sweeps batched: {K} steps x {R} cases in {tf:.3f}s, its max "
-          f"{np.max(runf.its[W:])}, relres max {rrf.max():.2e}", file=sys.stderr, flush=True)
-    del runf
+          f"relres max {rr32.max():.2e}; its per case (last slab) {run32.its[-1].tolist()}",
+          file=sys.stderr, flush=True)
     if not args.no_fp64:
-        # FP64 GMRES on the same batch (all cases in one solve)
+        # plain FP64 GPU GMRES, the reference's preconditioner A^-1 (x) K^-1 (the shifted
+        # factors of the panel are FP32 only), the whole batch in one solve
         runb = Runner(prob, op, pr, np.arange(R), mode=0, nsteps=nsteps)
         for _ in range(W):
             runb.step()
         tb = timed(runb, K)
         res["fp64_batched_slabs_per_s"] = K * R / tb
         res["fp64_batched_its_max"] = int(np.max(runb.its[W:]))
+        res["fp64_batched_relres_max"] = float(np.max(runb.relres[W:]))
+        res["fp64_batched_all_converged"] = bool(np.all(np.array(runb.ierr[W:]) == 0))
         print(f"# fp64 batched: {K} steps x {R} cases in {tb:.3f}s, its max "
               f"{np.max(runb.its[W:])}", file=sys.stderr, flush=True)
         del runb
-        # plain FP64 GPU GMRES on the same slabs, one load case per solve
+        # the same, one load case per solve
         runs = [Runner(prob, op, pr, np.array([r]), mode=0, nsteps=nsteps) for r in range(R)]
         for _ in range(W):
             for rn in runs:
@@ -740,64 +1002,50 @@ def main():
                 rn.step()
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
-        rr64 = np.array([rn.relres[W:] for rn in runs])
         its64 = np.array([rn.its[W:] for rn in runs])
-        res["fp64_slabs_per_s"] = K * R / t64
-        res["fp64_ms_per_slab"] = 1e3 * t64 / (K * R)
-        res["fp64_its_mean"] = float(its64.mean())
-        res["fp64_its_max"] = int(its64.max())
-        res["relres_fp64_max"] = float(rr64.max())
-        # every case of the batched FP32-IR run vs its FP64 run, both after W + K slabs
-        x64 = np.stack([from_interleaved(rn.xprev, prob.p, 6, 1)[0] for rn in runs])
-        err = np.linalg.norm(x32 - x64, axis=1) / np.linalg.norm(x64, axis=1)
-        res["rel_err_vs_fp64_max"] = float(err.max())
+        res["fp64_single_slabs_per_s"] = K * R / t64
+        res["fp64_single_its_mean"] = float(its64.mean())
+        res["fp64_single_relres_max"] = float(np.max([rn.relres[W:] for rn in runs]))
         print(f"# fp64 one case per solve: {K}x{R} slabs in {t64:.3f}s, its mean "
-              f"{its64.mean():.1f} max {its64.max()}, relres max {rr64.max():.2e}, "
-              f"rel err fp32-ir vs fp64 max {err.max():.2e}", file=sys.stderr, flush=True)
-    spmm = spmm_microbench(op, prob.p)
-    spmm["R32"] = spmm_microbench(op, prob.p, R=32)
-    spmm["panel"] = spmm_panel_leg()
-    print(f"# kron-spmm fp32: plate R=1 {spmm['gbs']:.0f} GB/s ({spmm['frac']:.2f}), R=32 "
-          f"{spmm['R32']['gbs']:.0f} GB/s; panel R=1 {spmm['panel']['R1']['gbs']:.0f} GB/s "
-          f"({spmm['panel']['R1']['frac']:.2f}), R=32 {spmm['panel']['R32']['gbs']:.0f} GB/s",
-          file=sys.stderr, flush=True)
-    if not args.headline_only:
+              f"{its64.mean():.1f}", file=sys.stderr, flush=True)
+        del runs
+        res["precision_ratio"] = (K * R / t32) / res["fp64_batched_slabs_per_s"]
+        res["batching_ratio"] = res["fp64_batched_slabs_per_s"] / res["fp64_single_slabs_per_s"]
+        res["speedup_vs_fp64_single"] = (K * R / t32) / res["fp64_single_slabs_per_s"]
+    res["accuracy_slab0"] = accuracy_check(prob, op, pr, args.xtol)
+    print(f"# accuracy (slab 0 vs FP64 at 1e-12): {res['accuracy_slab0']}", file=sys.stderr,
+          flush=True)
+    spmm = spmm_legs(op, prob.p)
+    print(f"# kron-spmm fp32: R=1 {spmm['R1']['gbs']:.0f} GB/s ({spmm['R1']['frac']:.2f}), "
+          f"R=32 {spmm['R32']['gbs']:.0f} GB/s ({spmm['R32']['frac']:.2f})", file=sys.stderr,
+          flush=True)
+    legs = set(args.legs.split(",")) - {""}
+    if "all" in legs:
+        legs = {"rod", "reassembly", "thermo", "driver_panel", "deletion", "damage", "fatigue"}
+    plate = prob if isinstance(prob, Problem) else None
+    if legs & {"reassembly", "thermo", "deletion"} and plate is None:
+        plate = Problem(100, 46, 4, 20, min(R, 8), "plate")
+    if "rod" in legs:
         res["rod_ilut"] = rod_leg(K)
-        print(f"# rod ILUT: fp64 {res['rod_ilut']['fp64']['ms_per_slab']:.2f} ms/slab, fp32-ir "
-              f"{res['rod_ilut']['fp32_ir']['ms_per_slab']:.2f} ms/slab", file=sys.stderr, flush=True)
-        res["reassembly"] = reassembly_leg(prob)
-        print(f"# re-assembly per slab: "
-              + ", ".join(f"{k} {v:.1f} ms" for k, v in res["reassembly"]["ms"].items())
-              + f"; host numpy assembly {res['reassembly']['host_assembly_ms']:.0f} ms",
-              file=sys.stderr, flush=True)
-        res["thermo"] = thermo_leg(prob)
-        for k in ("fp64_reference_prec", "fp64", "fp32_ir"):
-            r = res["thermo"][k]
-            print(f"# thermo-mechanical slab ({k}): {r['ms_per_slab']:.1f} ms/slab, its "
-                  f"{r['its_thermal']}/{r['its_mech']}; "
-                  + ", ".join(f"{a} {b:.1f}" for a, b in r["phases_ms"].items()) + " ms",
-                  file=sys.stderr, flush=True)
+    if "reassembly" in legs:
+        res["reassembly"] = reassembly_leg(plate)
+    if "thermo" in legs:
+        res["thermo"] = thermo_leg(plate)
+    if "driver_panel" in legs:
         res["driver_panel"] = driver_panel_leg()
-        print(f"# reference panel mesh, coupled slab: {res['driver_panel']['ms_per_slab']:.1f} ms, its "
-              f"{res['driver_panel']['its_thermal']}/{res['driver_panel']['its_mech']}, T max "
-              f"{res['driver_panel']['T_max']:.1f}; "
-              + ", ".join(f"{a} {b:.1f}" for a, b in res["driver_panel"]["phases_ms"].items()),
-              file=sys.stderr, flush=True)
-        res["deletion"] = deletion_leg(prob)
-        print(f"# deletion: {res['deletion']['nefail']} elements, "
-              + ", ".join(f"{k} {v:.1f} ms" for k, v in res["deletion"]["ms"].items())
-              + f", next slab its max {res['deletion']['its_max']}", file=sys.stderr, flush=True)
-        if not args.no_damage:
-            res["damage"] = damage_leg(args.damage_nip)
-            print(f"# damage: {res['damage']['ms_per_slab']:.1f} ms per slab, "
-                  f"{res['damage']['gp_steps_per_s']:.3e} GP-steps/s", file=sys.stderr, flush=True)
+    if "deletion" in legs:
+        res["deletion"] = deletion_leg(plate)
+    if "damage" in legs:
+        res["damage"] = damage_leg(args.damage_nip)
+    if "fatigue" in legs:
+        res["fatigue"] = fatigue_leg(args.fatigue_blocks, args.damage_nip)
+        print(f"# fatigue: {res['fatigue']}", file=sys.stderr, flush=True)
     # per-kernel durations: the batched FP32-IR steps once more, every launch of the
-    # operator and sweep kernels bracketed by HIP events (after the timed legs)
+    # operator and sweep kernels bracketed by HIP events on the launching stream
     prof = kernel_profile(run32, K, mask=0b11)
 
     value = K * R / t32
     dom_name = max(prof, key=lambda k: prof[k]["ms"])
-    dom = prof[dom_name]
     out = {
         "metric": "slab solves/s at rel.res 1e-8 (FP32-IR batched vs FP64 GPU); Kron-SpMM GB/s",
         "value": value,
@@ -810,26 +1058,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 operator/factor sweeps in f64 GMRES-IR",
-        "data": "synthetic (structured notched-plate mesh and seeded load cases)",
-        "config": {"workload": f"notched plate {nx}x{ny}x{nz} C3D8, notch {args.notch}x1, "
-                               f"3 DOF/node, p={prob.p}, n_t=6, {R} load cases per solve "
-                               f"({args.cases_kind}), "
-                               f"GMRES(30) eps 1e-8, exact banded factor "
-                               f"({pr.nblocks} blocks of {pr.block})",
+        "data": "synthetic (structured mesh, seeded load cases)",
+        "config": {"workload": prob.describe() + f"; GMRES(30) eps 1e-8, forward-error test "
+                               f"{args.xtol:g}; exact banded factors ({pr.nblocks} blocks of "
+                               f"{pr.block}), {len(sinfo['shifts'])} shifted",
                    "cases": R, "p": prob.p},
         "all_converged": ok32,
         "relres_max": float(rr32.max()),
         "its_max": int(its32.max()),
         "its_mean": float(its32.mean()),
-        "speedup_vs_fp64": (value / res["fp64_slabs_per_s"]) if "fp64_slabs_per_s" in res else None,
         **res,
+        "solver_setup": sinfo,
         "kron_spmm": spmm,
         "kernels": prof,
-        "roofline": roofline(dom_name, dom),
-        "setup_s": {"assembly": t_asm, "upload_factor": t_fac},
+        "roofline": roofline(dom_name, prof[dom_name], prob, pr),
+        "setup_s": {"assembly": t_asm},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prob)
+        del run32
+        out["cpu_baseline"] = run_cpu_baseline(args, prob)
     print(json.dumps(out))
 
 

@@ -208,6 +208,28 @@ void xt_prec_destroy(xt_prec pr);
  * device.  XT_E_SINGULAR if K is not positive definite. */
 int xt_prec_create_band(int p, const int *rowptr, const int *colidx, const double *vals,
                         int block, xt_prec *out);
+/* Exact factor K^-1 as xt_prec_create_band, plus the factors (K + s_g M)^-1 for the
+ * nshift positive shifts s_g (same blocks; K and M 1-based CSR, host, rows sorted by
+ * column).  A slab solve with this preconditioner splits each load case's temporal
+ * coupling C_r = A_r^-1 B_r: its dominant eigenvalue lambda_r (real, positive: the
+ * mass-like temporal mode that grows with the load frequency) with right and left
+ * eigenvectors v_r, u_r (u_r . v_r = 1) is preconditioned by the factor whose shift is
+ * nearest lambda_r (log scale; none when lambda_r is below a quarter of the smallest
+ * shift), the rest by K^-1:
+ *   P_r^-1 = [(I - v_r u_r^T) (x) K^-1 + v_r u_r^T (x) (K + s M)^-1] (A_r^-1 (x) I).
+ * With s = lambda_r this inverts the mode's spatial operator K + lambda_r M exactly.
+ * No counterpart in the reference, whose preconditioner A^-1 (x) K^-1 (mpilusol,
+ * kronecker_mumps.f90:120-183) drops the mass term B (x) M: at load frequencies where
+ * lambda_r reaches the structure's lowest K^-1 M eigenvalues, GMRES with a factor
+ * rounded to FP32 stagnates. */
+#define XT_SHIFT_FP32_ONLY 1   /* flags: keep the shifted factors in FP32 only (a third of
+                                  the memory); FP64 applies then use A^-1 (x) K^-1 */
+int xt_prec_create_band_shifted(int p, const int *krow, const int *kcol, const double *kval,
+                                const int *mrow, const int *mcol, const double *mval,
+                                int nshift, const double *shifts, int block, int flags,
+                                xt_prec *out);
+/* Number of shifted factors of a preconditioner and (when shifts != NULL) their shifts. */
+int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts);
 /* Refactor a banded-factor preconditioner with new values of K (1-based CSR in device
  * memory, within the band it was created for: XT_E_ARG otherwise).  Replaces the
  * per-slab refactorization mumps_solver(job = 1), main3d.f90:783. */
@@ -313,6 +335,17 @@ int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, const double *b
                   double *x, int im, double eps, int maxits, int mode, int *its,
                   int *ierr, double *relres, void *stream);
 
+/* xt_slab_solve with a forward-error test for mode 1: a case converges when its true
+ * relative residual meets eps and its last refinement correction is at most xtol times
+ * its solution norm (xtol = 0: the residual test alone, as xt_slab_solve).  At load
+ * frequencies where the slab operator is ill-conditioned the residual bounds the error
+ * of the displacement only loosely; the correction of a converging refinement does
+ * (each step contracts the error).  Modes 0 and 2 ignore xtol.  No counterpart in the
+ * reference (kronpgmres stops on the residual, kronecker.f90:312). */
+int xt_slab_solve_tol(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
+                      double *x, int im, double eps, double xtol, int maxits, int mode,
+                      int *its, int *ierr, double *relres, void *stream);
+
 /* chkelesta (delelement.f90:109-136) on device-resident state: failed[e] = 1 iff
  * some Gauss point of element e has D >= Dc, else 0; for a failed element
  * elesta[e] and its gpsta entries are set to 0 (others untouched).  Device
@@ -333,6 +366,17 @@ int xt_damage_device(int ne, int nn, int nip, const int *nodes, const double *Nx
                      double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
                      double *p, double *D, double *ws, double *del_po, double *sig_d,
                      void *stream);
+
+/* xt_damage_device with other constants of the damage law: params[7] = sig_f, sig_u,
+ * eps_D, Cy, h, S, s (damage.h:9-17 names; NULL: the reference's 180, 577, 0.08, 1740,
+ * 0.2, 0.5, 0.5).  The reference compiles them in; a material with another fatigue
+ * limit (e.g. a 350 MPa yield) needs them at run time. */
+int xt_damage_device_params(int ne, int nn, int nip, const int *nodes, const double *Nx,
+                            const double *Ny, const double *Nz, const double *Nt,
+                            const double *sol, const double *ym, const double *nu,
+                            double *eps_old, double *eps_pl, double *sig_eff, double *sig_back,
+                            double *p, double *D, double *ws, double *del_po, double *sig_d,
+                            const double *params, void *stream);
 
 /* Right-hand side of a slab for R load cases (main3d.f90:803-823):
  *   b[dof][r*q + j] := fixed[dof] ? 0 : b[dof][r*q + j] + coef[r*q + j] * rext_r[dof]
@@ -356,6 +400,11 @@ int xt_prof_enable(unsigned mask);
 /* Sum of per-launch kernel durations (ms), number of launches, and the launches'
  * algorithmic bytes and flops since xt_prof_enable.  Synchronises on the last event. */
 int xt_prof_read(int kind, double *ms, long long *launches, double *bytes, double *flops);
+
+/* Algorithmic bytes and flops recorded since xt_prof_enable: the work of the unpaired
+ * computation.  The banded-factor sweeps perform more flops than that (the pairing
+ * products of their recurrences); for the other kinds it equals xt_prof_read's. */
+int xt_prof_read_alg(int kind, double *bytes, double *flops);
 
 /* Replay the banded factor's sweeps (2 + 2 (nb - 1) launches of fixed buffers) from a
  * HIP graph captured on first use per column count and precision (on = 1, the

@@ -17,6 +17,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+# spatial solve of or_kronpgmres_cb: (ctx, p, q, x [q*p], w [q*p]) -> 0 on success
+SPATIAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                         C.POINTER(C.c_double))
 _i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
@@ -50,12 +53,17 @@ def lib():
                                          _d, _d, ci, cd, ci, _d, ci, C.POINTER(ci),
                                          C.POINTER(ci)]
         L.or_kronpgmres_band.restype = ci
+        L.or_kronpgmres_cb.argtypes = [_d, _d, ci, _i, _i, _d, _i, _i, _d, ci, SPATIAL_FN,
+                                       C.c_void_p, _d, _d, ci, cd, ci, _d, ci, C.POINTER(ci),
+                                       C.POINTER(ci)]
+        L.or_kronpgmres_cb.restype = ci
         L.or_stcoeff_m.argtypes = [cd, cd, cd, _d, _d, _d, _d, _d, _d]
         L.or_stcoeff_t.argtypes = [cd, cd, _d, _d, _d, _d, _d, _d]
         L.or_getfextc_m.argtypes = [cd, cd, cd, _d, _d]
         L.or_damage_nt.argtypes = [_d, cd, ci, _d]
         L.or_kokkos_host.argtypes = [ci] * 9 + [_i, _d, _d, _d, _d, _d, _d, _d] + [_d] * 9
         L.or_damage_step.argtypes = [_d, cd, cd, _d, _d, _d, _d, _d]
+        L.or_damage_set_params.argtypes = [C.c_void_p]
         L.or_chkelesta.argtypes = [ci, ci, cd, _d, _i, _i, _i]
         L.or_chkelesta.restype = ci
         _LIB = L
@@ -162,10 +170,40 @@ def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
     tail = (rhs, sol, im, eps, maxits, hist, cap, C.byref(nh), C.byref(its))
     if isinstance(prec, BandFactor):
         ierr = lib().or_kronpgmres_band(*args, prec.lb, prec.w, *tail)
+    elif isinstance(prec, LapackBandFactor):
+        ierr = lib().or_kronpgmres_cb(*args, prec.callback, None, *tail)
     else:
         alu, jlu, ju = prec
         ierr = lib().or_kronpgmres(*args, _f64(alu), _i32(jlu), _i32(ju), *tail)
     return sol, ierr, its.value, hist[:nh.value].copy()
+
+
+class LapackBandFactor:
+    """K^-1 for or_kronpgmres_cb: band Cholesky of an SPD CSR matrix by LAPACK (dpbtrf /
+    dpbtrs through scipy), for meshes too large for or_bandchol's scalar loop.  The same
+    exact factor (the live path's K^-1, up to rounding).  The solve is called back from
+    the oracle's GMRES for the q temporal blocks at once."""
+
+    def __init__(self, K):
+        import scipy.linalg as sla
+        ia, ja, a = _i32(K[0]), _i32(K[1]), _f64(K[2])
+        n = len(ia) - 1
+        rows = np.repeat(np.arange(n), np.diff(ia))
+        cols = ja - 1
+        low = cols <= rows
+        self.n = n
+        self.w = int(np.max(rows[low] - cols[low])) if n else 0
+        ab = np.zeros((self.w + 1, n))
+        ab[rows[low] - cols[low], cols[low]] = a[low]
+        self.cb = sla.cholesky_banded(ab, lower=True, overwrite_ab=True, check_finite=False)
+        self._sla = sla
+
+        def solve(_ctx, p, q, x, w):
+            X = np.ctypeslib.as_array(x, shape=(q, p))
+            W = np.ctypeslib.as_array(w, shape=(q, p))
+            W[:] = self._sla.cho_solve_banded((self.cb, True), X.T, check_finite=False).T
+            return 0
+        self.callback = SPATIAL_FN(solve)
 
 
 class BandFactor:
@@ -227,6 +265,16 @@ def damage_nt(tcoorde, w, nip):
     out = np.zeros(6 * nip)
     lib().or_damage_nt(_f64(tcoorde), w, nip, out)
     return out.reshape(nip, 6)
+
+
+def damage_params(params=None):
+    """Set the damage-law constants (sig_f, sig_u, eps_D, Cy, h, S, s) of the oracle's
+    functor; None restores the reference's (damage.h:9-17)."""
+    if params is None:
+        lib().or_damage_set_params(None)
+    else:
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        lib().or_damage_set_params(p.ctypes.data)
 
 
 def kokkos_host(ne, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu, state):

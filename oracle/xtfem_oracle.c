@@ -539,6 +539,44 @@ int or_kronpgmres_band(const double *a, const double *b, int q,
                               im, eps, maxits, hist, hist_cap, nhist, its_out);
 }
 
+/* kronpgmres (kronecker.f90:181-365) with the spatial solve K^-1 supplied by the
+   caller: solve(ctx, p, q, x, w) writes w_j = K^-1 x_j for the q temporal blocks of x
+   (any exact factorization of K: the live path's MUMPS role); the temporal mix
+   y_i = sum_j inva(i,j) w_j is kronlusol's (kronecker.f90:171-176). */
+typedef int (*or_spatial_fn)(void *ctx, int p, int q, const double *x, double *w);
+struct or_cb_ctx { or_spatial_fn solve; void *ctx; };
+static int or_prec_cb(const double *a, int q, const void *ctx, int p, const double *x,
+                      double *y)
+{
+    const struct or_cb_ctx *c = (const struct or_cb_ctx *)ctx;
+    double inva[64];
+    if (or_inverse(q, a, inva)) return 1;
+    size_t n = (size_t)p * q;
+    double *wv = (double *)calloc(n, sizeof(double));
+    if (c->solve(c->ctx, p, q, x, wv)) { free(wv); return 1; }
+    for (size_t k = 0; k < n; ++k) y[k] = 0.0;
+    for (int i = 0; i < q; ++i)
+        for (int j = 0; j < q; ++j) {
+            double s = inva[i + j * q];
+            double *yi = y + (size_t)i * p;
+            const double *wj = wv + (size_t)j * p;
+            for (int k = 0; k < p; ++k) yi[k] = yi[k] + s * wj[k];
+        }
+    free(wv);
+    return 0;
+}
+int or_kronpgmres_cb(const double *a, const double *b, int q,
+                     const int *ki, const int *kj, const double *kv,
+                     const int *mi, const int *mj, const double *mv, int p,
+                     or_spatial_fn solve, void *sctx,
+                     double *rhs, double *sol, int im, double eps, int maxits,
+                     double *hist, int hist_cap, int *nhist, int *its_out)
+{
+    struct or_cb_ctx c = {solve, sctx};
+    return or_kronpgmres_impl(a, b, q, ki, kj, kv, mi, mj, mv, p, or_prec_cb, &c, rhs, sol,
+                              im, eps, maxits, hist, hist_cap, nhist, its_out);
+}
+
 /* ------------------------------------------------------------------------ */
 /* stcoeff_m: closed-form 6x6 temporal matrices of one slab.                  */
 /* stcoeff.f90:62-356.  Outputs column-major 6x6.  Default-real literals of   */
@@ -787,13 +825,21 @@ void or_damage_nt(const double *tcoorde, double w, int nip, double *Nt)
 /* nodes[8*g + a] (1-based), Nx/Ny/Nz[8*g + a], Nt[jj*6 + k], sol temporal-   */
 /* major, per-GP state vectors [6*g + c] and scalars [g].                     */
 /* ------------------------------------------------------------------------ */
-#define D_CY 1740.0
-#define D_H 0.2
-#define D_S 0.5
-#define D_SEXP 0.5
-#define D_SIGF 180.0
-#define D_SIGU 577.0
-#define D_EPSD 0.08
+/* The constants are variables so that a caller may set another material (the
+   reference's are the defaults, damage.h:9-17): sig_f, sig_u, eps_D, Cy, h, S, s. */
+static double g_dp[7] = {180.0, 577.0, 0.08, 1740.0, 0.2, 0.5, 0.5};
+void or_damage_set_params(const double *params)
+{
+    static const double ref[7] = {180.0, 577.0, 0.08, 1740.0, 0.2, 0.5, 0.5};
+    for (int i = 0; i < 7; ++i) g_dp[i] = params ? params[i] : ref[i];
+}
+#define D_SIGF (g_dp[0])
+#define D_SIGU (g_dp[1])
+#define D_EPSD (g_dp[2])
+#define D_CY (g_dp[3])
+#define D_H (g_dp[4])
+#define D_S (g_dp[5])
+#define D_SEXP (g_dp[6])
 #define D_WD ((D_SIGU - D_SIGF) * D_EPSD)
 
 /* One return-map step of the functor for a given strain (kokkos_kernel.h:85-341).

@@ -369,3 +369,25 @@ def test_kokkos_host_strain_of_linear_field(oracle_lib):
         assert np.all(st["p"] == 0) and np.all(st["D"] == 0)          # elastic branch
         sig = np.stack([fem.elastic_dmat(ym[g], nu[g]) @ got[g] for g in range(ngp)])
         assert rel(st["sig_eff"].reshape(ngp, 6), sig) < 1e-13
+
+
+def test_lapack_band_factor_matches_band_cholesky(oracle_lib):
+    """The LAPACK band factor behind or_kronpgmres_cb (the CPU baseline's K^-1 for large
+    meshes) is the same exact factor as or_bandchol's: K^-1 y agrees to rounding, and
+    the oracle's GMRES takes the same iterations to the same solution."""
+    from xtfem_amd import slab
+    S = structured_system(9, 4, 3, 45.0, 20.0, 15.0)
+    Kb, Mb, p = S["Kb"], S["Mb"], S["p"]
+    bf, lf = oracle_lib.BandFactor(Kb), oracle_lib.LapackBandFactor(Kb)
+    assert bf.w == lf.w
+    rng = np.random.default_rng(2)
+    y = rng.standard_normal(p)
+    import scipy.linalg as sla
+    x = sla.cho_solve_banded((lf.cb, True), y)
+    assert rel(dense(Kb) @ x, y) < 1e-10 and rel(x, bf.solve(y)) < 1e-10
+    c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * 40.0)
+    A, B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+    rhs = rng.standard_normal(6 * p)
+    x1, i1, its1, _ = oracle_lib.kronpgmres(A, B, Kb, Mb, bf, rhs, eps=1e-12)
+    x2, i2, its2, _ = oracle_lib.kronpgmres(A, B, Kb, Mb, lf, rhs, eps=1e-12)
+    assert i1 == i2 == 0 and its1 == its2 and rel(x2, x1) < 1e-9

@@ -66,6 +66,8 @@ _SIGS = {
                                 vp]),
     "xt_prec_create": (ci, [ci, vp, vp, vp, vp]),
     "xt_prec_create_band": (ci, [ci, vp, vp, vp, ci, vp]),
+    "xt_prec_create_band_shifted": (ci, [ci, vp, vp, vp, vp, vp, vp, ci, vp, ci, ci, vp]),
+    "xt_prec_shifts": (ci, [vp, vp, vp]),
     "xt_prec_band_info": (ci, [vp, vp, vp, vp]),
     "xt_prec_destroy": (None, [vp]),
     "xt_prec_levels": (ci, [vp, vp, vp]),
@@ -74,11 +76,14 @@ _SIGS = {
     "xt_to_interleaved": (ci, [ci, ci, ci, vp, vp, vp]),
     "xt_from_interleaved": (ci, [ci, ci, ci, vp, vp, vp]),
     "xt_slab_solve": (ci, [vp, vp, ci, vp, vp, vp, ci, cd, ci, ci, vp, vp, vp, vp]),
+    "xt_slab_solve_tol": (ci, [vp, vp, ci, vp, vp, vp, ci, cd, cd, ci, ci, vp, vp, vp, vp]),
     "xt_damage_device": (ci, [ci, ci, ci] + [vp] * 18),
+    "xt_damage_device_params": (ci, [ci, ci, ci] + [vp] * 19),
     "xt_slab_rhs": (ci, [ci, ci, ci, vp, vp, C.c_longlong, vp, vp, vp]),
     "xt_prof_enable": (ci, [C.c_uint]),
     "xt_graphs_enable": (ci, [ci]),
     "xt_prof_read": (ci, [ci, vp, vp, vp, vp]),
+    "xt_prof_read_alg": (ci, [ci, vp, vp]),
     "xt_device_sync": (ci, []),
 }
 

@@ -61,6 +61,7 @@ struct BandPrec {
     DevBuf Dl[2], Du[2], G[2], H[2];
     DevBuf Lv[2][kLevels][2];
     int nlev = 0;
+    bool keep64 = true;          // false: the FP64 sets are released after the FP32 copies
     DevBuf w1, w2;
     SweepGraph graph[4];
     int next_graph = 0;
@@ -391,7 +392,7 @@ template <typename T>
 static void band_prof_end(int b, int c, int tri, int nblk, double out_elems, hipStream_t s) {
     const double mat = tri ? (double)b * (b + 1) / 2 : (double)b * b;
     const double bytes = sizeof(T) * nblk * (mat + (double)b * c + out_elems);
-    prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c);
+    prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c, false);
 }
 
 // Matrix-core launch for many columns.
@@ -903,6 +904,12 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
         XT_HIP(hipGetLastError());
     }
     XT_HIP(hipStreamSynchronize(s));
+    if (!bp.keep64)
+        for (int k = 0; k < 4 + 2 * kLevels; ++k) {
+            DevBuf *src = k == 0 ? bp.Dl : k == 1 ? bp.Du : k == 2 ? bp.G : k == 3 ? bp.H
+                        : bp.Lv[(k - 4) / kLevels][(k - 4) % kLevels];
+            src[0].release();
+        }
     return XT_OK;
 }
 
@@ -923,7 +930,8 @@ int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da
     return factor_dev(*bp, dia, dja, da, s);
 }
 
-int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out) {
+int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out,
+               bool keep64) {
     XT_ARG(p > 0 && ia && ja && a && out, "band prec arguments");
 #define XT_ROWS_ATTR(T, CM, RW)                                                                \
     XT_HIP(hipFuncSetAttribute((const void *)band_rows_kernel<T, CM, RW>,                      \
@@ -955,6 +963,7 @@ int band_build(int p, const int *ia, const int *ja, const double *a, int block, 
     bp->b = b;
     bp->nb = nb;
     bp->bw = bw;
+    bp->keep64 = keep64;
     hipStream_t s = nullptr;
     int rc = factor(*bp, ia, ja, a, s);
     if (rc != XT_OK) { delete bp; return rc; }
@@ -970,10 +979,16 @@ void band_dims(const BandPrec *bp, int *b, int *nb, int *bw) {
     if (bw) *bw = bp->bw;
 }
 
+bool band_has_fp64(const BandPrec *bp) { return bp->keep64; }
+
 template <typename T>
 int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s) {
     const int c = q * R, b = bp->b, nb = bp->nb;
     const int pi = sizeof(T) == 4 ? 1 : 0;
+    if (pi == 0 && !bp->keep64) {
+        set_error("band factor kept in FP32 only: no FP64 sweeps");
+        return XT_E_UNSUPPORTED;
+    }
     const long long npad = (long long)nb * b * c, nreal = (long long)bp->p * c;
     XT_TRY(bp->w1.reserve(sizeof(T) * npad));
     XT_TRY(bp->w2.reserve(sizeof(T) * npad));
@@ -1065,6 +1080,13 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
         XT_HIP(hipGraphLaunch(g->exec, s));
     } else {
         XT_TRY(sweeps(s));
+        // algorithmic work of the apply: the unpaired block recurrences, each factor block
+        // read once (D_i, D_i^T as triangles, G_i, H_i), each vector block read and written
+        // once per product (plus the read of the block it is subtracted from)
+        const double tri = (double)b * (b + 1) / 2, full = (double)b * b, vb = (double)b * c;
+        prof_alg(XT_PROF_BAND,
+                 sizeof(T) * (2.0 * (nb * tri + (nb - 1) * full) + 2.0 * (2.0 * nb + 3.0 * (nb - 1)) * vb),
+                 2.0 * (2.0 * nb * tri * c + 2.0 * (nb - 1) * full * c));
     }
     XT_HIP(hipGetLastError());
     XT_HIP(hipMemcpyAsync(y, w1, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));

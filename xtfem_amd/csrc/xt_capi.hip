@@ -13,9 +13,7 @@
 #include "xt_common.h"
 
 namespace xt {
-int slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b, double *x,
-                    int im, double eps, int maxits, int mode, int *its, int *ierr,
-                    double *relres, hipStream_t stream, std::vector<double> *hist);
+
 int elestatus_device(int ne, int ngpe, double Dc, const double *D, int *elesta, int *gpsta,
                      int *failed, hipStream_t s);
 template <typename T>

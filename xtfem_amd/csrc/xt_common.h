@@ -130,7 +130,45 @@ bool prof_on(int kind);
 // Replay fixed launch sequences (the band sweeps) from captured HIP graphs.
 bool graphs_on();
 void prof_begin(int kind, hipStream_t s);
-void prof_end(int kind, hipStream_t s, double bytes, double flops);
+// bytes / flops the launch performs; alg: they are also its algorithmic share (else the
+// launch site records the algorithmic totals with prof_alg)
+void prof_end(int kind, hipStream_t s, double bytes, double flops, bool alg = true);
+void prof_alg(int kind, double bytes, double flops);
+
+// Rank-1 temporal split of a preconditioner that holds shifted factors
+// (K + s_g M)^-1 beside K^-1.  For load case r with temporal matrices A_r, B_r let
+// lambda_r be the dominant eigenvalue of C_r = A_r^-1 B_r, v_r, u_r its right and left
+// eigenvectors (u_r . v_r = 1).  The preconditioner
+//   P_r^-1 = [(I - v_r u_r^T) (x) K^-1 + v_r u_r^T (x) (K + s_g M)^-1] (A_r^-1 (x) I)
+// applies K^-1 to the temporal mix (I - v u^T) A^-1 (passed as minv) and the shifted
+// factor to the single column m2_r = u_r^T A_r^-1 X of the case, added back as v_r times
+// the result.  Split cases are numbered t = 0..ns-1 group by group; the gathered column
+// of case t sits at base[t] + dof * stride[t] (group g: a contiguous p x n_g block).
+struct PrecSplit {
+    int ns = 0;
+    std::vector<int> goff, gfac;   // host: column offsets of the groups, their factor index
+    DevBuf dcase, dbase, dstride;  // device: int case[ns], long long base[ns], int stride[ns]
+    DevBuf m2v[2];                 // device [ns][2q]: m2 (q) then v (q); FP64 [0], FP32 [1]
+    // host sources of the uploads (kept until the next build: the copies are asynchronous)
+    std::vector<int> hcase, hstride;
+    std::vector<long long> hbase;
+    std::vector<double> h64;
+    std::vector<float> h32;
+};
+// Shifted factors of a preconditioner: their number, the shifts when non-NULL, and
+// whether they have FP64 copies (else the split serves the FP32 applies only).
+int prec_shifts(xt_prec pr, const double **shifts, bool *fp64 = nullptr);
+// Y = P^-1 X with the split above (sp may be NULL or empty: plain A^-1 (x) K^-1).
+template <typename T>
+int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *sp, const T *x,
+                     T *y, hipStream_t s);
+
+// Batched slab solve (xt_slab_solve); hist: residual history of a one-case FP64 solve;
+// xtol: forward-error test of the refinement (xt_slab_solve_tol).
+int slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b, double *x,
+                    int im, double eps, int maxits, int mode, int *its, int *ierr,
+                    double *relres, hipStream_t stream, std::vector<double> *hist,
+                    double xtol = 0.0);
 
 // Device workspace of the batched slab solver, owned by the operator handle and kept
 // across calls (buffers only grow), so a solve allocates nothing once warm.
@@ -138,18 +176,22 @@ struct SolverWorkspace {
     DevBuf tm64, tm32, mi64, mi32;   // temporal matrices and inverses
     DevBuf V, Z;                     // Arnoldi basis, preconditioned basis (flexible)
     DevBuf w32a, w32b, z64, t64;     // work vectors
+    DevBuf xprev;                    // iterate before a refinement step (forward-error test)
     DevBuf partial, hdev, hdev2, ndev, coef;
     // Givens state of a cycle in one device region (Hessenberg columns, rotations, rhs
     // of the least squares problem, last residual norms, counters, frozen flags), the
     // targets and the residual history; pinned host staging mirrors the state region
     DevBuf gstate, geps, ghist;
     PinnedBuf hstage;
+    PrecSplit split;                 // temporal split of a shifted-factor preconditioner
 };
 SolverWorkspace *op_workspace(xt_op op);
 
 // Exact block-tridiagonal factor of a banded SPD matrix (xt_band.hip).
 struct BandPrec;
-int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out);
+int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out,
+               bool keep64 = true);
+bool band_has_fp64(const BandPrec *bp);
 void band_delete(BandPrec *bp);
 void band_dims(const BandPrec *bp, int *b, int *nb, int *bw);
 int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da,

@@ -28,13 +28,17 @@
 
 namespace xt {
 
-#define XT_CY 1740.0
-#define XT_H 0.2
-#define XT_S 0.5
-#define XT_SIGF 180.0
-#define XT_SIGU 577.0
-#define XT_EPSD 0.08
-#define XT_WD ((XT_SIGU - XT_SIGF) * XT_EPSD)
+// Material constants of the damage law.  Defaults: damage.h:9-17 (Cy = 1740, h = 0.2,
+// S = 0.5, s = 0.5, sig_f = 180, sig_u = 577, eps_D = 0.08); the damage threshold is
+// wd = (sig_u - sig_f) eps_D (kokkos_kernel.h:47).
+struct DamageParams {
+    double sigf = 180.0, sigu = 577.0, epsd = 0.08, cy = 1740.0, h = 0.2, S = 0.5, sexp = 0.5;
+};
+#define XT_CY dp.cy
+#define XT_H dp.h
+#define XT_S dp.S
+#define XT_SIGF dp.sigf
+#define XT_WD ((dp.sigu - dp.sigf) * dp.epsd)
 
 // Block = one wave of 64 lanes = 8 elements x 8 Gauss points (GP ii = 8 e + g).
 __global__ void __launch_bounds__(64)
@@ -46,7 +50,7 @@ damage_kernel(int ngp, int nn, int nip, const int *__restrict__ nodes,
               double *__restrict__ eps_pl, double *__restrict__ sig_eff,
               double *__restrict__ sig_back, double *__restrict__ pp, double *__restrict__ dd,
               double *__restrict__ wsv, double *__restrict__ del_po_v,
-              double *__restrict__ sig_d_v) {
+              double *__restrict__ sig_d_v, const DamageParams dp) {
     __shared__ double xch[2][64][4];  // [parity][lane][disp of the lane's node, padded]
     const int lane = threadIdx.x;
     const int ii = blockIdx.x * 64 + lane;
@@ -223,7 +227,8 @@ damage_kernel(int ngp, int nn, int nip, const int *__restrict__ nodes,
             double Yn;
             if (trt >= 0.0) Yn = nui / (2.0 * ymi) * trt * trt;
             else Yn = nui / (2.0 * ymi) * XT_H * hr * hr * trt * trt;
-            d += sqrt((Yp - Yn) / XT_S) * del_p;   // (Y/S)^s, s = 0.5, correctly rounded
+            // (Y/S)^s: s = 0.5 as the correctly rounded sqrt (the reference's value)
+            d += (dp.sexp == 0.5 ? sqrt((Yp - Yn) / XT_S) : pow((Yp - Yn) / XT_S, dp.sexp)) * del_p;
         } else {
             d = 0.0;
         }
@@ -274,13 +279,20 @@ zero_failed_kernel(int ngp, const int *__restrict__ gpsta, double *D, double *ws
 
 using namespace xt;
 
-extern "C" int xt_damage_device(int ne, int nn, int nip, const int *nodes, const double *Nx,
-                                const double *Ny, const double *Nz, const double *Nt,
-                                const double *sol, const double *ym, const double *nu,
-                                double *eps_old, double *eps_pl, double *sig_eff,
-                                double *sig_back, double *p, double *D, double *ws,
-                                double *del_po, double *sig_d, void *stream) {
+extern "C" int xt_damage_device_params(int ne, int nn, int nip, const int *nodes,
+                                       const double *Nx, const double *Ny, const double *Nz,
+                                       const double *Nt, const double *sol, const double *ym,
+                                       const double *nu, double *eps_old, double *eps_pl,
+                                       double *sig_eff, double *sig_back, double *p, double *D,
+                                       double *ws, double *del_po, double *sig_d,
+                                       const double *params, void *stream) {
     XT_ARG(ne >= 0 && nn > 0 && nip >= 0, "damage sizes");
+    DamageParams dp;
+    if (params) {
+        dp.sigf = params[0]; dp.sigu = params[1]; dp.epsd = params[2]; dp.cy = params[3];
+        dp.h = params[4]; dp.S = params[5]; dp.sexp = params[6];
+        XT_ARG(dp.sigf > 0.0 && dp.sigu > dp.sigf && dp.S > 0.0, "damage parameters");
+    }
     XT_ARG(nodes && Nx && Ny && Nz && (nip == 0 || Nt) && sol && ym && nu && eps_old && eps_pl &&
                sig_eff && sig_back && p && D && ws && del_po && sig_d,
            "damage arrays");
@@ -290,12 +302,23 @@ extern "C" int xt_damage_device(int ne, int nn, int nip, const int *nodes, const
     if (prof) prof_begin(XT_PROF_DAMAGE, (hipStream_t)stream);
     hipLaunchKernelGGL(damage_kernel, dim3(ceil_div(ngp, 64)), dim3(64), 0,
                        (hipStream_t)stream, ngp, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu,
-                       eps_old, eps_pl, sig_eff, sig_back, p, D, ws, del_po, sig_d);
+                       eps_old, eps_pl, sig_eff, sig_back, p, D, ws, del_po, sig_d, dp);
     if (prof)   // state 31 FP64 read + written, 24 derivatives + 8 node ids, Nt, 24 disp gathers per step
         prof_end(XT_PROF_DAMAGE, (hipStream_t)stream,
                  (double)ngp * (31 * 16 + 24 * 8 + 8 * 4) + 8.0 * 6 * nip + 8.0 * 18 * nn, 0.0);
     XT_HIP(hipGetLastError());
     return XT_OK;
+}
+
+extern "C" int xt_damage_device(int ne, int nn, int nip, const int *nodes, const double *Nx,
+                                const double *Ny, const double *Nz, const double *Nt,
+                                const double *sol, const double *ym, const double *nu,
+                                double *eps_old, double *eps_pl, double *sig_eff,
+                                double *sig_back, double *p, double *D, double *ws,
+                                double *del_po, double *sig_d, void *stream) {
+    return xt_damage_device_params(ne, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu, eps_old,
+                                   eps_pl, sig_eff, sig_back, p, D, ws, del_po, sig_d, nullptr,
+                                   stream);
 }
 
 namespace xt {

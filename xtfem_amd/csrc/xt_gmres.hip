@@ -26,9 +26,6 @@ struct xt_op_s;
 struct xt_prec_s;
 
 namespace xt {
-template <typename T>
-int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
-                 hipStream_t s);
 
 static constexpr int JB = 8;  // basis vectors per multidot block row
 
@@ -209,6 +206,121 @@ static int small_inverse(int q, const double *A, double *Ainv) {
     return 0;
 }
 
+// Dominant eigenvalue of the q x q row-major C (q <= 8) by power iteration, with its
+// right (v) and left (u) eigenvectors scaled so that u . v = 1.  false unless it is a real
+// positive eigenvalue the iteration resolves (residual below 1e-10 of it).
+static bool dominant_mode(int q, const double *C, double &lam, double *u, double *v) {
+    auto power = [&](bool tr, double *x, double &l) -> bool {
+        for (int i = 0; i < q; ++i) x[i] = 1.0 + 0.37 * i;   // generic start
+        double nx = 0.0;
+        for (int i = 0; i < q; ++i) nx += x[i] * x[i];
+        for (int i = 0; i < q; ++i) x[i] /= std::sqrt(nx);
+        for (int it = 0; it < 5000; ++it) {
+            double y[8], ny = 0.0, xy = 0.0;
+            for (int i = 0; i < q; ++i) {
+                double t = 0.0;
+                for (int j = 0; j < q; ++j) t += (tr ? C[j * q + i] : C[i * q + j]) * x[j];
+                y[i] = t;
+                ny += t * t;
+                xy += x[i] * t;
+            }
+            if (ny == 0.0) return false;
+            l = xy;   // Rayleigh quotient (||x|| = 1)
+            double res = 0.0;
+            for (int i = 0; i < q; ++i) res += (y[i] - l * x[i]) * (y[i] - l * x[i]);
+            ny = std::sqrt(ny);
+            for (int i = 0; i < q; ++i) x[i] = y[i] / ny;
+            if (std::sqrt(res) <= 1e-10 * std::fabs(l)) return l > 0.0;
+        }
+        return false;
+    };
+    double lr = 0.0, ll = 0.0;
+    if (!power(false, v, lr) || !power(true, u, ll)) return false;
+    if (std::fabs(lr - ll) > 1e-8 * lr) return false;
+    double d = 0.0;
+    for (int i = 0; i < q; ++i) d += u[i] * v[i];
+    if (std::fabs(d) < 1e-8) return false;   // (nearly) defective: no stable split
+    for (int i = 0; i < q; ++i) u[i] /= d;
+    lam = lr;
+    return true;
+}
+
+// Temporal split of the preconditioner for the cases of one solve (PrecSplit): case r
+// goes to the shifted factor whose shift is nearest lambda_r on a log scale, or to none
+// when lambda_r is below a quarter of the smallest shift (the mass term is then small
+// against K for every mode the shifts were chosen for).  mi (row-major A_r^-1) becomes
+// (I - v_r u_r^T) A_r^-1 for a split case.
+static int build_split(PrecSplit &sp, int p, int q, int R, const std::vector<double> &tm,
+                       std::vector<double> &mi, int nsh, const double *shifts, hipStream_t s) {
+    double smin = shifts[0];
+    for (int g = 1; g < nsh; ++g) smin = std::min(smin, shifts[g]);
+    std::vector<int> grp(R, -1);
+    std::vector<double> uu((size_t)R * q), vv((size_t)R * q);
+    for (int r = 0; r < R; ++r) {
+        const double *Ai = &mi[(size_t)r * q * q];
+        const double *B = &tm[((size_t)r * 2 + 1) * q * q];
+        double C[64];
+        for (int i = 0; i < q; ++i)
+            for (int j = 0; j < q; ++j) {
+                double t = 0.0;
+                for (int k = 0; k < q; ++k) t += Ai[i * q + k] * B[k * q + j];
+                C[i * q + j] = t;
+            }
+        double lam = 0.0;
+        if (!dominant_mode(q, C, lam, &uu[(size_t)r * q], &vv[(size_t)r * q])) continue;
+        if (lam < 0.25 * smin) continue;
+        int best = 0;
+        for (int g = 1; g < nsh; ++g)
+            if (std::fabs(std::log(lam / shifts[g])) < std::fabs(std::log(lam / shifts[best])))
+                best = g;
+        grp[r] = best;
+    }
+    sp.ns = 0;
+    sp.goff.assign(1, 0);
+    sp.gfac.clear();
+    sp.hcase.clear();
+    sp.hbase.clear();
+    sp.hstride.clear();
+    sp.h64.clear();
+    for (int g = 0; g < nsh; ++g) {
+        int ng = 0;
+        for (int r = 0; r < R; ++r) ng += grp[r] == g;
+        if (!ng) continue;
+        const int o = sp.goff.back();
+        int k = 0;
+        for (int r = 0; r < R; ++r) {
+            if (grp[r] != g) continue;
+            const double *u = &uu[(size_t)r * q], *v = &vv[(size_t)r * q];
+            double *Ai = &mi[(size_t)r * q * q];
+            double m2[8];
+            for (int j = 0; j < q; ++j) {
+                double t = 0.0;
+                for (int kk = 0; kk < q; ++kk) t += u[kk] * Ai[kk * q + j];
+                m2[j] = t;
+            }
+            for (int i = 0; i < q; ++i)
+                for (int j = 0; j < q; ++j) Ai[i * q + j] -= v[i] * m2[j];
+            sp.hcase.push_back(r);
+            sp.hbase.push_back((long long)o * p + k);
+            sp.hstride.push_back(ng);
+            sp.h64.insert(sp.h64.end(), m2, m2 + q);
+            sp.h64.insert(sp.h64.end(), v, v + q);
+            ++k;
+        }
+        sp.goff.push_back(o + ng);
+        sp.gfac.push_back(g);
+    }
+    sp.ns = sp.goff.back();
+    if (!sp.ns) return XT_OK;
+    sp.h32.assign(sp.h64.begin(), sp.h64.end());
+    XT_TRY(sp.dcase.upload(sp.hcase.data(), sizeof(int) * sp.ns, s));
+    XT_TRY(sp.dbase.upload(sp.hbase.data(), sizeof(long long) * sp.ns, s));
+    XT_TRY(sp.dstride.upload(sp.hstride.data(), sizeof(int) * sp.ns, s));
+    XT_TRY(sp.m2v[0].upload(sp.h64.data(), sizeof(double) * sp.h64.size(), s));
+    XT_TRY(sp.m2v[1].upload(sp.h32.data(), sizeof(float) * sp.h32.size(), s));
+    return XT_OK;
+}
+
 }  // namespace xt
 
 using namespace xt;
@@ -263,6 +375,16 @@ struct Solver {
     int nblk = 0, chunk = 0;
     std::vector<double> *hist = nullptr;  // R == 1: ro/ro0 per iteration (kronpgmres log)
     double hist_ro0 = 1.0;
+    // temporal split (shifted factors) of the FP64 and FP32 applies, or none
+    const PrecSplit *split64 = nullptr, *split32 = nullptr;
+
+    // y = P^-1 x: the temporal inverses (or split mixes) mi64 / mi32, then the factor(s)
+    int papply64(const double *x, double *y) {
+        return prec_apply_split<double>(pr, R, q, mi64.as<double>(), split64, x, y, s);
+    }
+    int papply32(const float *x, float *y) {
+        return prec_apply_split<float>(pr, R, q, mi32.as<float>(), split32, x, y, s);
+    }
 
     double *Vj(int j) { return V.as<double>() + (size_t)j * n; }
     double *Zj(int j) { return Z.as<double>() + (size_t)j * n; }
@@ -342,12 +464,12 @@ struct Solver {
     // w = Op(P^-1 v) with FP64 or FP32 operator/preconditioner
     int op_prec(const double *v, double *w, bool f32) {
         if (!f32) {
-            XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), v, z64.as<double>(), false, s));
+            XT_TRY(papply64(v, z64.as<double>()));
             XT_TRY(op_apply_internal(op, R, tm64.ptr, 1.0, 0.0, z64.ptr, w, 0, s));
             return XT_OK;
         }
         hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, v, w32a.as<float>());
-        XT_TRY(prec_apply_t<float>(pr, R, q, mi32.as<float>(), w32a.as<float>(), w32b.as<float>(), true, s));
+        XT_TRY(papply32(w32a.as<float>(), w32b.as<float>()));
         XT_TRY(op_apply_internal(op, R, tm32.ptr, 1.0, 0.0, w32b.ptr, w32a.ptr, 1, s));
         hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32a.as<float>(), w);
         XT_HIP(hipGetLastError());
@@ -359,10 +481,10 @@ struct Solver {
         double *z = Zj(i);
         if (f32prec) {
             hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, v, w32a.as<float>());
-            XT_TRY(prec_apply_t<float>(pr, R, q, mi32.as<float>(), w32a.as<float>(), w32b.as<float>(), true, s));
+            XT_TRY(papply32(w32a.as<float>(), w32b.as<float>()));
             hipLaunchKernelGGL(cvt_f32_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), z);
         } else {
-            XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), v, z, false, s));
+            XT_TRY(papply64(v, z));
         }
         if (f32op && f32prec) {   // w32b holds Z_i in FP32 exactly (Z_i is its widening)
             XT_TRY(op_apply_internal(op, R, tm32.ptr, 1.0, 0.0, w32b.ptr, w32a.ptr, 1, s));
@@ -380,11 +502,11 @@ struct Solver {
     // x += P^-1 u
     int prec_add(const double *u, double *x, bool f32) {
         if (!f32) {
-            XT_TRY(prec_apply_t<double>(pr, R, q, mi64.as<double>(), u, z64.as<double>(), false, s));
+            XT_TRY(papply64(u, z64.as<double>()));
             hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, z64.as<double>(), x);
         } else {
             hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, u, w32a.as<float>());
-            XT_TRY(prec_apply_t<float>(pr, R, q, mi32.as<float>(), w32a.as<float>(), w32b.as<float>(), true, s));
+            XT_TRY(papply32(w32a.as<float>(), w32b.as<float>()));
             hipLaunchKernelGGL(add_into<float>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), x);
         }
         XT_HIP(hipGetLastError());
@@ -578,12 +700,6 @@ struct Solver {
 
 }  // namespace
 
-namespace xt {
-int slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b, double *x,
-                    int im, double eps, int maxits, int mode, int *its, int *ierr,
-                    double *relres, hipStream_t stream, std::vector<double> *hist);
-}
-
 extern "C" int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
                              double *x, int im, double eps, int maxits, int mode, int *its,
                              int *ierr, double *relres, void *stream) {
@@ -591,10 +707,21 @@ extern "C" int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, cons
                                (hipStream_t)stream, nullptr);
 }
 
+extern "C" int xt_slab_solve_tol(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
+                                 double *x, int im, double eps, double xtol, int maxits,
+                                 int mode, int *its, int *ierr, double *relres, void *stream) {
+    if (xtol < 0.0) {
+        set_error("argument error: xtol must be >= 0");
+        return XT_E_ARG;
+    }
+    return xt::slab_solve_impl(op, pr, R, ab, b, x, im, eps, maxits, mode, its, ierr, relres,
+                               (hipStream_t)stream, nullptr, xtol);
+}
+
 int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
                         double *x, int im, double eps, int maxits, int mode, int *its,
                         int *ierr, double *relres, hipStream_t stream,
-                        std::vector<double> *hist) {
+                        std::vector<double> *hist, double xtol) {
     XT_ARG(op && pr && R > 0 && ab && b && x && its && ierr && relres, "slab_solve arguments");
     XT_ARG(im >= 1 && im <= 256 && maxits >= 1 && mode >= 0 && mode <= 2, "slab_solve parameters");
     SolverWorkspace *ws = op_workspace(op);
@@ -627,7 +754,26 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             return XT_E_SINGULAR;
         }
     }
-    std::vector<float> tm32(tm.begin(), tm.end()), mi32(mi.begin(), mi.end());
+    // shifted factors: each case's dominant temporal mode goes to the nearest shift; the
+    // split mixes replace A_r^-1 for the applies whose precision the factors hold
+    std::vector<double> mis(mi);
+    {
+        const double *shifts = nullptr;
+        bool fp64 = false;
+        const int nsh = prec_shifts(pr, &shifts, &fp64);
+        if (nsh > 0) {
+            XT_HIP(hipStreamSynchronize(S.s));   // the previous split's uploads have finished
+            XT_TRY(build_split(ws->split, S.p, q, R, tm, mis, nsh, shifts, S.s));
+            if (ws->split.ns) {
+                S.split32 = &ws->split;
+                if (fp64) {
+                    S.split64 = &ws->split;
+                    mi = mis;
+                }
+            }
+        }
+    }
+    std::vector<float> tm32(tm.begin(), tm.end()), mi32(mis.begin(), mis.end());
     std::copy(tm.begin(), tm.end(), S.hstage.at<double>(S.o_tm));
     std::copy(mi.begin(), mi.end(), S.hstage.at<double>(S.o_mi));
     std::copy(tm32.begin(), tm32.end(), S.hstage.at<float>(S.o_tm32));
@@ -676,11 +822,24 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         // (ill-conditioned temporal matrices); a right-hand side whose step reduced its
         // residual by less than 2x continues with inner cycles in FP64 (operator and
         // preconditioner), where FP32 does not pay.
-        std::vector<double> rn(R), irel(R, 1.0e-3), prev(R, 0.0);
+        // With xtol > 0 a case also needs its last correction to be at most xtol times
+        // its solution (a forward-error test: the residual does not bound the error of an
+        // ill-conditioned slab); until then it keeps refining, each inner solve asked for
+        // a fixed reduction of the residual.
+        std::vector<double> rn(R), irel(R, 1.0e-3), prev(R, 0.0), dn(R, 0.0), xn(R, 0.0);
         std::vector<int> fb(R, 0);
+        if (xtol > 0.0) XT_TRY(ws->xprev.reserve(sizeof(double) * S.n));
         for (int outer = 0;; ++outer) {
+            if (xtol > 0.0 && outer > 0) {   // correction of the last step: x - xprev
+                double *d = ws->xprev.as<double>();
+                hipLaunchKernelGGL(sub_from, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n, x, d);
+                XT_HIP(hipGetLastError());
+                XT_TRY(S.norms2(d, dn));
+                XT_TRY(S.norms2(x, xn));
+            }
             // V_0 holds b - A x for the current x; its norms give the true residual (the
-            // first pass has them from the setup)
+            // first pass has them from the setup).  They are computed last: the first
+            // inner cycle normalises V_0 by the norms the device still holds.
             if (outer == 0) rn = r0sq;
             else XT_TRY(S.norms2(S.Vj(0), rn));
             if (outer == 64) break;
@@ -690,19 +849,24 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             for (int r = 0; r < R; ++r) {
                 if (ierr[r] == -1) continue;
                 const double nr = std::sqrt(rn[r]);
-                if (nr <= eps1[r]) { conv[r] = 1; continue; }
-                if (itv[r] >= maxits) continue;
+                const bool fwd = xtol == 0.0 || (outer > 0 && dn[r] <= xtol * xtol * xn[r]);
+                if (nr <= eps1[r] && fwd) { conv[r] = 1; continue; }
+                conv[r] = 0;
+                if (itv[r] >= maxits || nr == 0.0) continue;
                 if (prev[r] > 0.0) {
                     const double rho = nr / prev[r];
                     if (rho > 0.5) fb[r] = 1;
                     else if (rho > 10.0 * irel[r]) irel[r] = std::min(0.1, rho);
                 }
                 prev[r] = nr;
-                e1[r] = std::max(irel[r] * nr, 0.5 * eps1[r]);
+                e1[r] = nr <= eps1[r] ? irel[r] * nr : std::max(irel[r] * nr, 0.5 * eps1[r]);
                 if (fb[r]) { act64[r] = 1; any64 = true; }
                 else { act32[r] = 1; any32 = true; }
             }
             if (!any32 && !any64) break;
+            if (xtol > 0.0)
+                XT_HIP(hipMemcpyAsync(ws->xprev.ptr, x, sizeof(double) * S.n,
+                                      hipMemcpyDeviceToDevice, S.s));
             // inner solves for the correction on V_0 = the residual (already there): each
             // flexible cycle adds its correction Z y to x itself
             std::vector<int> cin(R, 0);

@@ -295,7 +295,14 @@ struct xt_prec_s {
     DevBuf dinv64, dinv32;
     DevBuf work;
     BandPrec *band = nullptr;  // or: exact banded factor (block sweeps)
-    ~xt_prec_s() { if (band) band_delete(band); }
+    // shifted factors (K + s_g M)^-1 of the temporal split, and their gathered columns
+    std::vector<double> shifts;
+    std::vector<BandPrec *> shifted;
+    DevBuf gin[2], gout[2];    // [0] FP64, [1] FP32
+    ~xt_prec_s() {
+        if (band) band_delete(band);
+        for (BandPrec *b : shifted) band_delete(b);
+    }
 };
 
 namespace xt {
@@ -445,6 +452,39 @@ static int ilut_apply(xt_prec pr, int R, int q, const T *minv, const T *x, T *y,
     return XT_OK;
 }
 
+// Gathered column of split case t: g[base_t + dof * stride_t] = sum_j m2_t[j] X[dof][r_t q + j]
+template <typename T>
+__global__ void __launch_bounds__(256)
+split_gather_kernel(int p, int q, int c, int ns, const int *__restrict__ cs,
+                    const long long *__restrict__ base, const int *__restrict__ stride,
+                    const T *__restrict__ m2v, const T *__restrict__ x, T *__restrict__ g) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)p * ns) return;
+    const long long dof = idx / ns;
+    const int t = (int)(idx - dof * ns);
+    const T *m2 = m2v + (size_t)t * 2 * q;
+    const T *xr = x + dof * c + (long long)cs[t] * q;
+    T acc = 0;
+    for (int j = 0; j < q; ++j) acc += m2[j] * xr[j];
+    g[base[t] + dof * stride[t]] = acc;
+}
+
+// Y[dof][r_t q + j] += v_t[j] h[base_t + dof * stride_t] (each case is one t at most)
+template <typename T>
+__global__ void __launch_bounds__(256)
+split_scatter_kernel(int p, int q, int c, int ns, const int *__restrict__ cs,
+                     const long long *__restrict__ base, const int *__restrict__ stride,
+                     const T *__restrict__ m2v, const T *__restrict__ h, T *__restrict__ y) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)p * ns) return;
+    const long long dof = idx / ns;
+    const int t = (int)(idx - dof * ns);
+    const T *v = m2v + (size_t)t * 2 * q + q;
+    const T hv = h[base[t] + dof * stride[t]];
+    T *yr = y + dof * c + (long long)cs[t] * q;
+    for (int j = 0; j < q; ++j) yr[j] += v[j] * hv;
+}
+
 template <typename T>
 int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
                  hipStream_t s) {
@@ -456,6 +496,46 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     }
     return ilut_apply<T>(pr, R, q, minv, x, y, f32, stage, true, s);
 }
+int prec_shifts(xt_prec pr, const double **shifts, bool *fp64) {
+    if (shifts) *shifts = pr->shifts.empty() ? nullptr : pr->shifts.data();
+    if (fp64) *fp64 = !pr->shifted.empty() && band_has_fp64(pr->shifted[0]);
+    return (int)pr->shifts.size();
+}
+
+template <typename T>
+int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *sp, const T *x,
+                     T *y, hipStream_t s) {
+    const bool f32 = sizeof(T) == 4;
+    if (!sp || sp->ns == 0 || !pr->band) return prec_apply_t<T>(pr, R, q, minv, x, y, f32, s);
+    const int pi = f32 ? 1 : 0, p = pr->p, ns = sp->ns, c = q * R;
+    XT_TRY(pr->gin[pi].reserve(sizeof(T) * (size_t)p * ns));
+    XT_TRY(pr->gout[pi].reserve(sizeof(T) * (size_t)p * ns));
+    T *gi = pr->gin[pi].as<T>(), *go = pr->gout[pi].as<T>();
+    const T *m2v = sp->m2v[pi].as<T>();
+    const int nblk = ceil_div((long long)p * ns, 256);
+    // the gather reads x before the K^-1 sweeps, which may write y over nothing it needs
+    hipLaunchKernelGGL(split_gather_kernel<T>, dim3(nblk), dim3(256), 0, s, p, q, c, ns,
+                       sp->dcase.as<int>(), sp->dbase.as<long long>(), sp->dstride.as<int>(), m2v,
+                       x, gi);
+    XT_HIP(hipGetLastError());
+    XT_TRY(band_apply<T>(pr->band, R, q, minv, x, y, s));
+    for (size_t g = 0; g + 1 < sp->goff.size(); ++g) {
+        const int o = sp->goff[g], ng = sp->goff[g + 1] - o;
+        XT_ARG(sp->gfac[g] >= 0 && sp->gfac[g] < (int)pr->shifted.size(), "split factor index");
+        XT_TRY(band_apply<T>(pr->shifted[sp->gfac[g]], ng, 1, nullptr, gi + (size_t)o * p,
+                             go + (size_t)o * p, s));
+    }
+    hipLaunchKernelGGL(split_scatter_kernel<T>, dim3(nblk), dim3(256), 0, s, p, q, c, ns,
+                       sp->dcase.as<int>(), sp->dbase.as<long long>(), sp->dstride.as<int>(), m2v,
+                       go, y);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
+template int prec_apply_split<double>(xt_prec, int, int, const double *, const PrecSplit *,
+                                      const double *, double *, hipStream_t);
+template int prec_apply_split<float>(xt_prec, int, int, const float *, const PrecSplit *,
+                                     const float *, float *, hipStream_t);
+
 template int prec_apply_t<double>(xt_prec, int, int, const double *, const double *, double *,
                                   bool, hipStream_t);
 template int prec_apply_t<float>(xt_prec, int, int, const float *, const float *, float *, bool,
@@ -553,9 +633,75 @@ int xt_prec_create_band(int p, const int *ia, const int *ja, const double *a, in
     return XT_OK;
 }
 
+int xt_prec_create_band_shifted(int p, const int *kia, const int *kja, const double *ka,
+                                const int *mia, const int *mja, const double *ma, int nshift,
+                                const double *shifts, int block, int flags, xt_prec *out) {
+    XT_ARG(out && p > 0 && kia && kja && ka && mia && mja && ma && nshift >= 1 && shifts,
+           "prec_create_band_shifted arguments");
+    XT_ARG(kia[0] == 1 && mia[0] == 1, "CSR must be 1-based");
+    for (int g = 0; g < nshift; ++g) XT_ARG(shifts[g] > 0.0, "shifts must be positive");
+    xt_prec pr = new (std::nothrow) xt_prec_s;
+    if (!pr) return XT_E_NOMEM;
+    pr->p = p;
+    int rc = band_build(p, kia, kja, ka, block, &pr->band);
+    int b = 0;
+    if (rc == XT_OK) band_dims(pr->band, &b, nullptr, nullptr);
+    // K + s M row by row (both rows sorted by column: merged; K's pattern holds M's for a
+    // mesh whose mass couples only DOFs that the stiffness couples, but any union works)
+    auto sorted = [p](const int *ia_, const int *ja_) {
+        for (int i = 0; i < p; ++i)
+            for (int k = ia_[i]; k + 1 < ia_[i + 1]; ++k)
+                if (ja_[k] <= ja_[k - 1]) return false;
+        return true;
+    };
+    if (rc == XT_OK && (!sorted(kia, kja) || !sorted(mia, mja))) {
+        set_error("prec_create_band_shifted: CSR rows must be sorted by column");
+        rc = XT_E_ARG;
+    }
+    std::vector<int> ia(p + 1, 1), ja;
+    std::vector<double> a;
+    for (int g = 0; g < nshift && rc == XT_OK; ++g) {
+        ja.clear();
+        a.clear();
+        for (int i = 0; i < p; ++i) {
+            int k = kia[i] - 1, ke = kia[i + 1] - 1, m = mia[i] - 1, me = mia[i + 1] - 1;
+            while (k < ke || m < me) {
+                const int jk = k < ke ? kja[k] : 1 << 30, jm = m < me ? mja[m] : 1 << 30;
+                if (jk == jm) { ja.push_back(jk); a.push_back(ka[k++] + shifts[g] * ma[m++]); }
+                else if (jk < jm) { ja.push_back(jk); a.push_back(ka[k++]); }
+                else { ja.push_back(jm); a.push_back(shifts[g] * ma[m++]); }
+            }
+            ia[i + 1] = (int)ja.size() + 1;
+        }
+        BandPrec *bp = nullptr;
+        if (rc == XT_OK)
+            rc = band_build(p, ia.data(), ja.data(), a.data(), b, &bp,
+                            !(flags & XT_SHIFT_FP32_ONLY));
+        if (rc == XT_OK) {
+            pr->shifted.push_back(bp);
+            pr->shifts.push_back(shifts[g]);
+        }
+    }
+    if (rc != XT_OK) { delete pr; return rc; }
+    *out = pr;
+    return XT_OK;
+}
+
+int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts) {
+    XT_ARG(pr, "prec");
+    if (nshift) *nshift = (int)pr->shifts.size();
+    if (shifts) std::copy(pr->shifts.begin(), pr->shifts.end(), shifts);
+    return XT_OK;
+}
+
 int xt_prec_refactor_band(xt_prec pr, const int *ia, const int *ja, const double *a,
                           void *stream) {
     XT_ARG(pr && pr->band && ia && ja && a, "prec_refactor_band arguments");
+    if (!pr->shifted.empty()) {
+        set_error("xt_prec_refactor_band: a preconditioner with shifted factors is rebuilt with "
+                  "xt_prec_create_band_shifted");
+        return XT_E_UNSUPPORTED;
+    }
     return band_refactor(pr->band, ia, ja, a, (hipStream_t)stream);
 }
 

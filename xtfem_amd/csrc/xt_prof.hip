@@ -14,7 +14,8 @@ namespace {
 struct Kind {
     std::vector<hipEvent_t> ev;   // pairs: ev[2i] start, ev[2i+1] stop
     size_t used = 0;              // events recorded since enable
-    double bytes = 0.0, flops = 0.0;
+    double bytes = 0.0, flops = 0.0;          // as the launches perform them
+    double alg_bytes = 0.0, alg_flops = 0.0;  // algorithmic (the unpaired computation)
 };
 unsigned g_mask = 0;
 Kind g_kind[XT_PROF_NKINDS];
@@ -44,7 +45,7 @@ void prof_begin(int kind, hipStream_t s) {
     if (e) (void)hipEventRecord(e, s);
 }
 
-void prof_end(int kind, hipStream_t s, double bytes, double flops) {
+void prof_end(int kind, hipStream_t s, double bytes, double flops, bool alg) {
     if (!prof_on(kind)) return;
     Kind &k = g_kind[kind];
     if (k.used % 2 == 0) return;
@@ -53,6 +54,16 @@ void prof_end(int kind, hipStream_t s, double bytes, double flops) {
     (void)hipEventRecord(e, s);
     k.bytes += bytes;
     k.flops += flops;
+    if (alg) {
+        k.alg_bytes += bytes;
+        k.alg_flops += flops;
+    }
+}
+
+void prof_alg(int kind, double bytes, double flops) {
+    if (!prof_on(kind)) return;
+    g_kind[kind].alg_bytes += bytes;
+    g_kind[kind].alg_flops += flops;
 }
 }  // namespace xt
 
@@ -69,7 +80,7 @@ int xt_prof_enable(unsigned mask) {
     g_mask = mask;
     for (auto &k : g_kind) {
         k.used = 0;
-        k.bytes = k.flops = 0.0;
+        k.bytes = k.flops = k.alg_bytes = k.alg_flops = 0.0;
         if (!mask) {   // disabling releases the event pool
             for (hipEvent_t e : k.ev) (void)hipEventDestroy(e);
             k.ev.clear();
@@ -93,6 +104,13 @@ int xt_prof_read(int kind, double *ms, long long *launches, double *bytes, doubl
     if (launches) *launches = (long long)pairs;
     if (bytes) *bytes = k.bytes;
     if (flops) *flops = k.flops;
+    return XT_OK;
+}
+
+int xt_prof_read_alg(int kind, double *bytes, double *flops) {
+    XT_ARG(kind >= 0 && kind < XT_PROF_NKINDS, "profile kind");
+    if (bytes) *bytes = g_kind[kind].alg_bytes;
+    if (flops) *flops = g_kind[kind].alg_flops;
     return XT_OK;
 }
 

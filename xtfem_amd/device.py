@@ -169,17 +169,32 @@ class BandPreconditioner(IlutPreconditioner):
     droptol = 0 limit, ilut.f:2169-2170).  Block Cholesky in FP64 on the device,
     applied by dense block sweeps in FP64 or FP32."""
 
-    def __init__(self, K, block=0):
+    def __init__(self, K, block=0, M=None, shifts=(), shifted_fp32_only=False):
+        """shifts (with M): also factor K + s M for each s; slab solves then split each
+        load case's dominant temporal mode onto the nearest shift (xt_prec_create_band_shifted;
+        shifted_fp32_only keeps those factors in FP32 only, for the FP32 applies)."""
         ia, ja, a = (np.ascontiguousarray(K[0], dtype=np.int32),
                      np.ascontiguousarray(K[1], dtype=np.int32),
                      np.ascontiguousarray(K[2], dtype=np.float64))
         self.p = len(ia) - 1
         self.factors = None
         self.nnz = None
+        self.shifts = np.asarray(shifts, dtype=np.float64).ravel()
         h = C.c_void_p()
         _dev()
-        check(lib().xt_prec_create_band(self.p, ia.ctypes.data, ja.ctypes.data, a.ctypes.data,
-                                        int(block), C.addressof(h)), "prec_create_band")
+        if len(self.shifts):
+            mi, mj, mv = (np.ascontiguousarray(M[0], dtype=np.int32),
+                          np.ascontiguousarray(M[1], dtype=np.int32),
+                          np.ascontiguousarray(M[2], dtype=np.float64))
+            check(lib().xt_prec_create_band_shifted(
+                self.p, ia.ctypes.data, ja.ctypes.data, a.ctypes.data, mi.ctypes.data,
+                mj.ctypes.data, mv.ctypes.data, len(self.shifts), self.shifts.ctypes.data,
+                int(block), 1 if shifted_fp32_only else 0, C.addressof(h)),
+                "prec_create_band_shifted")
+        else:
+            check(lib().xt_prec_create_band(self.p, ia.ctypes.data, ja.ctypes.data,
+                                            a.ctypes.data, int(block), C.addressof(h)),
+                  "prec_create_band")
         self.h = h
         b, nb, bw = C.c_int(0), C.c_int(0), C.c_int(0)
         check(lib().xt_prec_band_info(h, C.addressof(b), C.addressof(nb), C.addressof(bw)),
@@ -197,6 +212,59 @@ class BandPreconditioner(IlutPreconditioner):
         sv = 4 if fp32 else 8
         bb = self.block * self.block
         return sv * bb * (2 * self.nblocks + 2 * (self.nblocks - 1))
+
+
+def dominant_temporal_mode(A, B):
+    """Dominant eigenvalue of A_r^-1 B_r per load case ([..., q, q] arrays; NaN where it is
+    not real and positive), as the slab solver's split computes it."""
+    C_ = np.linalg.solve(np.asarray(A, dtype=np.float64), np.asarray(B, dtype=np.float64))
+    w = np.linalg.eigvals(C_)
+    i = np.argmax(np.abs(w), axis=-1)
+    lam = np.take_along_axis(w, i[..., None], axis=-1)[..., 0]
+    return np.where((np.abs(lam.imag) <= 1e-10 * np.abs(lam)) & (lam.real > 0), lam.real, np.nan)
+
+
+def lowest_mode(pr, M, iters=40, seed=0):
+    """Estimate of the smallest eigenvalue mu_min of K phi = mu M phi: power iteration on
+    K^-1 M with the device factor pr (K^-1) and a CSR operator for M.  Fixed DOFs of the
+    1-0 method (unit diagonal in K and M) would give mu = 1: the start vector is zero
+    there and K^-1 M keeps it so."""
+    opM = KronOperator([M], q=1)
+    ia = np.asarray(M[0])
+    free = torch.as_tensor(np.diff(ia) > 1, device=_dev())
+    p = pr.p
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    x = (torch.rand(p, generator=g, dtype=torch.float64) - 0.5).to(_dev()) * free
+    tm = torch.ones(1, dtype=torch.float64, device=_dev())
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    rho = 0.0
+    for _ in range(iters):
+        x /= torch.linalg.norm(x)
+        opM.apply(tm, x, y, 1)
+        pr.apply(None, y, z, 1, q=1)
+        z *= free
+        rho = float(torch.dot(x, z))
+        x, z = z, x
+    return 1.0 / rho
+
+
+def shifts_for_cases(lam, mu_min, ratio=4.0, floor=0.05):
+    """Shifts for xt_prec_create_band_shifted covering the dominant temporal eigenvalues
+    lam of the load cases: cases with lam below floor * mu_min need none (the mass term
+    is small against K there); the rest are grouped so that every case lies within a
+    factor sqrt(ratio) of its group's shift (geometric centres)."""
+    lam = np.asarray(lam, dtype=np.float64)
+    big = np.sort(lam[np.isfinite(lam) & (lam >= floor * mu_min)])
+    out = []
+    i = 0
+    while i < len(big):
+        j = i
+        while j + 1 < len(big) and big[j + 1] <= big[i] * ratio:
+            j += 1
+        out.append(float(np.sqrt(big[i] * big[j])))
+        i = j + 1
+    return out
 
 
 class Assembly:
@@ -301,13 +369,15 @@ class Assembly:
         return self.ym_t, self.nu_t, self.vv
 
 
-def slab_solve(op, pr, A, B, rhs, x0=None, im=30, eps=1e-8, maxits=500, mode=1):
+def slab_solve(op, pr, A, B, rhs, x0=None, im=30, eps=1e-8, maxits=500, mode=1, xtol=0.0):
     """Solve (A_r (x) K + B_r (x) M) x_r = rhs_r for every load case r on the GPU.
 
     mode 0: FP64 GMRES(im) with the FP64 Kronecker preconditioner.
     mode 1: GMRES iterative refinement: FP64 true residual, inner flexible GMRES cycles
             with the FP32 operator and FP32 preconditioner sweeps.
     mode 2: FP64 flexible GMRES(im) with the FP32 preconditioner sweeps.
+    xtol (mode 1): also require the last refinement correction to be at most xtol times
+    the solution (forward-error test, xt_slab_solve_tol).
     rhs: host [R, q*p] temporal-major.  Returns (x [R, q*p], its, ierr, relres)."""
     A = np.asarray(A, dtype=np.float64).reshape(-1, op.q, op.q)
     B = np.asarray(B, dtype=np.float64).reshape(-1, op.q, op.q)
@@ -315,19 +385,26 @@ def slab_solve(op, pr, A, B, rhs, x0=None, im=30, eps=1e-8, maxits=500, mode=1):
     R = rhs.shape[0]
     b = to_interleaved(rhs, op.p, op.q)
     x = torch.zeros_like(b) if x0 is None else to_interleaved(x0, op.p, op.q)
-    x, its, ierr, relres = slab_solve_device(op, pr, A, B, b, x, R, im, eps, maxits, mode)
+    x, its, ierr, relres = slab_solve_device(op, pr, A, B, b, x, R, im, eps, maxits, mode,
+                                             xtol)
     return from_interleaved(x, op.p, op.q, R), its, ierr, relres
 
 
-def slab_solve_device(op, pr, A, B, b, x, R, im=30, eps=1e-8, maxits=500, mode=1):
+def slab_solve_device(op, pr, A, B, b, x, R, im=30, eps=1e-8, maxits=500, mode=1, xtol=0.0):
     """Device-vector form of slab_solve (b, x interleaved FP64 device tensors; x in/out)."""
     ab = np.ascontiguousarray(np.stack([_colmajor(A), _colmajor(B)], axis=1))
     its = np.zeros(R, dtype=np.int32)
     ierr = np.zeros(R, dtype=np.int32)
     relres = np.zeros(R)
-    check(lib().xt_slab_solve(op.h, pr.h, R, ab.ctypes.data, b.data_ptr(), x.data_ptr(), int(im),
-                              float(eps), int(maxits), int(mode), its.ctypes.data,
-                              ierr.ctypes.data, relres.ctypes.data, _stream()), "slab_solve")
+    if xtol > 0.0:
+        check(lib().xt_slab_solve_tol(op.h, pr.h, R, ab.ctypes.data, b.data_ptr(), x.data_ptr(),
+                                      int(im), float(eps), float(xtol), int(maxits), int(mode),
+                                      its.ctypes.data, ierr.ctypes.data, relres.ctypes.data,
+                                      _stream()), "slab_solve_tol")
+    else:
+        check(lib().xt_slab_solve(op.h, pr.h, R, ab.ctypes.data, b.data_ptr(), x.data_ptr(),
+                                  int(im), float(eps), int(maxits), int(mode), its.ctypes.data,
+                                  ierr.ctypes.data, relres.ctypes.data, _stream()), "slab_solve")
     return x, its, ierr, relres
 
 
@@ -339,9 +416,12 @@ class DamageState:
     FIELDS6 = ("eps_old", "eps_pl", "sig_eff", "sig_back")
     FIELDS1 = ("p", "D", "ws", "del_po", "sig_d")
 
-    def __init__(self, mesh, Nx, Ny, Nz, ym, nu):
+    def __init__(self, mesh, Nx, Ny, Nz, ym, nu, params=None):
+        """params: the damage-law constants (sig_f, sig_u, eps_D, Cy, h, S, s) or None for
+        the reference's (damage.h:9-17)."""
         dev = _dev()
         self.ne, self.nn = mesh.ne, mesh.nn
+        self.params = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         ngp = 8 * mesh.ne
         t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), device=dev).to(dt).contiguous()
         self.nodes = t(np.repeat(mesh.xconn, 8, axis=0).ravel(), torch.int32)
@@ -356,12 +436,13 @@ class DamageState:
     def step(self, sol_dev, Nt_dev, nip):
         """sol_dev: temporal-major FP64 device vector [6*3*nn]; Nt_dev: [nip*6]."""
         s = self.s
-        check(lib().xt_damage_device(self.ne, self.nn, nip, self.nodes.data_ptr(), self.Nx.data_ptr(),
-                                     self.Ny.data_ptr(), self.Nz.data_ptr(), Nt_dev.data_ptr(),
-                                     sol_dev.data_ptr(), self.ym.data_ptr(), self.nu.data_ptr(),
-                                     *[s[k].data_ptr() for k in self.FIELDS6],
-                                     *[s[k].data_ptr() for k in self.FIELDS1], _stream()),
-              "damage_device")
+        check(lib().xt_damage_device_params(
+            self.ne, self.nn, nip, self.nodes.data_ptr(), self.Nx.data_ptr(), self.Ny.data_ptr(),
+            self.Nz.data_ptr(), Nt_dev.data_ptr(), sol_dev.data_ptr(), self.ym.data_ptr(),
+            self.nu.data_ptr(), *[s[k].data_ptr() for k in self.FIELDS6],
+            *[s[k].data_ptr() for k in self.FIELDS1],
+            None if self.params is None else self.params.ctypes.data, _stream()),
+            "damage_device")
         # kokkos_damage.f90:102-107: failed Gauss points carry D = ws = 0
         check(lib().xt_damage_zero_failed(8 * self.ne, self.gpsta.data_ptr(), s["D"].data_ptr(),
                                           s["ws"].data_ptr(), _stream()), "damage_zero_failed")

@@ -224,6 +224,9 @@ int xt_prec_create_band(int p, const int *rowptr, const int *colidx, const doubl
  * rounded to FP32 stagnates. */
 #define XT_SHIFT_FP32_ONLY 1   /* flags: keep the shifted factors in FP32 only (a third of
                                   the memory); FP64 applies then use A^-1 (x) K^-1 */
+#define XT_BAND_FP32_ONLY 2    /* flags: keep the factor of K itself in FP32 only: the
+                                  preconditioner then applies in FP32 only (GMRES-IR without
+                                  FP64 fallback, FGMRES with FP32 sweeps); nshift may be 0 */
 int xt_prec_create_band_shifted(int p, const int *krow, const int *kcol, const double *kval,
                                 const int *mrow, const int *mcol, const double *mval,
                                 int nshift, const double *shifts, int block, int flags,
@@ -346,6 +349,21 @@ int xt_slab_solve_tol(xt_op op, xt_prec pr, int R, const double *ab, const doubl
                       double *x, int im, double eps, double xtol, int maxits, int mode,
                       int *its, int *ierr, double *relres, void *stream);
 
+/* Batched slab solve of nseg independent systems stacked in one block-diagonal
+ * operator (e.g. meshes of different sizes: their K and M concatenated, one banded
+ * factor of the block-diagonal K), rows seg_dof[s] .. seg_dof[s+1]-1 for system s
+ * (host array of nseg + 1 increasing values, 0 and p at the ends).  Every (system s,
+ * load case r) is its own instance g = s*R + r: its own Krylov space, Hessenberg,
+ * rotations, residual norms and convergence (a converged instance is masked: its basis
+ * columns are zero and its solution frozen), while every kernel of a step covers all
+ * instances at once.  The temporal matrices are per load case, shared by the systems.
+ * its, ierr, relres: host arrays of nseg*R.  Otherwise as xt_slab_solve_tol.  No
+ * counterpart in the reference, which solves one mesh per run. */
+int xt_slab_solve_seg(xt_op op, xt_prec pr, int nseg, const long long *seg_dof, int R,
+                      const double *ab, const double *b, double *x, int im, double eps,
+                      double xtol, int maxits, int mode, int *its, int *ierr, double *relres,
+                      void *stream);
+
 /* chkelesta (delelement.f90:109-136) on device-resident state: failed[e] = 1 iff
  * some Gauss point of element e has D >= Dc, else 0; for a failed element
  * elesta[e] and its gpsta entries are set to 0 (others untouched).  Device
@@ -411,6 +429,27 @@ int xt_prof_read_alg(int kind, double *bytes, double *flops);
  * default), or launch them one by one (0).  No counterpart in the reference (launch
  * overhead control).  Process-wide. */
 int xt_graphs_enable(int on);
+
+/* Kernel-variant knobs for the autotuner (xtfem_amd.tune times the candidates on the
+ * caller's operator and preconditioner and sets the fastest).  0 = the built-in choice.
+ * Process-wide; set them between solves, not during one.  No counterpart in the
+ * reference.
+ *   XT_TUNE_SPMM_UNR      node-block SpMM: entries per pass (loads in flight)   2, 4, 8
+ *   XT_TUNE_SPMM_WPB      node-block SpMM: waves per workgroup                  1, 2, 4
+ *   XT_TUNE_SPMM_R1SPLIT  node-block SpMM, one load case: lanes per node row    4, 8
+ *   XT_TUNE_BAND_TILE     banded-factor sweeps, batched passes: tile rows       64, 128
+ *                         (128: FP32 with block rows a multiple of 128)
+ *   XT_TUNE_GMRES_CHECK   slab solve: Arnoldi steps between host convergence checks 1, 2, 4
+ * XT_E_ARG for a value a knob does not take. */
+#define XT_TUNE_SPMM_UNR 0
+#define XT_TUNE_SPMM_WPB 1
+#define XT_TUNE_SPMM_R1SPLIT 2
+#define XT_TUNE_BAND_TILE 3
+#define XT_TUNE_GMRES_CHECK 4
+#define XT_TUNE_NKNOBS 5
+int xt_tune_set(int knob, int value);
+/* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
+int xt_tune_get(int knob);
 
 /* Synchronise the device (all streams of this process). */
 int xt_device_sync(void);

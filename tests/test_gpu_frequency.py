@@ -60,14 +60,14 @@ def sweep(oracle_lib):
                                                 eps=1e-12, maxits=3000)
         assert ierr == 0, (r, its)
         xref.append(x)
-    return dict(Kb=Kb, Mb=Mb, p=p, A=A, B=B, rhs=rhs, xref=np.array(xref), freq=freq)
+    return dict(Kb=Kb, Mb=Mb, p=p, A=A, B=B, rhs=rhs, xref=np.array(xref), freq=freq, bc=bc)
 
 
 def test_shift_choice(dev, sweep):
     """The plate's lowest mode (device power iteration on K^-1 M) and the shifts the
     cases need: the high-frequency cases' dominant temporal eigenvalues exceed it."""
     pr0 = dev.BandPreconditioner(sweep["Kb"])
-    mu = dev.lowest_mode(pr0, sweep["Mb"])
+    mu = dev.lowest_mode(pr0, sweep["Mb"], sweep["bc"])
     lam = dev.dominant_temporal_mode(sweep["A"], sweep["B"])
     assert np.all(np.isfinite(lam))
     assert 3e7 < mu < 8e7, mu                      # 2.47e9 / 50 for this plate and density
@@ -84,7 +84,7 @@ def test_frequency_sweep_matches_oracle(dev, sweep, mode, eps, xtol, fp32_only):
     FP64 GMRES and FP64 FGMRES with FP32 sweeps, all with the split preconditioner: every
     case converges and matches the oracle."""
     pr0 = dev.BandPreconditioner(sweep["Kb"])
-    mu = dev.lowest_mode(pr0, sweep["Mb"])
+    mu = dev.lowest_mode(pr0, sweep["Mb"], sweep["bc"])
     lam = dev.dominant_temporal_mode(sweep["A"], sweep["B"])
     pr = dev.BandPreconditioner(sweep["Kb"], M=sweep["Mb"],
                                 shifts=dev.shifts_for_cases(lam, mu), shifted_fp32_only=fp32_only)
@@ -105,7 +105,7 @@ def test_split_fp32_converges_where_unsplit_stagnates(dev, sweep):
     r = [len(sweep["freq"]) - 1]
     op = dev.KronOperator([sweep["Kb"], sweep["Mb"]], q=6)
     pr0 = dev.BandPreconditioner(sweep["Kb"])
-    mu = dev.lowest_mode(pr0, sweep["Mb"])
+    mu = dev.lowest_mode(pr0, sweep["Mb"], sweep["bc"])
     lam = dev.dominant_temporal_mode(sweep["A"], sweep["B"])
     pr = dev.BandPreconditioner(sweep["Kb"], M=sweep["Mb"], shifts=dev.shifts_for_cases(lam, mu))
     args = (sweep["A"][r], sweep["B"][r], sweep["rhs"][r])

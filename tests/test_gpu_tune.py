@@ -1,0 +1,79 @@
+"""The autotuner's kernel variants (xt_tune_set knobs): every variant of the node-block
+Kron-SpMM and of the banded-factor sweeps computes the same operator, checked against
+the oracle's kronspmv (kronecker.f90:19-85) and the exact factor; the tuner leaves a
+valid pick set and reuses it for the same problem."""
+import numpy as np
+import pytest
+
+from conftest import rel, structured_system
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from xtfem_amd import device, tune
+    yield device
+    for k in tune.KNOBS:                 # leave the built-in choices for later tests
+        tune.set_knob(k, 0)
+
+
+@pytest.mark.parametrize("R", [1, 5, 32])
+def test_spmm_variants_match_oracle(dev, oracle_lib, R):
+    from xtfem_amd import slab, tune
+    S = structured_system(12, 5, 4, 24.0, 10.0, 8.0)
+    op = dev.KronOperator([S["Kb"], S["Mb"]], q=6)
+    assert op.format == "blocks"
+    rng = np.random.default_rng(R)
+    c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))
+    A, B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+    X = rng.standard_normal((R, 6 * S["p"]))
+    ref = np.stack([oracle_lib.kronspmv(A[r], B[r], S["Kb"], S["Mb"], X[r]) for r in range(R)])
+    splits = (4, 8) if R == 1 else (0,)
+    for unr in (2, 4, 8):
+        for wpb in (1, 2, 4):
+            for spl in splits:
+                tune.set_knob("spmm_unr", unr)
+                tune.set_knob("spmm_wpb", wpb)
+                tune.set_knob("spmm_r1split", spl)
+                for fp32, tol in ((False, 1e-12), (True, 2e-5)):
+                    Y = op.apply_host(A, B, X, fp32=fp32)
+                    for r in range(R):
+                        assert rel(Y[r], ref[r]) < tol, (unr, wpb, spl, fp32, r)
+    for k in tune.KNOBS:
+        tune.set_knob(k, 0)
+
+
+def test_band_tile_variants_and_autotune(dev, oracle_lib):
+    """The 64- and 128-row tiles of the FP32 batched passes against the exact factor, then
+    a full autotune: picks among the candidates, knobs left set, cached on reuse."""
+    import torch
+    from xtfem_amd import slab, tune
+    S = structured_system(130, 1, 1, 130.0, 1.0, 1.0)
+    pr = dev.BandPreconditioner(S["Kb"], block=128)
+    op = dev.KronOperator([S["Kb"], S["Mb"]], q=6)
+    p, R = S["p"], 32
+    alu, jlu, ju, ierr = oracle_lib.ilut(S["Kb"], p, 0.0)
+    rng = np.random.default_rng(3)
+    c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))
+    A = c["k"] + c["ktn"]
+    X = rng.standard_normal((R, 6 * p))
+    ref = np.stack([oracle_lib.kronlusol(A[r], alu, jlu, ju, X[r]) for r in range(R)])
+    minv = torch.as_tensor(np.linalg.inv(A), device="cuda").float().contiguous()
+    x = dev.to_interleaved(X, p, 6).float().contiguous()
+    for tile in (64, 128):
+        tune.set_knob("band_tile", tile)
+        y = torch.empty_like(x)
+        pr.apply(minv, x, y, R, fp32=True)
+        Y = dev.from_interleaved(y.double().contiguous(), p, 6, R)
+        for r in range(R):
+            assert rel(Y[r], ref[r]) < 1e-3, (tile, r)
+    out = tune.autotune(op, pr, R, fp32=True, reps=3)
+    assert out["picks"]["spmm_unr"] in (2, 4, 8) and out["picks"]["band_tile"] in (64, 128)
+    for k, v in out["picks"].items():
+        assert tune.get_knob(k) == v
+    again = tune.autotune(op, pr, R, fp32=True)
+    assert again.get("cached") and again["picks"] == out["picks"]

@@ -77,11 +77,15 @@ _SIGS = {
     "xt_from_interleaved": (ci, [ci, ci, ci, vp, vp, vp]),
     "xt_slab_solve": (ci, [vp, vp, ci, vp, vp, vp, ci, cd, ci, ci, vp, vp, vp, vp]),
     "xt_slab_solve_tol": (ci, [vp, vp, ci, vp, vp, vp, ci, cd, cd, ci, ci, vp, vp, vp, vp]),
+    "xt_slab_solve_seg": (ci, [vp, vp, ci, vp, ci, vp, vp, vp, ci, cd, cd, ci, ci, vp, vp, vp,
+                               vp]),
     "xt_damage_device": (ci, [ci, ci, ci] + [vp] * 18),
     "xt_damage_device_params": (ci, [ci, ci, ci] + [vp] * 19),
     "xt_slab_rhs": (ci, [ci, ci, ci, vp, vp, C.c_longlong, vp, vp, vp]),
     "xt_prof_enable": (ci, [C.c_uint]),
     "xt_graphs_enable": (ci, [ci]),
+    "xt_tune_set": (ci, [ci, ci]),
+    "xt_tune_get": (ci, [ci]),
     "xt_prof_read": (ci, [ci, vp, vp, vp, vp]),
     "xt_prof_read_alg": (ci, [ci, vp, vp]),
     "xt_device_sync": (ci, []),

@@ -48,7 +48,7 @@ constexpr int kLevels = 4;     // pairing levels of the block recurrences, at mo
 constexpr int kMinChain = 6;   // a level is added while its chain keeps this many steps
 constexpr int kKey = 6 + 2 * kLevels;
 struct SweepGraph {
-    int c = 0, pi = -1;
+    int c = 0, pi = -1, tile = 0;   // column count, precision, tile knob at capture
     const void *key[kKey] = {};
     hipGraphExec_t exec = nullptr;
 };
@@ -405,7 +405,9 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
     if (prof) prof_begin(XT_PROF_BAND, s);
     if (nblk > 1) {   // batched pass: 128-row tiles for large FP32 batches, else 64 (the
                       // FP64 128-row tile holds 2 waves a SIMD and measured slower)
-        const int tm = (sizeof(T) == 4 && nblk >= 23 && b % 128 == 0) ? 128 : 64;
+        const int tk = tune_knob(XT_TUNE_BAND_TILE);
+        const bool can128 = sizeof(T) == 4 && b % 128 == 0;
+        const int tm = (can128 && (tk == 128 || (tk == 0 && nblk >= 23))) ? 128 : 64;
         const long long tiles = (long long)(b / tm) * ceil_div(c, 64) * nblk;
         const dim3 grid((unsigned)(ceil_div(tiles, 8) * 8));
         if (tm == 128)
@@ -1048,7 +1050,8 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             for (int L = 0; L < kLevels; ++L) key[6 + dir * kLevels + L] = bp->Lv[dir][L][pi].ptr;
         SweepGraph *g = nullptr;
         for (SweepGraph &cand : bp->graph)
-            if (cand.exec && cand.c == c && cand.pi == pi && std::equal(key, key + kKey, cand.key)) {
+            if (cand.exec && cand.c == c && cand.pi == pi &&
+                cand.tile == tune_knob(XT_TUNE_BAND_TILE) && std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
                 break;
             }
@@ -1075,6 +1078,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             XT_HIP(ei);
             g->c = c;
             g->pi = pi;
+            g->tile = tune_knob(XT_TUNE_BAND_TILE);
             std::copy(key, key + kKey, g->key);
         }
         XT_HIP(hipGraphLaunch(g->exec, s));

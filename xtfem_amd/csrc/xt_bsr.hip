@@ -209,8 +209,9 @@ __device__ __forceinline__ long long xcd_block(long long b, long long grid) {
     return (b % 8) * per + b / 8;
 }
 
-// LPN: lanes per node (power of two >= load cases in this launch, <= 64)
-template <typename T, int LPN>
+// LPN: lanes per node (power of two >= load cases in this launch, <= 64); UNR: entries
+// per pass (loads in flight)
+template <typename T, int LPN, int UNR>
 __global__ void __launch_bounds__(256)
 kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
                  const int *__restrict__ slice_base, const int *__restrict__ slice_len,
@@ -240,7 +241,6 @@ kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
     // UNR entries per pass: their column indices, blocks and X rows are all loaded
     // before the products, so one pass costs two dependent memory latencies (column
     // index, then X) instead of two per entry.
-    constexpr int UNR = sizeof(T) == 4 ? 4 : 2;
     auto accumulate = [&](const T (&kv)[12], const T (&xv)[3][6]) {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -340,7 +340,7 @@ __device__ __forceinline__ T group_sum(T v) {
 // lane i < 3 applies the temporal matrices to row i of the node.  SPL times the waves
 // of the lane-per-node kernel, and 1/SPL of its dependent load chain per lane, for the
 // small launches where that chain is the bound.
-template <typename T, int SPL>
+template <typename T, int SPL, int UNR>
 __global__ void __launch_bounds__(256)
 kron_bsr3_r1_kernel(int nslices, const int *__restrict__ slot_node,
                     const int *__restrict__ slice_base, const int *__restrict__ slice_len,
@@ -385,7 +385,6 @@ kron_bsr3_r1_kernel(int nslices, const int *__restrict__ slot_node,
             }
         }
     };
-    constexpr int UNR = sizeof(T) == 4 ? 4 : 2;
     int e = sub;
     for (; e + (UNR - 1) * SPL < L; e += UNR * SPL) {
         int m[UNR];
@@ -471,6 +470,15 @@ int bsr_set_values(BsrPlan &P, const double *kv, const double *mv, hipStream_t s
 
 static constexpr int kR1Split = 4;   // lanes per node row for one load case (8: slower)
 
+// Kernel variant of a launch: the tuning knobs, else the built-in choices (4 entries in
+// flight in FP32 and 2 in FP64; one wave per workgroup when four-wave workgroups would
+// leave CUs idle; kR1Split lanes per node for one load case).
+template <typename T>
+static int spmm_unr() {
+    const int k = tune_knob(XT_TUNE_SPMM_UNR);
+    return k ? k : (sizeof(T) == 4 ? 4 : 2);
+}
+
 template <typename T>
 int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta, const T *x,
               T *y, bool f32, hipStream_t s) {
@@ -482,30 +490,48 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
         while (lpn < Rc) lpn <<= 1;
         const long long slots = (long long)P.nslices * kSlice;
         const long long waves = slots * lpn / 64;
-        // one wave per workgroup when four-wave workgroups would leave CUs idle
-        const int wpb = waves >= 4 * 1024 ? 4 : 1;
+        const int wk = tune_knob(XT_TUNE_SPMM_WPB);
+        const int wpb = wk ? wk : (waves >= 4 * 1024 ? 4 : 1);
         const int grid = (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
-#define XT_BSR_LAUNCH(N)                                                                   \
-    hipLaunchKernelGGL((kron_bsr3_kernel<T, N>), dim3(grid), dim3(64 * wpb), 0, s, P.nslices, \
-                       P.slot_node.as<int>(), P.slice_base.as<int>(), P.slice_len.as<int>(), \
-                       P.bcol.as<int>(), bv, R, r0, c, tmat, (T)alpha, (T)beta, x, y)
+        const int unr = spmm_unr<T>();
+#define XT_BSR_LAUNCH(N, U)                                                                   \
+    hipLaunchKernelGGL((kron_bsr3_kernel<T, N, U>), dim3(grid), dim3(64 * wpb), 0, s,         \
+                       P.nslices, P.slot_node.as<int>(), P.slice_base.as<int>(),              \
+                       P.slice_len.as<int>(), P.bcol.as<int>(), bv, R, r0, c, tmat, (T)alpha, \
+                       (T)beta, x, y)
+#define XT_BSR_UNR(N)                                                                         \
+    if (unr == 2) XT_BSR_LAUNCH(N, 2);                                                        \
+    else if (unr == 8) XT_BSR_LAUNCH(N, 8);                                                   \
+    else XT_BSR_LAUNCH(N, 4)
+#define XT_BSR_R1(S, U)                                                                       \
+    hipLaunchKernelGGL((kron_bsr3_r1_kernel<T, S, U>), dim3(grid1), dim3(64), 0, s, P.nslices, \
+                       P.slot_node.as<int>(), P.slice_base.as<int>(), P.slice_len.as<int>(),  \
+                       P.bcol.as<int>(), bv, tmat, (T)alpha, (T)beta, x, y)
         if (prof_on(XT_PROF_SPMM)) prof_begin(XT_PROF_SPMM, s);
-        if (R == 1) {   // kR1Split lanes per node
-            const int grid1 = (int)(((slots * kR1Split / 64) + 7) / 8 * 8);
-            hipLaunchKernelGGL((kron_bsr3_r1_kernel<T, kR1Split>), dim3(grid1), dim3(64), 0, s,
-                               P.nslices,
-                               P.slot_node.as<int>(), P.slice_base.as<int>(),
-                               P.slice_len.as<int>(), P.bcol.as<int>(), bv, tmat, (T)alpha,
-                               (T)beta, x, y);
+        if (R == 1) {   // spl lanes per node
+            const int sk = tune_knob(XT_TUNE_SPMM_R1SPLIT);
+            const int spl = sk ? sk : kR1Split;
+            const int grid1 = (int)(((slots * spl / 64) + 7) / 8 * 8);
+            if (spl == 8) {
+                if (unr == 2) XT_BSR_R1(8, 2);
+                else if (unr == 8) XT_BSR_R1(8, 8);
+                else XT_BSR_R1(8, 4);
+            } else {
+                if (unr == 2) XT_BSR_R1(4, 2);
+                else if (unr == 8) XT_BSR_R1(4, 8);
+                else XT_BSR_R1(4, 4);
+            }
         } else switch (lpn) {
-            case 1: XT_BSR_LAUNCH(1); break;
-            case 2: XT_BSR_LAUNCH(2); break;
-            case 4: XT_BSR_LAUNCH(4); break;
-            case 8: XT_BSR_LAUNCH(8); break;
-            case 16: XT_BSR_LAUNCH(16); break;
-            case 32: XT_BSR_LAUNCH(32); break;
-            default: XT_BSR_LAUNCH(64); break;
+            case 1: XT_BSR_UNR(1); break;
+            case 2: XT_BSR_UNR(2); break;
+            case 4: XT_BSR_UNR(4); break;
+            case 8: XT_BSR_UNR(8); break;
+            case 16: XT_BSR_UNR(16); break;
+            case 32: XT_BSR_UNR(32); break;
+            default: XT_BSR_UNR(64); break;
         }
+#undef XT_BSR_R1
+#undef XT_BSR_UNR
 #undef XT_BSR_LAUNCH
         if (prof_on(XT_PROF_SPMM)) {
             // 12 values (3x3 K block + diagonal of the M block) + 1 column index per

@@ -129,6 +129,8 @@ int small_inverse_rowmajor(int q, const double *A, double *Ainv);
 bool prof_on(int kind);
 // Replay fixed launch sequences (the band sweeps) from captured HIP graphs.
 bool graphs_on();
+// Kernel-variant knob (XT_TUNE_*): 0 = the built-in choice.
+int tune_knob(int knob);
 void prof_begin(int kind, hipStream_t s);
 // bytes / flops the launch performs; alg: they are also its algorithmic share (else the
 // launch site records the algorithmic totals with prof_alg)
@@ -168,7 +170,9 @@ int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *s
 int slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b, double *x,
                     int im, double eps, int maxits, int mode, int *its, int *ierr,
                     double *relres, hipStream_t stream, std::vector<double> *hist,
-                    double xtol = 0.0);
+                    double xtol = 0.0, int nseg = 1, const long long *seg_dof = nullptr);
+// The preconditioner can apply in FP64 (a banded factor kept in FP32 only cannot).
+bool prec_has_fp64(xt_prec pr);
 
 // Device workspace of the batched slab solver, owned by the operator handle and kept
 // across calls (buffers only grow), so a solve allocates nothing once warm.
@@ -177,6 +181,10 @@ struct SolverWorkspace {
     DevBuf V, Z;                     // Arnoldi basis, preconditioned basis (flexible)
     DevBuf w32a, w32b, z64, t64;     // work vectors
     DevBuf xprev;                    // iterate before a refinement step (forward-error test)
+    // segments (independent meshes in one block-diagonal system): row chunks of the dot
+    // products (never straddling a segment), the chunks of each segment, segment of each row
+    DevBuf chunks, segchunk, segof;
+    std::vector<int> hchunks, hsegchunk, hsegof;
     DevBuf partial, hdev, hdev2, ndev, coef;
     // Givens state of a cycle in one device region (Hessenberg columns, rotations, rhs
     // of the least squares problem, last residual norms, counters, frozen flags), the

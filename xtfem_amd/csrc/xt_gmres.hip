@@ -29,17 +29,18 @@ namespace xt {
 
 static constexpr int JB = 8;  // basis vectors per multidot block row
 
-// partial[b][j][col] = sum over rows of chunk b of V_j[row][col] * w[row][col]
+// partial[b][j][col] = sum over the rows of chunk b of V_j[row][col] * w[row][col]; chunk b
+// covers rows chunks[b].x .. chunks[b].y - 1 (chunks never straddle two segments)
 __global__ void __launch_bounds__(256)
-multidot_partial(int p, int c, int k, int chunk, const double *__restrict__ V, size_t ldv,
-                 const double *__restrict__ w, double *__restrict__ partial) {
+multidot_partial(int c, int k, const int2 *__restrict__ chunks, const double *__restrict__ V,
+                 size_t ldv, const double *__restrict__ w, double *__restrict__ partial) {
     __shared__ double sh[256];
     const int rpl = 256 / c;
     const int lr = threadIdx.x / c;
     const int col = threadIdx.x - lr * c;
     const int j0 = blockIdx.y * JB;
-    const int rbeg = blockIdx.x * chunk;
-    const int rend = min(p, rbeg + chunk);
+    const int rbeg = chunks[blockIdx.x].x;
+    const int rend = chunks[blockIdx.x].y;
     double acc[JB];
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) acc[jj] = 0.0;
@@ -66,17 +67,20 @@ multidot_partial(int p, int c, int k, int chunk, const double *__restrict__ V, s
     }
 }
 
-// out[j][r] = sum_b sum_{i<q} partial[b][j][r*q + i]: one workgroup per (j, r), its
-// threads stride over the nblk * q terms, then a tree sum in LDS.
+// out[j][g] = sum over the chunks b of segment s, sum_{i<q} partial[b][j][r*q + i] for
+// instance g = s*R + r (segment s = a mesh's rows, r = load case): one workgroup per
+// (j, g), its threads stride over the terms, then a tree sum in LDS.
 __global__ void __launch_bounds__(256)
-reduce_partials(int nblk, int k, int c, int q, int R, const double *__restrict__ partial,
-                double *__restrict__ out) {
+reduce_partials(int k, int c, int q, int R, int I, const int *__restrict__ seg_chunk,
+                const double *__restrict__ partial, double *__restrict__ out) {
     __shared__ double sh[256];
-    const int j = blockIdx.x, r = blockIdx.y;
-    const int n = nblk * q;
+    const int j = blockIdx.x, g = blockIdx.y;
+    const int sg = g / R, r = g - sg * R;
+    const int b0 = seg_chunk[sg];
+    const int n = (seg_chunk[sg + 1] - b0) * q;
     double s = 0.0;
     for (int t = threadIdx.x; t < n; t += 256) {
-        const int b = t / q, i = t - b * q;
+        const int b = b0 + t / q, i = t % q;
         s += partial[((size_t)b * k + j) * c + r * q + i];
     }
     sh[threadIdx.x] = s;
@@ -85,31 +89,40 @@ reduce_partials(int nblk, int k, int c, int q, int R, const double *__restrict__
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[(size_t)j * R + r] = sh[0];
+    if (threadIdx.x == 0) out[(size_t)j * I + g] = sh[0];
 }
 
-// y = (yin ? yin : 0) + alpha * sum_j coef[j][r] V_j
+// Instance of element idx of an interleaved vector: segment of its row (seg_of, NULL for
+// one segment) times R plus its load case.
+__device__ __forceinline__ int instance_of(long long idx, int c, int q, int R,
+                                           const int *__restrict__ seg_of) {
+    const long long row = idx / c;
+    const int r = (int)(idx - row * c) / q;
+    return (seg_of ? seg_of[row] * R : 0) + r;
+}
+
+// y = (yin ? yin : 0) + alpha * sum_j coef[j][g] V_j, g the element's instance
 __global__ void __launch_bounds__(256)
-multiaxpy(long long n, int c, int q, int R, int k, double alpha, const double *__restrict__ coef,
-          const double *__restrict__ V, size_t ldv, const double *yin, double *y) {
+multiaxpy(long long n, int c, int q, int R, int I, const int *__restrict__ seg_of, int k,
+          double alpha, const double *__restrict__ coef, const double *__restrict__ V,
+          size_t ldv, const double *yin, double *y) {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    const int col = (int)(idx % c);
-    const int r = col / q;
+    const int g = instance_of(idx, c, q, R, seg_of);
     double s = yin ? yin[idx] : 0.0;
-    for (int j = 0; j < k; ++j) s += (alpha * coef[j * R + r]) * V[(size_t)j * ldv + idx];
+    for (int j = 0; j < k; ++j) s += (alpha * coef[j * I + g]) * V[(size_t)j * ldv + idx];
     y[idx] = s;
 }
 
-// v *= 1/sqrt(nrm2[r]) (0 if frozen or zero norm)
+// v *= 1/sqrt(nrm2[g]) (0 if the instance is frozen or the norm zero)
 __global__ void __launch_bounds__(256)
-scale_by_norm(long long n, int c, int q, const double *__restrict__ nrm2,
-              const int *__restrict__ frozen, double *v) {
+scale_by_norm(long long n, int c, int q, int R, const int *__restrict__ seg_of,
+              const double *__restrict__ nrm2, const int *__restrict__ frozen, double *v) {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    const int r = (int)(idx % c) / q;
-    const double t = sqrt(nrm2[r]);
-    v[idx] = (frozen[r] || t == 0.0) ? 0.0 : v[idx] * (1.0 / t);
+    const int g = instance_of(idx, c, q, R, seg_of);
+    const double t = sqrt(nrm2[g]);
+    v[idx] = (frozen[g] || t == 0.0) ? 0.0 : v[idx] * (1.0 / t);
 }
 
 // One Arnoldi step's Givens update per right-hand side (kronecker.f90:281-316), on
@@ -350,6 +363,13 @@ struct Solver {
     DevBuf &partial, &hdev, &hdev2, &ndev, &coef;
     DevBuf &gstate, &geps, &ghist;
     PinnedBuf &hstage;
+    SolverWorkspace &ws;
+    // Segments: nseg independent systems (meshes) stacked block-diagonally, rows
+    // seg_dof[s]..seg_dof[s+1]-1; instance g = s*R + r has its own Krylov space,
+    // Hessenberg, rotations and convergence.  I = nseg*R instances.
+    int nseg = 1, I = 0;
+    std::vector<long long> seg_dof;
+    const int *segof = nullptr;   // device: segment of each row (NULL for one segment)
     // Packed cycle state, the same byte offsets in gstate (device) and hstage (pinned
     // host): doubles hh | cs | sn | rs | ro, then ints its | icyc | frozen.  A cycle
     // uploads rs..frozen in one copy and reads the whole state back in one.  hstage
@@ -364,14 +384,17 @@ struct Solver {
         : tm64(w.tm64), tm32(w.tm32), mi64(w.mi64), mi32(w.mi32), V(w.V), Z(w.Z), w32a(w.w32a),
           w32b(w.w32b), z64(w.z64), t64(w.t64), partial(w.partial), hdev(w.hdev),
           hdev2(w.hdev2), ndev(w.ndev), coef(w.coef), gstate(w.gstate), geps(w.geps),
-          ghist(w.ghist), hstage(w.hstage) {}
+          ghist(w.ghist), hstage(w.hstage), ws(w) {}
     template <class T> T *dst(size_t off) {
         return reinterpret_cast<T *>(static_cast<char *>(gstate.ptr) + off);
     }
     // Arnoldi steps between host checks of the frozen flags: a few for small systems,
     // where a step costs less than the host round trip; every step for large ones, where
     // a step past convergence would cost more than the check
-    int check_every() const { return n <= (1LL << 16) ? 4 : 1; }
+    int check_every() const {
+        const int k = tune_knob(XT_TUNE_GMRES_CHECK);
+        return k ? k : (n <= (1LL << 16) ? 4 : 1);
+    }
     int nblk = 0, chunk = 0;
     std::vector<double> *hist = nullptr;  // R == 1: ro/ro0 per iteration (kronpgmres log)
     double hist_ro0 = 1.0;
@@ -392,6 +415,7 @@ struct Solver {
     int init(bool flexible) {
         c = q * R;
         n = (long long)p * c;
+        I = nseg * R;
         XT_ARG(c <= 256, "q*R must be <= 256 for the batched solver");
         XT_TRY(V.reserve(sizeof(double) * n * (im + 1)));
         if (flexible) XT_TRY(Z.reserve(sizeof(double) * n * im));
@@ -399,33 +423,55 @@ struct Solver {
         XT_TRY(t64.reserve(sizeof(double) * n));
         XT_TRY(w32a.reserve(sizeof(float) * n));
         XT_TRY(w32b.reserve(sizeof(float) * n));
+        // row chunks of the dot products: about 1024 over the whole range, each inside one
+        // segment
         const int rpl = 256 / c;
         chunk = std::max(rpl, ((p + 1023) / 1024 + rpl - 1) / rpl * rpl);
-        nblk = ceil_div(p, chunk);
+        if (seg_dof.empty()) seg_dof = {0, (long long)p};
+        ws.hchunks.clear();
+        ws.hsegchunk.assign(1, 0);
+        for (int sg = 0; sg < nseg; ++sg) {
+            for (long long r0 = seg_dof[sg]; r0 < seg_dof[sg + 1]; r0 += chunk) {
+                ws.hchunks.push_back((int)r0);
+                ws.hchunks.push_back((int)std::min(seg_dof[sg + 1], r0 + chunk));
+            }
+            ws.hsegchunk.push_back((int)(ws.hchunks.size() / 2));
+        }
+        nblk = (int)(ws.hchunks.size() / 2);
+        XT_TRY(ws.chunks.upload(ws.hchunks.data(), sizeof(int) * ws.hchunks.size(), s));
+        XT_TRY(ws.segchunk.upload(ws.hsegchunk.data(), sizeof(int) * ws.hsegchunk.size(), s));
+        segof = nullptr;
+        if (nseg > 1) {
+            ws.hsegof.resize(p);
+            for (int sg = 0; sg < nseg; ++sg)
+                for (long long r0 = seg_dof[sg]; r0 < seg_dof[sg + 1]; ++r0) ws.hsegof[r0] = sg;
+            XT_TRY(ws.segof.upload(ws.hsegof.data(), sizeof(int) * p, s));
+            segof = ws.segof.as<int>();
+        }
         XT_TRY(partial.reserve(sizeof(double) * (size_t)nblk * (im + 1) * c));
-        XT_TRY(hdev.reserve(sizeof(double) * (im + 1) * R));
-        XT_TRY(hdev2.reserve(sizeof(double) * (im + 1) * R));
-        XT_TRY(ndev.reserve(sizeof(double) * R));
-        XT_TRY(coef.reserve(sizeof(double) * (im + 1) * R));
-        XT_TRY(geps.reserve(sizeof(double) * R));
+        XT_TRY(hdev.reserve(sizeof(double) * (im + 1) * I));
+        XT_TRY(hdev2.reserve(sizeof(double) * (im + 1) * I));
+        XT_TRY(ndev.reserve(sizeof(double) * I));
+        XT_TRY(coef.reserve(sizeof(double) * (im + 1) * I));
+        XT_TRY(geps.reserve(sizeof(double) * I));
         size_t o = 0;
         auto take = [&o](size_t bytes) {
             const size_t at = o;
             o += (bytes + 255) / 256 * 256;
             return at;
         };
-        o_hh = take(sizeof(double) * (size_t)R * (im + 1) * im);
-        o_cs = take(sizeof(double) * (size_t)R * im);
-        o_sn = take(sizeof(double) * (size_t)R * im);
-        o_rs = take(sizeof(double) * (size_t)R * (im + 1));
-        o_ro = take(sizeof(double) * R);
-        o_its = take(sizeof(int) * R);
-        o_icyc = take(sizeof(int) * R);
-        o_fz = take(sizeof(int) * R);
+        o_hh = take(sizeof(double) * (size_t)I * (im + 1) * im);
+        o_cs = take(sizeof(double) * (size_t)I * im);
+        o_sn = take(sizeof(double) * (size_t)I * im);
+        o_rs = take(sizeof(double) * (size_t)I * (im + 1));
+        o_ro = take(sizeof(double) * I);
+        o_its = take(sizeof(int) * I);
+        o_icyc = take(sizeof(int) * I);
+        o_fz = take(sizeof(int) * I);
         st_bytes = o;
-        o_nrm = take(sizeof(double) * R);
-        o_coef = take(sizeof(double) * (im + 1) * R);
-        o_eps = take(sizeof(double) * R);
+        o_nrm = take(sizeof(double) * I);
+        o_coef = take(sizeof(double) * (im + 1) * I);
+        o_eps = take(sizeof(double) * I);
         o_tm = take(sizeof(double) * (size_t)R * 2 * q * q);
         o_mi = take(sizeof(double) * (size_t)R * q * q);
         o_tm32 = take(sizeof(float) * (size_t)R * 2 * q * q);
@@ -443,20 +489,20 @@ struct Solver {
         return XT_OK;
     }
 
-    // out[j][r] = <V_j, w>_r for j < k
+    // out[j][g] = <V_j, w>_g for j < k and every instance g
     int dots(int k, const double *Vbase, const double *w, double *out) {
         dim3 grid(nblk, ceil_div(k, JB));
-        hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, p, c, k, chunk, Vbase,
-                           (size_t)n, w, partial.as<double>());
-        hipLaunchKernelGGL(reduce_partials, dim3(k, R), dim3(256), 0, s, nblk, k, c, q, R,
-                           partial.as<double>(), out);
+        hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, c, k,
+                           ws.chunks.as<const int2>(), Vbase, (size_t)n, w, partial.as<double>());
+        hipLaunchKernelGGL(reduce_partials, dim3(k, I), dim3(256), 0, s, k, c, q, R, I,
+                           ws.segchunk.as<int>(), partial.as<double>(), out);
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
     int maxpy(int k, double alpha, const double *cf, const double *Vbase, const double *yin,
               double *y) {
-        hipLaunchKernelGGL(multiaxpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R, k,
-                           alpha, cf, Vbase, (size_t)n, yin, y);
+        hipLaunchKernelGGL(multiaxpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R, I,
+                           segof, k, alpha, cf, Vbase, (size_t)n, yin, y);
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
@@ -522,9 +568,9 @@ struct Solver {
     int norms2(const double *v, std::vector<double> &out) {
         XT_TRY(dots(1, v, v, ndev.as<double>()));
         double *h = hstage.at<double>(o_nrm);
-        XT_HIP(hipMemcpyAsync(h, ndev.ptr, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+        XT_HIP(hipMemcpyAsync(h, ndev.ptr, sizeof(double) * I, hipMemcpyDeviceToHost, s));
         XT_HIP(hipStreamSynchronize(s));
-        out.assign(h, h + R);
+        out.assign(h, h + I);
         return XT_OK;
     }
 
@@ -540,21 +586,21 @@ struct Solver {
     int gmres(double *x, bool f32, const std::vector<double> &eps1, int maxits,
               std::vector<int> &active, std::vector<int> &its, std::vector<int> &conv,
               std::vector<double> &ro_last, int max_cycles = 1 << 30, bool flexible = false,
-              bool f32prec = false, const std::vector<double> *nrm0 = nullptr) {
-        const int K = im;
-        std::vector<int> ncyc(R, 0);
-        std::vector<double> hh((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K),
-            rs((size_t)R * (K + 1));
-        std::vector<int> icyc(R, 0), fz(R, 0);
+              bool f32prec = false, const std::vector<double> *nrm0 = nullptr, int kcyc = 0) {
+        const int K = (kcyc > 0 && kcyc < im) ? kcyc : im;   // cycle length
+        std::vector<int> ncyc(I, 0);
+        std::vector<double> hh((size_t)I * (K + 1) * K), cs((size_t)I * K), sn((size_t)I * K),
+            rs((size_t)I * (K + 1));
+        std::vector<int> icyc(I, 0), fz(I, 0);
         std::vector<double> nrm;
         auto HH = [&](int r, int a, int b) -> double & { return hh[(size_t)r * (K + 1) * K + a + (size_t)b * (K + 1)]; };
-        for (int r = 0; r < R; ++r) conv[r] = 0;
+        for (int r = 0; r < I; ++r) conv[r] = 0;
         const int hist_cap = maxits + K + 2;
         if (hist) XT_TRY(ghist.reserve(sizeof(double) * hist_cap));
-        std::copy(eps1.begin(), eps1.begin() + R, hstage.at<double>(o_eps));
-        XT_HIP(hipMemcpyAsync(geps.ptr, hstage.at<double>(o_eps), sizeof(double) * R,
+        std::copy(eps1.begin(), eps1.begin() + I, hstage.at<double>(o_eps));
+        XT_HIP(hipMemcpyAsync(geps.ptr, hstage.at<double>(o_eps), sizeof(double) * I,
                               hipMemcpyHostToDevice, s));
-        std::vector<int> its_before(R);
+        std::vector<int> its_before(I);
         for (;;) {
             // start of a cycle: normalise V_0
             if (nrm0) {
@@ -564,7 +610,7 @@ struct Solver {
                 XT_TRY(norms2(Vj(0), nrm));
             }
             bool any = false;
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < I; ++r) {
                 if (!active[r]) { fz[r] = 1; continue; }
                 const double ro = std::sqrt(nrm[r]);
                 ro_last[r] = ro;
@@ -582,14 +628,14 @@ struct Solver {
             // rs | ro | its | icyc | frozen in one copy (the previous copies that read this
             // staging area finished at the last synchronisation)
             std::copy(rs.begin(), rs.end(), hstage.at<double>(o_rs));
-            std::copy(ro_last.begin(), ro_last.begin() + R, hstage.at<double>(o_ro));
-            std::copy(its.begin(), its.begin() + R, hstage.at<int>(o_its));
+            std::copy(ro_last.begin(), ro_last.begin() + I, hstage.at<double>(o_ro));
+            std::copy(its.begin(), its.begin() + I, hstage.at<int>(o_its));
             std::copy(icyc.begin(), icyc.end(), hstage.at<int>(o_icyc));
             std::copy(fz.begin(), fz.end(), hstage.at<int>(o_fz));
             XT_HIP(hipMemcpyAsync(grs, hstage.at<char>(o_rs), st_bytes - o_rs,
                                   hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
-                               ndev.as<double>(), frozen, Vj(0));
+                               R, segof, ndev.as<double>(), frozen, Vj(0));
             int i = 0;
             for (;;) {
                 // Arnoldi step i -> i+1
@@ -603,8 +649,8 @@ struct Solver {
                 // normalise V_{i+1} for every RHS still in its cycle at this step
                 // (the restart recombination reads it); zero it for the others
                 hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
-                                   ndev.as<double>(), frozen, Vj(i + 1));
-                hipLaunchKernelGGL(givens_kernel, dim3(ceil_div(R, 64)), dim3(64), 0, s, R, i, K,
+                                   R, segof, ndev.as<double>(), frozen, Vj(i + 1));
+                hipLaunchKernelGGL(givens_kernel, dim3(ceil_div(I, 64)), dim3(64), 0, s, I, i, K,
                                    hdev.as<double>(), hdev2.as<double>(), ndev.as<double>(), ghh,
                                    gcs, gsn, grs, frozen, geps.as<double>(), gits, gicyc, gro,
                                    hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
@@ -615,9 +661,9 @@ struct Solver {
                     XT_HIP(hipMemcpyAsync(hstage.ptr, gstate.ptr, st_bytes, hipMemcpyDeviceToHost,
                                           s));
                     XT_HIP(hipStreamSynchronize(s));
-                    std::copy(hstage.at<int>(o_fz), hstage.at<int>(o_fz) + R, fz.begin());
+                    std::copy(hstage.at<int>(o_fz), hstage.at<int>(o_fz) + I, fz.begin());
                     bool cont = false;
-                    for (int r = 0; r < R; ++r) cont = cont || !fz[r];
+                    for (int r = 0; r < I; ++r) cont = cont || !fz[r];
                     if (!cont) break;
                 }
                 ++i;
@@ -632,11 +678,11 @@ struct Solver {
                 h = hstage.at<double>(o_rs);
                 std::copy(h, h + rs.size(), rs.begin());
                 h = hstage.at<double>(o_ro);
-                std::copy(h, h + R, ro_last.begin());
+                std::copy(h, h + I, ro_last.begin());
                 const int *hi = hstage.at<int>(o_icyc);
-                std::copy(hi, hi + R, icyc.begin());
+                std::copy(hi, hi + I, icyc.begin());
                 hi = hstage.at<int>(o_its);
-                std::copy(hi, hi + R, its.begin());
+                std::copy(hi, hi + I, its.begin());
             }
             if (hist && its[0] > its_before[0]) {
                 std::vector<double> hv(its[0] - its_before[0]);
@@ -646,8 +692,8 @@ struct Solver {
             }
             // back substitution and solution update, per RHS with its own cycle length
             const int kmax = i + 1;
-            std::vector<double> y((size_t)kmax * R, 0.0), rr((size_t)(kmax + 1) * R, 0.0);
-            for (int r = 0; r < R; ++r) {
+            std::vector<double> y((size_t)kmax * I, 0.0), rr((size_t)(kmax + 1) * I, 0.0);
+            for (int r = 0; r < I; ++r) {
                 if (!active[r] || icyc[r] == 0) continue;
                 const int ir = icyc[r];
                 double *rs_ = &rs[(size_t)r * (K + 1)];
@@ -658,10 +704,10 @@ struct Solver {
                     for (int j = kk + 1; j < ir; ++j) t -= HH(r, kk, j) * yy[j];
                     yy[kk] = t / HH(r, kk, kk);
                 }
-                for (int j = 0; j < ir; ++j) y[(size_t)j * R + r] = yy[j];
+                for (int j = 0; j < ir; ++j) y[(size_t)j * I + r] = yy[j];
             }
             std::copy(y.begin(), y.end(), hstage.at<double>(o_coef));
-            XT_HIP(hipMemcpyAsync(coef.ptr, hstage.at<double>(o_coef), sizeof(double) * kmax * R,
+            XT_HIP(hipMemcpyAsync(coef.ptr, hstage.at<double>(o_coef), sizeof(double) * kmax * I,
                                   hipMemcpyHostToDevice, s));
             if (flexible) {
                 XT_TRY(maxpy(kmax, 1.0, coef.as<double>(), Zj(0), x, x));
@@ -671,7 +717,7 @@ struct Solver {
             }
             // decide per RHS; build restart residual V_0 = sum_j rs'_j V_j
             bool again = false;
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < I; ++r) {
                 if (!active[r]) continue;
                 const int ir = icyc[r];
                 if (ro_last[r] <= eps1[r]) { conv[r] = 1; active[r] = 0; continue; }
@@ -682,14 +728,14 @@ struct Solver {
                     rs_[jj - 2] = -s_[jj - 2] * rs_[jj - 1];
                     rs_[jj - 1] = c_[jj - 2] * rs_[jj - 1];
                 }
-                for (int j = 0; j <= ir; ++j) rr[(size_t)j * R + r] = rs_[j];
+                for (int j = 0; j <= ir; ++j) rr[(size_t)j * I + r] = rs_[j];
                 again = true;
             }
             if (!again) break;
             XT_HIP(hipStreamSynchronize(s));   // y's copy read the staging area rr reuses
             std::copy(rr.begin(), rr.end(), hstage.at<double>(o_coef));
             XT_HIP(hipMemcpyAsync(coef.ptr, hstage.at<double>(o_coef),
-                                  sizeof(double) * (kmax + 1) * R, hipMemcpyHostToDevice, s));
+                                  sizeof(double) * (kmax + 1) * I, hipMemcpyHostToDevice, s));
             XT_TRY(maxpy(kmax + 1, 1.0, coef.as<double>(), Vj(0), nullptr, t64.as<double>()));
             XT_HIP(hipMemcpyAsync(Vj(0), t64.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             XT_HIP(hipStreamSynchronize(s));
@@ -718,10 +764,23 @@ extern "C" int xt_slab_solve_tol(xt_op op, xt_prec pr, int R, const double *ab, 
                                (hipStream_t)stream, nullptr, xtol);
 }
 
+extern "C" int xt_slab_solve_seg(xt_op op, xt_prec pr, int nseg, const long long *seg_dof,
+                                 int R, const double *ab, const double *b, double *x, int im,
+                                 double eps, double xtol, int maxits, int mode, int *its,
+                                 int *ierr, double *relres, void *stream) {
+    if (xtol < 0.0 || nseg < 1 || !seg_dof) {
+        set_error("argument error: xtol >= 0, nseg >= 1 and seg_dof are required");
+        return XT_E_ARG;
+    }
+    return xt::slab_solve_impl(op, pr, R, ab, b, x, im, eps, maxits, mode, its, ierr, relres,
+                               (hipStream_t)stream, nullptr, xtol, nseg, seg_dof);
+}
+
 int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
                         double *x, int im, double eps, int maxits, int mode, int *its,
                         int *ierr, double *relres, hipStream_t stream,
-                        std::vector<double> *hist, double xtol) {
+                        std::vector<double> *hist, double xtol, int nseg,
+                        const long long *seg_dof) {
     XT_ARG(op && pr && R > 0 && ab && b && x && its && ierr && relres, "slab_solve arguments");
     XT_ARG(im >= 1 && im <= 256 && maxits >= 1 && mode >= 0 && mode <= 2, "slab_solve parameters");
     SolverWorkspace *ws = op_workspace(op);
@@ -732,13 +791,21 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
     S.R = R;
     S.im = im;
     S.s = stream;
-    S.hist = (R == 1 && mode == 0) ? hist : nullptr;
+    S.hist = (R == 1 && nseg == 1 && mode == 0) ? hist : nullptr;
     {
         int p = 0, q = 0, nt = 0;
         XT_TRY(op_dims(op, &p, &q, &nt));
         XT_ARG(nt == 2, "slab_solve needs a two-term (K, M) operator");
         S.p = p;
         S.q = q;
+    }
+    if (nseg > 1 || seg_dof) {
+        XT_ARG(nseg >= 1 && seg_dof && seg_dof[0] == 0 && seg_dof[nseg] == S.p,
+               "segments must cover the rows 0..p");
+        for (int sg = 0; sg < nseg; ++sg)
+            XT_ARG(seg_dof[sg + 1] > seg_dof[sg], "segments must be non-empty and increasing");
+        S.nseg = nseg;
+        S.seg_dof.assign(seg_dof, seg_dof + nseg + 1);
     }
     XT_TRY(S.init(mode != 0));
     const int q = S.q;
@@ -783,15 +850,18 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
     XT_TRY(S.tm32.upload(S.hstage.at<float>(S.o_tm32), sizeof(float) * tm32.size(), S.s));
     XT_TRY(S.mi32.upload(S.hstage.at<float>(S.o_mi32), sizeof(float) * mi32.size(), S.s));
 
-    std::vector<int> active(R, 1), itv(R, 0), conv(R, 0);
-    std::vector<double> ro_last(R, 0.0), eps1(R, 0.0), r0(R), bn(R);
+    // FP64 preconditioner sweeps available (a factor kept in FP32 only has none: no
+    // FP64 fallback then)
+    const bool fp64_prec = prec_has_fp64(pr);
+    std::vector<int> active(S.I, 1), itv(S.I, 0), conv(S.I, 0);
+    std::vector<double> ro_last(S.I, 0.0), eps1(S.I, 0.0), r0(S.I), bn(S.I);
     // V_0 = b - A x0; when every x0 is zero that is b itself (A 0 = 0 exactly), so the
     // operator apply and a second norm pass are skipped (ndev then holds b's norms,
     // which are V_0's)
     std::vector<double> xn;
     XT_TRY(S.norms2(x, xn));
     bool x0zero = true;
-    for (int r = 0; r < R; ++r) x0zero = x0zero && xn[r] == 0.0;
+    for (int r = 0; r < S.I; ++r) x0zero = x0zero && xn[r] == 0.0;
     XT_TRY(S.norms2(b, bn));
     S.hist_ro0 = std::sqrt(bn[0]);
     if (x0zero) {
@@ -802,7 +872,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         XT_TRY(S.norms2(S.Vj(0), r0));
     }
     const std::vector<double> r0sq = r0;   // squared norms of V_0 (ndev still holds them)
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < S.I; ++r) {
         r0[r] = std::sqrt(r0[r]);
         eps1[r] = eps * r0[r];
         ierr[r] = 0;
@@ -812,22 +882,25 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         // mode 2: FP64 flexible GMRES, FP32 preconditioner
         XT_TRY(S.gmres(x, false, eps1, maxits, active, itv, conv, ro_last, 1 << 30, mode == 2,
                        mode == 2));
-        for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
+        for (int r = 0; r < S.I; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
     } else {
-        // GMRES-IR: one flexible GMRES(im) cycle per refinement step on the current
-        // residual with the FP32 operator and FP32 preconditioner (FP64 basis, stored
+        // GMRES-IR: one flexible GMRES cycle per refinement step on the current residual
+        // with the FP32 operator and FP32 preconditioner (FP64 basis, stored
         // preconditioned vectors), FP64 outer residual.  The inner target starts at 1e-3
         // of the residual (one inner iteration a step with an exact factor) and is
         // relaxed to the reduction the last step achieved when FP32 rounding is the floor
-        // (ill-conditioned temporal matrices); a right-hand side whose step reduced its
-        // residual by less than 2x continues with inner cycles in FP64 (operator and
-        // preconditioner), where FP32 does not pay.
+        // (ill-conditioned temporal matrices).  A right-hand side continues with inner
+        // cycles in FP64 (operator and preconditioner), where FP32 does not pay, when its
+        // step reduced the residual by less than 2x, or when its FP32 cycle did not meet
+        // the inner target within kFp32Cycle iterations (an FP32 factor of an
+        // ill-conditioned K preconditions poorly: FP64 cycles then need a few iterations
+        // where FP32 ones need tens).
         // With xtol > 0 a case also needs its last correction to be at most xtol times
         // its solution (a forward-error test: the residual does not bound the error of an
         // ill-conditioned slab); until then it keeps refining, each inner solve asked for
         // a fixed reduction of the residual.
-        std::vector<double> rn(R), irel(R, 1.0e-3), prev(R, 0.0), dn(R, 0.0), xn(R, 0.0);
-        std::vector<int> fb(R, 0);
+        std::vector<double> rn(S.I), irel(S.I, 1.0e-3), prev(S.I, 0.0), dn(S.I, 0.0), xn(S.I, 0.0);
+        std::vector<int> fb(S.I, 0);
         if (xtol > 0.0) XT_TRY(ws->xprev.reserve(sizeof(double) * S.n));
         for (int outer = 0;; ++outer) {
             if (xtol > 0.0 && outer > 0) {   // correction of the last step: x - xprev
@@ -844,9 +917,9 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             else XT_TRY(S.norms2(S.Vj(0), rn));
             if (outer == 64) break;
             bool any32 = false, any64 = false;
-            std::vector<double> e1(R, 0.0);
-            std::vector<int> act32(R, 0), act64(R, 0);
-            for (int r = 0; r < R; ++r) {
+            std::vector<double> e1(S.I, 0.0);
+            std::vector<int> act32(S.I, 0), act64(S.I, 0);
+            for (int r = 0; r < S.I; ++r) {
                 if (ierr[r] == -1) continue;
                 const double nr = std::sqrt(rn[r]);
                 const bool fwd = xtol == 0.0 || (outer > 0 && dn[r] <= xtol * xtol * xn[r]);
@@ -855,7 +928,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                 if (itv[r] >= maxits || nr == 0.0) continue;
                 if (prev[r] > 0.0) {
                     const double rho = nr / prev[r];
-                    if (rho > 0.5) fb[r] = 1;
+                    if (rho > 0.5 && fp64_prec) fb[r] = 1;
                     else if (rho > 10.0 * irel[r]) irel[r] = std::min(0.1, rho);
                 }
                 prev[r] = nr;
@@ -869,17 +942,23 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                                       hipMemcpyDeviceToDevice, S.s));
             // inner solves for the correction on V_0 = the residual (already there): each
             // flexible cycle adds its correction Z y to x itself
-            std::vector<int> cin(R, 0);
+            std::vector<int> cin(S.I, 0);
             // the first inner call starts from the norms just computed (V_0 unchanged)
-            if (any32)
-                XT_TRY(S.gmres(x, true, e1, maxits, act32, itv, cin, ro_last, 1, true, true, &rn));
+            if (any32) {
+                constexpr int kFp32Cycle = 10;
+                const std::vector<int> was32 = act32;   // the call clears the active flags
+                XT_TRY(S.gmres(x, true, e1, maxits, act32, itv, cin, ro_last, 1, true, true, &rn,
+                               kFp32Cycle));
+                for (int r = 0; r < S.I; ++r)
+                    if (was32[r] && !cin[r] && fp64_prec) fb[r] = 1;
+            }
             if (any64)
                 XT_TRY(S.gmres(x, false, e1, maxits, act64, itv, cin, ro_last, 1, true, false,
                                any32 ? nullptr : &rn));
             XT_TRY(S.residual(b, x, S.Vj(0)));
         }
-        for (int r = 0; r < R; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
-        for (int r = 0; r < R; ++r) {   // true residual: the loop's last b - A x
+        for (int r = 0; r < S.I; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
+        for (int r = 0; r < S.I; ++r) {   // true residual: the loop's last b - A x
             its[r] = itv[r];
             relres[r] = bn[r] > 0 ? std::sqrt(rn[r]) / std::sqrt(bn[r]) : 0.0;
         }
@@ -889,7 +968,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
     std::vector<double> rn;
     XT_TRY(S.residual(b, x, S.t64.as<double>()));
     XT_TRY(S.norms2(S.t64.as<double>(), rn));
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < S.I; ++r) {
         its[r] = itv[r];
         relres[r] = bn[r] > 0 ? std::sqrt(rn[r]) / std::sqrt(bn[r]) : 0.0;
     }

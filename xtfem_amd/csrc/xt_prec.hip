@@ -496,6 +496,8 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     }
     return ilut_apply<T>(pr, R, q, minv, x, y, f32, stage, true, s);
 }
+bool prec_has_fp64(xt_prec pr) { return !pr->band || band_has_fp64(pr->band); }
+
 int prec_shifts(xt_prec pr, const double **shifts, bool *fp64) {
     if (shifts) *shifts = pr->shifts.empty() ? nullptr : pr->shifts.data();
     if (fp64) *fp64 = !pr->shifted.empty() && band_has_fp64(pr->shifted[0]);
@@ -636,14 +638,14 @@ int xt_prec_create_band(int p, const int *ia, const int *ja, const double *a, in
 int xt_prec_create_band_shifted(int p, const int *kia, const int *kja, const double *ka,
                                 const int *mia, const int *mja, const double *ma, int nshift,
                                 const double *shifts, int block, int flags, xt_prec *out) {
-    XT_ARG(out && p > 0 && kia && kja && ka && mia && mja && ma && nshift >= 1 && shifts,
-           "prec_create_band_shifted arguments");
-    XT_ARG(kia[0] == 1 && mia[0] == 1, "CSR must be 1-based");
+    XT_ARG(out && p > 0 && kia && kja && ka && nshift >= 0 && (nshift == 0 || (shifts && mia &&
+           mja && ma)), "prec_create_band_shifted arguments");
+    XT_ARG(kia[0] == 1 && (nshift == 0 || mia[0] == 1), "CSR must be 1-based");
     for (int g = 0; g < nshift; ++g) XT_ARG(shifts[g] > 0.0, "shifts must be positive");
     xt_prec pr = new (std::nothrow) xt_prec_s;
     if (!pr) return XT_E_NOMEM;
     pr->p = p;
-    int rc = band_build(p, kia, kja, ka, block, &pr->band);
+    int rc = band_build(p, kia, kja, ka, block, &pr->band, !(flags & XT_BAND_FP32_ONLY));
     int b = 0;
     if (rc == XT_OK) band_dims(pr->band, &b, nullptr, nullptr);
     // K + s M row by row (both rows sorted by column: merged; K's pattern holds M's for a
@@ -654,7 +656,7 @@ int xt_prec_create_band_shifted(int p, const int *kia, const int *kja, const dou
                 if (ja_[k] <= ja_[k - 1]) return false;
         return true;
     };
-    if (rc == XT_OK && (!sorted(kia, kja) || !sorted(mia, mja))) {
+    if (rc == XT_OK && nshift > 0 && (!sorted(kia, kja) || !sorted(mia, mja))) {
         set_error("prec_create_band_shifted: CSR rows must be sorted by column");
         rc = XT_E_ARG;
     }

@@ -34,8 +34,10 @@ bool prof_on(int kind) { return (g_mask >> kind) & 1u; }
 
 namespace {
 bool g_graphs = true;
+int g_knob[XT_TUNE_NKNOBS] = {};   // 0 = the built-in choice
 }
 bool graphs_on() { return g_graphs; }
+int tune_knob(int knob) { return (knob >= 0 && knob < XT_TUNE_NKNOBS) ? g_knob[knob] : 0; }
 
 void prof_begin(int kind, hipStream_t s) {
     if (!prof_on(kind)) return;
@@ -70,6 +72,24 @@ void prof_alg(int kind, double bytes, double flops) {
 using namespace xt;
 
 extern "C" {
+
+int xt_tune_set(int knob, int value) {
+    XT_ARG(knob >= 0 && knob < XT_TUNE_NKNOBS, "tuning knob");
+    static const int allowed[XT_TUNE_NKNOBS][4] = {
+        {0, 2, 4, 8},      // XT_TUNE_SPMM_UNR
+        {0, 1, 2, 4},      // XT_TUNE_SPMM_WPB
+        {0, 4, 8, 4},      // XT_TUNE_SPMM_R1SPLIT
+        {0, 64, 128, 64},  // XT_TUNE_BAND_TILE
+        {0, 1, 2, 4},      // XT_TUNE_GMRES_CHECK
+    };
+    bool ok = false;
+    for (int v : allowed[knob]) ok = ok || v == value;
+    XT_ARG(ok, "value not allowed for this knob");
+    g_knob[knob] = value;
+    return XT_OK;
+}
+
+int xt_tune_get(int knob) { return (knob >= 0 && knob < XT_TUNE_NKNOBS) ? g_knob[knob] : -1; }
 
 int xt_graphs_enable(int on) {
     g_graphs = on != 0;

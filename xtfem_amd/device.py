@@ -169,10 +169,11 @@ class BandPreconditioner(IlutPreconditioner):
     droptol = 0 limit, ilut.f:2169-2170).  Block Cholesky in FP64 on the device,
     applied by dense block sweeps in FP64 or FP32."""
 
-    def __init__(self, K, block=0, M=None, shifts=(), shifted_fp32_only=False):
+    def __init__(self, K, block=0, M=None, shifts=(), shifted_fp32_only=False, fp32_only=False):
         """shifts (with M): also factor K + s M for each s; slab solves then split each
         load case's dominant temporal mode onto the nearest shift (xt_prec_create_band_shifted;
-        shifted_fp32_only keeps those factors in FP32 only, for the FP32 applies)."""
+        shifted_fp32_only keeps those factors in FP32 only, for the FP32 applies;
+        fp32_only keeps the factor of K in FP32 only as well)."""
         ia, ja, a = (np.ascontiguousarray(K[0], dtype=np.int32),
                      np.ascontiguousarray(K[1], dtype=np.int32),
                      np.ascontiguousarray(K[2], dtype=np.float64))
@@ -182,15 +183,19 @@ class BandPreconditioner(IlutPreconditioner):
         self.shifts = np.asarray(shifts, dtype=np.float64).ravel()
         h = C.c_void_p()
         _dev()
-        if len(self.shifts):
-            mi, mj, mv = (np.ascontiguousarray(M[0], dtype=np.int32),
-                          np.ascontiguousarray(M[1], dtype=np.int32),
-                          np.ascontiguousarray(M[2], dtype=np.float64))
+        if len(self.shifts) or fp32_only:
+            if len(self.shifts):
+                mi, mj, mv = (np.ascontiguousarray(M[0], dtype=np.int32),
+                              np.ascontiguousarray(M[1], dtype=np.int32),
+                              np.ascontiguousarray(M[2], dtype=np.float64))
+                mp = (mi.ctypes.data, mj.ctypes.data, mv.ctypes.data)
+            else:
+                mp = (None, None, None)
+            flags = (1 if shifted_fp32_only else 0) | (2 if fp32_only else 0)
             check(lib().xt_prec_create_band_shifted(
-                self.p, ia.ctypes.data, ja.ctypes.data, a.ctypes.data, mi.ctypes.data,
-                mj.ctypes.data, mv.ctypes.data, len(self.shifts), self.shifts.ctypes.data,
-                int(block), 1 if shifted_fp32_only else 0, C.addressof(h)),
-                "prec_create_band_shifted")
+                self.p, ia.ctypes.data, ja.ctypes.data, a.ctypes.data, *mp, len(self.shifts),
+                self.shifts.ctypes.data if len(self.shifts) else None, int(block), flags,
+                C.addressof(h)), "prec_create_band_shifted")
         else:
             check(lib().xt_prec_create_band(self.p, ia.ctypes.data, ja.ctypes.data,
                                             a.ctypes.data, int(block), C.addressof(h)),
@@ -224,14 +229,13 @@ def dominant_temporal_mode(A, B):
     return np.where((np.abs(lam.imag) <= 1e-10 * np.abs(lam)) & (lam.real > 0), lam.real, np.nan)
 
 
-def lowest_mode(pr, M, iters=40, seed=0):
+def lowest_mode(pr, M, fixed, iters=40, seed=0):
     """Estimate of the smallest eigenvalue mu_min of K phi = mu M phi: power iteration on
-    K^-1 M with the device factor pr (K^-1) and a CSR operator for M.  Fixed DOFs of the
-    1-0 method (unit diagonal in K and M) would give mu = 1: the start vector is zero
-    there and K^-1 M keeps it so."""
+    K^-1 M with the device factor pr (K^-1) and a CSR operator for M.  The fixed DOFs of
+    the 1-0 method (unit diagonal in K and M, per-DOF mask `fixed`) would give mu = 1:
+    the iterate is kept zero there."""
     opM = KronOperator([M], q=1)
-    ia = np.asarray(M[0])
-    free = torch.as_tensor(np.diff(ia) > 1, device=_dev())
+    free = torch.as_tensor(np.asarray(fixed) == 0, device=_dev())
     p = pr.p
     g = torch.Generator(device="cpu").manual_seed(seed)
     x = (torch.rand(p, generator=g, dtype=torch.float64) - 0.5).to(_dev()) * free
@@ -406,6 +410,43 @@ def slab_solve_device(op, pr, A, B, b, x, R, im=30, eps=1e-8, maxits=500, mode=1
                                   int(im), float(eps), int(maxits), int(mode), its.ctypes.data,
                                   ierr.ctypes.data, relres.ctypes.data, _stream()), "slab_solve")
     return x, its, ierr, relres
+
+
+def block_diagonal(mats):
+    """Stack 1-based CSR matrices (ia, ja, a) into one block-diagonal CSR; returns it and the
+    row offsets of the blocks [n_blocks + 1]."""
+    ia_out, ja_out, a_out = [np.zeros(1, dtype=np.int64)], [], []
+    off = [0]
+    nnz = 0
+    for ia, ja, a in mats:
+        ia = np.asarray(ia, dtype=np.int64)
+        n = len(ia) - 1
+        ia_out.append(ia[1:] - 1 + nnz)
+        ja_out.append(np.asarray(ja, dtype=np.int64) + off[-1])
+        a_out.append(np.asarray(a, dtype=np.float64))
+        nnz += int(ia[-1] - 1)
+        off.append(off[-1] + n)
+    ia = (np.concatenate(ia_out) + 1).astype(np.int32)
+    return (ia, np.concatenate(ja_out).astype(np.int32), np.concatenate(a_out)), np.array(off)
+
+
+def slab_solve_seg(op, pr, seg_dof, A, B, b, x, R, im=30, eps=1e-8, maxits=500, mode=1,
+                   xtol=0.0):
+    """xt_slab_solve_seg: independent systems stacked block-diagonally (rows
+    seg_dof[s]..seg_dof[s+1]-1), R load cases each, one instance per (system, case).
+    b, x: interleaved FP64 device tensors; returns (x, its, ierr, relres) with the
+    per-instance arrays shaped [nseg, R]."""
+    nseg = len(seg_dof) - 1
+    sd = np.ascontiguousarray(seg_dof, dtype=np.int64)
+    ab = np.ascontiguousarray(np.stack([_colmajor(A), _colmajor(B)], axis=1))
+    its = np.zeros(nseg * R, dtype=np.int32)
+    ierr = np.zeros(nseg * R, dtype=np.int32)
+    relres = np.zeros(nseg * R)
+    check(lib().xt_slab_solve_seg(op.h, pr.h, nseg, sd.ctypes.data, R, ab.ctypes.data,
+                                  b.data_ptr(), x.data_ptr(), int(im), float(eps), float(xtol),
+                                  int(maxits), int(mode), its.ctypes.data, ierr.ctypes.data,
+                                  relres.ctypes.data, _stream()), "slab_solve_seg")
+    return x, its.reshape(nseg, R), ierr.reshape(nseg, R), relres.reshape(nseg, R)
 
 
 class DamageState:

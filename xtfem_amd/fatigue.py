@@ -27,7 +27,7 @@ import time
 import numpy as np
 import torch
 
-from . import device, slab
+from . import api, device, slab
 from ._lib import check, lib
 from .deletion import ElementDeletion
 
@@ -64,6 +64,7 @@ class FatigueSlabs:
         self._build(self.deletion.K, self.deletion.M, self.deletion.fixed)
         self.n = 0
         self.log = []
+        self.separated = False
 
     def _build(self, K, M, fixed):
         """Device operator, banded factor and fixed mask of the current matrices."""
@@ -74,13 +75,27 @@ class FatigueSlabs:
 
     def step(self):
         """One slab: deletion check, right-hand side, solve, damage.  Returns the slab's
-        record (iterations, relative residual, newly failed elements, timings)."""
+        record (iterations, relative residual, newly failed elements, timings); when the
+        deletion leaves K singular (the specimen is cut through) the record says
+        ``separated`` and no solve runs (nor may later steps)."""
+        if self.separated:
+            raise RuntimeError("the specimen has separated: no further slabs")
         rec = {"slab": self.n}
         t0 = time.perf_counter()
         ev = self.deletion.step(self.damage)           # chkelesta + edits (main3d.f90:625-686)
         rec["failed"] = [] if ev is None else ev.nefail.tolist()
         if ev is not None:
-            self._build(ev.K, ev.M, ev.fixed)
+            try:
+                self._build(ev.K, ev.M, ev.fixed)
+            except api.XtError as e:
+                if "code -3" not in str(e):
+                    raise
+                # K singular: the deleted elements cut the specimen in two (a part is no
+                # longer held by the clamp); the specimen has failed, the loop ends here
+                self.separated = True
+                rec["separated"] = True
+                self.log.append(rec)
+                return rec
         torch.cuda.synchronize()
         rec["deletion_ms"] = 1e3 * (time.perf_counter() - t0)
         t0 = time.perf_counter()

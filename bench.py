@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cases", type=int, default=32, help="load cases per batched solve")
-    ap.add_argument("--workload", choices=("panel", "plate"), default="panel")
+    ap.add_argument("--workload", choices=("panel", "plate", "multimesh"), default="panel",
+                    help="multimesh: only the uneven multi-mesh batch leg")
     ap.add_argument("--grid", type=int, nargs=3, default=None,
                     help="elements per axis (panel: 240 200 2, plate: 100 46 4)")
     ap.add_argument("--notch", type=int, default=20, help="plate: notch length in elements")
@@ -70,7 +71,7 @@ def parse():
                          "(0: no shifted factors)")
     ap.add_argument("--legs", default="",
                     help="extra legs, comma-separated: rod, reassembly, thermo, driver_panel, "
-                         "deletion, damage, fatigue (or 'all')")
+                         "deletion, damage, fatigue, multimesh (or 'all')")
     ap.add_argument("--fatigue-blocks", type=int, default=40,
                     help="timed cycle blocks of the fatigue leg")
     ap.add_argument("--damage-nip", type=int, default=2560,
@@ -86,6 +87,8 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=45.0,
                     help="seconds of solving per CPU worker in the CPU baseline")
     ap.add_argument("--cpu-matrices", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the setup and the timed FP32-IR steps (for rocprofv3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     return ap.parse_args()
@@ -478,6 +481,127 @@ def fatigue_leg(blocks=40, nip=2560, amp=250.0):
                 all_converged=bool(all(r["ierr"] == 0 for r in recs)),
                 max_D=float(st["D"].max()), yielded_fraction=float(np.mean(st["p"] > 0)),
                 setup_s=t_setup)
+
+
+def multimesh_meshes(nmesh=64, target_dof=6.0e6):
+    """The uneven multi-mesh batch: nmesh prismatic rods (n x n cross-section, n in 2..8)
+    and plates (width 8..20, thickness 2..3 elements), long along x, element counts drawn
+    log-uniform in [2k, 400k] (seeded) and scaled by one common factor so that the batch
+    holds about target_dof spatial DOFs; each clamped at x = 0 with an axial traction on
+    its far end.  Returns [(dims, kind)] and the scale factor."""
+    rng = np.random.default_rng(SEED + 5)
+    ne = np.exp(rng.uniform(np.log(2e3), np.log(4e5), nmesh))
+    kind = rng.integers(0, 2, nmesh)
+    sec = rng.integers(2, 9, nmesh)
+    width = rng.integers(8, 21, nmesh)
+    thick = rng.integers(2, 4, nmesh)
+
+    def dims(i, f):
+        e = ne[i] * f
+        if kind[i] == 0:
+            n = int(sec[i])
+            return (max(4, int(round(e / (n * n)))), n, n)
+        ny, nz = int(width[i]), int(thick[i])
+        return (max(4, int(round(e / (ny * nz)))), ny, nz)
+
+    def total(f):
+        return sum(3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1) for d in (dims(i, f) for i in range(nmesh)))
+
+    lo, hi = 1e-3, 1.0
+    for _ in range(60):
+        mid = np.sqrt(lo * hi)
+        lo, hi = (mid, hi) if total(mid) < target_dof else (lo, mid)
+    f = hi if total(hi) <= 1.02 * target_dof else lo
+    return [(dims(i, f), "rod" if kind[i] == 0 else "plate") for i in range(nmesh)], f
+
+
+def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
+    """The uneven multi-mesh batch: every mesh's K and M stacked block-diagonally, one
+    banded factor (FP32), one batched FP32-IR slab solve of nmesh x R instances with
+    per-instance convergence (xt_slab_solve_seg), timed over K solves; against the same
+    meshes solved one at a time in FP64 on the GPU (each with its own operator and FP64
+    banded factor, R load cases per solve)."""
+    import torch
+    from xtfem_amd import device, fem, slab
+    t0 = time.perf_counter()
+    spec, scale = multimesh_meshes(nmesh, target_dof)
+    amp, freq, dt = load_cases(R, SEED + 6)
+    w = 2 * np.pi * freq
+    c = slab.stcoeff_m(0.0, dt, w)
+    A, B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+    fx, _ = slab.getfextc_m(0.0, w, dt)
+    systems = []
+    for (nx, ny, nz), kind in spec:
+        mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz))
+        bc = fem.clamp_mask(mesh, 0, [0.0])
+        asm = device.Assembly(mesh, bc)
+        asm.run(2.0e5, 0.3, 7.85e-9)
+        Kb, Mb = asm.host()
+        del asm
+        rext = fem.traction_load(mesh, 0, float(nx), 0, float(ny * nz))
+        rhs = np.array([slab.slab_rhs(3 * mesh.nn, fx[r] * amp[r], None, rext,
+                                      fixed=np.tile(bc, 6)) for r in range(R)])
+        systems.append(dict(p=3 * mesh.nn, ne=mesh.ne, Kb=Kb, Mb=Mb, rhs=rhs, kind=kind))
+    Kc, seg = device.block_diagonal([s["Kb"] for s in systems])
+    Mc, _ = device.block_diagonal([s["Mb"] for s in systems])
+    t_mesh = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    op = device.KronOperator([Kc, Mc], q=6)
+    pr = device.BandPreconditioner(Kc, fp32_only=True)
+    b = torch.cat([device.to_interleaved(s["rhs"], s["p"], 6) for s in systems]).contiguous()
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    device.slab_solve_seg(op, pr, seg, A, B, b, x, R, mode=1, xtol=xtol)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        x.zero_()
+        _, its, ierr, relres = device.slab_solve_seg(op, pr, seg, A, B, b, x, R, mode=1,
+                                                     xtol=xtol)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / K
+    x32 = x.clone()
+    del op, pr
+    torch.cuda.empty_cache()
+    # one mesh at a time in FP64 (setup excluded from the timed solves)
+    t64, its64, err = 0.0, [], []
+    for si, s in enumerate(systems):
+        op1 = device.KronOperator([s["Kb"], s["Mb"]], q=6)
+        pr1 = device.BandPreconditioner(s["Kb"])
+        b1 = device.to_interleaved(s["rhs"], s["p"], 6)
+        x1 = torch.zeros_like(b1)
+        device.slab_solve_device(op1, pr1, A, B, b1, x1, R, mode=0)       # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x1.zero_()
+        _, it1, ie1, rr1 = device.slab_solve_device(op1, pr1, A, B, b1, x1, R, mode=0)
+        torch.cuda.synchronize()
+        t64 += time.perf_counter() - t1
+        its64 += it1.tolist()
+        xs = x32[seg[si] * 6 * R:seg[si + 1] * 6 * R]
+        d = (xs - x1).view(-1, 6 * R)
+        ref = x1.view(-1, 6 * R)
+        for r in range(R):
+            cols = slice(6 * r, 6 * r + 6)
+            err.append(float(torch.linalg.norm(d[:, cols]) / torch.linalg.norm(ref[:, cols])))
+        del op1, pr1
+    n_inst = nmesh * R
+    elems = [s["ne"] for s in systems]
+    return dict(workload=f"{nmesh} meshes ({sum(1 for s in systems if s['kind'] == 'rod')} "
+                         f"rods, the rest plates), {min(elems)}-{max(elems)} C3D8 elements "
+                         f"(log-uniform draw in [2k, 400k] scaled by {scale:.3f}), "
+                         f"{int(seg[-1])} spatial DOFs in all, {R} load cases each, one "
+                         f"batched FP32-IR solve of {n_inst} instances",
+                fp32_ir_batched_slabs_per_s=n_inst / t32, ms_per_batched_solve=1e3 * t32,
+                fp64_one_at_a_time_slabs_per_s=n_inst / t64,
+                speedup=(n_inst / t32) / (n_inst / t64),
+                its_min=int(its.min()), its_max=int(its.max()),
+                fp64_its_max=int(max(its64)),
+                all_converged=bool(np.all(ierr == 0) and np.all(relres <= 1e-8)),
+                relres_max=float(relres.max()),
+                rel_err_vs_fp64_max=float(max(err)),
+                setup_s=dict(meshes=t_mesh, operator_and_factor=t_setup))
 
 
 def deletion_leg(prob, R=8):
@@ -956,6 +1080,9 @@ def main():
         print(json.dumps(rod_leg(args.steps)))
         return
     R, K, W = args.cases, args.steps, args.warmup
+    if args.workload == "multimesh":
+        print(json.dumps(multimesh_leg(K=K)))
+        return
     if args.thermo_only:
         grid = args.grid or [100, 46, 4]
         print(json.dumps(thermo_leg(Problem(*grid, args.notch, 1), nslab=args.steps)))
@@ -977,6 +1104,9 @@ def main():
     for _ in range(W):
         run32.step()
     t32 = timed(run32, K)
+    if args.profile_only:       # the batched FP32-IR steps alone (for a kernel trace)
+        print(json.dumps(dict(ms_per_step=1e3 * t32 / K, its=run32.its[-1].tolist())))
+        return
     its32 = np.array(run32.its[W:])
     rr32 = np.array(run32.relres[W:])
     ok32 = bool(np.all(np.array(run32.ierr[W:]) == 0) and np.all(rr32 <= 1e-8))
@@ -1040,9 +1170,21 @@ def main():
     print(f"# kron-spmm fp32: R=1 {spmm['R1']['gbs']:.0f} GB/s ({spmm['R1']['frac']:.2f}), "
           f"R=32 {spmm['R32']['gbs']:.0f} GB/s ({spmm['R32']['frac']:.2f})", file=sys.stderr,
           flush=True)
+    # per-kernel durations: the batched FP32-IR steps once more, every launch of the
+    # operator and sweep kernels bracketed by HIP events on the launching stream
+    prof = kernel_profile(run32, K, mask=0b11)
+    dom_name = max(prof, key=lambda k: prof[k]["ms"])
+    roof = roofline(dom_name, prof[dom_name], prob, pr)
+    value = K * R / t32
+    ms_step = 1e3 * t32 / K
+    nblocks, block = pr.nblocks, pr.block
+    del run32, op, pr                  # the extra legs need the memory
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     legs = set(args.legs.split(",")) - {""}
     if "all" in legs:
-        legs = {"rod", "reassembly", "thermo", "driver_panel", "deletion", "damage", "fatigue"}
+        legs = {"rod", "reassembly", "thermo", "driver_panel", "deletion", "damage", "fatigue",
+                "multimesh"}
     plate = prob if isinstance(prob, Problem) else None
     if legs & {"reassembly", "thermo", "deletion"} and plate is None:
         plate = Problem(100, 46, 4, 20, min(R, 8), "plate")
@@ -1058,15 +1200,12 @@ def main():
         res["deletion"] = deletion_leg(plate)
     if "damage" in legs:
         res["damage"] = damage_leg(args.damage_nip)
+    if "multimesh" in legs:
+        res["multimesh"] = multimesh_leg()
+        print(f"# multimesh: {res['multimesh']}", file=sys.stderr, flush=True)
     if "fatigue" in legs:
         res["fatigue"] = fatigue_leg(args.fatigue_blocks, args.damage_nip)
         print(f"# fatigue: {res['fatigue']}", file=sys.stderr, flush=True)
-    # per-kernel durations: the batched FP32-IR steps once more, every launch of the
-    # operator and sweep kernels bracketed by HIP events on the launching stream
-    prof = kernel_profile(run32, K, mask=0b11)
-
-    value = K * R / t32
-    dom_name = max(prof, key=lambda k: prof[k]["ms"])
     out = {
         "metric": "slab solves/s at rel.res 1e-8 (FP32-IR batched vs FP64 GPU); Kron-SpMM GB/s",
         "value": value,
@@ -1074,15 +1213,15 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": 1e3 * t32 / K,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 operator/factor sweeps in f64 GMRES-IR",
         "data": "synthetic (structured mesh, seeded load cases)",
         "config": {"workload": prob.describe() + f"; GMRES(30) eps 1e-8, forward-error test "
-                               f"{args.xtol:g}; exact banded factors ({pr.nblocks} blocks of "
-                               f"{pr.block}), {len(sinfo['shifts'])} shifted",
+                               f"{args.xtol:g}; exact banded factors ({nblocks} blocks of "
+                               f"{block}), {len(sinfo['shifts'])} shifted",
                    "cases": R, "p": prob.p},
         "all_converged": ok32,
         "relres_max": float(rr32.max()),
@@ -1092,11 +1231,10 @@ def main():
         "solver_setup": sinfo,
         "kron_spmm": spmm,
         "kernels": prof,
-        "roofline": roofline(dom_name, prof[dom_name], prob, pr),
+        "roofline": roof,
         "setup_s": {"assembly": t_asm},
     }
     if not args.no_cpu:
-        del run32
         out["cpu_baseline"] = run_cpu_baseline(args, prob)
     print(json.dumps(out))
 

@@ -440,13 +440,21 @@ int xt_graphs_enable(int on);
  *   XT_TUNE_BAND_TILE     banded-factor sweeps, batched passes: tile rows       64, 128
  *                         (128: FP32 with block rows a multiple of 128)
  *   XT_TUNE_GMRES_CHECK   slab solve: Arnoldi steps between host convergence checks 1, 2, 4
+ *   XT_TUNE_SPMM_LDS      node-block SpMM, several load cases: the LDS-staged kernel
+ *                         (neighbour rows of X staged per slice) off 1, on 2 (where the
+ *                         neighbour rows fit), built-in: off
+ *   XT_TUNE_BAND_SMALLC   banded-factor sweeps with at most 16 columns: VALU rows kernel 1,
+ *                         matrix-core kernels 2; built-in: rows kernel below 1024-row
+ *                         blocks
  * XT_E_ARG for a value a knob does not take. */
 #define XT_TUNE_SPMM_UNR 0
 #define XT_TUNE_SPMM_WPB 1
 #define XT_TUNE_SPMM_R1SPLIT 2
 #define XT_TUNE_BAND_TILE 3
 #define XT_TUNE_GMRES_CHECK 4
-#define XT_TUNE_NKNOBS 5
+#define XT_TUNE_SPMM_LDS 5
+#define XT_TUNE_BAND_SMALLC 6
+#define XT_TUNE_NKNOBS 7
 int xt_tune_set(int knob, int value);
 /* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
 int xt_tune_get(int knob);

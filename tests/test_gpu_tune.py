@@ -21,7 +21,7 @@ def dev():
         tune.set_knob(k, 0)
 
 
-@pytest.mark.parametrize("R", [1, 5, 32])
+@pytest.mark.parametrize("R", [1, 5, 8, 32])
 def test_spmm_variants_match_oracle(dev, oracle_lib, R):
     from xtfem_amd import slab, tune
     S = structured_system(12, 5, 4, 24.0, 10.0, 8.0)
@@ -36,13 +36,15 @@ def test_spmm_variants_match_oracle(dev, oracle_lib, R):
     for unr in (2, 4, 8):
         for wpb in (1, 2, 4):
             for spl in splits:
-                tune.set_knob("spmm_unr", unr)
-                tune.set_knob("spmm_wpb", wpb)
-                tune.set_knob("spmm_r1split", spl)
-                for fp32, tol in ((False, 1e-12), (True, 2e-5)):
-                    Y = op.apply_host(A, B, X, fp32=fp32)
-                    for r in range(R):
-                        assert rel(Y[r], ref[r]) < tol, (unr, wpb, spl, fp32, r)
+                for lds in ((0,) if R == 1 else (1, 2)):
+                    tune.set_knob("spmm_unr", unr)
+                    tune.set_knob("spmm_wpb", wpb)
+                    tune.set_knob("spmm_r1split", spl)
+                    tune.set_knob("spmm_lds", lds)
+                    for fp32, tol in ((False, 1e-12), (True, 2e-5)):
+                        Y = op.apply_host(A, B, X, fp32=fp32)
+                        for r in range(R):
+                            assert rel(Y[r], ref[r]) < tol, (unr, wpb, spl, lds, fp32, r)
     for k in tune.KNOBS:
         tune.set_knob(k, 0)
 

@@ -48,7 +48,7 @@ constexpr int kLevels = 4;     // pairing levels of the block recurrences, at mo
 constexpr int kMinChain = 6;   // a level is added while its chain keeps this many steps
 constexpr int kKey = 6 + 2 * kLevels;
 struct SweepGraph {
-    int c = 0, pi = -1, tile = 0;   // column count, precision, tile knob at capture
+    int c = 0, pi = -1, tile = 0, smallc = 0;   // column count, precision, knobs at capture
     const void *key[kKey] = {};
     hipGraphExec_t exec = nullptr;
 };
@@ -93,6 +93,7 @@ struct BandPrec {
 // of the 256 threads are summed through LDS: 16 threads per output add 16 partials
 // each, then a 16-lane butterfly (one value per lane).  Grid x = b / RW.
 constexpr size_t kRowsLdsMax = 96 * 1024;
+constexpr int kRowsMaxB = 1024;   // block rows from which few columns go to matrix cores
 
 template <typename T, int CM, int RW>
 __global__ void __launch_bounds__(256)
@@ -431,6 +432,10 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
     const int b = bp.b;
     const bool prof = prof_on(XT_PROF_BAND);
     const size_t vlds = sizeof(T) * (size_t)b * c;
+    // few columns: the VALU rows kernel stages V once per RW rows, which for large blocks
+    // re-reads V from L2 b / RW times per block; the matrix-core kernels read it once per
+    // tile and win there (the knob overrides)
+    const int sk = tune_knob(XT_TUNE_BAND_SMALLC);
 #define XT_ROWS(CM, RW)                                                                        \
     if (prof) prof_begin(XT_PROF_BAND, s);                                                     \
     hipLaunchKernelGGL((band_rows_kernel<T, CM, RW>), dim3(b / RW, 1, nblk), dim3(256), vlds, s, \
@@ -438,10 +443,9 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
     if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s)
     if (vlds > kRowsLdsMax) {
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
-    } else if (c <= 8) {
-        XT_ROWS(8, 2);
-    } else if (c <= 16) {
-        XT_ROWS(16, 1);
+    } else if (c <= 16 && (sk == 1 || (sk == 0 && b < kRowsMaxB))) {
+        if (c <= 8) XT_ROWS(8, 2);
+        else XT_ROWS(16, 1);
     } else {
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
     }
@@ -1051,7 +1055,9 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
         SweepGraph *g = nullptr;
         for (SweepGraph &cand : bp->graph)
             if (cand.exec && cand.c == c && cand.pi == pi &&
-                cand.tile == tune_knob(XT_TUNE_BAND_TILE) && std::equal(key, key + kKey, cand.key)) {
+                cand.tile == tune_knob(XT_TUNE_BAND_TILE) &&
+                cand.smallc == tune_knob(XT_TUNE_BAND_SMALLC) &&
+                std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
                 break;
             }
@@ -1079,6 +1085,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             g->c = c;
             g->pi = pi;
             g->tile = tune_knob(XT_TUNE_BAND_TILE);
+            g->smallc = tune_knob(XT_TUNE_BAND_SMALLC);
             std::copy(key, key + kKey, g->key);
         }
         XT_HIP(hipGraphLaunch(g->exec, s));

@@ -44,6 +44,12 @@ struct BsrPlan {
     DevBuf bval32, bval64;  // [nblocks][12]
     DevBuf kmap, mmap;      // value position in bval of every CSR entry of K and of M
     long long nk = 0, nm = 0;
+    // Staged form: the distinct column nodes of each slice (its neighbour set, sorted),
+    // unode[ubase[s] .. ubase[s] + ulen[s]), and for every stored block the position of
+    // its column node in that list (uint16), so a workgroup can stage the slice's rows
+    // of X in LDS once and read them from there.
+    DevBuf unode, ubase, ulen, lidx;
+    int umax = 0;           // longest neighbour set (0: no staged form)
 };
 
 static constexpr int kSlice = 64;
@@ -151,6 +157,35 @@ int bsr_build(int p, const int *krp, const int *kci, const double *kv, const int
             }
             for (int b : c) pos[b] = -1;
         }
+    // staged form: per slice the sorted distinct column nodes and each block's position
+    {
+        std::vector<int> unode, ub(nslices), ul(nslices);
+        std::vector<unsigned short> li(nb);
+        int umax = 0;
+        bool ok = true;
+        std::vector<int> tmp;
+        for (int sl = 0; sl < nslices && ok; ++sl) {
+            const long long b0 = base[sl], b1 = b0 + (long long)len[sl] * kSlice;
+            tmp.assign(bcol.begin() + b0, bcol.begin() + b1);
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            if (tmp.size() > 65535) { ok = false; break; }
+            ub[sl] = (int)unode.size();
+            ul[sl] = (int)tmp.size();
+            umax = std::max(umax, (int)tmp.size());
+            for (long long i = b0; i < b1; ++i)
+                li[i] = (unsigned short)(std::lower_bound(tmp.begin(), tmp.end(), bcol[i]) -
+                                         tmp.begin());
+            unode.insert(unode.end(), tmp.begin(), tmp.end());
+        }
+        if (ok && !unode.empty()) {
+            XT_TRY(P.unode.upload(unode.data(), sizeof(int) * unode.size()));
+            XT_TRY(P.ubase.upload(ub.data(), sizeof(int) * nslices));
+            XT_TRY(P.ulen.upload(ul.data(), sizeof(int) * nslices));
+            XT_TRY(P.lidx.upload(li.data(), sizeof(unsigned short) * nb));
+            P.umax = umax;
+        }
+    }
     P.nn = nn;
     P.nslices = nslices;
     P.nblocks = nb;
@@ -298,6 +333,119 @@ kron_bsr3_kernel(int nslices, const int *__restrict__ slot_node,
             out[o] = sacc;
         }
         T *yr = y + (size_t)(3 * node + i) * c + rc;
+#pragma unroll
+        for (int o = 0; o < 6; o += 2) {
+            V2 v;
+            if (beta == T(0)) {
+                v.x = alpha * out[o];
+                v.y = alpha * out[o + 1];
+            } else {
+                const V2 old = *reinterpret_cast<const V2 *>(yr + o);
+                v.x = alpha * out[o] + beta * old.x;
+                v.y = alpha * out[o + 1] + beta * old.y;
+            }
+            *reinterpret_cast<V2 *>(yr + o) = v;
+        }
+    }
+}
+
+// Staged variant for many load cases: one workgroup per (slice, chunk of CC columns =
+// CC/6 load cases).  The workgroup first copies the rows of X of the slice's neighbour
+// nodes, restricted to its columns, into LDS (16-byte loads; each row of X is read once
+// per slice instead of once per neighbouring node row), then thread (slot k, case rl)
+// walks the slot's entries reading the block values from memory (consecutive slots
+// consecutive) and the neighbour's X rows from LDS through the block's position in the
+// neighbour list.  Same products and summation order as kron_bsr3_kernel.
+template <typename T, int CC>
+__global__ void __launch_bounds__(64 * (CC / 6))
+kron_bsr3_lds_kernel(int nslices, int nch, const int *__restrict__ slot_node,
+                     const int *__restrict__ slice_base, const int *__restrict__ slice_len,
+                     const int *__restrict__ unode, const int *__restrict__ ubase,
+                     const int *__restrict__ ulen, const unsigned short *__restrict__ lidx,
+                     const T *__restrict__ bval, int c, const T *__restrict__ tmat, T alpha,
+                     T beta, const T *__restrict__ x, T *__restrict__ y) {
+    using V2 = typename Vec2<T>::type;
+    constexpr int VEC = 16 / sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *xs = reinterpret_cast<T *>(smem_raw);          // [u][3][CC]
+    const long long blk = xcd_block(blockIdx.x, gridDim.x);
+    const int s = (int)(blk / nch), ch = (int)(blk - (long long)s * nch);
+    if (s >= nslices) return;                           // (grid padded to a multiple of 8)
+    const int col0 = ch * CC;
+    const int U = ulen[s];
+    const int *un = unode + ubase[s];
+    constexpr int RPN = 3 * CC / VEC;                   // vector loads per neighbour node
+    for (int t = threadIdx.x; t < U * RPN; t += blockDim.x) {
+        const int u = t / RPN, v = t - u * RPN;
+        const int d = v / (CC / VEC), cv = v - d * (CC / VEC);
+        *reinterpret_cast<vec_t *>(xs + ((size_t)u * 3 + d) * CC + cv * VEC) =
+            *reinterpret_cast<const vec_t *>(x + ((size_t)un[u] * 3 + d) * c + col0 + cv * VEC);
+    }
+    __syncthreads();
+    const int k = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    const int r = ch * (CC / 6) + rl;
+    const int node = slot_node[(size_t)s * kSlice + k];
+    const int L = slice_len[s];
+    const long long b0 = slice_base[s];
+    T aK[3][6], aM[3][6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) { aK[i][j] = T(0); aM[i][j] = T(0); }
+    constexpr int UNR = 4;
+    int e = 0;
+    auto step = [&](const T (&kv)[12], int li) {
+        T xv[3][6];
+        const T *xr = xs + (size_t)li * 3 * CC + rl * 6;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int j = 0; j < 6; j += 2) {
+                const V2 v = *reinterpret_cast<const V2 *>(xr + d * CC + j);
+                xv[d][j] = v.x;
+                xv[d][j + 1] = v.y;
+            }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                aK[i][j] += kv[3 * i + 0] * xv[0][j] + kv[3 * i + 1] * xv[1][j] + kv[3 * i + 2] * xv[2][j];
+                aM[i][j] += kv[9 + i] * xv[i][j];
+            }
+    };
+    for (; e + UNR <= L; e += UNR) {
+        T kv[UNR][12];
+        int li[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const long long idx = b0 + (long long)(e + u) * kSlice + k;
+            li[u] = lidx[idx];
+            load_block(bval + idx * 12, kv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) step(kv[u], li[u]);
+    }
+    for (; e < L; ++e) {
+        const long long idx = b0 + (long long)e * kSlice + k;
+        T kv[12];
+        load_block(bval + idx * 12, kv);
+        step(kv, lidx[idx]);
+    }
+    if (node < 0) return;
+    const T *TA = tmat + (size_t)(r * 2 + 0) * 36;
+    const T *TB = tmat + (size_t)(r * 2 + 1) * 36;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T out[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) {
+            T sacc = T(0);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) sacc += TA[o * 6 + j] * aK[i][j] + TB[o * 6 + j] * aM[i][j];
+            out[o] = sacc;
+        }
+        T *yr = y + (size_t)(3 * node + i) * c + r * 6;
 #pragma unroll
         for (int o = 0; o < 6; o += 2) {
             V2 v;
@@ -479,11 +627,58 @@ static int spmm_unr() {
     return k ? k : (sizeof(T) == 4 ? 4 : 2);
 }
 
+// LDS budget of one staged workgroup (two fit a CU's 160 KB).
+constexpr size_t kStageLds = 64 * 1024;
+
+// Columns per staged workgroup: the largest of 48, 24, 12 (whole load cases, 16-byte
+// rows) that divides c and whose neighbour rows fit kStageLds; 0 when none does.
+template <typename T>
+static int stage_cols(const BsrPlan &P, int c) {
+    if (!P.umax) return 0;
+    for (int cc : {48, 24, 12})
+        if (c % cc == 0 && (size_t)P.umax * 3 * cc * sizeof(T) <= kStageLds) return cc;
+    return 0;
+}
+
 template <typename T>
 int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta, const T *x,
               T *y, bool f32, hipStream_t s) {
     const T *bv = f32 ? (const T *)P.bval32.ptr : (const T *)P.bval64.ptr;
     const int c = 6 * R;
+    // staged kernel: only when the tuning knob asks for it (2; measured slower than the
+    // lane-per-case kernel on the structured meshes here at 32 load cases)
+    const int lk = tune_knob(XT_TUNE_SPMM_LDS);
+    const int cc = (lk != 2 || R < 2) ? 0 : stage_cols<T>(P, c);
+    if (cc) {
+        const int nch = c / cc;
+        const int grid = (int)(((long long)P.nslices * nch + 7) / 8 * 8);
+        const size_t lds = (size_t)P.umax * 3 * cc * sizeof(T);
+        if (prof_on(XT_PROF_SPMM)) prof_begin(XT_PROF_SPMM, s);
+#define XT_BSR_LDS(CCV)                                                                        \
+    hipLaunchKernelGGL((kron_bsr3_lds_kernel<T, CCV>), dim3(grid), dim3(64 * (CCV / 6)), lds, s, \
+                       P.nslices, nch, P.slot_node.as<int>(), P.slice_base.as<int>(),          \
+                       P.slice_len.as<int>(), P.unode.as<int>(), P.ubase.as<int>(),            \
+                       P.ulen.as<int>(), P.lidx.as<unsigned short>(), bv, c, tmat, (T)alpha,   \
+                       (T)beta, x, y)
+        if (cc == 48) XT_BSR_LDS(48);
+        else if (cc == 24) XT_BSR_LDS(24);
+        else XT_BSR_LDS(12);
+#undef XT_BSR_LDS
+        if (prof_on(XT_PROF_SPMM)) {
+            // as the lane-per-case kernel's: the stored format read once, X read once,
+            // Y written (and read when beta != 0); the neighbour lists (2 B per block
+            // in place of the 4-byte column index, 4 B per listed node) are counted
+            const double sv = sizeof(T);
+            const double vec = (double)P.nn * 3 * c * sv;
+            const double bytes = (double)P.nblocks * (12 * sv + 2) +
+                                 (double)P.nslices * (kSlice * 4 + 16) +
+                                 (double)(P.unode.bytes) + vec * (beta != 0 ? 3 : 2);
+            const double flops = 2.0 * 12 * P.nreal * c + 2.0 * 2 * 6 * (3.0 * P.nn) * c;
+            prof_end(XT_PROF_SPMM, s, bytes, flops);
+        }
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
     for (int r0 = 0; r0 < R; r0 += 64) {
         const int Rc = std::min(64, R - r0);
         int lpn = 1;

@@ -928,11 +928,19 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                 if (itv[r] >= maxits || nr == 0.0) continue;
                 if (prev[r] > 0.0) {
                     const double rho = nr / prev[r];
-                    if (rho > 0.5 && fp64_prec) fb[r] = 1;
-                    else if (rho > 10.0 * irel[r]) irel[r] = std::min(0.1, rho);
+                    if (rho > 0.5 && fp64_prec && !fb[r]) {
+                        fb[r] = 1;
+                        irel[r] = 1.0e-3;
+                    } else if (!fb[r] && rho > 10.0 * irel[r]) {
+                        irel[r] = std::min(0.1, rho);   // the FP32 floor
+                    }
                 }
                 prev[r] = nr;
-                e1[r] = nr <= eps1[r] ? irel[r] * nr : std::max(irel[r] * nr, 0.5 * eps1[r]);
+                // FP64 cycles after a fallback aim at the final target at once (one long
+                // Krylov cycle converges faster than several restarted ones)
+                e1[r] = nr <= eps1[r] ? irel[r] * nr
+                        : fb[r]       ? 0.5 * eps1[r]
+                                      : std::max(irel[r] * nr, 0.5 * eps1[r]);
                 if (fb[r]) { act64[r] = 1; any64 = true; }
                 else { act32[r] = 1; any32 = true; }
             }
@@ -945,12 +953,15 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             std::vector<int> cin(S.I, 0);
             // the first inner call starts from the norms just computed (V_0 unchanged)
             if (any32) {
-                constexpr int kFp32Cycle = 10;
+                constexpr int kFp32Cycle = 4;
                 const std::vector<int> was32 = act32;   // the call clears the active flags
                 XT_TRY(S.gmres(x, true, e1, maxits, act32, itv, cin, ro_last, 1, true, true, &rn,
                                kFp32Cycle));
                 for (int r = 0; r < S.I; ++r)
-                    if (was32[r] && !cin[r] && fp64_prec) fb[r] = 1;
+                    if (was32[r] && !cin[r] && fp64_prec && !fb[r]) {
+                        fb[r] = 1;
+                        irel[r] = 1.0e-3;   // FP64 cycles reach the full reduction
+                    }
             }
             if (any64)
                 XT_TRY(S.gmres(x, false, e1, maxits, act64, itv, cin, ro_last, 1, true, false,

@@ -81,6 +81,8 @@ int xt_tune_set(int knob, int value) {
         {0, 4, 8, 4},      // XT_TUNE_SPMM_R1SPLIT
         {0, 64, 128, 64},  // XT_TUNE_BAND_TILE
         {0, 1, 2, 4},      // XT_TUNE_GMRES_CHECK
+        {0, 1, 2, 1},      // XT_TUNE_SPMM_LDS
+        {0, 1, 2, 1},      // XT_TUNE_BAND_SMALLC
     };
     bool ok = false;
     for (int v : allowed[knob]) ok = ok || v == value;

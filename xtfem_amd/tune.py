@@ -5,8 +5,10 @@ For a given operator, preconditioner and load-case count R it times, with HIP ev
 the launching stream (the library's per-launch profiler), the candidates of
 
 * the node-block Kron-SpMM: entries in flight per pass (2, 4, 8), waves per workgroup
-  (1, 2, 4) and, for one load case, lanes per node row (4, 8);
-* the banded-factor sweeps: the tile rows of the batched passes (64, 128 in FP32);
+  (1, 2, 4), for one load case lanes per node row (4, 8), for several the kernel that
+  stages each slice's neighbour rows of X in LDS (off, on);
+* the banded-factor sweeps: the tile rows of the batched passes (64, 128 in FP32), and
+  for a few columns the VALU rows kernel or the matrix-core kernels;
 * the slab solve: Arnoldi steps between host convergence checks (1, 2, 4), timed on a
   whole solve when a right-hand side is given;
 
@@ -26,7 +28,8 @@ import torch
 
 from ._lib import check, lib
 
-KNOBS = {"spmm_unr": 0, "spmm_wpb": 1, "spmm_r1split": 2, "band_tile": 3, "gmres_check": 4}
+KNOBS = {"spmm_unr": 0, "spmm_wpb": 1, "spmm_r1split": 2, "band_tile": 3, "gmres_check": 4,
+         "spmm_lds": 5, "band_smallc": 6}
 XT_PROF_SPMM, XT_PROF_BAND = 0, 1
 _CACHE: dict = {}
 
@@ -124,7 +127,7 @@ def autotune(op, pr, R, fp32=True, reps=7, solve=None):
     out = {"signature": key, "picks": {}, "timings_ms": {}}
     if op.format == "blocks":
         for name, cands in (("spmm_unr", (2, 4, 8)), ("spmm_wpb", (1, 2, 4))) + \
-                ((("spmm_r1split", (4, 8)),) if R == 1 else ()):
+                ((("spmm_r1split", (4, 8)),) if R == 1 else (("spmm_lds", (1, 2)),)):
             best, res = _pick(name, cands, XT_PROF_SPMM, spmm, reps)
             out["picks"][name] = best
             out["timings_ms"][name] = res
@@ -136,6 +139,13 @@ def autotune(op, pr, R, fp32=True, reps=7, solve=None):
         best, res = _pick("band_tile", cands, XT_PROF_BAND, sweep, max(3, reps // 2))
         out["picks"]["band_tile"] = best
         out["timings_ms"]["band_tile"] = res
+        # few columns (a shifted factor's group, one load case): rows kernel or matrix cores
+        xs = x[:p * q].contiguous()                 # one load case: q columns
+        ys = torch.empty_like(xs)
+        sweep_small = lambda: pr.apply(None, xs, ys, 1, q=q, fp32=fp32)
+        best, res = _pick("band_smallc", (1, 2), XT_PROF_BAND, sweep_small, max(3, reps // 2))
+        out["picks"]["band_smallc"] = best
+        out["timings_ms"]["band_smallc"] = res
     if solve is not None:
         res = {}
         for v in (1, 2, 4):

@@ -484,24 +484,29 @@ def fatigue_leg(blocks=40, nip=2560, amp=250.0):
 
 
 def multimesh_meshes(nmesh=64, target_dof=6.0e6):
-    """The uneven multi-mesh batch: nmesh prismatic rods (n x n cross-section, n in 2..8)
-    and plates (width 8..20, thickness 2..3 elements), long along x, element counts drawn
+    """The uneven multi-mesh batch: nmesh prismatic rods (n x n cross-section, length
+    6-14 sides) and plates (length 10-25 thicknesses, 2-6 widths), element counts drawn
     log-uniform in [2k, 400k] (seeded) and scaled by one common factor so that the batch
-    holds about target_dof spatial DOFs; each clamped at x = 0 with an axial traction on
-    its far end.  Returns [(dims, kind)] and the scale factor."""
+    holds about target_dof spatial DOFs; each clamped at x = 0, its far end x = L held
+    transversely (free to stretch) and loaded by an axial traction.  The length bounds
+    and the guided far end keep every stiffness matrix's condition number below about
+    1e7 (a cantilever 10^4 sides long reaches 10^16, beyond FP64).  Returns
+    [(dims, kind)] sorted by node-slice size (x is the slowest node axis, so the
+    bandwidth grows with the (ny+1)(nz+1) slice) and the scale factor."""
     rng = np.random.default_rng(SEED + 5)
     ne = np.exp(rng.uniform(np.log(2e3), np.log(4e5), nmesh))
     kind = rng.integers(0, 2, nmesh)
-    sec = rng.integers(2, 9, nmesh)
-    width = rng.integers(8, 21, nmesh)
-    thick = rng.integers(2, 4, nmesh)
+    slender = np.where(kind == 0, rng.uniform(6.0, 14.0, nmesh), rng.uniform(10.0, 25.0, nmesh))
+    aspect = rng.uniform(2.0, 6.0, nmesh)
 
     def dims(i, f):
         e = ne[i] * f
         if kind[i] == 0:
-            n = int(sec[i])
+            n = max(2, int(round((e / slender[i]) ** (1.0 / 3.0))))
             return (max(4, int(round(e / (n * n)))), n, n)
-        ny, nz = int(width[i]), int(thick[i])
+        L = (e * aspect[i] * slender[i]) ** (1.0 / 3.0)
+        nz = max(2, int(round(L / slender[i])))
+        ny = max(nz, int(round(L / aspect[i])))
         return (max(4, int(round(e / (ny * nz)))), ny, nz)
 
     def total(f):
@@ -512,12 +517,15 @@ def multimesh_meshes(nmesh=64, target_dof=6.0e6):
         mid = np.sqrt(lo * hi)
         lo, hi = (mid, hi) if total(mid) < target_dof else (lo, mid)
     f = hi if total(hi) <= 1.02 * target_dof else lo
-    return [(dims(i, f), "rod" if kind[i] == 0 else "plate") for i in range(nmesh)], f
+    out = [(dims(i, f), "rod" if kind[i] == 0 else "plate") for i in range(nmesh)]
+    out.sort(key=lambda t: ((t[0][1] + 1) * (t[0][2] + 1), t[0][0]))
+    return out, f
 
 
 def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
-    """The uneven multi-mesh batch: every mesh's K and M stacked block-diagonally, one
-    banded factor (FP32), one batched FP32-IR slab solve of nmesh x R instances with
+    """The uneven multi-mesh batch: every mesh's K and M stacked block-diagonally, a
+    banded factor per run of meshes of similar bandwidth (FP32, xt_prec_create_band_seg),
+    one batched FP32-IR slab solve of nmesh x R instances with
     per-instance convergence (xt_slab_solve_seg), timed over K solves; against the same
     meshes solved one at a time in FP64 on the GPU (each with its own operator and FP64
     banded factor, R load cases per solve)."""
@@ -534,6 +542,9 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
     for (nx, ny, nz), kind in spec:
         mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz))
         bc = fem.clamp_mask(mesh, 0, [0.0])
+        far = np.flatnonzero(mesh.xcoord[:, 0] == float(nx))
+        bc[3 * far + 1] = 1                    # far end guided: y, z held, x free
+        bc[3 * far + 2] = 1
         asm = device.Assembly(mesh, bc)
         asm.run(2.0e5, 0.3, 7.85e-9)
         Kb, Mb = asm.host()
@@ -547,7 +558,9 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
     t_mesh = time.perf_counter() - t0
     t0 = time.perf_counter()
     op = device.KronOperator([Kc, Mc], q=6)
-    pr = device.BandPreconditioner(Kc, fp32_only=True)
+    pr = device.BandPreconditioner(Kc, fp32_only=True, seg=seg)
+    groups = dict(runs=len(pr.group_blocks), blocks=[int(v) for v in pr.group_blocks],
+                  factor_gb=pr.matrix_bytes(True) / 1e9)
     b = torch.cat([device.to_interleaved(s["rhs"], s["p"], 6) for s in systems]).contiguous()
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
@@ -601,6 +614,7 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
                 all_converged=bool(np.all(ierr == 0) and np.all(relres <= 1e-8)),
                 relres_max=float(relres.max()),
                 rel_err_vs_fp64_max=float(max(err)),
+                banded_factor=groups,
                 setup_s=dict(meshes=t_mesh, operator_and_factor=t_setup))
 
 

@@ -231,6 +231,23 @@ int xt_prec_create_band_shifted(int p, const int *krow, const int *kcol, const d
                                 const int *mrow, const int *mcol, const double *mval,
                                 int nshift, const double *shifts, int block, int flags,
                                 xt_prec *out);
+/* Exact factor K^-1 of a block-diagonal K whose diagonal blocks are nseg independent
+ * systems (meshes) on the consecutive DOF ranges seg_dof[s]..seg_dof[s+1]-1 (seg_dof
+ * [nseg+1], seg_dof[0] = 0, seg_dof[nseg] = p; 1-based CSR on the host, no entry may
+ * couple two ranges: XT_E_ARG).  The ranges are grouped into runs of consecutive
+ * systems and each run gets a banded factor of its own, with the block size of its
+ * widest band, so a batch of uneven meshes does not pad every mesh to the widest band
+ * of the batch (order the systems by bandwidth for the fewest, tightest runs).  The
+ * runs are chosen to minimise the factor bytes an apply streams plus a per-run launch
+ * cost.  flags: XT_BAND_FP32_ONLY as for xt_prec_create_band_shifted.  The temporal
+ * split (shifted factors) and xt_prec_refactor_band are not available for it.  No
+ * counterpart in the reference, which factors one mesh per process (mumps.f90:22-145). */
+int xt_prec_create_band_seg(int p, const int *rowptr, const int *colidx, const double *vals,
+                            int nseg, const int *seg_dof, int block, int flags, xt_prec *out);
+/* Runs of a preconditioner from xt_prec_create_band_seg (1 for any other banded factor,
+ * 0 for ILUT): ngroup, and when non-NULL the first DOF of each run and the end
+ * (goff [ngroup+1]) and each run's block size (gblock [ngroup]). */
+int xt_prec_band_groups(xt_prec pr, int *ngroup, int *goff, int *gblock);
 /* Number of shifted factors of a preconditioner and (when shifts != NULL) their shifts. */
 int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts);
 /* Refactor a banded-factor preconditioner with new values of K (1-based CSR in device
@@ -238,7 +255,8 @@ int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts);
  * per-slab refactorization mumps_solver(job = 1), main3d.f90:783. */
 int xt_prec_refactor_band(xt_prec pr, const int *rowptr, const int *colidx,
                           const double *vals, void *stream);
-/* Block size, number of blocks and bandwidth of a banded-factor preconditioner. */
+/* Block size, number of blocks and bandwidth of a banded-factor preconditioner (of a
+ * grouped one: the largest block, the blocks of all runs, the largest bandwidth). */
 int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth);
 /* Sequential steps of the two sweeps: ILUT levels, or blocks of a banded factor. */
 int xt_prec_levels(xt_prec pr, int *nlev_lower, int *nlev_upper);
@@ -446,6 +464,9 @@ int xt_graphs_enable(int on);
  *   XT_TUNE_BAND_SMALLC   banded-factor sweeps with at most 16 columns: VALU rows kernel 1,
  *                         matrix-core kernels 2; built-in: rows kernel below 1024-row
  *                         blocks
+ *   XT_TUNE_IR_FP32_CYCLE GMRES-IR: iterations an FP32 inner cycle gets to meet its target
+ *                         before the case moves to FP64 cycles              1, 2, 4, 8
+ *                         (built-in 2)
  * XT_E_ARG for a value a knob does not take. */
 #define XT_TUNE_SPMM_UNR 0
 #define XT_TUNE_SPMM_WPB 1
@@ -454,7 +475,8 @@ int xt_graphs_enable(int on);
 #define XT_TUNE_GMRES_CHECK 4
 #define XT_TUNE_SPMM_LDS 5
 #define XT_TUNE_BAND_SMALLC 6
-#define XT_TUNE_NKNOBS 7
+#define XT_TUNE_IR_FP32_CYCLE 7
+#define XT_TUNE_NKNOBS 8
 int xt_tune_set(int knob, int value);
 /* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
 int xt_tune_get(int knob);

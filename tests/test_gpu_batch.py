@@ -82,3 +82,64 @@ def test_segmented_batch_matches_single_meshes(dev, oracle_lib, mode, xtol):
             assert ie == 0 and rel(xs[r], xo) <= 1e-6, (s, r, rel(xs[r], xo))
             assert abs(int(its[s, r]) - int(its1[r])) <= 1, (s, r, its[s, r], its1[r])
             assert rel(xs[r], x1[r]) <= 1e-7, (s, r)
+
+
+@pytest.mark.parametrize("mode,eps,xtol", [(1, 1e-8, 1e-7), (0, 1e-10, 0.0)])
+def test_grouped_band_factor_uneven_meshes(dev, oracle_lib, mode, eps, xtol):
+    """xt_prec_create_band_seg on meshes of uneven bandwidth (a thin rod, a 5 x 5 rod, a
+    plate whose node slices hold 287 nodes): the narrow meshes share a run with a small
+    block, the plate gets its own; each instance must reach the true residual eps (FP64
+    GMRES: 1e-10, since a residual of 1e-8 leaves the plate's displacement about 2e-6
+    from the solution; FP32-IR: 1e-8 with the forward-error test) and match the oracle's
+    FP64 kronpgmres (exact K^-1 through LAPACK) on its own mesh to 1e-6, and the whole
+    batch must agree with the same solve preconditioned by one banded factor padded to
+    the plate's block."""
+    import torch
+    from xtfem_amd import fem, slab
+    ms = []
+    for nx, ny, nz in ((9, 2, 2), (48, 5, 5), (30, 40, 6)):   # x slowest: slices (ny+1)(nz+1)
+        mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz))
+        K, M, *_ = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+        bc = fem.clamp_mask(mesh, 0, [0.0])
+        rext = fem.traction_load(mesh, 0, float(nx), 0, float(ny * nz))
+        ms.append(dict(p=3 * mesh.nn, Kb=fem.applybc(K, bc), Mb=fem.applybc(M, bc), bc=bc,
+                       rext=rext))
+    R = 2
+    freq, amp = np.array([25.0, 70.0]), np.array([90.0, 45.0])
+    dt = np.round(0.5 * freq) / freq
+    w = 2 * np.pi * freq
+    c = slab.stcoeff_m(0.0, dt, w)
+    A, B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+    fx, _ = slab.getfextc_m(0.0, w, dt)
+    rhs = [np.array([slab.slab_rhs(m["p"], fx[r] * amp[r], None, m["rext"],
+                                   fixed=np.tile(m["bc"], 6)) for r in range(R)]) for m in ms]
+    Kc, seg = dev.block_diagonal([m["Kb"] for m in ms])
+    Mc, _ = dev.block_diagonal([m["Mb"] for m in ms])
+    op = dev.KronOperator([Kc, Mc], q=6)
+    pr = dev.BandPreconditioner(Kc, seg=seg, fp32_only=(mode == 1))
+    assert list(pr.group_offsets) == [0, int(seg[2]), int(seg[3])], pr.group_offsets
+    assert list(pr.group_blocks) == [192, 896], pr.group_blocks
+    assert pr.block == 896
+    prw = dev.BandPreconditioner(Kc)                 # one factor, every mesh at block 896
+    assert prw.block == 896
+    b = torch.cat([dev.to_interleaved(rh, m["p"], 6) for rh, m in zip(rhs, ms)]).contiguous()
+    x = torch.zeros_like(b)
+    x, its, ierr, relres = dev.slab_solve_seg(op, pr, seg, A, B, b, x, R, eps=eps,
+                                              maxits=300, mode=mode, xtol=xtol)
+    xw = torch.zeros_like(b)
+    xw, itsw, ierrw, _ = dev.slab_solve_seg(op, prw, seg, A, B, b, xw, R, eps=eps,
+                                            maxits=300, mode=mode, xtol=xtol)
+    assert np.all(ierr == 0) and np.all(ierrw == 0)
+    assert float(torch.linalg.norm(x - xw) / torch.linalg.norm(xw)) <= 1e-7
+    for s, m in enumerate(ms):
+        xs = dev.from_interleaved(x[seg[s] * 6 * R:seg[s + 1] * 6 * R].contiguous(), m["p"], 6, R)
+        lf = oracle_lib.LapackBandFactor(m["Kb"])
+        for r in range(R):
+            assert relres[s, r] <= eps, (s, r, its[s, r], relres[s, r])
+            res = rhs[s][r] - oracle_lib.kronspmv(A[r], B[r], m["Kb"], m["Mb"], xs[r])
+            assert np.linalg.norm(res) <= 1.01 * eps * np.linalg.norm(rhs[s][r]), (s, r)
+            xo, ie, _, _ = oracle_lib.kronpgmres(A[r], B[r], m["Kb"], m["Mb"], lf, rhs[s][r],
+                                                 eps=1e-12, maxits=500)
+            assert ie == 0 and rel(xs[r], xo) <= 1e-6, (s, r, rel(xs[r], xo))
+            if mode == 0:                            # both exact FP64 factors
+                assert abs(int(its[s, r]) - int(itsw[s, r])) <= 1, (s, r)

@@ -437,15 +437,20 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
     // tile and win there (the knob overrides)
     const int sk = tune_knob(XT_TUNE_BAND_SMALLC);
 #define XT_ROWS(CM, RW)                                                                        \
-    if (prof) prof_begin(XT_PROF_BAND, s);                                                     \
-    hipLaunchKernelGGL((band_rows_kernel<T, CM, RW>), dim3(b / RW, 1, nblk), dim3(256), vlds, s, \
-                       b, c, tri, A, as, V, vs, add, ads, out, os);                            \
-    if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s)
+    do {                                                                                       \
+        if (prof) prof_begin(XT_PROF_BAND, s);                                                 \
+        hipLaunchKernelGGL((band_rows_kernel<T, CM, RW>), dim3(b / RW, 1, nblk), dim3(256),    \
+                           vlds, s, b, c, tri, A, as, V, vs, add, ads, out, os);               \
+        if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s);         \
+    } while (0)
     if (vlds > kRowsLdsMax) {
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
     } else if (c <= 16 && (sk == 1 || (sk == 0 && b < kRowsMaxB))) {
-        if (c <= 8) XT_ROWS(8, 2);
-        else XT_ROWS(16, 1);
+        if (c <= 8) {
+            XT_ROWS(8, 2);
+        } else {
+            XT_ROWS(16, 1);
+        }
     } else {
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
     }
@@ -996,8 +1001,11 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
         return XT_E_UNSUPPORTED;
     }
     const long long npad = (long long)nb * b * c, nreal = (long long)bp->p * c;
-    XT_TRY(bp->w1.reserve(sizeof(T) * npad));
-    XT_TRY(bp->w2.reserve(sizeof(T) * npad));
+    // sized for FP64 sweeps when the factor has them, so switching precision does not
+    // reallocate (captured graphs hold these pointers)
+    const size_t es = bp->keep64 ? sizeof(double) : sizeof(T);
+    XT_TRY(bp->w1.reserve(es * npad));
+    XT_TRY(bp->w2.reserve(es * npad));
     T *w1 = bp->w1.as<T>(), *w2 = bp->w2.as<T>();
     if (minv) XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w1, s));
     else XT_HIP(hipMemcpyAsync(w1, x, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
@@ -1048,7 +1056,10 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
         XT_TRY(chain(1, w1, H, st));
         return XT_OK;
     };
-    if (graphs_on() && !prof_on(XT_PROF_BAND)) {
+    hipStreamCaptureStatus capst = hipStreamCaptureStatusNone;
+    XT_HIP(hipStreamIsCapturing(s, &capst));
+    // (inside a caller's capture the sweeps join that graph as plain launches)
+    if (graphs_on() && !prof_on(XT_PROF_BAND) && capst == hipStreamCaptureStatusNone) {
         const void *key[kKey] = {w1, w2, Dl, Du, G, H};
         for (int dir = 0; dir < 2; ++dir)
             for (int L = 0; L < kLevels; ++L) key[6 + dir * kLevels + L] = bp->Lv[dir][L][pi].ptr;

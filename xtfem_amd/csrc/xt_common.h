@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <atomic>
 #include <vector>
 
 #include "../../include/xtfem.h"
@@ -39,6 +40,10 @@ const char *get_error();
         }                                                                               \
     } while (0)
 
+// Count of device allocations and frees made through DevBuf: a captured HIP graph holds
+// raw device pointers, so one recorded at an older count may name freed memory.
+extern std::atomic<unsigned long long> g_dev_epoch;
+
 // Owning device allocation (hipMalloc / hipFree); move-only.
 struct DevBuf {
     void *ptr = nullptr;
@@ -52,7 +57,14 @@ struct DevBuf {
         return *this;
     }
     ~DevBuf() { release(); }
-    void release() { if (ptr) (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+    void release() {
+        if (ptr) {
+            (void)hipFree(ptr);
+            ++g_dev_epoch;
+        }
+        ptr = nullptr;
+        bytes = 0;
+    }
     // Grow to at least n bytes (contents not preserved).
     int reserve(size_t n) {
         if (n <= bytes) return XT_OK;
@@ -64,6 +76,7 @@ struct DevBuf {
             set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
             return XT_E_HIP;
         }
+        ++g_dev_epoch;
         bytes = n;
         return XT_OK;
     }
@@ -192,6 +205,25 @@ struct SolverWorkspace {
     DevBuf gstate, geps, ghist;
     PinnedBuf hstage;
     PrecSplit split;                 // temporal split of a shifted-factor preconditioner
+    // Arnoldi steps replayed from HIP graphs: one per step index and variant, keyed on
+    // every buffer and handle the step's launches use; captured on the second use of a
+    // key (the first sizes the buffers), on a stream of its own
+    struct StepGraph {
+        std::vector<const void *> key;
+        int uses = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<StepGraph> steps;
+    unsigned long long step_epoch = 0;   // g_dev_epoch the cached step graphs were made at
+    hipStream_t cap = nullptr;
+    SolverWorkspace() = default;
+    SolverWorkspace(const SolverWorkspace &) = delete;
+    SolverWorkspace &operator=(const SolverWorkspace &) = delete;
+    ~SolverWorkspace() {
+        for (StepGraph &g : steps)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
 };
 SolverWorkspace *op_workspace(xt_op op);
 

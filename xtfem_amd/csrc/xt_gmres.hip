@@ -574,6 +574,112 @@ struct Solver {
         return XT_OK;
     }
 
+    // The launches of Arnoldi step i -> i+1 on stream s: w = Op P^-1 V_i (or the flexible
+    // form), CGS2 against V_0..V_i, the new norm, V_{i+1} normalised (zeroed for frozen
+    // instances), the Givens update of every instance.
+    int step_launches(int i, int K, bool flexible, bool f32, bool f32prec, int hist_cap) {
+        if (flexible) XT_TRY(op_prec_flex(i, Vj(i), Vj(i + 1), f32prec, f32));
+        else XT_TRY(op_prec(Vj(i), Vj(i + 1), f32));
+        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
+        XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
+        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>()));
+        XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
+        XT_TRY(dots(1, Vj(i + 1), Vj(i + 1), ndev.as<double>()));
+        // normalise V_{i+1} for every instance still in its cycle at this step (the
+        // restart recombination reads it); zero it for the others
+        hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R,
+                           segof, ndev.as<double>(), frozen, Vj(i + 1));
+        hipLaunchKernelGGL(givens_kernel, dim3(ceil_div(I, 64)), dim3(64), 0, s, I, i, K,
+                           hdev.as<double>(), hdev2.as<double>(), ndev.as<double>(), ghh, gcs,
+                           gsn, grs, frozen, geps.as<double>(), gits, gicyc, gro,
+                           hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+
+    // One Arnoldi step.  With graphs enabled (and no per-launch timing), the step's
+    // launches are captured into a HIP graph on the second use of the same step index,
+    // variant and buffers, and replayed from it afterwards: the dozens of launches of a
+    // step (operator, preconditioner sweeps, CGS2, Givens) cost one graph launch.  The key
+    // holds every pointer and size the launches read, so a grown buffer, another operator,
+    // preconditioner or split captures anew, and any device allocation or free since the
+    // graphs were made drops them (the key cannot list the buffers inside the operator
+    // and preconditioner handles).
+    int arnoldi_step(int i, int K, bool flexible, bool f32, bool f32prec, int hist_cap) {
+        bool plain = !graphs_on();
+        for (int k = 0; k < XT_PROF_NKINDS && !plain; ++k) plain = prof_on(k);
+        hipStreamCaptureStatus cs_ = hipStreamCaptureStatusNone;
+        if (!plain) {
+            XT_HIP(hipStreamIsCapturing(s, &cs_));
+            plain = cs_ != hipStreamCaptureStatusNone;
+        }
+        if (plain) return step_launches(i, K, flexible, f32, f32prec, hist_cap);
+        // a device buffer allocated or freed anywhere since the cached graphs were made
+        // (a preconditioner workspace grown for FP64 sweeps, a handle destroyed) may be
+        // one they name: drop them all
+        const unsigned long long epoch = g_dev_epoch.load();
+        if (epoch != ws.step_epoch) {
+            for (SolverWorkspace::StepGraph &old : ws.steps)
+                if (old.exec) (void)hipGraphExecDestroy(old.exec);
+            ws.steps.clear();
+            ws.step_epoch = epoch;
+        }
+        const std::vector<const void *> key = {
+            (const void *)op, (const void *)pr, V.ptr, Z.ptr, w32a.ptr, w32b.ptr, z64.ptr,
+            partial.ptr, hdev.ptr, hdev2.ptr, ndev.ptr, gstate.ptr, geps.ptr, ghist.ptr,
+            mi64.ptr, mi32.ptr, tm64.ptr, tm32.ptr, ws.chunks.ptr, ws.segchunk.ptr, segof,
+            split64, split32, ws.split.dcase.ptr, ws.split.m2v[0].ptr, ws.split.m2v[1].ptr,
+            (const void *)(intptr_t)i, (const void *)(intptr_t)K,
+            (const void *)(intptr_t)(flexible * 4 + f32 * 2 + f32prec),
+            (const void *)(intptr_t)n, (const void *)(intptr_t)I, (const void *)(intptr_t)nblk,
+            (const void *)(intptr_t)(hist != nullptr), (const void *)(intptr_t)hist_cap,
+            (const void *)(intptr_t)(ws.split.ns), (const void *)(intptr_t)R};
+        SolverWorkspace::StepGraph *g = nullptr;
+        for (SolverWorkspace::StepGraph &cand : ws.steps)
+            if (cand.key == key) { g = &cand; break; }
+        if (!g) {
+            if (ws.steps.size() >= 256) {   // a bounded cache: start over
+                for (SolverWorkspace::StepGraph &old : ws.steps)
+                    if (old.exec) (void)hipGraphExecDestroy(old.exec);
+                ws.steps.clear();
+            }
+            ws.steps.push_back({key, 0, nullptr});
+            g = &ws.steps.back();
+        }
+        if (!g->exec && ++g->uses < 2) return step_launches(i, K, flexible, f32, f32prec, hist_cap);
+        if (!g->exec) {
+            if (!ws.cap) XT_HIP(hipStreamCreateWithFlags(&ws.cap, hipStreamNonBlocking));
+            XT_HIP(hipStreamBeginCapture(ws.cap, hipStreamCaptureModeThreadLocal));
+            const hipStream_t saved = s;
+            s = ws.cap;
+            const int rc = step_launches(i, K, flexible, f32, f32prec, hist_cap);
+            s = saved;
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(ws.cap, &graph);
+            if (rc != XT_OK || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                if (rc != XT_OK) return rc;
+                XT_HIP(e);
+            }
+            if (g_dev_epoch.load() != ws.step_epoch) {
+                // a launch of the step allocated while it was recorded: the graph may name
+                // the memory it replaced; the recorded launches never ran, so run them
+                (void)hipGraphDestroy(graph);
+                for (SolverWorkspace::StepGraph &old : ws.steps)
+                    if (old.exec) (void)hipGraphExecDestroy(old.exec);
+                ws.steps.clear();
+                ws.step_epoch = g_dev_epoch.load();
+                return step_launches(i, K, flexible, f32, f32prec, hist_cap);
+            }
+            const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) g->exec = nullptr;
+            XT_HIP(ei);
+        }
+        XT_HIP(hipGraphLaunch(g->exec, s));
+        return XT_OK;
+    }
+
     // GMRES on the residual system already stored in V_0 (unnormalised, = r0).
     // x is updated in place.  eps1[r]: absolute target on |rs(i+1)|.
     // active[r] in: RHS to solve; out: its_r, conv_r (1 converged, 0 not).
@@ -638,23 +744,8 @@ struct Solver {
                                R, segof, ndev.as<double>(), frozen, Vj(0));
             int i = 0;
             for (;;) {
-                // Arnoldi step i -> i+1
-                if (flexible) XT_TRY(op_prec_flex(i, Vj(i), Vj(i + 1), f32prec, f32));
-                else XT_TRY(op_prec(Vj(i), Vj(i + 1), f32));
-                XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
-                XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
-                XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>()));
-                XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
-                XT_TRY(dots(1, Vj(i + 1), Vj(i + 1), ndev.as<double>()));
-                // normalise V_{i+1} for every RHS still in its cycle at this step
-                // (the restart recombination reads it); zero it for the others
-                hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q,
-                                   R, segof, ndev.as<double>(), frozen, Vj(i + 1));
-                hipLaunchKernelGGL(givens_kernel, dim3(ceil_div(I, 64)), dim3(64), 0, s, I, i, K,
-                                   hdev.as<double>(), hdev2.as<double>(), ndev.as<double>(), ghh,
-                                   gcs, gsn, grs, frozen, geps.as<double>(), gits, gicyc, gro,
-                                   hist ? ghist.as<double>() : nullptr, hist_ro0, hist_cap);
-                XT_HIP(hipGetLastError());
+                // Arnoldi step i -> i+1, replayed from a HIP graph when one is captured
+                XT_TRY(arnoldi_step(i, K, flexible, f32, f32prec, hist_cap));
                 if ((i + 1) % check_every() == 0 || i + 1 == K) {
                     // the whole state in one copy: the cycle ends at a check where every
                     // case is frozen (all are at i + 1 == K), so this is then its final state
@@ -892,7 +983,9 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         // (ill-conditioned temporal matrices).  A right-hand side continues with inner
         // cycles in FP64 (operator and preconditioner), where FP32 does not pay, when its
         // step reduced the residual by less than 2x, or when its FP32 cycle did not meet
-        // the inner target within kFp32Cycle iterations (an FP32 factor of an
+        // the inner target within kFp32Cycle iterations (2, or the XT_TUNE_IR_FP32_CYCLE
+        // knob; with an exact factor of a well-conditioned K one iteration meets it; an
+        // FP32 factor of an
         // ill-conditioned K preconditions poorly: FP64 cycles then need a few iterations
         // where FP32 ones need tens).
         // With xtol > 0 a case also needs its last correction to be at most xtol times
@@ -953,7 +1046,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             std::vector<int> cin(S.I, 0);
             // the first inner call starts from the norms just computed (V_0 unchanged)
             if (any32) {
-                constexpr int kFp32Cycle = 4;
+                const int kFp32Cycle = tune_knob(XT_TUNE_IR_FP32_CYCLE) ? tune_knob(XT_TUNE_IR_FP32_CYCLE) : 2;
                 const std::vector<int> was32 = act32;   // the call clears the active flags
                 XT_TRY(S.gmres(x, true, e1, maxits, act32, itv, cin, ro_last, 1, true, true, &rn,
                                kFp32Cycle));

@@ -299,9 +299,13 @@ struct xt_prec_s {
     std::vector<double> shifts;
     std::vector<BandPrec *> shifted;
     DevBuf gin[2], gout[2];    // [0] FP64, [1] FP32
+    // or: banded factors of runs of independent systems, run g on DOFs goff[g]..goff[g+1]-1
+    std::vector<BandPrec *> groups;
+    std::vector<int> goff;
     ~xt_prec_s() {
         if (band) band_delete(band);
         for (BandPrec *b : shifted) band_delete(b);
+        for (BandPrec *b : groups) band_delete(b);
     }
 };
 
@@ -489,14 +493,25 @@ template <typename T>
 int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
                  hipStream_t s) {
     if (pr->band) return band_apply<T>(pr->band, R, q, minv, x, y, s);
+    if (!pr->groups.empty()) {   // rows of run g: goff[g] .. goff[g+1]-1, all R q columns
+        const long long c = (long long)q * R;
+        for (size_t g = 0; g < pr->groups.size(); ++g)
+            XT_TRY(band_apply<T>(pr->groups[g], R, q, minv, x + pr->goff[g] * c,
+                                 y + pr->goff[g] * c, s));
+        return XT_OK;
+    }
     T *stage = nullptr;
-    if (minv) {
-        XT_TRY(pr->work.reserve(sizeof(T) * (size_t)pr->p * q * R));
+    if (minv) {   // sized for FP64, so switching precision does not reallocate
+        XT_TRY(pr->work.reserve(sizeof(double) * (size_t)pr->p * q * R));
         stage = pr->work.as<T>();
     }
     return ilut_apply<T>(pr, R, q, minv, x, y, f32, stage, true, s);
 }
-bool prec_has_fp64(xt_prec pr) { return !pr->band || band_has_fp64(pr->band); }
+bool prec_has_fp64(xt_prec pr) {
+    for (const BandPrec *b : pr->groups)
+        if (!band_has_fp64(b)) return false;
+    return !pr->band || band_has_fp64(pr->band);
+}
 
 int prec_shifts(xt_prec pr, const double **shifts, bool *fp64) {
     if (shifts) *shifts = pr->shifts.empty() ? nullptr : pr->shifts.data();
@@ -689,6 +704,115 @@ int xt_prec_create_band_shifted(int p, const int *kia, const int *kja, const dou
     return XT_OK;
 }
 
+// Runs of consecutive systems for xt_prec_create_band_seg: minimise over the partitions
+// of the systems into runs  sum_runs (dofs of the run x its block size) + kRunCost,
+// the factor entries an apply streams plus a launch cost per run in the same units
+// (about 8 us of sweeps at the HBM rate, FP32: 8e-6 s x 5e12 B/s / 16 B per entry).
+static void band_runs(const std::vector<long long> &n, const std::vector<int> &b,
+                      std::vector<int> &cut) {
+    constexpr double kRunCost = 2.5e6;
+    const int S = (int)n.size();
+    std::vector<double> best(S + 1, 0.0);
+    std::vector<int> from(S + 1, 0);
+    for (int j = 1; j <= S; ++j) {
+        best[j] = 1e300;
+        long long dofs = 0;
+        int bmax = 0;
+        for (int i = j - 1; i >= 0; --i) {   // run i..j-1
+            dofs += n[i];
+            bmax = std::max(bmax, b[i]);
+            const double c = best[i] + (double)dofs * bmax + kRunCost;
+            if (c < best[j]) {
+                best[j] = c;
+                from[j] = i;
+            }
+        }
+    }
+    cut.clear();
+    for (int j = S; j > 0; j = from[j]) cut.push_back(j);
+    cut.push_back(0);
+    std::reverse(cut.begin(), cut.end());
+}
+
+int xt_prec_create_band_seg(int p, const int *ia, const int *ja, const double *a, int nseg,
+                            const int *seg_dof, int block, int flags, xt_prec *out) {
+    XT_ARG(out && p > 0 && ia && ja && a && nseg > 0 && seg_dof && block >= 0,
+           "prec_create_band_seg arguments");
+    XT_ARG(ia[0] == 1, "CSR must be 1-based");
+    XT_ARG(seg_dof[0] == 0 && seg_dof[nseg] == p, "seg_dof must run from 0 to p");
+    for (int t = 0; t < nseg; ++t) XT_ARG(seg_dof[t + 1] > seg_dof[t], "empty or unordered segment");
+    std::vector<long long> n(nseg);
+    std::vector<int> b(nseg);
+    for (int t = 0; t < nseg; ++t) {
+        const int lo = seg_dof[t], hi = seg_dof[t + 1];
+        int bw = 0;
+        for (int i = lo; i < hi; ++i) {
+            XT_ARG(ia[i + 1] >= ia[i] && ia[i + 1] <= ia[p], "CSR row pointers must not decrease");
+            for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+                const int j = ja[k] - 1;
+                if (j < lo || j >= hi) {
+                    set_error("prec_create_band_seg: row %d (segment %d) couples column %d "
+                              "outside its segment", i + 1, t, j + 1);
+                    return XT_E_ARG;
+                }
+                if (j <= i) bw = std::max(bw, i - j);
+            }
+        }
+        n[t] = hi - lo;
+        b[t] = (std::max(std::max(block, bw), 64) + 63) / 64 * 64;
+    }
+    std::vector<int> cut;
+    band_runs(n, b, cut);
+    xt_prec pr = new (std::nothrow) xt_prec_s;
+    if (!pr) return XT_E_NOMEM;
+    pr->p = p;
+    int rc = XT_OK;
+    std::vector<int> sia, sja;
+    std::vector<double> sa;
+    for (size_t g = 0; g + 1 < cut.size() && rc == XT_OK; ++g) {
+        const int lo = seg_dof[cut[g]], hi = seg_dof[cut[g + 1]], pg = hi - lo;
+        int bg = 0;
+        for (int t = cut[g]; t < cut[g + 1]; ++t) bg = std::max(bg, b[t]);
+        // the run's rows, columns shifted to the run (1-based)
+        const long long k0 = ia[lo] - 1, k1 = ia[hi] - 1;
+        sia.resize(pg + 1);
+        for (int i = 0; i <= pg; ++i) sia[i] = (int)(ia[lo + i] - k0);
+        sja.resize(k1 - k0);
+        for (long long k = k0; k < k1; ++k) sja[k - k0] = ja[k] - lo;
+        sa.assign(a + k0, a + k1);
+        BandPrec *bp = nullptr;
+        rc = band_build(pg, sia.data(), sja.data(), sa.data(), bg, &bp,
+                        !(flags & XT_BAND_FP32_ONLY));
+        if (rc == XT_OK) {
+            pr->groups.push_back(bp);
+            pr->goff.push_back(lo);
+        }
+    }
+    if (rc != XT_OK) { delete pr; return rc; }
+    pr->goff.push_back(p);
+    *out = pr;
+    return XT_OK;
+}
+
+int xt_prec_band_groups(xt_prec pr, int *ngroup, int *goff, int *gblock) {
+    XT_ARG(pr && ngroup, "prec_band_groups arguments");
+    if (!pr->groups.empty()) {
+        *ngroup = (int)pr->groups.size();
+        for (size_t g = 0; g < pr->groups.size(); ++g) {
+            if (goff) goff[g] = pr->goff[g];
+            if (gblock) band_dims(pr->groups[g], &gblock[g], nullptr, nullptr);
+        }
+        if (goff) goff[pr->groups.size()] = pr->p;
+        return XT_OK;
+    }
+    *ngroup = pr->band ? 1 : 0;
+    if (pr->band) {
+        if (goff) { goff[0] = 0; goff[1] = pr->p; }
+        if (gblock) band_dims(pr->band, &gblock[0], nullptr, nullptr);
+    }
+    return XT_OK;
+}
+
 int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts) {
     XT_ARG(pr, "prec");
     if (nshift) *nshift = (int)pr->shifts.size();
@@ -698,7 +822,13 @@ int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts) {
 
 int xt_prec_refactor_band(xt_prec pr, const int *ia, const int *ja, const double *a,
                           void *stream) {
-    XT_ARG(pr && pr->band && ia && ja && a, "prec_refactor_band arguments");
+    XT_ARG(pr && (pr->band || !pr->groups.empty()) && ia && ja && a,
+           "prec_refactor_band arguments");
+    if (!pr->groups.empty()) {
+        set_error("xt_prec_refactor_band: a grouped factor is rebuilt with "
+                  "xt_prec_create_band_seg");
+        return XT_E_UNSUPPORTED;
+    }
     if (!pr->shifted.empty()) {
         set_error("xt_prec_refactor_band: a preconditioner with shifted factors is rebuilt with "
                   "xt_prec_create_band_shifted");
@@ -708,16 +838,35 @@ int xt_prec_refactor_band(xt_prec pr, const int *ia, const int *ja, const double
 }
 
 int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth) {
-    XT_ARG(pr && pr->band, "not a banded-factor preconditioner");
-    band_dims(pr->band, block, nblocks, bandwidth);
+    XT_ARG(pr && (pr->band || !pr->groups.empty()), "not a banded-factor preconditioner");
+    if (pr->band) {
+        band_dims(pr->band, block, nblocks, bandwidth);
+        return XT_OK;
+    }
+    int bmax = 0, nbs = 0, bwmax = 0;
+    for (const BandPrec *g : pr->groups) {
+        int b = 0, nb = 0, bw = 0;
+        band_dims(g, &b, &nb, &bw);
+        bmax = std::max(bmax, b);
+        nbs += nb;
+        bwmax = std::max(bwmax, bw);
+    }
+    if (block) *block = bmax;
+    if (nblocks) *nblocks = nbs;
+    if (bandwidth) *bandwidth = bwmax;
     return XT_OK;
 }
 
 int xt_prec_levels(xt_prec pr, int *nlev_lower, int *nlev_upper) {
     XT_ARG(pr, "prec");
-    if (pr->band) {   // sequential block steps of each sweep
+    if (pr->band || !pr->groups.empty()) {   // sequential block steps of each sweep
         int nb = 0;
-        band_dims(pr->band, nullptr, &nb, nullptr);
+        if (pr->band) band_dims(pr->band, nullptr, &nb, nullptr);
+        for (size_t g = 0; g < pr->groups.size(); ++g) {   // the runs follow one another
+            int nbg = 0;
+            band_dims(pr->groups[g], nullptr, &nbg, nullptr);
+            nb += nbg;
+        }
         if (nlev_lower) *nlev_lower = nb;
         if (nlev_upper) *nlev_upper = nb;
         return XT_OK;
@@ -741,7 +890,7 @@ int xt_prec_apply(xt_prec pr, int R, int q, const void *minv, const void *x, voi
 int xt_prec_apply_noalloc(xt_prec pr, int R, int q, const void *minv, const void *x, void *y,
                           int fp32, void *stream) {
     XT_ARG(pr && R > 0 && q > 0 && x && y && x != y, "prec_apply_noalloc arguments");
-    if (pr->band) {
+    if (pr->band || !pr->groups.empty()) {
         set_error("xt_prec_apply_noalloc: ILUT factors only (the banded factor keeps its own "
                   "workspace)");
         return XT_E_UNSUPPORTED;

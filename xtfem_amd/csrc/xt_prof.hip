@@ -35,7 +35,8 @@ bool prof_on(int kind) { return (g_mask >> kind) & 1u; }
 namespace {
 bool g_graphs = true;
 int g_knob[XT_TUNE_NKNOBS] = {};   // 0 = the built-in choice
-}
+}  // namespace
+std::atomic<unsigned long long> g_dev_epoch{0};
 bool graphs_on() { return g_graphs; }
 int tune_knob(int knob) { return (knob >= 0 && knob < XT_TUNE_NKNOBS) ? g_knob[knob] : 0; }
 
@@ -75,14 +76,15 @@ extern "C" {
 
 int xt_tune_set(int knob, int value) {
     XT_ARG(knob >= 0 && knob < XT_TUNE_NKNOBS, "tuning knob");
-    static const int allowed[XT_TUNE_NKNOBS][4] = {
-        {0, 2, 4, 8},      // XT_TUNE_SPMM_UNR
-        {0, 1, 2, 4},      // XT_TUNE_SPMM_WPB
-        {0, 4, 8, 4},      // XT_TUNE_SPMM_R1SPLIT
-        {0, 64, 128, 64},  // XT_TUNE_BAND_TILE
-        {0, 1, 2, 4},      // XT_TUNE_GMRES_CHECK
-        {0, 1, 2, 1},      // XT_TUNE_SPMM_LDS
-        {0, 1, 2, 1},      // XT_TUNE_BAND_SMALLC
+    static const int allowed[XT_TUNE_NKNOBS][5] = {
+        {0, 2, 4, 8, 8},       // XT_TUNE_SPMM_UNR
+        {0, 1, 2, 4, 4},       // XT_TUNE_SPMM_WPB
+        {0, 4, 8, 4, 4},       // XT_TUNE_SPMM_R1SPLIT
+        {0, 64, 128, 64, 64},  // XT_TUNE_BAND_TILE
+        {0, 1, 2, 4, 4},       // XT_TUNE_GMRES_CHECK
+        {0, 1, 2, 1, 1},       // XT_TUNE_SPMM_LDS
+        {0, 1, 2, 1, 1},       // XT_TUNE_BAND_SMALLC
+        {0, 1, 2, 4, 8},       // XT_TUNE_IR_FP32_CYCLE
     };
     bool ok = false;
     for (int v : allowed[knob]) ok = ok || v == value;

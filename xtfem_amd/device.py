@@ -169,11 +169,15 @@ class BandPreconditioner(IlutPreconditioner):
     droptol = 0 limit, ilut.f:2169-2170).  Block Cholesky in FP64 on the device,
     applied by dense block sweeps in FP64 or FP32."""
 
-    def __init__(self, K, block=0, M=None, shifts=(), shifted_fp32_only=False, fp32_only=False):
+    def __init__(self, K, block=0, M=None, shifts=(), shifted_fp32_only=False, fp32_only=False,
+                 seg=None):
         """shifts (with M): also factor K + s M for each s; slab solves then split each
         load case's dominant temporal mode onto the nearest shift (xt_prec_create_band_shifted;
         shifted_fp32_only keeps those factors in FP32 only, for the FP32 applies;
-        fp32_only keeps the factor of K in FP32 only as well)."""
+        fp32_only keeps the factor of K in FP32 only as well).  seg (DOF offsets [nseg+1]
+        of independent systems of a block-diagonal K, as block_diagonal returns them):
+        a banded factor per run of systems with the run's own block size
+        (xt_prec_create_band_seg; no shifts)."""
         ia, ja, a = (np.ascontiguousarray(K[0], dtype=np.int32),
                      np.ascontiguousarray(K[1], dtype=np.int32),
                      np.ascontiguousarray(K[2], dtype=np.float64))
@@ -183,7 +187,15 @@ class BandPreconditioner(IlutPreconditioner):
         self.shifts = np.asarray(shifts, dtype=np.float64).ravel()
         h = C.c_void_p()
         _dev()
-        if len(self.shifts) or fp32_only:
+        if seg is not None:
+            if len(self.shifts):
+                raise ValueError("a grouped banded factor takes no shifts")
+            sd = np.ascontiguousarray(seg, dtype=np.int32)
+            check(lib().xt_prec_create_band_seg(
+                self.p, ia.ctypes.data, ja.ctypes.data, a.ctypes.data, len(sd) - 1,
+                sd.ctypes.data, int(block), 2 if fp32_only else 0, C.addressof(h)),
+                "prec_create_band_seg")
+        elif len(self.shifts) or fp32_only:
             if len(self.shifts):
                 mi, mj, mv = (np.ascontiguousarray(M[0], dtype=np.int32),
                               np.ascontiguousarray(M[1], dtype=np.int32),
@@ -205,6 +217,13 @@ class BandPreconditioner(IlutPreconditioner):
         check(lib().xt_prec_band_info(h, C.addressof(b), C.addressof(nb), C.addressof(bw)),
               "prec_band_info")
         self.block, self.nblocks, self.bandwidth = b.value, nb.value, bw.value
+        ng = C.c_int(0)
+        check(lib().xt_prec_band_groups(h, C.addressof(ng), None, None), "prec_band_groups")
+        goff = np.zeros(ng.value + 1, dtype=np.int32)
+        gblock = np.zeros(ng.value, dtype=np.int32)
+        check(lib().xt_prec_band_groups(h, C.addressof(ng), goff.ctypes.data,
+                                        gblock.ctypes.data), "prec_band_groups")
+        self.group_offsets, self.group_blocks = goff, gblock
 
     def refactor(self, rowptr, colidx, vals):
         """Refactor from K on the device (1-based CSR int32/FP64 tensors, same band)."""
@@ -213,10 +232,13 @@ class BandPreconditioner(IlutPreconditioner):
 
     def matrix_bytes(self, fp32):
         """Bytes of the four block-matrix sets one apply streams (D, D^T halves counted
-        as stored: whole b x b blocks)."""
+        as stored: whole b x b blocks; summed over the runs of a grouped factor)."""
         sv = 4 if fp32 else 8
-        bb = self.block * self.block
-        return sv * bb * (2 * self.nblocks + 2 * (self.nblocks - 1))
+        tot = 0
+        for g, b in enumerate(self.group_blocks):
+            nb = -(-int(self.group_offsets[g + 1] - self.group_offsets[g]) // int(b))
+            tot += sv * int(b) * int(b) * (2 * nb + 2 * (nb - 1))
+        return tot
 
 
 def dominant_temporal_mode(A, B):

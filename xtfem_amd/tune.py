@@ -9,8 +9,9 @@ the launching stream (the library's per-launch profiler), the candidates of
   stages each slice's neighbour rows of X in LDS (off, on);
 * the banded-factor sweeps: the tile rows of the batched passes (64, 128 in FP32), and
   for a few columns the VALU rows kernel or the matrix-core kernels;
-* the slab solve: Arnoldi steps between host convergence checks (1, 2, 4), timed on a
-  whole solve when a right-hand side is given;
+* the slab solve, timed on whole solves when a solve is given: the iterations an FP32
+  inner cycle of GMRES-IR gets before a case moves to FP64 cycles (1, 2, 4, 8) and the
+  Arnoldi steps between host convergence checks (1, 2, 4);
 
 keeps the fastest of each (knobs tuned one after another, the earlier ones fixed at their
 pick) and leaves them set.  The choice is cached per problem signature (p, nonzeros,
@@ -29,7 +30,7 @@ import torch
 from ._lib import check, lib
 
 KNOBS = {"spmm_unr": 0, "spmm_wpb": 1, "spmm_r1split": 2, "band_tile": 3, "gmres_check": 4,
-         "spmm_lds": 5, "band_smallc": 6}
+         "spmm_lds": 5, "band_smallc": 6, "ir_fp32_cycle": 7}
 XT_PROF_SPMM, XT_PROF_BAND = 0, 1
 _CACHE: dict = {}
 
@@ -147,15 +148,16 @@ def autotune(op, pr, R, fp32=True, reps=7, solve=None):
         out["picks"]["band_smallc"] = best
         out["timings_ms"]["band_smallc"] = res
     if solve is not None:
-        res = {}
-        for v in (1, 2, 4):
-            set_knob("gmres_check", v)
-            solve()
-            res[v] = _wall(solve, 3)
-        best = min(res, key=res.get)
-        set_knob("gmres_check", best)
-        out["picks"]["gmres_check"] = best
-        out["timings_ms"]["gmres_check"] = res
+        for name, cands in (("ir_fp32_cycle", (1, 2, 4, 8)), ("gmres_check", (1, 2, 4))):
+            res = {}
+            for v in cands:
+                set_knob(name, v)
+                solve()
+                res[v] = _wall(solve, 2)
+            best = min(res, key=res.get)
+            set_knob(name, best)
+            out["picks"][name] = best
+            out["timings_ms"][name] = res
     _CACHE[key] = out
     _save_file(key, out)
     return out

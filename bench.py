@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--shift-ratio", type=float, default=16.0,
                     help="spread of the dominant temporal eigenvalues one shifted factor covers "
                          "(0: no shifted factors)")
+    ap.add_argument("--band-levels", type=int, default=None,
+                    help="XT_TUNE_BAND_LEVELS for the factors (1: no pairing levels; "
+                         "default: 1 for the panel, the built-in choice otherwise)")
     ap.add_argument("--legs", default="",
                     help="extra legs, comma-separated: rod, reassembly, thermo, driver_panel, "
                          "deletion, damage, fatigue, multimesh (or 'all')")
@@ -272,6 +275,7 @@ class Runner:
         self.eps = 1e-8
         self.n = 0
         self.its, self.relres, self.ierr = [], [], []
+        self.it_prec = []
 
     def step(self):
         from xtfem_amd._lib import check, lib
@@ -289,6 +293,7 @@ class Runner:
         self.its.append(its)
         self.ierr.append(ierr)
         self.relres.append(relres)
+        self.it_prec.append(self.device.slab_stats(self.op))
         self.xprev, self.x = self.x, self.xprev
         self.n += 1
 
@@ -310,7 +315,8 @@ def kernel_profile(run, K, mask):
     check(lib().xt_prof_enable(mask), "prof_enable")
     timed(run, K)
     out = {}
-    for kind, name in ((0, "kron_spmm"), (1, "band_sweep"), (3, "damage")):
+    for kind, name in ((0, "kron_spmm"), (1, "band_sweep"), (3, "damage"),
+                       (4, "band_sweep_f64")):
         if not (mask >> kind) & 1:
             continue
         ms, n, by, fl = C.c_double(), C.c_longlong(), C.c_double(), C.c_double()
@@ -973,38 +979,46 @@ def cpu_damage_sample(ne_s=1000, nip=2560):
                        f"{t:.1f} s on one host core")
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "band_sweep_kernels_notched_plate_pmc.md")
+PMC_SUMMARY = {"panel": os.path.join(ROOT, "profiles", "band_sweep_kron_spmm_panel_32cases_pmc.md"),
+               "plate": os.path.join(ROOT, "profiles", "band_sweep_kernels_notched_plate_pmc.md")}
 
 
-def pmc_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+def pmc_traffic(name, workload):
+    """HBM bytes per launch of kernel `name` in `workload` from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or
+    None when that summary does not give it."""
     import re
-    if not os.path.exists(PMC_SUMMARY):
+    path = PMC_SUMMARY.get(workload)
+    if not path or not os.path.exists(path):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PMC_SUMMARY).read(), re.M)
+    text = open(path).read()
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(name) + r"\]:\s*([0-9.eE+]+)",
+                  text, re.M)
+    if not m and name == "band_sweep":        # the older summary's single FP32 figure
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
     return float(m.group(1)) if m else None
 
 
-def roofline(name, k, prob, pr):
+def roofline(name, k, prob, pr, workload):
     """Roofline entry of the dominant kernel, on its algorithmic work (xt_prof_read_alg):
-    matrix-core bound when the algorithmic intensity is above the FP32 ridge (157.3
-    TFLOP/s / 8 TB/s), else HBM bound.  For the factor sweeps the algorithmic flops are
-    those of the unpaired block recurrences (the pairing's extra products are reported as
-    performed); the banded minimum 4 p w c flops per apply (two triangular sweeps over
-    half-bandwidth w) is given beside it."""
+    matrix-core bound when the algorithmic intensity is above the ridge of its precision
+    (FP32 157.3, FP64 78.6 TFLOP/s over 8 TB/s), else HBM bound.  For the factor sweeps
+    the algorithmic flops are those of the unpaired block recurrences (the pairing's extra
+    products are reported as performed); the banded minimum 4 p w c flops per apply (two
+    triangular sweeps over half-bandwidth w) is given beside it."""
+    prec = "f64" if name.endswith("_f64") else "f32"
+    peak = PEAK_MFMA_TFLOPS[prec]
     intensity = k["flops"] / k["bytes"]
-    out = {"kernel": name, "traffic": pmc_traffic(),
+    out = {"kernel": name, "traffic": pmc_traffic(name, workload),
            "algorithmic_bytes_per_launch": k["bytes"] / k["launches"],
            "algorithmic_flops_per_launch": k["flops"] / k["launches"],
            "performed_flops_per_launch": k["performed_flops"] / k["launches"],
            "avg_launch_us": k["avg_us"], "intensity_flop_per_byte": intensity}
-    if intensity > PEAK_MFMA_TFLOPS["f32"] * 1e12 / (PEAK_HBM_GBS * 1e9):
+    if intensity > peak * 1e12 / (PEAK_HBM_GBS * 1e9):
         tf = k["flops"] / (k["ms"] * 1e-3) / 1e12
-        out.update(bound="mfma", achieved=tf, peak=PEAK_MFMA_TFLOPS["f32"], unit="TFLOP/s",
-                   frac=tf / PEAK_MFMA_TFLOPS["f32"],
+        out.update(bound="mfma", achieved=tf, peak=peak, unit="TFLOP/s", frac=tf / peak,
                    performed_tflops=k["performed_flops"] / (k["ms"] * 1e-3) / 1e12)
-        if name == "band_sweep":
+        if name.startswith("band_sweep"):
             out["banded_minimum_flops_per_apply_c192"] = 4.0 * prob.p * pr.bandwidth * 192
     else:
         out.update(bound="hbm", achieved=k["gbs"], peak=PEAK_HBM_GBS, unit="GB/s",
@@ -1030,7 +1044,12 @@ def setup_solver(prob, args, fp32_only):
     0.05 of the structure's lowest K^-1 M eigenvalue (device power iteration)."""
     import torch
     from xtfem_amd import device
+    from xtfem_amd import tune
     t0 = time.perf_counter()
+    # the factors' pairing levels (fixed when a factor is built): none for the panel, whose
+    # chain steps (1856 x 1856 blocks times 192 columns, split over k) fill the GPU by
+    # themselves (FP64 apply 27.7 ms without pairing, 31.7 ms with four levels)
+    tune.set_knob("band_levels", args.band_levels)
     op = device.KronOperator([prob.Kb, prob.Mb], q=6)
     pr0 = device.BandPreconditioner(prob.Kb, block=args.block)
     mu = device.lowest_mode(pr0, prob.Mb, prob.bc)
@@ -1042,6 +1061,7 @@ def setup_solver(prob, args, fp32_only):
     pr = device.BandPreconditioner(prob.Kb, block=args.block, M=prob.Mb, shifts=shifts,
                                    shifted_fp32_only=fp32_only)
     torch.cuda.synchronize()
+    tune.set_knob("band_levels", 0)
     info = dict(mu_min=mu, lambda_min=float(np.nanmin(lam)), lambda_max=float(np.nanmax(lam)),
                 shifts=[float(v) for v in shifts], shifted_fp32_only=fp32_only,
                 bandwidth=pr.bandwidth, blocks=pr.nblocks, block=pr.block,
@@ -1075,6 +1095,8 @@ def accuracy_check(prob, op, pr, xtol):
 
 def main():
     args = parse()
+    if args.band_levels is None:
+        args.band_levels = 1 if args.workload == "panel" else 0
     if args.copy_probe:
         copy_probe()
         return
@@ -1118,6 +1140,7 @@ def main():
     for _ in range(W):
         run32.step()
     t32 = timed(run32, K)
+    it_prec = np.array(run32.it_prec[W:]).sum(axis=0)      # (FP32, FP64) sweep iterations
     if args.profile_only:       # the batched FP32-IR steps alone (for a kernel trace)
         print(json.dumps(dict(ms_per_step=1e3 * t32 / K, its=run32.its[-1].tolist())))
         return
@@ -1186,11 +1209,19 @@ def main():
           flush=True)
     # per-kernel durations: the batched FP32-IR steps once more, every launch of the
     # operator and sweep kernels bracketed by HIP events on the launching stream
-    prof = kernel_profile(run32, K, mask=0b11)
+    prof = kernel_profile(run32, K, mask=0b10011)
     dom_name = max(prof, key=lambda k: prof[k]["ms"])
-    roof = roofline(dom_name, prof[dom_name], prob, pr)
+    roof = roofline(dom_name, prof[dom_name], prob, pr, args.workload)
     value = K * R / t32
     ms_step = 1e3 * t32 / K
+    share32 = float(it_prec[0] / max(1, it_prec.sum()))
+    # the arithmetic of the timed steps: GMRES-IR runs FP32 cycles where they pay and FP64
+    # cycles elsewhere (the solver probes FP32 and remembers when it did not pay)
+    dtype = ("f32 sweeps/operator in f64 GMRES-IR" if share32 >= 0.9 else
+             "f64 (GMRES-IR: FP32 cycles probed, not profitable for this K)" if share32 <= 0.1
+             else "f32+f64 (GMRES-IR, mixed cycles)")
+    res["iterations_by_sweep_precision"] = {"f32": int(it_prec[0]), "f64": int(it_prec[1]),
+                                            "f32_share": share32}
     nblocks, block = pr.nblocks, pr.block
     del run32, op, pr                  # the extra legs need the memory
     torch.cuda.synchronize()
@@ -1231,7 +1262,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 operator/factor sweeps in f64 GMRES-IR",
+        "dtype": dtype,
         "data": "synthetic (structured mesh, seeded load cases)",
         "config": {"workload": prob.describe() + f"; GMRES(30) eps 1e-8, forward-error test "
                                f"{args.xtol:g}; exact banded factors ({nblocks} blocks of "

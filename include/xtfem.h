@@ -382,6 +382,12 @@ int xt_slab_solve_seg(xt_op op, xt_prec pr, int nseg, const long long *seg_dof, 
                       double xtol, int maxits, int mode, int *its, int *ierr, double *relres,
                       void *stream);
 
+/* Iterations of the last slab solve on op (any of the entry points above), summed over
+ * its instances: it32 those whose preconditioner sweeps ran in FP32 (GMRES-IR's FP32
+ * cycles, mode 2), it64 those in FP64 (mode 0, GMRES-IR's FP64 cycles).  No counterpart
+ * in the reference (one precision). */
+int xt_slab_stats(xt_op op, long long *it32, long long *it64);
+
 /* chkelesta (delelement.f90:109-136) on device-resident state: failed[e] = 1 iff
  * some Gauss point of element e has D >= Dc, else 0; for a failed element
  * elesta[e] and its gpsta entries are set to 0 (others untouched).  Device
@@ -425,10 +431,11 @@ int xt_slab_rhs(int p, int q, int R, const double *coef, const double *rext, lon
 
 /* ---- kernel timing ----------------------------------------------------------- */
 #define XT_PROF_SPMM 0     /* Kronecker operator (node-block or CSR kernel)         */
-#define XT_PROF_BAND 1     /* banded-factor block sweeps                            */
+#define XT_PROF_BAND 1     /* banded-factor block sweeps in FP32                    */
 #define XT_PROF_TRSV 2     /* ILUT level-scheduled triangular solves                */
 #define XT_PROF_DAMAGE 3   /* damage kernel                                         */
-#define XT_PROF_NKINDS 4
+#define XT_PROF_BAND64 4   /* banded-factor block sweeps in FP64                    */
+#define XT_PROF_NKINDS 5
 /* Enable per-launch HIP-event timing for the kinds in mask (bit k = kind k) and
  * clear the counters; 0 disables and releases the events.  Process-wide; one host
  * thread. */
@@ -467,6 +474,21 @@ int xt_graphs_enable(int on);
  *   XT_TUNE_IR_FP32_CYCLE GMRES-IR: iterations an FP32 inner cycle gets to meet its target
  *                         before the case moves to FP64 cycles              1, 2, 4, 8
  *                         (built-in 2)
+ *   XT_TUNE_IR_MIN_GAIN   GMRES-IR: least reduction of the true residual an FP32
+ *                         refinement step must reach for the case to stay in FP32
+ *                         cycles (when the preconditioner has FP64 sweeps)  2, 8, 32, 128
+ *                         (built-in 16)
+ *   XT_TUNE_IR_SKIP32     GMRES-IR precision probe per operator and preconditioner (with
+ *                         FP64 sweeps): one solve FP32 cycles first, the next FP64 cycles
+ *                         only, the following n - 2 the faster of the two by wall time,
+ *                         then again; 1: FP32 cycles first always           1, 4, 16, 64
+ *                         (built-in 16)
+ *   XT_TUNE_BAND_CHAIN    banded-factor sweeps, the dependent chain steps with more than
+ *                         16 columns: one workgroup per 16 x 16 output tile 1, 64 x 64
+ *                         tiles split over k into about 512 / 256 / 1024 workgroups with
+ *                         a fixed-order sum 2 / 3 / 4 (built-in 4)          1, 2, 3, 4
+ *   XT_TUNE_BAND_LEVELS   banded factor: pairing levels of the block recurrences at most
+ *                         v - 1 (read when a factor is built; built-in 4)   1, 2, 3, 5
  * XT_E_ARG for a value a knob does not take. */
 #define XT_TUNE_SPMM_UNR 0
 #define XT_TUNE_SPMM_WPB 1
@@ -476,7 +498,11 @@ int xt_graphs_enable(int on);
 #define XT_TUNE_SPMM_LDS 5
 #define XT_TUNE_BAND_SMALLC 6
 #define XT_TUNE_IR_FP32_CYCLE 7
-#define XT_TUNE_NKNOBS 8
+#define XT_TUNE_IR_MIN_GAIN 8
+#define XT_TUNE_IR_SKIP32 9
+#define XT_TUNE_BAND_CHAIN 10
+#define XT_TUNE_BAND_LEVELS 11
+#define XT_TUNE_NKNOBS 12
 int xt_tune_set(int knob, int value);
 /* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
 int xt_tune_get(int knob);

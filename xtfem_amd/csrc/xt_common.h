@@ -186,6 +186,8 @@ int slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const double 
                     double xtol = 0.0, int nseg = 1, const long long *seg_dof = nullptr);
 // The preconditioner can apply in FP64 (a banded factor kept in FP32 only cannot).
 bool prec_has_fp64(xt_prec pr);
+// A number that identifies the preconditioner for the life of the process.
+unsigned long long prec_id(xt_prec pr);
 
 // Device workspace of the batched slab solver, owned by the operator handle and kept
 // across calls (buffers only grow), so a solve allocates nothing once warm.
@@ -215,6 +217,13 @@ struct SolverWorkspace {
     };
     std::vector<StepGraph> steps;
     unsigned long long step_epoch = 0;   // g_dev_epoch the cached step graphs were made at
+    // GMRES-IR precision probe for the preconditioner ir_prec (prec_id): phase 0 next
+    // solve runs FP32 cycles first, 1 FP64 only, 2 the faster (ir_fp64) for ir_left more
+    unsigned long long ir_prec = 0;
+    int ir_phase = 0, ir_left = 0;
+    double ir_cost_a = 0.0;
+    bool ir_fp64 = false;
+    long long last_it32 = 0, last_it64 = 0;   // iterations of the last solve by sweep precision
     hipStream_t cap = nullptr;
     SolverWorkspace() = default;
     SolverWorkspace(const SolverWorkspace &) = delete;

@@ -17,6 +17,7 @@
 // outer loop recomputes the true residual b - A x with the FP64 operator and
 // adds the inner correction, until ||b - A x|| <= eps * ||b - A x0||.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -855,6 +856,14 @@ extern "C" int xt_slab_solve_tol(xt_op op, xt_prec pr, int R, const double *ab, 
                                (hipStream_t)stream, nullptr, xtol);
 }
 
+extern "C" int xt_slab_stats(xt_op op, long long *it32, long long *it64) {
+    XT_ARG(op, "slab_stats: op");
+    SolverWorkspace *ws = op_workspace(op);
+    if (it32) *it32 = ws->last_it32;
+    if (it64) *it64 = ws->last_it64;
+    return XT_OK;
+}
+
 extern "C" int xt_slab_solve_seg(xt_op op, xt_prec pr, int nseg, const long long *seg_dof,
                                  int R, const double *ab, const double *b, double *x, int im,
                                  double eps, double xtol, int maxits, int mode, int *its,
@@ -969,10 +978,17 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         ierr[r] = 0;
         if (r0[r] == 0.0) { ierr[r] = -1; active[r] = 0; }
     }
+    ws->last_it32 = ws->last_it64 = 0;
+    auto itsum = [&]() {
+        long long t = 0;
+        for (int r = 0; r < S.I; ++r) t += itv[r];
+        return t;
+    };
     if (mode == 0 || mode == 2) {
         // mode 2: FP64 flexible GMRES, FP32 preconditioner
         XT_TRY(S.gmres(x, false, eps1, maxits, active, itv, conv, ro_last, 1 << 30, mode == 2,
                        mode == 2));
+        (mode == 2 ? ws->last_it32 : ws->last_it64) = itsum();
         for (int r = 0; r < S.I; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
     } else {
         // GMRES-IR: one flexible GMRES cycle per refinement step on the current residual
@@ -982,21 +998,51 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         // relaxed to the reduction the last step achieved when FP32 rounding is the floor
         // (ill-conditioned temporal matrices).  A right-hand side continues with inner
         // cycles in FP64 (operator and preconditioner), where FP32 does not pay, when its
-        // step reduced the residual by less than 2x, or when its FP32 cycle did not meet
+        // step reduced the true residual by less than 16x (the XT_TUNE_IR_MIN_GAIN knob:
+        // an FP64 cycle with the exact factor gains orders of magnitude per iteration, an
+        // FP32 iteration costs about half of one), or when its FP32 cycle did not meet
         // the inner target within kFp32Cycle iterations (2, or the XT_TUNE_IR_FP32_CYCLE
         // knob; with an exact factor of a well-conditioned K one iteration meets it; an
         // FP32 factor of an
         // ill-conditioned K preconditions poorly: FP64 cycles then need a few iterations
         // where FP32 ones need tens).
-        // With xtol > 0 a case also needs its last correction to be at most xtol times
-        // its solution (a forward-error test: the residual does not bound the error of an
-        // ill-conditioned slab); until then it keeps refining, each inner solve asked for
-        // a fixed reduction of the residual.
+        // With xtol > 0 a case also needs a forward-error test (the residual eps does not
+        // bound the error of an ill-conditioned slab): see etight below.
         std::vector<double> rn(S.I), irel(S.I, 1.0e-3), prev(S.I, 0.0), dn(S.I, 0.0), xn(S.I, 0.0);
         std::vector<int> fb(S.I, 0);
-        if (xtol > 0.0) XT_TRY(ws->xprev.reserve(sizeof(double) * S.n));
+        // Forward-error target xtol > 0 with FP64 sweeps at hand: a case converges when its
+        // residual reaches etight = 1e-5 xtol of the initial residual (no tighter than
+        // 1e-14): FP32 refinement steps keep going while each gains at least gmin32 (the
+        // FP64 residual makes their progress real), FP64 cycles aim there in one long
+        // Krylov cycle.  The
+        // last correction does not bound the error here: FP32 cycles of an ill-conditioned
+        // slab leave the modes the FP32 factor resolves poorly almost untouched, so their
+        // corrections are small while the error is not (the thin panel: corrections below
+        // 1e-7 of the solution, error 1.5e-5); its slab operator turns a relative residual
+        // into up to ~2.5e4 times that error, which 1e-5 xtol covers.  With an FP32-only
+        // factor the last-correction test is the one available.
+        std::vector<double> etight(S.I);
+        for (int r = 0; r < S.I; ++r) etight[r] = std::max(1.0e-5 * xtol, 1.0e-14) * r0[r];
+        // Whether FP32 cycles pay is measured, per operator and preconditioner: the first
+        // solve runs FP32 cycles first (probe A), the next one FP64 cycles only (probe B),
+        // and the following solves take whichever was faster by wall time, until the
+        // next probe pair after XT_TUNE_IR_SKIP32 = n solves in all (built-in 16; 1: FP32
+        // first always).  An FP32 factor of an ill-conditioned K (the thin panel) can
+        // leave FP32 cycles a net loss even when each one converges.
+        const int nskip = tune_knob(XT_TUNE_IR_SKIP32) ? tune_knob(XT_TUNE_IR_SKIP32) : 16;
+        const bool probing = fp64_prec && nskip > 1;
+        if (probing && ws->ir_prec != prec_id(pr)) {
+            ws->ir_prec = prec_id(pr);
+            ws->ir_phase = 0;
+        }
+        const bool fp64only = probing && (ws->ir_phase == 1 || (ws->ir_phase == 2 && ws->ir_fp64));
+        if (fp64only) std::fill(fb.begin(), fb.end(), 1);
+        const auto tstart = std::chrono::steady_clock::now();
+        const double gmin32 = tune_knob(XT_TUNE_IR_MIN_GAIN) ? tune_knob(XT_TUNE_IR_MIN_GAIN) : 16;
+        if (xtol > 0.0 && !fp64_prec) XT_TRY(ws->xprev.reserve(sizeof(double) * S.n));
         for (int outer = 0;; ++outer) {
-            if (xtol > 0.0 && outer > 0) {   // correction of the last step: x - xprev
+            const bool dtest = xtol > 0.0 && !fp64_prec;   // the last-correction test
+            if (dtest && outer > 0) {   // correction of the last step: x - xprev
                 double *d = ws->xprev.as<double>();
                 hipLaunchKernelGGL(sub_from, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.s, S.n, x, d);
                 XT_HIP(hipGetLastError());
@@ -1015,13 +1061,15 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             for (int r = 0; r < S.I; ++r) {
                 if (ierr[r] == -1) continue;
                 const double nr = std::sqrt(rn[r]);
-                const bool fwd = xtol == 0.0 || (outer > 0 && dn[r] <= xtol * xtol * xn[r]);
+                const bool fwd = xtol == 0.0 ||
+                                 (fp64_prec ? nr <= etight[r]
+                                            : outer > 0 && dn[r] <= xtol * xtol * xn[r]);
                 if (nr <= eps1[r] && fwd) { conv[r] = 1; continue; }
                 conv[r] = 0;
                 if (itv[r] >= maxits || nr == 0.0) continue;
                 if (prev[r] > 0.0) {
                     const double rho = nr / prev[r];
-                    if (rho > 0.5 && fp64_prec && !fb[r]) {
+                    if (rho * gmin32 > 1.0 && fp64_prec && !fb[r]) {
                         fb[r] = 1;
                         irel[r] = 1.0e-3;
                     } else if (!fb[r] && rho > 10.0 * irel[r]) {
@@ -1031,14 +1079,15 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
                 prev[r] = nr;
                 // FP64 cycles after a fallback aim at the final target at once (one long
                 // Krylov cycle converges faster than several restarted ones)
-                e1[r] = nr <= eps1[r] ? irel[r] * nr
-                        : fb[r]       ? 0.5 * eps1[r]
-                                      : std::max(irel[r] * nr, 0.5 * eps1[r]);
+                e1[r] = fb[r] && xtol > 0.0 && fp64_prec ? etight[r]
+                        : nr <= eps1[r]         ? irel[r] * nr
+                        : fb[r]                 ? 0.5 * eps1[r]
+                                                : std::max(irel[r] * nr, 0.5 * eps1[r]);
                 if (fb[r]) { act64[r] = 1; any64 = true; }
                 else { act32[r] = 1; any32 = true; }
             }
             if (!any32 && !any64) break;
-            if (xtol > 0.0)
+            if (dtest)
                 XT_HIP(hipMemcpyAsync(ws->xprev.ptr, x, sizeof(double) * S.n,
                                       hipMemcpyDeviceToDevice, S.s));
             // inner solves for the correction on V_0 = the residual (already there): each
@@ -1048,18 +1097,37 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
             if (any32) {
                 const int kFp32Cycle = tune_knob(XT_TUNE_IR_FP32_CYCLE) ? tune_knob(XT_TUNE_IR_FP32_CYCLE) : 2;
                 const std::vector<int> was32 = act32;   // the call clears the active flags
+                const long long t0 = itsum();
                 XT_TRY(S.gmres(x, true, e1, maxits, act32, itv, cin, ro_last, 1, true, true, &rn,
                                kFp32Cycle));
+                ws->last_it32 += itsum() - t0;
                 for (int r = 0; r < S.I; ++r)
                     if (was32[r] && !cin[r] && fp64_prec && !fb[r]) {
                         fb[r] = 1;
                         irel[r] = 1.0e-3;   // FP64 cycles reach the full reduction
                     }
             }
-            if (any64)
+            if (any64) {
+                const long long t0 = itsum();
                 XT_TRY(S.gmres(x, false, e1, maxits, act64, itv, cin, ro_last, 1, true, false,
                                any32 ? nullptr : &rn));
+                ws->last_it64 += itsum() - t0;
+            }
             XT_TRY(S.residual(b, x, S.Vj(0)));
+        }
+        if (probing) {
+            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                           tstart).count();
+            if (ws->ir_phase == 0) {
+                ws->ir_cost_a = t;
+                ws->ir_phase = 1;
+            } else if (ws->ir_phase == 1) {
+                ws->ir_fp64 = t < ws->ir_cost_a;
+                ws->ir_phase = 2;
+                ws->ir_left = nskip - 2;
+            } else if (--ws->ir_left <= 0) {
+                ws->ir_phase = 0;
+            }
         }
         for (int r = 0; r < S.I; ++r) if (ierr[r] == 0 && !conv[r]) ierr[r] = 1;
         for (int r = 0; r < S.I; ++r) {   // true residual: the loop's last b - A x

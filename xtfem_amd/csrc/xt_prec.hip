@@ -289,7 +289,11 @@ struct Tri {
 
 using namespace xt;
 
+namespace xt {
+std::atomic<unsigned long long> g_prec_ids{0};
+}
 struct xt_prec_s {
+    unsigned long long id = ++xt::g_prec_ids;   // never reused, unlike the handle's address
     int p = 0;
     Tri L, U;                  // ILUT factors (level-scheduled solves)
     DevBuf dinv64, dinv32;
@@ -507,6 +511,7 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     }
     return ilut_apply<T>(pr, R, q, minv, x, y, f32, stage, true, s);
 }
+unsigned long long prec_id(xt_prec pr) { return pr->id; }
 bool prec_has_fp64(xt_prec pr) {
     for (const BandPrec *b : pr->groups)
         if (!band_has_fp64(b)) return false;
@@ -707,10 +712,14 @@ int xt_prec_create_band_shifted(int p, const int *kia, const int *kja, const dou
 // Runs of consecutive systems for xt_prec_create_band_seg: minimise over the partitions
 // of the systems into runs  sum_runs (dofs of the run x its block size) + kRunCost,
 // the factor entries an apply streams plus a launch cost per run in the same units
-// (about 8 us of sweeps at the HBM rate, FP32: 8e-6 s x 5e12 B/s / 16 B per entry).
+// (about 8 us of sweeps at the HBM rate, FP32: 8e-6 s x 5e12 B/s / 16 B per entry),
+// with runs of several systems capped in size.
 static void band_runs(const std::vector<long long> &n, const std::vector<int> &b,
                       std::vector<int> &cut) {
     constexpr double kRunCost = 2.5e6;
+    // a run of several systems holds at most this many factor entries per block set (the
+    // FP64 factorisation of a run needs about six such sets at once)
+    constexpr double kRunMaxEntries = 1.0e9;
     const int S = (int)n.size();
     std::vector<double> best(S + 1, 0.0);
     std::vector<int> from(S + 1, 0);
@@ -721,6 +730,7 @@ static void band_runs(const std::vector<long long> &n, const std::vector<int> &b
         for (int i = j - 1; i >= 0; --i) {   // run i..j-1
             dofs += n[i];
             bmax = std::max(bmax, b[i]);
+            if (i < j - 1 && (double)dofs * bmax > kRunMaxEntries) break;
             const double c = best[i] + (double)dofs * bmax + kRunCost;
             if (c < best[j]) {
                 best[j] = c;

@@ -434,6 +434,14 @@ def slab_solve_device(op, pr, A, B, b, x, R, im=30, eps=1e-8, maxits=500, mode=1
     return x, its, ierr, relres
 
 
+def slab_stats(op):
+    """(it32, it64): iterations of the last slab solve on op whose preconditioner sweeps
+    ran in FP32 and in FP64, summed over its instances (xt_slab_stats)."""
+    a, b = C.c_longlong(), C.c_longlong()
+    check(lib().xt_slab_stats(op.h, C.addressof(a), C.addressof(b)), "slab_stats")
+    return a.value, b.value
+
+
 def block_diagonal(mats):
     """Stack 1-based CSR matrices (ia, ja, a) into one block-diagonal CSR; returns it and the
     row offsets of the blocks [n_blocks + 1]."""

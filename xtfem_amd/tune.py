@@ -9,9 +9,10 @@ the launching stream (the library's per-launch profiler), the candidates of
   stages each slice's neighbour rows of X in LDS (off, on);
 * the banded-factor sweeps: the tile rows of the batched passes (64, 128 in FP32), and
   for a few columns the VALU rows kernel or the matrix-core kernels;
-* the slab solve, timed on whole solves when a solve is given: the iterations an FP32
-  inner cycle of GMRES-IR gets before a case moves to FP64 cycles (1, 2, 4, 8) and the
-  Arnoldi steps between host convergence checks (1, 2, 4);
+* the slab solve, timed on whole solves when a solve is given: the least residual
+  reduction an FP32 refinement step of GMRES-IR must reach for a case to stay in FP32
+  (2, 8, 32, 128), the iterations an FP32 inner cycle gets before a case moves to FP64
+  cycles (1, 2, 4, 8) and the Arnoldi steps between host convergence checks (1, 2, 4);
 
 keeps the fastest of each (knobs tuned one after another, the earlier ones fixed at their
 pick) and leaves them set.  The choice is cached per problem signature (p, nonzeros,
@@ -30,8 +31,9 @@ import torch
 from ._lib import check, lib
 
 KNOBS = {"spmm_unr": 0, "spmm_wpb": 1, "spmm_r1split": 2, "band_tile": 3, "gmres_check": 4,
-         "spmm_lds": 5, "band_smallc": 6, "ir_fp32_cycle": 7}
-XT_PROF_SPMM, XT_PROF_BAND = 0, 1
+         "spmm_lds": 5, "band_smallc": 6, "ir_fp32_cycle": 7, "ir_min_gain": 8,
+         "ir_skip32": 9, "band_chain": 10, "band_levels": 11}
+XT_PROF_SPMM, XT_PROF_BAND, XT_PROF_BAND64 = 0, 1, 4
 _CACHE: dict = {}
 
 
@@ -137,18 +139,20 @@ def autotune(op, pr, R, fp32=True, reps=7, solve=None):
         minv = minv.expand(R, q, q).contiguous()
         sweep = lambda: pr.apply(minv, x, y, R, fp32=fp32)
         cands = (64, 128) if fp32 and pr.block % 128 == 0 else (64,)
-        best, res = _pick("band_tile", cands, XT_PROF_BAND, sweep, max(3, reps // 2))
+        kb = XT_PROF_BAND if fp32 else XT_PROF_BAND64
+        best, res = _pick("band_tile", cands, kb, sweep, max(3, reps // 2))
         out["picks"]["band_tile"] = best
         out["timings_ms"]["band_tile"] = res
         # few columns (a shifted factor's group, one load case): rows kernel or matrix cores
         xs = x[:p * q].contiguous()                 # one load case: q columns
         ys = torch.empty_like(xs)
         sweep_small = lambda: pr.apply(None, xs, ys, 1, q=q, fp32=fp32)
-        best, res = _pick("band_smallc", (1, 2), XT_PROF_BAND, sweep_small, max(3, reps // 2))
+        best, res = _pick("band_smallc", (1, 2), kb, sweep_small, max(3, reps // 2))
         out["picks"]["band_smallc"] = best
         out["timings_ms"]["band_smallc"] = res
     if solve is not None:
-        for name, cands in (("ir_fp32_cycle", (1, 2, 4, 8)), ("gmres_check", (1, 2, 4))):
+        for name, cands in (("ir_min_gain", (2, 8, 32, 128)), ("ir_fp32_cycle", (1, 2, 4, 8)),
+                            ("gmres_check", (1, 2, 4))):
             res = {}
             for v in cands:
                 set_knob(name, v)

@@ -81,6 +81,8 @@ def parse():
                     help="temporal points per slab of the damage leg")
     ap.add_argument("--rod-only", action="store_true",
                     help="only run the prismatic-rod ILUT leg (K timed slabs) and print it")
+    ap.add_argument("--damage-only", action="store_true",
+                    help="only the damage-kernel leg (60x60x12 block), one JSON line")
     ap.add_argument("--thermo-only", action="store_true",
                     help="only run the thermo-mechanical plate leg and print it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -979,8 +981,11 @@ def cpu_damage_sample(ne_s=1000, nip=2560):
                        f"{t:.1f} s on one host core")
 
 
-PMC_SUMMARY = {"panel": os.path.join(ROOT, "profiles", "band_sweep_kron_spmm_panel_32cases_pmc.md"),
-               "plate": os.path.join(ROOT, "profiles", "band_sweep_kernels_notched_plate_pmc.md")}
+# rocprofv3 --pmc summaries (FETCH_SIZE / WRITE_SIZE per launch, calibrated on a plain
+# copy), per workload; none is committed for the current kernels (the counter runs
+# crashed in the profiler's initialisation on this image), so traffic reads null
+PMC_SUMMARY = {"panel": os.path.join(ROOT, "profiles", "band_sweep_panel_32cases_pmc.md"),
+               "plate": os.path.join(ROOT, "profiles", "band_sweep_notched_plate_32cases_pmc.md")}
 
 
 def pmc_traffic(name, workload):
@@ -994,8 +999,6 @@ def pmc_traffic(name, workload):
     text = open(path).read()
     m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(name) + r"\]:\s*([0-9.eE+]+)",
                   text, re.M)
-    if not m and name == "band_sweep":        # the older summary's single FP32 figure
-        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
     return float(m.group(1)) if m else None
 
 
@@ -1115,6 +1118,9 @@ def main():
     if args.rod_only:
         print(json.dumps(rod_leg(args.steps)))
         return
+    if args.damage_only:
+        print(json.dumps(damage_leg(args.damage_nip, slabs=args.steps)))
+        return
     R, K, W = args.cases, args.steps, args.warmup
     if args.workload == "multimesh":
         print(json.dumps(multimesh_leg(K=K)))
@@ -1139,11 +1145,20 @@ def main():
     run32 = Runner(prob, op, pr, np.arange(R), mode=1, nsteps=2 * nsteps, xtol=args.xtol)
     for _ in range(W):
         run32.step()
+    if args.profile_only:
+        # the batched GMRES-IR steps with per-launch HIP events only (for a kernel trace of
+        # this command: its per-kernel averages against the events' averages printed here)
+        prof = kernel_profile(run32, K, mask=0b10011)
+        dom = max(prof, key=lambda k: prof[k]["ms"])
+        print(json.dumps(dict(workload=prob.describe(), steps=K, warmup=W,
+                              its=run32.its[-1].tolist(),
+                              iterations_by_sweep_precision=np.array(run32.it_prec[W:]).sum(
+                                  axis=0).tolist(),
+                              kernels=prof, roofline=roofline(dom, prof[dom], prob, pr,
+                                                              args.workload))))
+        return
     t32 = timed(run32, K)
     it_prec = np.array(run32.it_prec[W:]).sum(axis=0)      # (FP32, FP64) sweep iterations
-    if args.profile_only:       # the batched FP32-IR steps alone (for a kernel trace)
-        print(json.dumps(dict(ms_per_step=1e3 * t32 / K, its=run32.its[-1].tolist())))
-        return
     its32 = np.array(run32.its[W:])
     rr32 = np.array(run32.relres[W:])
     ok32 = bool(np.all(np.array(run32.ierr[W:]) == 0) and np.all(rr32 <= 1e-8))

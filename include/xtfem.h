@@ -470,7 +470,7 @@ int xt_graphs_enable(int on);
  *                         neighbour rows fit), built-in: off
  *   XT_TUNE_BAND_SMALLC   banded-factor sweeps with at most 16 columns: VALU rows kernel 1,
  *                         matrix-core kernels 2; built-in: rows kernel below 1024-row
- *                         blocks
+ *                         blocks (FP32) or 512-row blocks (FP64)
  *   XT_TUNE_IR_FP32_CYCLE GMRES-IR: iterations an FP32 inner cycle gets to meet its target
  *                         before the case moves to FP64 cycles              1, 2, 4, 8
  *                         (built-in 2)

@@ -95,6 +95,8 @@ struct BandPrec {
 // each, then a 16-lane butterfly (one value per lane).  Grid x = b / RW.
 constexpr size_t kRowsLdsMax = 96 * 1024;
 constexpr int kRowsMaxB = 1024;   // block rows from which few columns go to matrix cores
+constexpr int kRowsMaxB64 = 512;  // the same in FP64 (768-row blocks, 6-16 columns: the
+                                  // matrix-core sweeps took a plate step from 49.7 to 38 ms)
 constexpr bool kChainSplitDefault = true;    // chain steps as split-k tile products
 
 template <typename T, int CM, int RW>
@@ -500,7 +502,7 @@ static int launch_gemm(const BandPrec &bp, int c, int tri, const T *A, long long
     } while (0)
     if (vlds > kRowsLdsMax) {
         XT_TRY(launch_tile<T>(bp, c, tri, A, as, V, vs, add, ads, out, os, nblk, s));
-    } else if (c <= 16 && (sk == 1 || (sk == 0 && b < kRowsMaxB))) {
+    } else if (c <= 16 && (sk == 1 || (sk == 0 && b < (sizeof(T) == 8 ? kRowsMaxB64 : kRowsMaxB)))) {
         if (c <= 8) {
             XT_ROWS(8, 2);
         } else {

@@ -489,6 +489,9 @@ int xt_graphs_enable(int on);
  *                         a fixed-order sum 2 / 3 / 4 (built-in 4)          1, 2, 3, 4
  *   XT_TUNE_BAND_LEVELS   banded factor: pairing levels of the block recurrences at most
  *                         v - 1 (read when a factor is built; built-in 4)   1, 2, 3, 5
+ *   XT_TUNE_GMRES_REORTH  Arnoldi orthogonalisation: classical Gram-Schmidt twice always
+ *                         1, the second pass only where the first lost more than half of
+ *                         |w|^2 for some instance 2 (built-in 2)            1, 2
  * XT_E_ARG for a value a knob does not take. */
 #define XT_TUNE_SPMM_UNR 0
 #define XT_TUNE_SPMM_WPB 1
@@ -502,7 +505,8 @@ int xt_graphs_enable(int on);
 #define XT_TUNE_IR_SKIP32 9
 #define XT_TUNE_BAND_CHAIN 10
 #define XT_TUNE_BAND_LEVELS 11
-#define XT_TUNE_NKNOBS 12
+#define XT_TUNE_GMRES_REORTH 12
+#define XT_TUNE_NKNOBS 13
 int xt_tune_set(int knob, int value);
 /* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
 int xt_tune_get(int knob);

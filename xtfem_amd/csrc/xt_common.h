@@ -201,6 +201,7 @@ struct SolverWorkspace {
     DevBuf chunks, segchunk, segof;
     std::vector<int> hchunks, hsegchunk, hsegof;
     DevBuf partial, hdev, hdev2, ndev, coef;
+    DevBuf reorth;                   // device flag: skip the second Gram-Schmidt pass
     // Givens state of a cycle in one device region (Hessenberg columns, rotations, rhs
     // of the least squares problem, last residual norms, counters, frozen flags), the
     // targets and the residual history; pinned host staging mirrors the state region

@@ -31,10 +31,13 @@ namespace xt {
 static constexpr int JB = 8;  // basis vectors per multidot block row
 
 // partial[b][j][col] = sum over the rows of chunk b of V_j[row][col] * w[row][col]; chunk b
-// covers rows chunks[b].x .. chunks[b].y - 1 (chunks never straddle two segments)
+// covers rows chunks[b].x .. chunks[b].y - 1 (chunks never straddle two segments).
+// skip (device flag, may be NULL): nonzero makes the launch a no-op.
 __global__ void __launch_bounds__(256)
 multidot_partial(int c, int k, const int2 *__restrict__ chunks, const double *__restrict__ V,
-                 size_t ldv, const double *__restrict__ w, double *__restrict__ partial) {
+                 size_t ldv, const double *__restrict__ w, double *__restrict__ partial,
+                 const int *skip) {
+    if (skip && *skip) return;   // a second Gram-Schmidt pass found unnecessary
     __shared__ double sh[256];
     const int rpl = 256 / c;
     const int lr = threadIdx.x / c;
@@ -70,12 +73,18 @@ multidot_partial(int c, int k, const int2 *__restrict__ chunks, const double *__
 
 // out[j][g] = sum over the chunks b of segment s, sum_{i<q} partial[b][j][r*q + i] for
 // instance g = s*R + r (segment s = a mesh's rows, r = load case): one workgroup per
-// (j, g), its threads stride over the terms, then a tree sum in LDS.
+// (j, g), its threads stride over the terms, then a tree sum in LDS.  A nonzero skip
+// flag writes zeros instead.
 __global__ void __launch_bounds__(256)
 reduce_partials(int k, int c, int q, int R, int I, const int *__restrict__ seg_chunk,
-                const double *__restrict__ partial, double *__restrict__ out) {
+                const double *__restrict__ partial, double *__restrict__ out,
+                const int *skip) {
     __shared__ double sh[256];
     const int j = blockIdx.x, g = blockIdx.y;
+    if (skip && *skip) {   // skipped pass: zero coefficients
+        if (threadIdx.x == 0) out[(size_t)j * I + g] = 0.0;
+        return;
+    }
     const int sg = g / R, r = g - sg * R;
     const int b0 = seg_chunk[sg];
     const int n = (seg_chunk[sg + 1] - b0) * q;
@@ -106,13 +115,34 @@ __device__ __forceinline__ int instance_of(long long idx, int c, int q, int R,
 __global__ void __launch_bounds__(256)
 multiaxpy(long long n, int c, int q, int R, int I, const int *__restrict__ seg_of, int k,
           double alpha, const double *__restrict__ coef, const double *__restrict__ V,
-          size_t ldv, const double *yin, double *y) {
+          size_t ldv, const double *yin, double *y, const int *skip) {
+    if (skip && *skip) return;   // (in place: y keeps its values)
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const int g = instance_of(idx, c, q, R, seg_of);
     double s = yin ? yin[idx] : 0.0;
     for (int j = 0; j < k; ++j) s += (alpha * coef[j * I + g]) * V[(size_t)j * ldv + idx];
     y[idx] = s;
+}
+
+// Whether the second Gram-Schmidt pass of a step can be skipped ("twice is enough",
+// Kahan-Parlett): after the first pass w' = w - sum_j h_j V_j, |w'|^2 = |w|^2 - sum_j
+// h_j^2; when every instance keeps at least half of |w|^2 there was no cancellation
+// worth a second pass.  h: k first-pass coefficients then <w, w> as row k, per
+// instance.  Sets *skip to 1 (skip) or 0; one workgroup.
+__global__ void __launch_bounds__(256)
+reorth_check_kernel(int k, int I, const double *__restrict__ h, int *skip) {
+    __shared__ int need;
+    if (threadIdx.x == 0) need = 0;
+    __syncthreads();
+    for (int g = threadIdx.x; g < I; g += 256) {
+        const double ww = h[(size_t)k * I + g];
+        double s = 0.0;
+        for (int j = 0; j < k; ++j) s += h[(size_t)j * I + g] * h[(size_t)j * I + g];
+        if (ww > 0.0 && !(ww - s >= 0.5 * ww)) atomicOr(&need, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *skip = need ? 0 : 1;
 }
 
 // v *= 1/sqrt(nrm2[g]) (0 if the instance is frozen or the norm zero)
@@ -453,6 +483,7 @@ struct Solver {
         XT_TRY(hdev.reserve(sizeof(double) * (im + 1) * I));
         XT_TRY(hdev2.reserve(sizeof(double) * (im + 1) * I));
         XT_TRY(ndev.reserve(sizeof(double) * I));
+        XT_TRY(ws.reorth.reserve(sizeof(int)));
         XT_TRY(coef.reserve(sizeof(double) * (im + 1) * I));
         XT_TRY(geps.reserve(sizeof(double) * I));
         size_t o = 0;
@@ -491,19 +522,21 @@ struct Solver {
     }
 
     // out[j][g] = <V_j, w>_g for j < k and every instance g
-    int dots(int k, const double *Vbase, const double *w, double *out) {
+    int dots(int k, const double *Vbase, const double *w, double *out,
+             const int *skip = nullptr) {
         dim3 grid(nblk, ceil_div(k, JB));
         hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, c, k,
-                           ws.chunks.as<const int2>(), Vbase, (size_t)n, w, partial.as<double>());
+                           ws.chunks.as<const int2>(), Vbase, (size_t)n, w, partial.as<double>(),
+                           skip);
         hipLaunchKernelGGL(reduce_partials, dim3(k, I), dim3(256), 0, s, k, c, q, R, I,
-                           ws.segchunk.as<int>(), partial.as<double>(), out);
+                           ws.segchunk.as<int>(), partial.as<double>(), out, skip);
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
     int maxpy(int k, double alpha, const double *cf, const double *Vbase, const double *yin,
-              double *y) {
+              double *y, const int *skip = nullptr) {
         hipLaunchKernelGGL(multiaxpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R, I,
-                           segof, k, alpha, cf, Vbase, (size_t)n, yin, y);
+                           segof, k, alpha, cf, Vbase, (size_t)n, yin, y, skip);
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
@@ -581,10 +614,20 @@ struct Solver {
     int step_launches(int i, int K, bool flexible, bool f32, bool f32prec, int hist_cap) {
         if (flexible) XT_TRY(op_prec_flex(i, Vj(i), Vj(i + 1), f32prec, f32));
         else XT_TRY(op_prec(Vj(i), Vj(i + 1), f32));
-        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
+        // CGS2; with selective reorthogonalisation the first pass also takes <w, w> (V_{i+1}
+        // follows V_i in the basis) and the second pass runs only where a check on the
+        // device finds cancellation (XT_TUNE_GMRES_REORTH: 1 always two passes)
+        const bool sel = tune_knob(XT_TUNE_GMRES_REORTH) != 1;
+        XT_TRY(dots(sel ? i + 2 : i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
         XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
-        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>()));
-        XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
+        const int *skip = nullptr;
+        if (sel) {
+            hipLaunchKernelGGL(reorth_check_kernel, dim3(1), dim3(256), 0, s, i + 1, I,
+                               hdev.as<double>(), ws.reorth.as<int>());
+            skip = ws.reorth.as<int>();
+        }
+        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>(), skip));
+        XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1), skip));
         XT_TRY(dots(1, Vj(i + 1), Vj(i + 1), ndev.as<double>()));
         // normalise V_{i+1} for every instance still in its cycle at this step (the
         // restart recombination reads it); zero it for the others
@@ -634,7 +677,8 @@ struct Solver {
             (const void *)(intptr_t)(flexible * 4 + f32 * 2 + f32prec),
             (const void *)(intptr_t)n, (const void *)(intptr_t)I, (const void *)(intptr_t)nblk,
             (const void *)(intptr_t)(hist != nullptr), (const void *)(intptr_t)hist_cap,
-            (const void *)(intptr_t)(ws.split.ns), (const void *)(intptr_t)R};
+            (const void *)(intptr_t)(ws.split.ns), (const void *)(intptr_t)R,
+            (const void *)(intptr_t)tune_knob(XT_TUNE_GMRES_REORTH), ws.reorth.ptr};
         SolverWorkspace::StepGraph *g = nullptr;
         for (SolverWorkspace::StepGraph &cand : ws.steps)
             if (cand.key == key) { g = &cand; break; }

@@ -148,3 +148,54 @@ def test_grouped_band_factor_uneven_meshes(dev, oracle_lib, mode, eps, xtol):
             assert ie == 0 and rel(xs[r], xo) <= 1e-6, (s, r, rel(xs[r], xo))
             if mode == 0:                            # both exact FP64 factors
                 assert abs(int(its[s, r]) - int(itsw[s, r])) <= 1, (s, r)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_active_case_compaction_matches_full_sweeps(dev, oracle_lib, mode):
+    """The factor sweeps of a batched solve over the load cases still iterating only
+    (XT_TUNE_GMRES_COMPACT, built-in) against the sweeps over every case: 8 cases from
+    12 to 95 Hz converge after different iteration counts, so the compacted solves drop
+    columns as they go; both must reach relres 1e-8, agree to 1e-6 relative (the split-k
+    sums of fewer columns round differently, which can move a stop by an iteration) and
+    take the same iterations within one."""
+    import torch
+    from xtfem_amd import fem, slab, tune
+    mesh = fem.structured_hex(24, 10, 3, 24.0, 10.0, 3.0)
+    # a heavy material (200 x steel's density): the mass term the A^-1 (x) K^-1
+    # preconditioner leaves out grows with the frequency, so the cases need 2 to 6
+    # iterations (the oracle's kronpgmres with the exact factor)
+    K, M, *_ = fem.assemble(mesh, 2.0e5, 0.3, 200 * 7.85e-9)
+    bc = fem.clamp_mask(mesh, 0, [0.0])
+    Kb, Mb, p = fem.applybc(K, bc), fem.applybc(M, bc), 3 * mesh.nn
+    rext = fem.traction_load(mesh, 0, 24.0, 0, 30.0)
+    freq = np.array([12.0, 21.0, 33.0, 47.0, 58.0, 71.0, 84.0, 95.0])
+    amp = np.linspace(30.0, 110.0, 8)
+    dt = np.round(0.5 * freq) / freq
+    w = 2 * np.pi * freq
+    c = slab.stcoeff_m(0.0, dt, w)
+    A, B = c["k"] + c["ktn"], c["m"] + c["mtn"]
+    fx, _ = slab.getfextc_m(0.0, w, dt)
+    rhs = np.array([slab.slab_rhs(p, fx[r] * amp[r], None, rext, fixed=np.tile(bc, 6))
+                    for r in range(8)])
+    op = dev.KronOperator([Kb, Mb], q=6)
+    pr = dev.BandPreconditioner(Kb, block=128)
+    b = dev.to_interleaved(rhs, p, 6)
+    out = {}
+    tune.set_knob("ir_skip32", 1)
+    try:
+        for comp in (1, 2):
+            tune.set_knob("gmres_compact", comp)
+            x = torch.zeros_like(b)
+            _, its, ierr, relres = dev.slab_solve_device(op, pr, A, B, b, x, 8, eps=1e-8,
+                                                         maxits=300, mode=mode,
+                                                         xtol=1e-7 if mode else 0.0)
+            torch.cuda.synchronize()
+            assert np.all(ierr == 0) and np.all(relres <= 1e-8), (comp, its, relres)
+            out[comp] = (x.clone(), its.copy())
+    finally:
+        tune.set_knob("gmres_compact", 0)
+        tune.set_knob("ir_skip32", 0)
+    (xf, itf), (xc, itc) = out[1], out[2]
+    assert float(torch.linalg.norm(xc - xf) / torch.linalg.norm(xf)) <= 1e-6
+    assert np.all(np.abs(itc - itf) <= 1), (itc, itf)
+    assert itf.max() > itf.min()           # the cases do converge at different counts

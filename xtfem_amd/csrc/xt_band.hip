@@ -463,7 +463,9 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
         else
             hipLaunchKernelGGL((band_tile_kernel<T, 64>), grid, dim3(256), 0, s, b, c, tri, nblk,
                                A, as, V, vs, add, ads, out, os, 0);
-    } else if (tri == 0 && chain_split_on(c) && bp.part.ptr) {
+    } else if (tri == 0 && chain_split_on(c) && bp.part.ptr &&
+               [&] { int ks = 0, ns = 1; chain_split(b, c, &ks, &ns);
+                     return sizeof(T) * (size_t)ns * b * c <= bp.part.bytes; }()) {
         // one product split over k: the partials of all splits, then their fixed-order sum
         int ks = 0, ns = 1;
         chain_split(b, c, &ks, &ns);
@@ -1051,28 +1053,80 @@ void band_dims(const BandPrec *bp, int *b, int *nb, int *bw) {
 
 bool band_has_fp64(const BandPrec *bp) { return bp->keep64; }
 
+// Columns of the active load cases, gathered (out[row][a q + j] = in[row][cols[a] q + j],
+// a < Ra) or scattered back (out[row][cols[a] q + j] = in[row][a q + j] for a < Ra, zero
+// for the inactive cases a >= Ra), rows < p.
 template <typename T>
-int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s) {
-    const int c = q * R, b = bp->b, nb = bp->nb;
+__global__ void __launch_bounds__(256)
+gather_cols_kernel(long long n, int q, int R, int Ra, const int *__restrict__ cols,
+                   const T *__restrict__ in, T *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;   // over p * Ra q
+    if (i >= n) return;
+    const int ca = Ra * q;
+    const long long row = i / ca;
+    const int e = (int)(i - row * ca), a = e / q, j = e - a * q;
+    out[i] = in[row * (long long)R * q + (long long)cols[a] * q + j];
+}
+template <typename T>
+__global__ void __launch_bounds__(256)
+scatter_cols_kernel(long long n, int q, int R, int Ra, const int *__restrict__ cols,
+                    const T *__restrict__ in, T *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;   // over p * R q
+    if (i >= n) return;
+    const int c = R * q;
+    const long long row = i / c;
+    const int e = (int)(i - row * c), a = e / q, j = e - a * q;
+    out[row * c + (long long)cols[a] * q + j] =
+        a < Ra ? in[row * (long long)Ra * q + (long long)a * q + j] : T(0);
+}
+
+template <typename T>
+int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s,
+               int Ra, const int *dcols) {
+    const int b = bp->b, nb = bp->nb;
     const int pi = sizeof(T) == 4 ? 1 : 0;
     if (pi == 0 && !bp->keep64) {
         set_error("band factor kept in FP32 only: no FP64 sweeps");
         return XT_E_UNSUPPORTED;
     }
+    const bool compact = dcols && Ra >= 0 && Ra < R;
+    if (compact && Ra == 0) {   // no active case: P^-1 0 = 0
+        hipLaunchKernelGGL(fill_zero_kernel<T>, dim3(ceil_div((long long)bp->p * q * R, 256)),
+                           dim3(256), 0, s, (long long)bp->p * q * R, y);
+        XT_HIP(hipGetLastError());
+        return XT_OK;
+    }
+    const int c = q * (compact ? Ra : R), cfull = q * R;
     const long long npad = (long long)nb * b * c, nreal = (long long)bp->p * c;
-    // sized for FP64 sweeps when the factor has them, so switching precision does not
-    // reallocate (captured graphs hold these pointers)
+    // sized for FP64 sweeps when the factor has them, and for all cases, so switching
+    // precision or the active set does not reallocate (captured graphs hold these pointers)
     const size_t es = bp->keep64 ? sizeof(double) : sizeof(T);
-    XT_TRY(bp->w1.reserve(es * npad));
-    XT_TRY(bp->w2.reserve(es * npad));
-    if (chain_split_on(c)) {
-        int ks = 0, ns = 1;
-        chain_split(b, c, &ks, &ns);
-        XT_TRY(bp->part.reserve(es * (size_t)ns * b * c));
+    XT_TRY(bp->w1.reserve(es * (size_t)nb * b * cfull));
+    XT_TRY(bp->w2.reserve(es * (size_t)nb * b * cfull));
+    if (chain_split_on(cfull)) {   // the largest ns x c over the column counts up to cfull
+        size_t most = 0;
+        for (int cc = q; cc <= cfull; cc += q) {
+            if (!chain_split_on(cc)) continue;
+            int ks = 0, ns = 1;
+            chain_split(b, cc, &ks, &ns);
+            most = std::max(most, (size_t)ns * cc);
+        }
+        XT_TRY(bp->part.reserve(es * most * b));
     }
     T *w1 = bp->w1.as<T>(), *w2 = bp->w2.as<T>();
-    if (minv) XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w1, s));
-    else XT_HIP(hipMemcpyAsync(w1, x, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
+    if (compact) {   // temporal mix of every case into w2, the active columns into w1
+        const T *src = x;
+        if (minv) {
+            XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w2, s));
+            src = w2;
+        }
+        hipLaunchKernelGGL(gather_cols_kernel<T>, dim3(ceil_div(nreal, 256)), dim3(256), 0, s,
+                           nreal, q, R, Ra, dcols, src, w1);
+    } else if (minv) {
+        XT_TRY(temporal_mix<T, T>(bp->p, q, R, minv, x, w1, s));
+    } else {
+        XT_HIP(hipMemcpyAsync(w1, x, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
+    }
     if (npad > nreal)
         hipLaunchKernelGGL(fill_zero_kernel<T>, dim3(ceil_div(npad - nreal, 256)), dim3(256), 0,
                            s, npad - nreal, w1 + nreal);
@@ -1177,12 +1231,19 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
                  2.0 * (2.0 * nb * tri * c + 2.0 * (nb - 1) * full * c));
     }
     XT_HIP(hipGetLastError());
-    XT_HIP(hipMemcpyAsync(y, w1, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
+    if (compact) {
+        const long long nfull = (long long)bp->p * cfull;
+        hipLaunchKernelGGL(scatter_cols_kernel<T>, dim3(ceil_div(nfull, 256)), dim3(256), 0, s,
+                           nfull, q, R, Ra, dcols, (const T *)w1, y);
+        XT_HIP(hipGetLastError());
+    } else {
+        XT_HIP(hipMemcpyAsync(y, w1, sizeof(T) * nreal, hipMemcpyDeviceToDevice, s));
+    }
     return XT_OK;
 }
 template int band_apply<double>(BandPrec *, int, int, const double *, const double *, double *,
-                                hipStream_t);
+                                hipStream_t, int, const int *);
 template int band_apply<float>(BandPrec *, int, int, const float *, const float *, float *,
-                               hipStream_t);
+                               hipStream_t, int, const int *);
 
 }  // namespace xt

@@ -16,9 +16,6 @@ namespace xt {
 
 int elestatus_device(int ne, int ngpe, double Dc, const double *D, int *elesta, int *gpsta,
                      int *failed, hipStream_t s);
-template <typename T>
-int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
-                 hipStream_t s);
 }  // namespace xt
 
 using namespace xt;

@@ -170,13 +170,28 @@ struct PrecSplit {
     std::vector<double> h64;
     std::vector<float> h32;
 };
+// Load cases whose columns the banded-factor sweeps must compute (the others' inputs are
+// zero: instances frozen in the current GMRES cycle): per run of the factor (one for a
+// single factor), Ra[g] active cases listed first in dcols[g R .. g R + R - 1] (device),
+// the inactive ones after them.  Ra[g] == R: every case, no compaction.
+struct ActiveCols {
+    std::vector<int> Ra;
+    const int *dcols = nullptr;
+};
+// DOF offsets of the runs of a banded-factor preconditioner ({0, p} for a single factor;
+// empty for ILUT factors).
+void prec_runs(xt_prec pr, std::vector<long long> &off);
 // Shifted factors of a preconditioner: their number, the shifts when non-NULL, and
 // whether they have FP64 copies (else the split serves the FP32 applies only).
 int prec_shifts(xt_prec pr, const double **shifts, bool *fp64 = nullptr);
 // Y = P^-1 X with the split above (sp may be NULL or empty: plain A^-1 (x) K^-1).
 template <typename T>
 int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *sp, const T *x,
-                     T *y, hipStream_t s);
+                     T *y, hipStream_t s, const ActiveCols *ac = nullptr);
+// Y = P^-1 X with the plain A^-1 (x) K^-1 (or ILUT) preconditioner.
+template <typename T>
+int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
+                 hipStream_t s, const ActiveCols *ac = nullptr);
 
 // Batched slab solve (xt_slab_solve); hist: residual history of a one-case FP64 solve;
 // xtol: forward-error test of the refinement (xt_slab_solve_tol).
@@ -202,6 +217,8 @@ struct SolverWorkspace {
     std::vector<int> hchunks, hsegchunk, hsegof;
     DevBuf partial, hdev, hdev2, ndev, coef;
     DevBuf reorth;                   // device flag: skip the second Gram-Schmidt pass
+    DevBuf actcols;                  // active load cases per factor run (compaction)
+    long long act_seq = 0;           // versions of actcols' lists
     // Givens state of a cycle in one device region (Hessenberg columns, rotations, rhs
     // of the least squares problem, last residual norms, counters, frozen flags), the
     // targets and the residual history; pinned host staging mirrors the state region
@@ -246,8 +263,11 @@ void band_delete(BandPrec *bp);
 void band_dims(const BandPrec *bp, int *b, int *nb, int *bw);
 int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da,
                   hipStream_t s);
+// Ra >= 0 with dcols (device, R entries, the Ra active cases first): the sweeps run on the
+// active cases' columns only, gathered after the temporal mix, and y is zero elsewhere.
 template <typename T>
-int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s);
+int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipStream_t s,
+               int Ra = -1, const int *dcols = nullptr);
 
 // Temporal mixing Y[dof][r*q+i] = sum_j M_r(i,j) X[dof][r*q+j] (M row-major [R][q][q]).
 template <typename TI, typename TO>

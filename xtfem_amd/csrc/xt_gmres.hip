@@ -432,12 +432,59 @@ struct Solver {
     // temporal split (shifted factors) of the FP64 and FP32 applies, or none
     const PrecSplit *split64 = nullptr, *split32 = nullptr;
 
-    // y = P^-1 x: the temporal inverses (or split mixes) mi64 / mi32, then the factor(s)
-    int papply64(const double *x, double *y) {
-        return prec_apply_split<double>(pr, R, q, mi64.as<double>(), split64, x, y, s);
+    // Active-column compaction of the banded-factor sweeps: the load cases with an
+    // instance still iterating in the current cycle, per run of the factor (instances
+    // frozen in the cycle have zero basis columns, so their preconditioned columns are
+    // zero and need no sweep).  Updated by set_active at the host's checks of the frozen
+    // flags; act_version changes with the lists (it keys the step graphs).
+    ActiveCols act;
+    bool act_on = false, act_full = true;
+    std::vector<long long> run_off;
+    std::vector<int> act_host;
+    long long act_version = -1;   // ws.act_seq when the lists were made (unique per process)
+    int set_active(const std::vector<int> &fz) {
+        if (!act_on) return XT_OK;
+        const int nrun = (int)run_off.size() - 1;
+        std::vector<int> lists((size_t)nrun * R), Ra(nrun, 0);
+        bool full = true;
+        for (int g = 0; g < nrun; ++g) {
+            int *l = &lists[(size_t)g * R];
+            std::vector<char> on(R, 0);
+            for (int sg = 0; sg < nseg; ++sg) {
+                const long long lo = nseg > 1 ? seg_dof[sg] : 0, hi = nseg > 1 ? seg_dof[sg + 1] : p;
+                if (lo >= run_off[g + 1] || hi <= run_off[g]) continue;   // not in this run
+                for (int r = 0; r < R; ++r) on[r] |= !fz[(size_t)sg * R + r];
+            }
+            int k = 0;
+            for (int r = 0; r < R; ++r) if (on[r]) l[k++] = r;
+            Ra[g] = k;
+            for (int r = 0; r < R; ++r) if (!on[r]) l[k++] = r;
+            full = full && Ra[g] == R;
+        }
+        act_full = full;
+        if (full) return XT_OK;
+        if (lists == act_host && Ra == act.Ra) return XT_OK;
+        act_host = lists;
+        act.Ra = Ra;
+        XT_TRY(ws.actcols.reserve(sizeof(int) * (size_t)nrun * R));
+        XT_HIP(hipMemcpyAsync(ws.actcols.ptr, act_host.data(), sizeof(int) * act_host.size(),
+                              hipMemcpyHostToDevice, s));
+        XT_HIP(hipStreamSynchronize(s));   // act_host may change at the next check
+        act.dcols = ws.actcols.as<int>();
+        act_version = ++ws.act_seq;
+        return XT_OK;
     }
-    int papply32(const float *x, float *y) {
-        return prec_apply_split<float>(pr, R, q, mi32.as<float>(), split32, x, y, s);
+    const ActiveCols *cur_act() const { return act_on && !act_full ? &act : nullptr; }
+
+    // y = P^-1 x: the temporal inverses (or split mixes) mi64 / mi32, then the factor(s);
+    // compact: only the active cases' columns (zero elsewhere)
+    int papply64(const double *x, double *y, bool compact = true) {
+        return prec_apply_split<double>(pr, R, q, mi64.as<double>(), split64, x, y, s,
+                                        compact ? cur_act() : nullptr);
+    }
+    int papply32(const float *x, float *y, bool compact = true) {
+        return prec_apply_split<float>(pr, R, q, mi32.as<float>(), split32, x, y, s,
+                                       compact ? cur_act() : nullptr);
     }
 
     double *Vj(int j) { return V.as<double>() + (size_t)j * n; }
@@ -447,6 +494,8 @@ struct Solver {
         c = q * R;
         n = (long long)p * c;
         I = nseg * R;
+        prec_runs(pr, run_off);   // banded factors: compact the sweeps (XT_TUNE_GMRES_COMPACT)
+        act_on = !run_off.empty() && R > 1 && tune_knob(XT_TUNE_GMRES_COMPACT) != 1;
         XT_ARG(c <= 256, "q*R must be <= 256 for the batched solver");
         XT_TRY(V.reserve(sizeof(double) * n * (im + 1)));
         if (flexible) XT_TRY(Z.reserve(sizeof(double) * n * im));
@@ -581,12 +630,13 @@ struct Solver {
     }
     // x += P^-1 u
     int prec_add(const double *u, double *x, bool f32) {
+        // every case of the cycle: those frozen during it need their correction too
         if (!f32) {
-            XT_TRY(papply64(u, z64.as<double>()));
+            XT_TRY(papply64(u, z64.as<double>(), false));
             hipLaunchKernelGGL(add_into<double>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, z64.as<double>(), x);
         } else {
             hipLaunchKernelGGL(cvt_f64_f32, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, u, w32a.as<float>());
-            XT_TRY(papply32(w32a.as<float>(), w32b.as<float>()));
+            XT_TRY(papply32(w32a.as<float>(), w32b.as<float>(), false));
             hipLaunchKernelGGL(add_into<float>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w32b.as<float>(), x);
         }
         XT_HIP(hipGetLastError());
@@ -678,7 +728,8 @@ struct Solver {
             (const void *)(intptr_t)n, (const void *)(intptr_t)I, (const void *)(intptr_t)nblk,
             (const void *)(intptr_t)(hist != nullptr), (const void *)(intptr_t)hist_cap,
             (const void *)(intptr_t)(ws.split.ns), (const void *)(intptr_t)R,
-            (const void *)(intptr_t)tune_knob(XT_TUNE_GMRES_REORTH), ws.reorth.ptr};
+            (const void *)(intptr_t)tune_knob(XT_TUNE_GMRES_REORTH), ws.reorth.ptr,
+            (const void *)(intptr_t)(cur_act() ? act_version : -1)};
         SolverWorkspace::StepGraph *g = nullptr;
         for (SolverWorkspace::StepGraph &cand : ws.steps)
             if (cand.key == key) { g = &cand; break; }
@@ -773,6 +824,7 @@ struct Solver {
                 icyc[r] = 0;
             }
             if (!any) break;
+            XT_TRY(set_active(fz));
             // the cycle's Givens state lives on the device; the host reads the frozen
             // flags every kCheck steps and the whole state at the end of the cycle
             its_before = its;
@@ -801,6 +853,7 @@ struct Solver {
                     bool cont = false;
                     for (int r = 0; r < I; ++r) cont = cont || !fz[r];
                     if (!cont) break;
+                    XT_TRY(set_active(fz));
                 }
                 ++i;
             }

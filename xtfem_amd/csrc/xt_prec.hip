@@ -495,13 +495,16 @@ split_scatter_kernel(int p, int q, int c, int ns, const int *__restrict__ cs,
 
 template <typename T>
 int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool f32,
-                 hipStream_t s) {
-    if (pr->band) return band_apply<T>(pr->band, R, q, minv, x, y, s);
+                 hipStream_t s, const ActiveCols *ac) {
+    if (pr->band)
+        return band_apply<T>(pr->band, R, q, minv, x, y, s, ac ? ac->Ra[0] : -1,
+                             ac ? ac->dcols : nullptr);
     if (!pr->groups.empty()) {   // rows of run g: goff[g] .. goff[g+1]-1, all R q columns
         const long long c = (long long)q * R;
         for (size_t g = 0; g < pr->groups.size(); ++g)
             XT_TRY(band_apply<T>(pr->groups[g], R, q, minv, x + pr->goff[g] * c,
-                                 y + pr->goff[g] * c, s));
+                                 y + pr->goff[g] * c, s, ac ? ac->Ra[g] : -1,
+                                 ac ? ac->dcols + g * R : nullptr));
         return XT_OK;
     }
     T *stage = nullptr;
@@ -512,6 +515,14 @@ int prec_apply_t(xt_prec pr, int R, int q, const T *minv, const T *x, T *y, bool
     return ilut_apply<T>(pr, R, q, minv, x, y, f32, stage, true, s);
 }
 unsigned long long prec_id(xt_prec pr) { return pr->id; }
+void prec_runs(xt_prec pr, std::vector<long long> &off) {
+    off.clear();
+    if (pr->band) {
+        off = {0, (long long)pr->p};
+    } else {
+        for (int o : pr->goff) off.push_back(o);
+    }
+}
 bool prec_has_fp64(xt_prec pr) {
     for (const BandPrec *b : pr->groups)
         if (!band_has_fp64(b)) return false;
@@ -526,9 +537,9 @@ int prec_shifts(xt_prec pr, const double **shifts, bool *fp64) {
 
 template <typename T>
 int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *sp, const T *x,
-                     T *y, hipStream_t s) {
+                     T *y, hipStream_t s, const ActiveCols *ac) {
     const bool f32 = sizeof(T) == 4;
-    if (!sp || sp->ns == 0 || !pr->band) return prec_apply_t<T>(pr, R, q, minv, x, y, f32, s);
+    if (!sp || sp->ns == 0 || !pr->band) return prec_apply_t<T>(pr, R, q, minv, x, y, f32, s, ac);
     const int pi = f32 ? 1 : 0, p = pr->p, ns = sp->ns, c = q * R;
     XT_TRY(pr->gin[pi].reserve(sizeof(T) * (size_t)p * ns));
     XT_TRY(pr->gout[pi].reserve(sizeof(T) * (size_t)p * ns));
@@ -540,7 +551,8 @@ int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *s
                        sp->dcase.as<int>(), sp->dbase.as<long long>(), sp->dstride.as<int>(), m2v,
                        x, gi);
     XT_HIP(hipGetLastError());
-    XT_TRY(band_apply<T>(pr->band, R, q, minv, x, y, s));
+    XT_TRY(band_apply<T>(pr->band, R, q, minv, x, y, s, ac ? ac->Ra[0] : -1,
+                         ac ? ac->dcols : nullptr));
     for (size_t g = 0; g + 1 < sp->goff.size(); ++g) {
         const int o = sp->goff[g], ng = sp->goff[g + 1] - o;
         XT_ARG(sp->gfac[g] >= 0 && sp->gfac[g] < (int)pr->shifted.size(), "split factor index");
@@ -554,14 +566,14 @@ int prec_apply_split(xt_prec pr, int R, int q, const T *minv, const PrecSplit *s
     return XT_OK;
 }
 template int prec_apply_split<double>(xt_prec, int, int, const double *, const PrecSplit *,
-                                      const double *, double *, hipStream_t);
+                                      const double *, double *, hipStream_t, const ActiveCols *);
 template int prec_apply_split<float>(xt_prec, int, int, const float *, const PrecSplit *,
-                                     const float *, float *, hipStream_t);
+                                     const float *, float *, hipStream_t, const ActiveCols *);
 
 template int prec_apply_t<double>(xt_prec, int, int, const double *, const double *, double *,
-                                  bool, hipStream_t);
+                                  bool, hipStream_t, const ActiveCols *);
 template int prec_apply_t<float>(xt_prec, int, int, const float *, const float *, float *, bool,
-                                 hipStream_t);
+                                 hipStream_t, const ActiveCols *);
 
 // Staging of the allocation-free apply (xt_prec_apply_noalloc): one array per device,
 // held until the process ends, shared by every call on that device.  Host code never

@@ -494,6 +494,9 @@ int xt_graphs_enable(int on);
  *                         |w|^2 for some instance 2 (built-in 2)            1, 2
  *   XT_TUNE_GMRES_COMPACT banded-factor sweeps of a batched solve over the load cases still
  *                         iterating in the cycle only 2, over all 1 (built-in 2)   1, 2
+ *   XT_TUNE_BAND_CHAIN_FEW as XT_TUNE_BAND_CHAIN for chain steps with at most 16 columns on
+ *                         the matrix cores: one 16 x 16 tile per workgroup 1, split-k 64 x
+ *                         64 tiles 2 (built-in 1)                            1, 2
  * XT_E_ARG for a value a knob does not take. */
 #define XT_TUNE_SPMM_UNR 0
 #define XT_TUNE_SPMM_WPB 1
@@ -509,7 +512,8 @@ int xt_graphs_enable(int on);
 #define XT_TUNE_BAND_LEVELS 11
 #define XT_TUNE_GMRES_REORTH 12
 #define XT_TUNE_GMRES_COMPACT 13
-#define XT_TUNE_NKNOBS 14
+#define XT_TUNE_BAND_CHAIN_FEW 14
+#define XT_TUNE_NKNOBS 15
 int xt_tune_set(int knob, int value);
 /* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
 int xt_tune_get(int knob);

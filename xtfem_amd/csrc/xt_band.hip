@@ -98,6 +98,7 @@ constexpr int kRowsMaxB = 1024;   // block rows from which few columns go to mat
 constexpr int kRowsMaxB64 = 512;  // the same in FP64 (768-row blocks, 6-16 columns: the
                                   // matrix-core sweeps took a plate step from 49.7 to 38 ms)
 constexpr bool kChainSplitDefault = true;    // chain steps as split-k tile products
+constexpr bool kChainSplitFewDefault = false;   // the same with at most 16 columns
 
 template <typename T, int CM, int RW>
 __global__ void __launch_bounds__(256)
@@ -425,7 +426,11 @@ static void chain_split(int b, int c, int *ks, int *ns) {
 // tile with its 4 waves splitting k): XT_TUNE_BAND_CHAIN 2-4 / 1, built-in: split.
 static bool chain_split_on(int c) {
     const int k = tune_knob(XT_TUNE_BAND_CHAIN);
-    return c > 16 && (k >= 2 || (k == 0 && kChainSplitDefault));
+    if (c <= 16) {   // few columns (XT_TUNE_BAND_CHAIN_FEW): split-k tiles 2, one 16 x 16
+        const int kf = tune_knob(XT_TUNE_BAND_CHAIN_FEW);   // tile per workgroup 1
+        return kf == 2 || (kf == 0 && kChainSplitFewDefault);
+    }
+    return k >= 2 || (k == 0 && kChainSplitDefault);
 }
 
 // Algorithmic bytes and flops of one sweep kernel launch: the matrix elements of the
@@ -1186,7 +1191,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             if (cand.exec && cand.c == c && cand.pi == pi &&
                 cand.tile == tune_knob(XT_TUNE_BAND_TILE) &&
                 cand.smallc == tune_knob(XT_TUNE_BAND_SMALLC) &&
-                cand.chain == tune_knob(XT_TUNE_BAND_CHAIN) &&
+                cand.chain == tune_knob(XT_TUNE_BAND_CHAIN) * 4 + tune_knob(XT_TUNE_BAND_CHAIN_FEW) &&
                 std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
                 break;
@@ -1216,7 +1221,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             g->pi = pi;
             g->tile = tune_knob(XT_TUNE_BAND_TILE);
             g->smallc = tune_knob(XT_TUNE_BAND_SMALLC);
-            g->chain = tune_knob(XT_TUNE_BAND_CHAIN);
+            g->chain = tune_knob(XT_TUNE_BAND_CHAIN) * 4 + tune_knob(XT_TUNE_BAND_CHAIN_FEW);
             std::copy(key, key + kKey, g->key);
         }
         XT_HIP(hipGraphLaunch(g->exec, s));

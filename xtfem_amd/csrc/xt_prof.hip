@@ -91,6 +91,7 @@ int xt_tune_set(int knob, int value) {
         {0, 1, 2, 3, 5},       // XT_TUNE_BAND_LEVELS
         {0, 1, 2, 2, 2},       // XT_TUNE_GMRES_REORTH
         {0, 1, 2, 2, 2},       // XT_TUNE_GMRES_COMPACT
+        {0, 1, 2, 2, 2},       // XT_TUNE_BAND_CHAIN_FEW
     };
     bool ok = false;
     for (int v : allowed[knob]) ok = ok || v == value;

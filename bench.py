@@ -83,6 +83,8 @@ def parse():
                     help="only run the prismatic-rod ILUT leg (K timed slabs) and print it")
     ap.add_argument("--damage-only", action="store_true",
                     help="only the damage-kernel leg (60x60x12 block), one JSON line")
+    ap.add_argument("--fatigue-only", action="store_true",
+                    help="only the fatigue loop leg (notched 60x60x12 block), one JSON line")
     ap.add_argument("--thermo-only", action="store_true",
                     help="only run the thermo-mechanical plate leg and print it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -1120,6 +1122,9 @@ def main():
         return
     if args.damage_only:
         print(json.dumps(damage_leg(args.damage_nip, slabs=args.steps)))
+        return
+    if args.fatigue_only:
+        print(json.dumps(fatigue_leg(args.fatigue_blocks, args.damage_nip)))
         return
     R, K, W = args.cases, args.steps, args.warmup
     if args.workload == "multimesh":

@@ -163,6 +163,16 @@ int xt_adddiagonal(int nnse, int ndofn, int ndf, const int *ndfail, int *ki, int
  * spmtx.f90:268-307. */
 int xt_csrapplybc(int *ia, int *ja, double *a, int n, int *nz, const int *bc);
 
+/* Values of the 1-based CSR (ia, ja, a) of order n in the pattern (pia, pja) of the same
+ * order (both with rows sorted by column): out[k] = a at pattern entry k, 0 where the
+ * matrix has no entry; XT_E_ARG when the matrix has an entry outside the pattern.  For
+ * the matrices a deletion event leaves (delelement / adddiagonal / csrapplybc only drop
+ * entries of the assembled pattern or set its diagonal), so xt_op_set_values and
+ * xt_prec_refactor_band update the device handles in place.  No counterpart in the
+ * reference, which rebuilds its MUMPS structure (main3d.f90:683-684). */
+int xt_csr_values_in_pattern(int n, const int *pia, const int *pja, const int *ia,
+                             const int *ja, const double *a, double *out);
+
 /* ---- 2. device-resident handle API ------------------------------------------- */
 
 typedef struct xt_op_s *xt_op;        /* Kronecker operator sum_k T_k (x) S_k   */

@@ -251,3 +251,50 @@ def test_delelement_errors(bar):
     # an empty failed list changes nothing
     assert same(api.delelement(mesh.xconn, mesh.xcoord, dns, dnxyz, fem.SD, stiff, wqs,
                                np.zeros(0, np.int32), bar["K"]), bar["K"])
+
+
+def test_csr_values_in_pattern_after_deletion():
+    """The matrices a deletion event leaves fit the element-coupling patterns of K and M
+    (fem.dof_pattern, checked against the elements' DOF pairs by brute force): their
+    values placed in them (zeros for dropped entries) reproduce them exactly as dense
+    matrices, and a matrix with an entry outside the pattern is refused."""
+    from xtfem_amd import api, fem
+    from xtfem_amd.deletion import ElementDeletion
+    nx, ny, nz = 6, 5, 2
+    mesh = fem.structured_hex(nx, ny, nz, float(nx), float(ny), float(nz))
+    K, M, *_ = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+    bc = fem.clamp_mask(mesh, 1, [0.0])
+    for readd in (False, True):
+        dl = ElementDeletion(mesh, K, M, bc, 2.0e5, 0.3, Dc=0.3, readd=readd)
+        ev = dl.apply(np.array([1, 2, nx * ny * nz], dtype=np.int32))   # incl. a corner
+        n = len(K[0]) - 1
+        # the element-coupling patterns (the assembly drops entries that sum to zero)
+        PK, PM = fem.dof_pattern(mesh, "all"), fem.dof_pattern(mesh, "equal")
+        for S, Pm in ((ev.K, PK), (ev.M, PM), (dl.K, PK), (K, PK), (M, PM)):
+            v = api.csr_values_in_pattern(Pm, S)
+            got = np.zeros((n, n))
+            rows = np.repeat(np.arange(n), np.diff(Pm[0]))
+            got[rows, Pm[1] - 1] = v
+            want = np.zeros((n, n))
+            rs = np.repeat(np.arange(n), np.diff(S[0]))
+            want[rs, np.asarray(S[1]) - 1] = S[2]
+            assert np.array_equal(got, want)
+        for comp, Pm in (("all", PK), ("equal", PM)):   # the pattern itself, brute force
+            want = np.zeros((n, n), bool)
+            for e in mesh.xconn - 1:
+                d = (3 * e[:, None] + np.arange(3)).ravel()
+                if comp == "all":
+                    want[np.ix_(d, d)] = True
+                else:
+                    for a in range(3):
+                        want[np.ix_(3 * e + a, 3 * e + a)] = True
+            got = np.zeros((n, n), bool)
+            got[np.repeat(np.arange(n), np.diff(Pm[0])), Pm[1] - 1] = True
+            assert np.array_equal(got, want), comp
+    # an entry outside the pattern
+    ia = np.array([1, 2, 3], dtype=np.int32)
+    ja = np.array([1, 2], dtype=np.int32)
+    bad = (np.array([1, 3, 4], dtype=np.int32), np.array([1, 2, 2], dtype=np.int32),
+           np.array([1.0, 5.0, 2.0]))
+    with pytest.raises(api.XtError):
+        api.csr_values_in_pattern((ia, ja), bad)

@@ -71,6 +71,7 @@ _SIGS = {
     "xt_prec_create_band_seg": (ci, [ci, vp, vp, vp, ci, vp, ci, ci, vp]),
     "xt_prec_band_groups": (ci, [vp, vp, vp, vp]),
     "xt_slab_stats": (ci, [vp, vp, vp]),
+    "xt_csr_values_in_pattern": (ci, [ci, vp, vp, vp, vp, vp, vp]),
     "xt_prec_band_info": (ci, [vp, vp, vp, vp]),
     "xt_prec_destroy": (None, [vp]),
     "xt_prec_levels": (ci, [vp, vp, vp]),

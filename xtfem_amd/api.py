@@ -250,6 +250,17 @@ def csrapplybc(S, bc):
     return ia, ja[:nz.value].copy(), a[:nz.value].copy()
 
 
+def csr_values_in_pattern(P, S):
+    """Values of the 1-based CSR S in the pattern of the 1-based CSR P (zeros where S has
+    no entry); XtError when S has an entry outside P (xt_csr_values_in_pattern)."""
+    pia, pja = _i32(P[0]), _i32(P[1])
+    ia, ja, a = _i32(S[0]), _i32(S[1]), np.ascontiguousarray(S[2], dtype=np.float64)
+    out = np.zeros(len(pja))
+    check(lib().xt_csr_values_in_pattern(len(pia) - 1, _P(pia), _P(pja), _P(ia), _P(ja), _P(a),
+                                         _P(out)), "csr_values_in_pattern")
+    return out
+
+
 __all__ = ["kronspmv", "kronspgemv", "amux", "ilut", "lusol", "kronlusol", "kronpgmres",
            "kokkos_host", "kokkos_damage", "chkelesta", "delelement", "chkcnsta", "chkndsta",
-           "adddiagonal", "csrapplybc", "XtError"]
+           "adddiagonal", "csrapplybc", "csr_values_in_pattern", "XtError"]

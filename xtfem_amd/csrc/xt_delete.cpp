@@ -354,4 +354,24 @@ int xt_csrapplybc(int *ia, int *ja, double *a, int n, int *nz, const int *bc) {
     return XT_OK;
 }
 
+int xt_csr_values_in_pattern(int n, const int *pia, const int *pja, const int *ia, const int *ja,
+                             const double *a, double *out) {
+    XT_ARG(n > 0 && pia && pja && ia && ja && a && out, "csr_values_in_pattern arguments");
+    for (int r = 0; r < n; ++r) {   // row-wise merge of two column-sorted rows
+        int k = pia[r] - 1;
+        const int k1 = pia[r + 1] - 1;
+        for (int m = ia[r] - 1; m < ia[r + 1] - 1; ++m) {
+            while (k < k1 && pja[k] < ja[m]) out[k++] = 0.0;
+            if (k == k1 || pja[k] != ja[m]) {
+                xt::set_error("csr_values_in_pattern: row %d column %d is not in the pattern", r + 1,
+                          ja[m]);
+                return XT_E_ARG;
+            }
+            out[k++] = a[m];
+        }
+        while (k < k1) out[k++] = 0.0;
+    }
+    return XT_OK;
+}
+
 }  // extern "C"

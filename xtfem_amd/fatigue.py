@@ -16,7 +16,8 @@ state belongs to the specimen): the solve is ``xt_slab_solve``
 (FP32-IR or FP64), the solution goes to the damage kernel in the reference's
 temporal-major layout (``xt_from_interleaved``), the element status is the device
 reduction, and a deletion event edits the matrices on the host (``ElementDeletion``)
-and rebuilds the device operator and banded factor from them.  The damage law's
+and puts their values into the device operator and banded factor (refactored on the
+device).  The damage law's
 constants are parameters (the reference's by default, damage.h:9-17).
 """
 from __future__ import annotations
@@ -27,7 +28,7 @@ import time
 import numpy as np
 import torch
 
-from . import api, device, slab
+from . import api, device, fem, slab
 from ._lib import check, lib
 from .deletion import ElementDeletion
 
@@ -67,11 +68,28 @@ class FatigueSlabs:
         self.separated = False
 
     def _build(self, K, M, fixed):
-        """Device operator, banded factor and fixed mask of the current matrices."""
-        self.op = device.KronOperator([K, M], q=6)
-        self.pr = device.BandPreconditioner(K)
+        """Device operator, banded factor and fixed mask of the current matrices, stored in
+        the element-coupling patterns of K and M (every coupling the elements make, stored
+        zeros included: the assembly and the 1-0 method drop entries that sum to zero,
+        which a deletion can make nonzero): every matrix a deletion event leaves fits them,
+        so after the first build the new values go into the existing handles
+        (xt_op_set_values, xt_prec_refactor_band from the device CSR) with no host-side
+        operator build or upload."""
         self.fixed = torch.as_tensor((np.asarray(fixed) != 0).astype(np.uint8), device="cuda")
-        self.tj = self.op.tmat(self.J, False)
+        if getattr(self, "op", None) is None:
+            self.patK = fem.dof_pattern(self.mesh, "all")
+            self.patM = fem.dof_pattern(self.mesh, "equal")
+            kv = api.csr_values_in_pattern(self.patK, K)
+            mv = api.csr_values_in_pattern(self.patM, M)
+            self.op = device.KronOperator([self.patK + (kv,), self.patM + (mv,)], q=6)
+            self.pr = device.BandPreconditioner(self.patK + (kv,))
+            self.patK_dev = tuple(torch.as_tensor(v, device="cuda") for v in self.patK)
+            self.tj = self.op.tmat(self.J, False)
+            return
+        kd = torch.as_tensor(api.csr_values_in_pattern(self.patK, K), device="cuda")
+        md = torch.as_tensor(api.csr_values_in_pattern(self.patM, M), device="cuda")
+        self.op.set_values(kd, md)
+        self.pr.refactor(self.patK_dev[0], self.patK_dev[1], kd)
 
     def step(self):
         """One slab: deletion check, right-hand side, solve, damage.  Returns the slab's

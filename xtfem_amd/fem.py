@@ -360,6 +360,39 @@ def node_adjacency(mesh: Mesh):
     return np.cumsum(ptr), cc
 
 
+def dof_pattern(mesh: Mesh, components="all"):
+    """1-based CSR pattern (rowptr, colidx; int32) of the 3-DOF matrices of the mesh with
+    every coupling its elements make, stored zeros included: all 3 x 3 component pairs of
+    nodes sharing an element (components "all": K) or equal components only ("equal":
+    M), columns sorted.  The assembled matrices (before the 1-0 method) and every matrix
+    an element deletion leaves have their entries in it."""
+    ptr, adj = node_adjacency(mesh)
+    nn = mesh.nn
+    deg = np.diff(ptr) + 1                          # neighbours and the node itself
+    allv = np.concatenate([adj, np.arange(nn)])
+    allr = np.concatenate([np.repeat(np.arange(nn), np.diff(ptr)), np.arange(nn)])
+    nodes = allv[np.lexsort((allv, allr))]          # each node row sorted by node
+    off = np.concatenate([[0], np.cumsum(deg)])     # node row n: nodes[off[n]:off[n+1]]
+    idx = np.repeat(np.arange(nn), 3)               # node of DOF row 3 n + a
+    if components == "all":
+        # DOF row 3 n + a: the columns 3 m + b for the nodes m of row n, b = 0, 1, 2
+        cols_node = (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+        lens = 3 * deg[idx]
+        starts = 3 * off[idx]
+    else:
+        # DOF row 3 n + a: the columns 3 m + a
+        lens = deg[idx]
+        starts = off[idx]
+    rp = np.zeros(3 * nn + 1, dtype=np.int64)
+    rp[1:] = np.cumsum(lens)
+    g = np.repeat(starts - rp[:-1], lens) + np.arange(rp[-1])
+    if components == "all":
+        cols = cols_node[g]
+    else:
+        cols = 3 * nodes[g] + np.repeat(np.tile(np.arange(3), nn), lens)
+    return (rp + 1).astype(np.int32), (cols + 1).astype(np.int32)
+
+
 def rcm_order(mesh: Mesh):
     """Reverse Cuthill-McKee order of the nodes: perm[new] = old.  Breadth-first from a
     pseudo-peripheral node of each connected part (lowest degree, then the farthest node

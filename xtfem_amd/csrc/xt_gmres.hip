@@ -32,11 +32,13 @@ static constexpr int JB = 8;  // basis vectors per multidot block row
 
 // partial[b][j][col] = sum over the rows of chunk b of V_j[row][col] * w[row][col]; chunk b
 // covers rows chunks[b].x .. chunks[b].y - 1 (chunks never straddle two segments).
-// skip (device flag, may be NULL): nonzero makes the launch a no-op.
+// skip (device flag, may be NULL): nonzero makes the launch a no-op.  frozen (device,
+// per instance, may be NULL): the columns of frozen instances are taken as zero.
 __global__ void __launch_bounds__(256)
 multidot_partial(int c, int k, const int2 *__restrict__ chunks, const double *__restrict__ V,
                  size_t ldv, const double *__restrict__ w, double *__restrict__ partial,
-                 const int *skip) {
+                 const int *skip, int q, int R, const int *__restrict__ seg_of,
+                 const int *__restrict__ frozen) {
     if (skip && *skip) return;   // a second Gram-Schmidt pass found unnecessary
     __shared__ double sh[256];
     const int rpl = 256 / c;
@@ -45,10 +47,12 @@ multidot_partial(int c, int k, const int2 *__restrict__ chunks, const double *__
     const int j0 = blockIdx.y * JB;
     const int rbeg = chunks[blockIdx.x].x;
     const int rend = chunks[blockIdx.x].y;
+    // a frozen instance's columns are zero: no loads (a chunk lies in one segment)
+    const bool off = frozen && frozen[(seg_of ? seg_of[rbeg] * R : 0) + col / q];
     double acc[JB];
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) acc[jj] = 0.0;
-    if (lr < rpl) {
+    if (lr < rpl && !off) {
         for (int row = rbeg + lr; row < rend; row += rpl) {
             const size_t o = (size_t)row * c + col;
             const double wv = w[o];
@@ -111,15 +115,18 @@ __device__ __forceinline__ int instance_of(long long idx, int c, int q, int R,
     return (seg_of ? seg_of[row] * R : 0) + r;
 }
 
-// y = (yin ? yin : 0) + alpha * sum_j coef[j][g] V_j, g the element's instance
+// y = (yin ? yin : 0) + alpha * sum_j coef[j][g] V_j, g the element's instance; frozen
+// (device, may be NULL, only with y == yin): frozen instances' elements are left alone
 __global__ void __launch_bounds__(256)
 multiaxpy(long long n, int c, int q, int R, int I, const int *__restrict__ seg_of, int k,
           double alpha, const double *__restrict__ coef, const double *__restrict__ V,
-          size_t ldv, const double *yin, double *y, const int *skip) {
+          size_t ldv, const double *yin, double *y, const int *skip,
+          const int *__restrict__ frozen) {
     if (skip && *skip) return;   // (in place: y keeps its values)
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const int g = instance_of(idx, c, q, R, seg_of);
+    if (frozen && frozen[g]) return;   // in place on a frozen (zero) column: nothing to do
     double s = yin ? yin[idx] : 0.0;
     for (int j = 0; j < k; ++j) s += (alpha * coef[j * I + g]) * V[(size_t)j * ldv + idx];
     y[idx] = s;
@@ -572,20 +579,20 @@ struct Solver {
 
     // out[j][g] = <V_j, w>_g for j < k and every instance g
     int dots(int k, const double *Vbase, const double *w, double *out,
-             const int *skip = nullptr) {
+             const int *skip = nullptr, const int *frz = nullptr) {
         dim3 grid(nblk, ceil_div(k, JB));
         hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, c, k,
                            ws.chunks.as<const int2>(), Vbase, (size_t)n, w, partial.as<double>(),
-                           skip);
+                           skip, q, R, segof, frz);
         hipLaunchKernelGGL(reduce_partials, dim3(k, I), dim3(256), 0, s, k, c, q, R, I,
                            ws.segchunk.as<int>(), partial.as<double>(), out, skip);
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
     int maxpy(int k, double alpha, const double *cf, const double *Vbase, const double *yin,
-              double *y, const int *skip = nullptr) {
+              double *y, const int *skip = nullptr, const int *frz = nullptr) {
         hipLaunchKernelGGL(multiaxpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R, I,
-                           segof, k, alpha, cf, Vbase, (size_t)n, yin, y, skip);
+                           segof, k, alpha, cf, Vbase, (size_t)n, yin, y, skip, frz);
         XT_HIP(hipGetLastError());
         return XT_OK;
     }
@@ -668,17 +675,20 @@ struct Solver {
         // follows V_i in the basis) and the second pass runs only where a check on the
         // device finds cancellation (XT_TUNE_GMRES_REORTH: 1 always two passes)
         const bool sel = tune_knob(XT_TUNE_GMRES_REORTH) != 1;
-        XT_TRY(dots(sel ? i + 2 : i + 1, Vj(0), Vj(i + 1), hdev.as<double>()));
-        XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1)));
+        // (instances frozen in this cycle have zero columns: the CGS2 kernels skip them;
+        // the device flags are current here, the Givens update of the last step set them)
+        XT_TRY(dots(sel ? i + 2 : i + 1, Vj(0), Vj(i + 1), hdev.as<double>(), nullptr, frozen));
+        XT_TRY(maxpy(i + 1, -1.0, hdev.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1), nullptr,
+                     frozen));
         const int *skip = nullptr;
         if (sel) {
             hipLaunchKernelGGL(reorth_check_kernel, dim3(1), dim3(256), 0, s, i + 1, I,
                                hdev.as<double>(), ws.reorth.as<int>());
             skip = ws.reorth.as<int>();
         }
-        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>(), skip));
-        XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1), skip));
-        XT_TRY(dots(1, Vj(i + 1), Vj(i + 1), ndev.as<double>()));
+        XT_TRY(dots(i + 1, Vj(0), Vj(i + 1), hdev2.as<double>(), skip, frozen));
+        XT_TRY(maxpy(i + 1, -1.0, hdev2.as<double>(), Vj(0), Vj(i + 1), Vj(i + 1), skip, frozen));
+        XT_TRY(dots(1, Vj(i + 1), Vj(i + 1), ndev.as<double>(), nullptr, frozen));
         // normalise V_{i+1} for every instance still in its cycle at this step (the
         // restart recombination reads it); zero it for the others
         hipLaunchKernelGGL(scale_by_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c, q, R,

@@ -69,9 +69,9 @@ def parse():
     ap.add_argument("--shift-ratio", type=float, default=16.0,
                     help="spread of the dominant temporal eigenvalues one shifted factor covers "
                          "(0: no shifted factors)")
-    ap.add_argument("--band-levels", type=int, default=None,
+    ap.add_argument("--band-levels", type=int, default=0,
                     help="XT_TUNE_BAND_LEVELS for the factors (1: no pairing levels; "
-                         "default: 1 for the panel, the built-in choice otherwise)")
+                         "0: the built-in choice)")
     ap.add_argument("--legs", default="",
                     help="extra legs, comma-separated: rod, reassembly, thermo, driver_panel, "
                          "deletion, damage, fatigue, multimesh (or 'all')")
@@ -1051,9 +1051,9 @@ def setup_solver(prob, args, fp32_only):
     from xtfem_amd import device
     from xtfem_amd import tune
     t0 = time.perf_counter()
-    # the factors' pairing levels (fixed when a factor is built): none for the panel, whose
-    # chain steps (1856 x 1856 blocks times 192 columns, split over k) fill the GPU by
-    # themselves (FP64 apply 27.7 ms without pairing, 31.7 ms with four levels)
+    # the factors' pairing levels (built-in unless --band-levels; the sweeps use them per
+    # apply by column count: at 192 columns the panel's FP64 chain steps fill the GPU
+    # unpaired, 27.6 vs 31.0 ms, at a few columns pairing wins, 11.0 vs 11.9 ms)
     tune.set_knob("band_levels", args.band_levels)
     op = device.KronOperator([prob.Kb, prob.Mb], q=6)
     pr0 = device.BandPreconditioner(prob.Kb, block=args.block)
@@ -1100,8 +1100,6 @@ def accuracy_check(prob, op, pr, xtol):
 
 def main():
     args = parse()
-    if args.band_levels is None:
-        args.band_levels = 1 if args.workload == "panel" else 0
     if args.copy_probe:
         copy_probe()
         return

@@ -507,6 +507,8 @@ int xt_graphs_enable(int on);
  *   XT_TUNE_BAND_CHAIN_FEW as XT_TUNE_BAND_CHAIN for chain steps with at most 16 columns on
  *                         the matrix cores: one 16 x 16 tile per workgroup 1, split-k 64 x
  *                         64 tiles 2 (built-in 1)                            1, 2
+ *   XT_TUNE_BAND_PAIR     banded-factor sweeps: the factor's pairing levels never 1,
+ *                         always 2 (built-in: FP32 always, FP64 up to 64 columns)   1, 2
  * XT_E_ARG for a value a knob does not take. */
 #define XT_TUNE_SPMM_UNR 0
 #define XT_TUNE_SPMM_WPB 1
@@ -523,7 +525,8 @@ int xt_graphs_enable(int on);
 #define XT_TUNE_GMRES_REORTH 12
 #define XT_TUNE_GMRES_COMPACT 13
 #define XT_TUNE_BAND_CHAIN_FEW 14
-#define XT_TUNE_NKNOBS 15
+#define XT_TUNE_BAND_PAIR 15
+#define XT_TUNE_NKNOBS 16
 int xt_tune_set(int knob, int value);
 /* Current value of a knob (0 = built-in choice), -1 for an unknown knob. */
 int xt_tune_get(int knob);

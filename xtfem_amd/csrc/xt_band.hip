@@ -48,7 +48,8 @@ constexpr int kLevels = 4;     // pairing levels of the block recurrences, at mo
 constexpr int kMinChain = 6;   // a level is added while its chain keeps this many steps
 constexpr int kKey = 6 + 2 * kLevels;
 struct SweepGraph {
-    int c = 0, pi = -1, tile = 0, smallc = 0, chain = 0;   // columns, precision, knobs at capture
+    int c = 0, pi = -1, tile = 0, smallc = 0, chain = 0, nl = 0;   // columns, precision,
+                                                                     // knobs, levels used
     const void *key[kKey] = {};
     hipGraphExec_t exec = nullptr;
 };
@@ -99,6 +100,10 @@ constexpr int kRowsMaxB64 = 512;  // the same in FP64 (768-row blocks, 6-16 colu
                                   // matrix-core sweeps took a plate step from 49.7 to 38 ms)
 constexpr bool kChainSplitDefault = true;    // chain steps as split-k tile products
 constexpr bool kChainSplitFewDefault = false;   // the same with at most 16 columns
+// Columns up to which the FP64 sweeps use the factor's pairing levels (the 1856-row panel
+// factor: pairing 11.6 vs 14.5 ms per apply at 48 columns, 20.7 vs 18.4 at 72, 31.0 vs
+// 27.6 at 192); FP32 sweeps use them at every width (16.1 vs 17.5 ms at 192).
+constexpr int kPairMaxC64 = 64;
 
 template <typename T, int CM, int RW>
 __global__ void __launch_bounds__(256)
@@ -1138,7 +1143,12 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
     const long long bb = (long long)b * b, bc = (long long)b * c;
     const T *Dl = bp->Dl[pi].as<T>(), *Du = bp->Du[pi].as<T>();
     const T *G = bp->G[pi].as<T>(), *H = bp->H[pi].as<T>();
-    const int m = nb - 1, nlev = bp->nlev;
+    // pairing levels used by this apply: the factor's, or none when the columns alone
+    // give the chain steps enough work (XT_TUNE_BAND_PAIR: 1 never, 2 always; built-in:
+    // FP32 always, FP64 up to kPairMaxC64 columns)
+    const int pk = tune_knob(XT_TUNE_BAND_PAIR);
+    const int m = nb - 1,
+              nlev = (pk == 2 || (pk == 0 && (sizeof(T) == 4 || c <= kPairMaxC64))) ? bp->nlev : 0;
     // One recurrence in chain coordinates: block v (0..m) at W + v S, level-L matrix of
     // step v at Mb[L] + (v - 1) MS[L] (level 0: G forward, H in reverse backward).
     auto chain = [&](int dir, T *w, const T *M0, hipStream_t st) -> int {
@@ -1192,6 +1202,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
                 cand.tile == tune_knob(XT_TUNE_BAND_TILE) &&
                 cand.smallc == tune_knob(XT_TUNE_BAND_SMALLC) &&
                 cand.chain == tune_knob(XT_TUNE_BAND_CHAIN) * 4 + tune_knob(XT_TUNE_BAND_CHAIN_FEW) &&
+                cand.nl == nlev &&
                 std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
                 break;
@@ -1222,6 +1233,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             g->tile = tune_knob(XT_TUNE_BAND_TILE);
             g->smallc = tune_knob(XT_TUNE_BAND_SMALLC);
             g->chain = tune_knob(XT_TUNE_BAND_CHAIN) * 4 + tune_knob(XT_TUNE_BAND_CHAIN_FEW);
+            g->nl = nlev;
             std::copy(key, key + kKey, g->key);
         }
         XT_HIP(hipGraphLaunch(g->exec, s));

@@ -92,6 +92,7 @@ int xt_tune_set(int knob, int value) {
         {0, 1, 2, 2, 2},       // XT_TUNE_GMRES_REORTH
         {0, 1, 2, 2, 2},       // XT_TUNE_GMRES_COMPACT
         {0, 1, 2, 2, 2},       // XT_TUNE_BAND_CHAIN_FEW
+        {0, 1, 2, 2, 2},       // XT_TUNE_BAND_PAIR
     };
     bool ok = false;
     for (int v : allowed[knob]) ok = ok || v == value;

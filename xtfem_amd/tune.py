@@ -33,7 +33,8 @@ from ._lib import check, lib
 KNOBS = {"spmm_unr": 0, "spmm_wpb": 1, "spmm_r1split": 2, "band_tile": 3, "gmres_check": 4,
          "spmm_lds": 5, "band_smallc": 6, "ir_fp32_cycle": 7, "ir_min_gain": 8,
          "ir_skip32": 9, "band_chain": 10, "band_levels": 11,
-         "gmres_reorth": 12, "gmres_compact": 13, "band_chain_few": 14}
+         "gmres_reorth": 12, "gmres_compact": 13, "band_chain_few": 14,
+         "band_pair": 15}
 XT_PROF_SPMM, XT_PROF_BAND, XT_PROF_BAND64 = 0, 1, 4
 _CACHE: dict = {}
 

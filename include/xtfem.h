@@ -479,8 +479,9 @@ int xt_graphs_enable(int on);
  *                         (neighbour rows of X staged per slice) off 1, on 2 (where the
  *                         neighbour rows fit), built-in: off
  *   XT_TUNE_BAND_SMALLC   banded-factor sweeps with at most 16 columns: VALU rows kernel 1,
- *                         matrix-core kernels 2; built-in: rows kernel below 1024-row
- *                         blocks (FP32) or 512-row blocks (FP64)
+ *                         matrix-core kernels with 64-column tiles in batched passes 2,
+ *                         with 16-column tiles 3; built-in: rows kernel below 1024-row
+ *                         blocks (FP32) or 512-row blocks (FP64), else 3    1, 2, 3
  *   XT_TUNE_IR_FP32_CYCLE GMRES-IR: iterations an FP32 inner cycle gets to meet its target
  *                         before the case moves to FP64 cycles              1, 2, 4, 8
  *                         (built-in 2)

@@ -124,3 +124,43 @@ def test_band_chain_and_level_variants(dev, oracle_lib, block):
     finally:
         tune.set_knob("band_levels", 0)
         tune.set_knob("band_chain", 0)
+
+
+@pytest.mark.parametrize("smallc", [1, 2, 3])
+def test_band_few_column_variants(dev, oracle_lib, smallc):
+    """Sweeps with 6 and 12 columns on 256-row blocks (13 blocks: batched passes of several
+    blocks, one pairing level): the VALU rows kernel (1), the matrix-core kernels with
+    64-column tiles (2) and with 16-column tiles in the batched passes (3), FP64 and
+    FP32, with and without the pairing level, against the exact factor
+    (ilut.f:2169-2170)."""
+    import torch
+    from xtfem_amd import fem, slab, tune
+    mesh = fem.structured_hex(260, 1, 1, 260.0, 1.0, 1.0)
+    K, M, *_ = fem.assemble(mesh, 2.0e5, 0.3, 7.85e-9)
+    Kb, p = fem.applybc(K, fem.clamp_mask(mesh, 1, [0.0])), 3 * mesh.nn
+    alu, jlu, ju, ierr = oracle_lib.ilut(Kb, p, 0.0)
+    assert ierr == 0
+    rng = np.random.default_rng(smallc)
+    pr = dev.BandPreconditioner(Kb, block=256)
+    assert pr.block == 256 and pr.nblocks == 13
+    try:
+        tune.set_knob("band_smallc", smallc)
+        for pair in (1, 2):
+            tune.set_knob("band_pair", pair)
+            for R in (1, 2):
+                c = slab.stcoeff_m(0.0, 0.5, 2 * np.pi * rng.uniform(10, 100, R))
+                A = c["k"] + c["ktn"]
+                X = rng.standard_normal((R, 6 * p))
+                ref = np.stack([oracle_lib.kronlusol(A[r], alu, jlu, ju, X[r]) for r in range(R)])
+                for fp32, tol in ((False, 1e-9), (True, 1e-3)):
+                    dt = torch.float32 if fp32 else torch.float64
+                    minv = torch.as_tensor(np.linalg.inv(A), device="cuda").to(dt).contiguous()
+                    x = dev.to_interleaved(X, p, 6).to(dt).contiguous()
+                    y = torch.empty_like(x)
+                    pr.apply(minv, x, y, R, fp32=fp32)
+                    Y = dev.from_interleaved(y.double().contiguous(), p, 6, R)
+                    for r in range(R):
+                        assert rel(Y[r], ref[r]) < tol, (smallc, pair, R, fp32, r)
+    finally:
+        tune.set_knob("band_smallc", 0)
+        tune.set_knob("band_pair", 0)

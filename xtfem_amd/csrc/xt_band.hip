@@ -104,6 +104,9 @@ constexpr bool kChainSplitFewDefault = false;   // the same with at most 16 colu
 // factor: pairing 11.6 vs 14.5 ms per apply at 48 columns, 20.7 vs 18.4 at 72, 31.0 vs
 // 27.6 at 192); FP32 sweeps use them at every width (16.1 vs 17.5 ms at 192).
 constexpr int kPairMaxC64 = 64;
+// Batched passes with at most 16 columns on 16 x 16 tiles instead of 64 x 64 ones (the
+// panel factor at one load case: FP64 apply 11.3 -> 8.0 ms, FP32 6.0 -> 4.6 ms).
+constexpr bool kSmallcMfmaBatched = true;
 
 template <typename T, int CM, int RW>
 __global__ void __launch_bounds__(256)
@@ -460,7 +463,13 @@ static int launch_tile(const BandPrec &bp, int c, int tri, const T *A, long long
     const int b = bp.b;
     const bool prof = prof_on(band_kind<T>());
     if (prof) prof_begin(band_kind<T>(), s);
-    if (nblk > 1) {   // batched pass: 128-row tiles for large FP32 batches, else 64 (the
+    const int skn = tune_knob(XT_TUNE_BAND_SMALLC);
+    if (nblk > 1 && c <= 16 && (skn == 3 || (skn == 0 && kSmallcMfmaBatched))) {
+        // few columns, many blocks: 16 x 16 output tiles (k split over the 4 waves), a
+        // quarter of the 64-column tiles' padded products
+        hipLaunchKernelGGL(band_mfma_kernel<T>, dim3(b / 16, ceil_div(c, 16), nblk), dim3(256),
+                           0, s, b, c, tri, A, as, V, vs, add, ads, out, os);
+    } else if (nblk > 1) {   // batched pass: 128-row tiles for large FP32 batches, else 64 (the
                       // FP64 128-row tile holds 2 waves a SIMD and measured slower)
         const int tk = tune_knob(XT_TUNE_BAND_TILE);
         const bool can128 = sizeof(T) == 4 && b % 128 == 0;

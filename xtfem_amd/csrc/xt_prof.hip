@@ -83,7 +83,7 @@ int xt_tune_set(int knob, int value) {
         {0, 64, 128, 64, 64},  // XT_TUNE_BAND_TILE
         {0, 1, 2, 4, 4},       // XT_TUNE_GMRES_CHECK
         {0, 1, 2, 1, 1},       // XT_TUNE_SPMM_LDS
-        {0, 1, 2, 1, 1},       // XT_TUNE_BAND_SMALLC
+        {0, 1, 2, 3, 3},       // XT_TUNE_BAND_SMALLC
         {0, 1, 2, 4, 8},       // XT_TUNE_IR_FP32_CYCLE
         {0, 2, 8, 32, 128},    // XT_TUNE_IR_MIN_GAIN
         {0, 1, 4, 16, 64},     // XT_TUNE_IR_SKIP32

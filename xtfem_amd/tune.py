@@ -8,7 +8,8 @@ the launching stream (the library's per-launch profiler), the candidates of
   (1, 2, 4), for one load case lanes per node row (4, 8), for several the kernel that
   stages each slice's neighbour rows of X in LDS (off, on);
 * the banded-factor sweeps: the tile rows of the batched passes (64, 128 in FP32), and
-  for a few columns the VALU rows kernel or the matrix-core kernels;
+  for a few columns the VALU rows kernel or the matrix-core kernels (64- or 16-column
+  tiles in the batched passes);
 * the slab solve, timed on whole solves when a solve is given: the least residual
   reduction an FP32 refinement step of GMRES-IR must reach for a case to stay in FP32
   (2, 8, 32, 128), the iterations an FP32 inner cycle gets before a case moves to FP64
@@ -149,7 +150,7 @@ def autotune(op, pr, R, fp32=True, reps=7, solve=None):
         xs = x[:p * q].contiguous()                 # one load case: q columns
         ys = torch.empty_like(xs)
         sweep_small = lambda: pr.apply(None, xs, ys, 1, q=q, fp32=fp32)
-        best, res = _pick("band_smallc", (1, 2), kb, sweep_small, max(3, reps // 2))
+        best, res = _pick("band_smallc", (1, 2, 3), kb, sweep_small, max(3, reps // 2))
         out["picks"]["band_smallc"] = best
         out["timings_ms"]["band_smallc"] = res
     if solve is not None:

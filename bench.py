@@ -1213,11 +1213,32 @@ def main():
         print(f"# fp64 one case per solve: {K}x{R} slabs in {t64:.3f}s, its mean "
               f"{its64.mean():.1f}", file=sys.stderr, flush=True)
         del runs
+        # the same to 1e-12: one case per solve at the headline's accuracy
+        runs = [Runner(prob, op, pr, np.array([r]), mode=0, nsteps=nsteps) for r in range(R)]
+        for rn in runs:
+            rn.eps = 1e-12
+        for _ in range(W):
+            for rn in runs:
+                rn.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            for rn in runs:
+                rn.step()
+        torch.cuda.synchronize()
+        t64t = time.perf_counter() - t0
+        res["fp64_single_eps1e-12_slabs_per_s"] = K * R / t64t
+        res["fp64_single_eps1e-12_its_mean"] = float(np.mean([rn.its[W:] for rn in runs]))
+        print(f"# fp64 one case per solve to 1e-12: {K}x{R} slabs in {t64t:.3f}s",
+              file=sys.stderr, flush=True)
+        del runs
         res["precision_ratio"] = (K * R / t32) / res["fp64_batched_slabs_per_s"]
         res["precision_ratio_equal_accuracy"] = ((K * R / t32) /
                                                  res["fp64_batched_eps1e-12_slabs_per_s"])
         res["batching_ratio"] = res["fp64_batched_slabs_per_s"] / res["fp64_single_slabs_per_s"]
         res["speedup_vs_fp64_single"] = (K * R / t32) / res["fp64_single_slabs_per_s"]
+        res["speedup_vs_fp64_single_equal_accuracy"] = ((K * R / t32) /
+                                                        res["fp64_single_eps1e-12_slabs_per_s"])
     res["accuracy_slab0"] = accuracy_check(prob, op, pr, args.xtol)
     print(f"# accuracy (slab 0 vs FP64 at 1e-12): {res['accuracy_slab0']}", file=sys.stderr,
           flush=True)

@@ -449,7 +449,8 @@ def fatigue_leg(blocks=40, nip=2560, amp=250.0):
     tensile traction on y = 60, damage law with a 350 MPa fatigue limit and critical
     damage 0.95.  One cycle block = one slab of 10 load periods: FP32-IR slab solve, the
     damage kernel over nip temporal points of the slab, element status, and deletion with
-    the device rebuild when elements fail (xtfem_amd.fatigue)."""
+    the device rebuild when elements fail (xtfem_amd.fatigue: the operator's values
+    replaced, the banded factor refreshed from the first changed row on)."""
     import torch
     from xtfem_amd import fem
     from xtfem_amd.fatigue import FatigueSlabs
@@ -486,6 +487,8 @@ def fatigue_leg(blocks=40, nip=2560, amp=250.0):
                 ms_solve_and_damage_median=float(np.median(plain)) if plain else None,
                 deletion_events=len(dels), deleted_elements=int(len(fs.deleted())),
                 deletion_ms=[r["deletion_ms"] for r in dels][:8],
+                refresh_from_row_frac=[round(r.get("refresh_row", 0) / (3 * mesh.nn), 3)
+                                       for r in dels][:8],
                 its_max=int(max(r["its"] for r in recs)),
                 relres_max=float(max(r["relres"] for r in recs)),
                 all_converged=bool(all(r["ierr"] == 0 for r in recs)),

@@ -265,6 +265,16 @@ int xt_prec_shifts(xt_prec pr, int *nshift, double *shifts);
  * per-slab refactorization mumps_solver(job = 1), main3d.f90:783. */
 int xt_prec_refactor_band(xt_prec pr, const int *rowptr, const int *colidx,
                           const double *vals, void *stream);
+/* xt_prec_refactor_band for a K that changed only in rows and columns >= first_row
+ * (0-based; the caller guarantees it, e.g. the DOFs of deleted elements and of the nodes
+ * they orphan): the factor's blocks before first_row / block are kept and the rest, with
+ * the recurrence and pairing matrices that depend on them, recomputed from the kept
+ * D_{j-1} and K (equal to a full refactorization up to rounding).  A factor kept in FP32
+ * only is refactored whole.  Plain banded factors only (no shifted or grouped factors).
+ * Replaces the refactorization after element deletion, mumps_solver(job = 1),
+ * main3d.f90:686. */
+int xt_prec_refresh_band(xt_prec pr, const int *rowptr, const int *colidx,
+                         const double *vals, int first_row, void *stream);
 /* Block size, number of blocks and bandwidth of a banded-factor preconditioner (of a
  * grouped one: the largest block, the blocks of all runs, the largest bandwidth). */
 int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth);

@@ -298,3 +298,20 @@ def test_csr_values_in_pattern_after_deletion():
            np.array([1.0, 5.0, 2.0]))
     with pytest.raises(api.XtError):
         api.csr_values_in_pattern((ia, ja), bad)
+
+
+def test_refresh_row_first_changed_row():
+    """The first changed row of a 1-based CSR (the row a deletion's refresh starts at)."""
+    from xtfem_amd.fatigue import FatigueSlabs
+    ia = np.array([1, 3, 4, 7, 9])        # rows of 2, 1, 3, 2 entries
+    old = np.arange(8, dtype=float)
+    new = old.copy()
+    assert FatigueSlabs.refresh_row(ia, old, new) == 4
+    new[4] += 1.0                         # entry 4 is in row 2
+    assert FatigueSlabs.refresh_row(ia, old, new) == 2
+    new[3] -= 1.0                         # entry 3: row 2 still
+    assert FatigueSlabs.refresh_row(ia, old, new) == 2
+    new[2] = 9.0                          # entry 2: row 1
+    assert FatigueSlabs.refresh_row(ia, old, new) == 1
+    new[0] = -1.0
+    assert FatigueSlabs.refresh_row(ia, old, new) == 0

@@ -206,3 +206,46 @@ def test_band_factor_after_renumbering_and_wide_blocks(dev):
         A = sp.csr_matrix((Kb[2], Kb[1] - 1, Kb[0] - 1), shape=(p, p))
         for j in range(6):
             assert np.linalg.norm(A @ Y[j] - X[j]) / np.linalg.norm(X[j]) < 1e-8, (p, j)
+
+
+@pytest.mark.parametrize("where", ["block1", "middle", "boundary", "last", "full"])
+def test_band_refresh_matches_fresh_factor(dev, oracle_lib, where):
+    """xt_prec_refresh_band: K changed only in rows and columns >= r0 (K' = K + 0.3 P K22
+    P^T, the trailing principal part scaled: still SPD); the factor of K refreshed from
+    r0 on gives the sweeps of a factor of K' built afresh (FP64 to 1e-10, FP32 to 1e-4,
+    with and without the pairing levels) and solves K' y = x to 1e-9, on a 49-block
+    factor with three pairing levels.  Rows r0 in block 1 (E_0 from the kept D_0), in a
+    middle block, at a block boundary, in the last block, and r0 = 0 (whole refactor)."""
+    import torch
+    S = notched_plate(260, 1, 1, 0)
+    p, K = S["p"], S["Kb"]
+    ia, ja, a = (np.asarray(v) for v in K)
+    b = 64
+    r0 = {"block1": b + 10, "middle": 23 * b + 37, "boundary": 30 * b,
+          "last": p - 5, "full": 0}[where]
+    rows = np.repeat(np.arange(p), np.diff(ia))
+    a2 = a * np.where((rows >= r0) & (ja - 1 >= r0), 1.3, 1.0)
+    K2 = (ia, ja, a2)
+    pr = dev.BandPreconditioner(K, block=b)
+    assert pr.block == b and pr.nblocks >= 49
+    dv = lambda v, t: torch.as_tensor(np.ascontiguousarray(v, dtype=t), device="cuda")
+    pr.refactor(dv(ia, np.int32), dv(ja, np.int32), dv(a2, np.float64), first_row=r0)
+    fresh = dev.BandPreconditioner(K2, block=b)
+    rng = np.random.default_rng(11)
+    for R in (2, 32):                      # FP64: paired (<= 64 columns) and unpaired
+        X = rng.standard_normal((R, 6 * p))
+        x = dev.to_interleaved(X, p, 6)
+        for fp32, tol in ((False, 1e-10), (True, 1e-4)):
+            dt = torch.float32 if fp32 else torch.float64
+            xi = x.to(dt).contiguous()
+            y1, y2 = torch.empty_like(xi), torch.empty_like(xi)
+            pr.apply(None, xi, y1, R, fp32=fp32)
+            fresh.apply(None, xi, y2, R, fp32=fp32)
+            Y1, Y2 = (dev.from_interleaved(y.double().contiguous(), p, 6, R) for y in (y1, y2))
+            for r in range(R):
+                assert rel(Y1[r], Y2[r]) < tol, (where, R, fp32, r, rel(Y1[r], Y2[r]))
+            if not fp32:
+                for r in range(0, R, 7):
+                    Kx = np.concatenate([oracle_lib.amux(ia, ja, a2, Y1[r, j * p:(j + 1) * p])
+                                         for j in range(6)])
+                    assert rel(Kx, X[r]) < 1e-9, (where, R, r)

@@ -54,6 +54,7 @@ _SIGS = {
     "xt_op_apply": (ci, [vp, ci, vp, cd, cd, vp, vp, ci, vp]),
     "xt_op_set_values": (ci, [vp, vp, vp]),
     "xt_prec_refactor_band": (ci, [vp, vp, vp, vp, vp]),
+    "xt_prec_refresh_band": (ci, [vp, vp, vp, vp, ci, vp]),
     "xt_asm_create": (ci, [ci, ci, vp, vp, vp, vp]),
     "xt_asm_destroy": (None, [vp]),
     "xt_asm_pattern": (ci, [vp, ci, vp, vp, vp]),

@@ -26,9 +26,10 @@
 // contiguous) in FP64 and FP32.
 //
 // The factorization runs on the device once per K, and again at every refactorization
-// (xt_prec_refactor_band: per slab when K follows the temperature): this file's panel
-// kernels for the block Cholesky, rocSOLVER trtri and rocBLAS gemm for D_i, G_i, H_i and
-// the level matrices.  None of it is on the per-iteration path.
+// (xt_prec_refactor_band: per slab when K follows the temperature; from the first
+// changed block on after an element deletion): this file's panel kernels for the block
+// Cholesky, rocSOLVER trtri for D_i and rocBLAS gemm for G_i, H_i and the level matrices
+// (b^3 products near the FP64 matrix-core peak).  None of it is on the per-iteration path.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -812,19 +813,21 @@ fill_zero_kernel(long long n, T *d) {
         d[i] = T(0);
 }
 
-// Lower triangle of the 1-based CSR K into the column-major b x b diagonal blocks kd
-// and sub-diagonal blocks ko (row block i+1, column block i), one lane per row in
-// entry order (a repeated entry: the last one stays).  The rows of the last block
+// Lower triangle of the 1-based CSR K, rows of blocks fb.. on, into the column-major
+// b x b diagonal blocks kd (block fb first) and sub-diagonal blocks ko (row block i+1,
+// column block i; the first is column block fb - 1 when fb > 0, else 0), one lane per
+// row in entry order (a repeated entry: the last one stays).  The rows of the last block
 // beyond p get a unit diagonal.  The blocks are zero on entry.
 __global__ void __launch_bounds__(256)
-band_scatter_kernel(int p, int b, int nb, const int *__restrict__ ia, const int *__restrict__ ja,
-                    const double *__restrict__ a, double *kd, double *ko, int *outside) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+band_scatter_kernel(int p, int b, int nb, int fb, const int *__restrict__ ia,
+                    const int *__restrict__ ja, const double *__restrict__ a, double *kd,
+                    double *ko, int *outside) {
+    const int i = fb * b + blockIdx.x * 256 + threadIdx.x;
     const long long bb = (long long)b * b;
     if (i >= nb * b) return;
-    const int bi = i / b, li = i - bi * b;
+    const int bi = i / b, li = i - bi * b, ko0 = fb > 0 ? fb - 1 : 0;
     if (i >= p) {
-        kd[bi * bb + (long long)li * b + li] = 1.0;
+        kd[(bi - fb) * bb + (long long)li * b + li] = 1.0;
         return;
     }
     for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
@@ -835,7 +838,8 @@ band_scatter_kernel(int p, int b, int nb, const int *__restrict__ ia, const int 
             continue;
         }
         const int bj = j / b, lj = j - bj * b;
-        double *dst = bi == bj ? kd + bi * bb : ko + bj * bb;   // bi == bj + 1 otherwise
+        // bi == bj + 1 otherwise
+        double *dst = bi == bj ? kd + (bi - fb) * bb : ko + (bj - ko0) * bb;
         dst[(long long)lj * b + li] = a[k];
     }
 }
@@ -855,25 +859,92 @@ struct RbHandle {
     ~RbHandle() { if (h) rocblas_destroy_handle(h); }
 };
 
+static int zero_fill(double *d, long long n, hipStream_t s) {
+    if (n <= 0) return XT_OK;
+    hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, n, d);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
+
+// Level-L matrices t0 <= t < t1 of direction dir: M^L_t = -(M^{L-1}_{2t+1} M^{L-1}_{2t}),
+// M^0_t = G_t forward and H_{m-1-t} backward.  Row-major X is column-major X^T, so the
+// row-major product is the column-major product of the stored blocks in the order
+// M_{2t}, M_{2t+1}.
+static int level_products(rocblas_handle h, int b, int m, int dir, int L, int t0, int t1,
+                          const double *G, const double *H, double *const lv[2][kLevels]) {
+    if (t1 <= t0) return XT_OK;
+    const long long bb = (long long)b * b;
+    const double minus = -1.0, zero = 0.0;
+    double *out = lv[dir][L - 1];
+    if (L == 1 && dir == 1) {   // level 0 backward runs through H in reverse
+        for (int t = t0; t < t1; ++t) {
+            const double *A = H + (long long)(m - 1 - 2 * t) * bb;
+            XT_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, b, b, b,
+                                &minus, A, b, A - bb, b, &zero, out + (long long)t * bb, b));
+        }
+        return XT_OK;
+    }
+    // consecutive pairs of a contiguous set: one batched call
+    const double *A = (L == 1 ? G : lv[dir][L - 2]) + (long long)(2 * t0) * bb;
+    XT_RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b, b,
+                                        b, &minus, A, b, 2 * bb, A + bb, b, 2 * bb, &zero,
+                                        out + (long long)t0 * bb, b, bb, t1 - t0));
+    return XT_OK;
+}
+
+// G_i = (D_{i+1} E_i)^T column-major, i.e. row-major D_{i+1} E_i, for n consecutive i
+// (E_i, D_{i+1} column-major with stride bb); H_i = E_i D_i column-major, row-major
+// (E_i D_i)^T.
+static int recurrence_products(rocblas_handle h, int b, const double *E, const double *Dnext,
+                               double *G, int ng, const double *Dsame, const double *Eh,
+                               double *Hout, int nh) {
+    const long long bb = (long long)b * b;
+    const double one = 1.0, zero = 0.0;
+    if (ng > 0)
+        XT_RB(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose,
+                                            rocblas_operation_transpose, b, b, b, &one, E, b, bb,
+                                            Dnext, b, bb, &zero, G, b, bb, ng));
+    if (nh > 0)
+        XT_RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, b,
+                                            b, b, &one, Eh, b, bb, Dsame, b, bb, &zero, Hout, b,
+                                            bb, nh));
+    return XT_OK;
+}
+
+// FP32 copy of elements [off, off + n) of an FP64 set.
+static int to_f32_range(DevBuf *set, long long off, long long n, hipStream_t s) {
+    if (n <= 0 || !set[0].ptr) return XT_OK;
+    hipLaunchKernelGGL(to_f32_kernel, dim3(4096), dim3(256), 0, s, n,
+                       set[0].as<double>() + off, set[1].as<float>() + off);
+    XT_HIP(hipGetLastError());
+    return XT_OK;
+}
+
 // Block Cholesky of K given as a 1-based CSR in device memory (lower triangle read).
+// fb > 0 (a factor with its FP64 sets, K changed only in rows and columns of blocks fb..):
+// blocks before fb are kept and the factor is recomputed from block fb on, starting from
+// E_{fb-1} = K_{fb,fb-1} D_{fb-1}^T and the Schur complement K_{fb,fb} - E E^T; the
+// pairing levels are recomputed where they depend on a changed block.  The result equals
+// a full factorization up to rounding.
 static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double *da,
-                      hipStream_t s) {
+                      hipStream_t s, int fb = 0) {
     const int b = bp.b, nb = bp.nb, p = bp.p;
     const long long bb = (long long)b * b;
-    DevBuf L, E, Dl, G, H, info;
-    XT_TRY(L.reserve(sizeof(double) * bb * nb));
-    hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, bb * nb,
-                       L.as<double>());
-    if (nb > 1) {
-        XT_TRY(E.reserve(sizeof(double) * bb * (nb - 1)));
-        hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(8192), dim3(256), 0, s, bb * (nb - 1),
-                           E.as<double>());
+    if (fb > 0 && (fb >= nb || !band_has_fp64(&bp) || !bp.Dl[0].ptr)) fb = 0;
+    const int nr = nb - fb;                       // blocks factored here
+    DevBuf L, E, Dl, G, H, info, E0;
+    XT_TRY(L.reserve(sizeof(double) * bb * nr));
+    XT_TRY(zero_fill(L.as<double>(), bb * nr, s));
+    const int ne = fb > 0 ? nr : nb - 1;          // sub-diagonal blocks held
+    if (ne > 0) {
+        XT_TRY(E.reserve(sizeof(double) * bb * ne));
+        XT_TRY(zero_fill(E.as<double>(), bb * ne, s));
     }
     XT_TRY(info.reserve(sizeof(int) * (nb + 1)));
     XT_HIP(hipMemsetAsync(info.ptr, 0, sizeof(int) * (nb + 1), s));
-    hipLaunchKernelGGL(band_scatter_kernel, dim3(ceil_div((long long)nb * b, 256)), dim3(256), 0,
-                       s, p, b, nb, dia, dja, da, L.as<double>(), nb > 1 ? E.as<double>() : nullptr,
-                       info.as<int>() + nb);
+    hipLaunchKernelGGL(band_scatter_kernel, dim3(ceil_div((long long)nr * b, 256)), dim3(256), 0,
+                       s, p, b, nb, fb, dia, dja, da, L.as<double>(),
+                       ne > 0 ? E.as<double>() : nullptr, info.as<int>() + nb);
     XT_HIP(hipGetLastError());
     {
         int outside = 0;
@@ -885,21 +956,36 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
         }
     }
     XT_HIP(hipMemsetAsync(info.ptr, 0, sizeof(int) * (nb + 1), s));
+    double *Lp = L.as<double>(), *Ep = E.as<double>();
     RbHandle rb;
     XT_RB(rocblas_create_handle(&rb.h));
     XT_RB(rocblas_set_stream(rb.h, s));
     XT_RB(rocblas_set_pointer_mode(rb.h, rocblas_pointer_mode_host));
-    const double one = 1.0, zero = 0.0;
-    double *Lp = L.as<double>(), *Ep = E.as<double>();
+    // Ep + e(i) bb holds E_i (first K_{i+1,i}): e(i) = i for the full factorization, and
+    // i - fb + 1 (E_{fb-1} at 0) for the partial one
+    const int eo = fb > 0 ? 1 - fb : 0;
+    if (fb > 0) {
+        // E_{fb-1} = K_{fb,fb-1} L_{fb-1}^-T = K_{fb,fb-1} D_{fb-1}^T (column-major, D kept
+        // column-major in the backward set), in place of the K block; then the Schur
+        // complement K_{fb,fb} -= E E^T (lower triangle)
+        const double one = 1.0, zero = 0.0, minus = -1.0;
+        XT_TRY(E0.reserve(sizeof(double) * bb));
+        XT_RB(rocblas_dgemm(rb.h, rocblas_operation_none, rocblas_operation_transpose, b, b, b,
+                            &one, Ep, b, bp.Du[0].as<double>() + (long long)(fb - 1) * bb, b,
+                            &zero, E0.as<double>(), b));
+        XT_HIP(hipMemcpyAsync(Ep, E0.ptr, sizeof(double) * bb, hipMemcpyDeviceToDevice, s));
+        XT_RB(rocblas_dsyrk(rb.h, rocblas_fill_lower, rocblas_operation_none, b, b, &minus, Ep,
+                            b, &one, Lp, b));
+    }
     int *ip = info.as<int>();
     // L_ii, E_i = K_{i+1,i} L_ii^-T and K_{i+1,i+1} -= E_i E_i^T, panel by panel
     const int nt = b / kCT;
     DevBuf Wb;
     XT_TRY(Wb.reserve(sizeof(double) * 2 * nt * kCT * kCT));
-    for (int i = 0; i < nb; ++i) {
-        double *Li = Lp + i * bb;
-        double *Ei = i + 1 < nb ? Ep + i * bb : nullptr;
-        double *Kn = i + 1 < nb ? Lp + (i + 1) * bb : nullptr;
+    for (int i = fb; i < nb; ++i) {
+        double *Li = Lp + (long long)(i - fb) * bb;
+        double *Ei = i + 1 < nb ? Ep + (long long)(i + eo) * bb : nullptr;
+        double *Kn = i + 1 < nb ? Li + bb : nullptr;
         const int ntot = Ei ? 2 * nt : nt;
         for (int k = 0; k < nt; ++k) {
             const int T = nt - 1 - k;
@@ -914,23 +1000,62 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
     std::vector<int> hinfo(nb + 1);
     XT_HIP(hipMemcpyAsync(hinfo.data(), ip, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
     XT_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < nb; ++i)
+    for (int i = fb; i < nb; ++i)
         if (hinfo[i] != 0) {
             set_error("K is not positive definite (block %d, minor %d)", i, hinfo[i]);
             return XT_E_SINGULAR;
         }
-    hipLaunchKernelGGL(zero_upper_kernel, dim3(ceil_div(bb, 256 * 8), nb), dim3(256), 0, s, b, Lp);
+    hipLaunchKernelGGL(zero_upper_kernel, dim3(ceil_div(bb, 256 * 8), nr), dim3(256), 0, s, b, Lp);
     XT_HIP(hipGetLastError());
     // D_i = L_ii^-1 in place (column-major lower = D_i^T row-major: the backward set)
     XT_RB(rocsolver_dtrtri_strided_batched(rb.h, rocblas_fill_lower, rocblas_diagonal_non_unit,
-                                           b, Lp, b, bb, ip, nb));
+                                           b, Lp, b, bb, ip + fb, nr));
     XT_HIP(hipMemcpyAsync(hinfo.data(), ip, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
     XT_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < nb; ++i)
+    for (int i = fb; i < nb; ++i)
         if (hinfo[i] != 0) {
             set_error("singular diagonal factor block %d", i);
             return XT_E_SINGULAR;
         }
+    const int m = nb - 1;
+    if (fb > 0) {
+        // nothing can fail numerically from here on: the new blocks go into the kept sets
+        double *Du = bp.Du[0].as<double>(), *Dlp = bp.Dl[0].as<double>();
+        double *Gp = bp.G[0].as<double>(), *Hp = bp.H[0].as<double>();
+        XT_HIP(hipMemcpyAsync(Du + (long long)fb * bb, Lp, sizeof(double) * bb * nr,
+                              hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(transpose_kernel, dim3(b / 32, b / 32, nr), dim3(256), 0, s, b,
+                           (const double *)Lp, Dlp + (long long)fb * bb);
+        XT_HIP(hipGetLastError());
+        // G_i for i = fb-1..nb-2 (E_{fb-1} with the new D_fb on), H_i for i = fb..nb-2
+        XT_TRY(recurrence_products(rb.h, b, Ep, Lp, Gp + (long long)(fb - 1) * bb, nr, Lp,
+                                   Ep + bb, Hp + (long long)fb * bb, nr - 1));
+        // levels: forward entries whose G range reaches index fb - 1, backward entries
+        // whose H range (H_{m-1-t}) reaches index fb
+        double *lvp[2][kLevels] = {};
+        for (int dir = 0; dir < 2; ++dir)
+            for (int L = 0; L < bp.nlev; ++L) lvp[dir][L] = bp.Lv[dir][L][0].as<double>();
+        for (int L = 1; L <= bp.nlev; ++L) {
+            const int cnt = m >> L;
+            XT_TRY(level_products(rb.h, b, m, 0, L, (fb - 1) >> L, cnt, Gp, Hp, lvp));
+            if (m - 1 - fb >= 0)
+                XT_TRY(level_products(rb.h, b, m, 1, L, 0,
+                                      std::min(cnt, ((m - 1 - fb) >> L) + 1), Gp, Hp, lvp));
+        }
+        XT_TRY(to_f32_range(bp.Du, (long long)fb * bb, bb * nr, s));
+        XT_TRY(to_f32_range(bp.Dl, (long long)fb * bb, bb * nr, s));
+        XT_TRY(to_f32_range(bp.G, (long long)(fb - 1) * bb, bb * nr, s));
+        XT_TRY(to_f32_range(bp.H, (long long)fb * bb, bb * (nr - 1), s));
+        for (int L = 1; L <= bp.nlev; ++L) {
+            const int cnt = m >> L, f0 = (fb - 1) >> L;
+            XT_TRY(to_f32_range(bp.Lv[0][L - 1], (long long)f0 * bb, bb * (cnt - f0), s));
+            if (m - 1 - fb >= 0)
+                XT_TRY(to_f32_range(bp.Lv[1][L - 1], 0,
+                                    bb * std::min(cnt, ((m - 1 - fb) >> L) + 1), s));
+        }
+        XT_HIP(hipStreamSynchronize(s));
+        return XT_OK;
+    }
     XT_TRY(Dl.reserve(sizeof(double) * bb * nb));
     hipLaunchKernelGGL(transpose_kernel, dim3(b / 32, b / 32, nb), dim3(256), 0, s, b, Lp,
                        Dl.as<double>());
@@ -938,45 +1063,26 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
     if (nb > 1) {
         XT_TRY(G.reserve(sizeof(double) * bb * (nb - 1)));
         XT_TRY(H.reserve(sizeof(double) * bb * (nb - 1)));
-        // G_i row-major = (D_{i+1} E_i)^T column-major = E_i^T D_{i+1}^T
-        XT_RB(rocblas_dgemm_strided_batched(rb.h, rocblas_operation_transpose,
-                                            rocblas_operation_transpose, b, b, b, &one, Ep, b,
-                                            bb, Lp + bb, b, bb, &zero, G.as<double>(), b, bb,
-                                            nb - 1));
-        // H_i row-major = (E_i D_i)^T row-major = E_i D_i column-major
-        XT_RB(rocblas_dgemm_strided_batched(rb.h, rocblas_operation_none,
-                                            rocblas_operation_none, b, b, b, &one, Ep, b, bb,
-                                            Lp, b, bb, &zero, H.as<double>(), b, bb, nb - 1));
+        XT_TRY(recurrence_products(rb.h, b, Ep, Lp + bb, G.as<double>(), nb - 1, Lp, Ep,
+                                   H.as<double>(), nb - 1));
     }
     // Level matrices: M^L_t = -(M^{L-1}_{2t+1} M^{L-1}_{2t}), t < m >> L, where
-    // M^0_t = G_t forward and H_{m-1-t} backward.  Row-major X is column-major X^T, so
-    // the row-major product is the column-major product of the stored blocks in the
-    // order M_{2t}, M_{2t+1}.
-    const int m = nb - 1;
+    // M^0_t = G_t forward and H_{m-1-t} backward.
     int nlev = 0;
     // XT_TUNE_BAND_LEVELS v caps the pairing levels at v - 1 (set before the factor is built)
     const int maxlev = tune_knob(XT_TUNE_BAND_LEVELS) ? tune_knob(XT_TUNE_BAND_LEVELS) - 1 : kLevels;
     while (nlev < maxlev && (m >> (nlev + 1)) >= kMinChain) ++nlev;
     DevBuf lv[2][kLevels];
-    const double minus = -1.0;
-    for (int dir = 0; dir < 2 && nlev > 0; ++dir)
+    double *lvp[2][kLevels] = {};
+    for (int dir = 0; dir < 2; ++dir)
         for (int L = 1; L <= nlev; ++L) {
-            const int cnt = m >> L;
-            XT_TRY(lv[dir][L - 1].reserve(sizeof(double) * bb * cnt));
-            if (L == 1 && dir == 1) {   // level 0 backward runs through H in reverse
-                for (int t = 0; t < cnt; ++t) {
-                    const double *A = H.as<double>() + (long long)(m - 1 - 2 * t) * bb;
-                    XT_RB(rocblas_dgemm(rb.h, rocblas_operation_none, rocblas_operation_none, b,
-                                        b, b, &minus, A, b, A - bb, b, &zero,
-                                        lv[dir][L - 1].as<double>() + (long long)t * bb, b));
-                }
-            } else {   // consecutive pairs of a contiguous set: one batched call
-                const double *A = L == 1 ? G.as<double>() : lv[dir][L - 2].as<double>();
-                XT_RB(rocblas_dgemm_strided_batched(
-                    rb.h, rocblas_operation_none, rocblas_operation_none, b, b, b, &minus, A, b,
-                    2 * bb, A + bb, b, 2 * bb, &zero, lv[dir][L - 1].as<double>(), b, bb, cnt));
-            }
+            XT_TRY(lv[dir][L - 1].reserve(sizeof(double) * bb * (m >> L)));
+            lvp[dir][L - 1] = lv[dir][L - 1].as<double>();
         }
+    for (int dir = 0; dir < 2; ++dir)
+        for (int L = 1; L <= nlev; ++L)
+            XT_TRY(level_products(rb.h, b, m, dir, L, 0, m >> L, G.as<double>(), H.as<double>(),
+                                  lvp));
     bp.Du[0] = std::move(L);
     bp.Dl[0] = std::move(Dl);
     bp.G[0] = std::move(G);
@@ -990,9 +1096,7 @@ static int factor_dev(BandPrec &bp, const int *dia, const int *dja, const double
         if (!src[0].ptr) continue;
         const long long n = (long long)(src[0].bytes / sizeof(double));
         XT_TRY(src[1].reserve(sizeof(float) * n));
-        hipLaunchKernelGGL(to_f32_kernel, dim3(4096), dim3(256), 0, s, n, src[0].as<double>(),
-                           src[1].as<float>());
-        XT_HIP(hipGetLastError());
+        XT_TRY(to_f32_range(src, 0, n, s));
     }
     XT_HIP(hipStreamSynchronize(s));
     if (!bp.keep64)
@@ -1016,9 +1120,12 @@ static int factor(BandPrec &bp, const int *ia, const int *ja, const double *a, h
 
 // Refactor with new values of a K whose lower-triangle pattern fits the existing band
 // (the same mesh, e.g. new material or boundary conditions), CSR in device memory.
+// first_row > 0: K is unchanged in rows and columns before first_row (0-based), and the
+// blocks before first_row / b are kept (a factor without FP64 sets is refactored whole).
 int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da,
-                  hipStream_t s) {
-    return factor_dev(*bp, dia, dja, da, s);
+                  hipStream_t s, int first_row) {
+    const int fb = first_row > 0 ? std::min(first_row / bp->b, bp->nb) : 0;
+    return factor_dev(*bp, dia, dja, da, s, fb);
 }
 
 int band_build(int p, const int *ia, const int *ja, const double *a, int block, BandPrec **out,

@@ -262,7 +262,7 @@ bool band_has_fp64(const BandPrec *bp);
 void band_delete(BandPrec *bp);
 void band_dims(const BandPrec *bp, int *b, int *nb, int *bw);
 int band_refactor(BandPrec *bp, const int *dia, const int *dja, const double *da,
-                  hipStream_t s);
+                  hipStream_t s, int first_row = 0);
 // Ra >= 0 with dcols (device, R entries, the Ra active cases first): the sweeps run on the
 // active cases' columns only, gathered after the temporal mix, and y is zero elsewhere.
 template <typename T>

@@ -859,6 +859,14 @@ int xt_prec_refactor_band(xt_prec pr, const int *ia, const int *ja, const double
     return band_refactor(pr->band, ia, ja, a, (hipStream_t)stream);
 }
 
+int xt_prec_refresh_band(xt_prec pr, const int *ia, const int *ja, const double *a,
+                         int first_row, void *stream) {
+    XT_ARG(pr && pr->band && pr->groups.empty() && pr->shifted.empty() && ia && ja && a,
+           "prec_refresh_band arguments (a plain banded factor)");
+    XT_ARG(first_row >= 0 && first_row < pr->p, "prec_refresh_band: first_row out of range");
+    return band_refactor(pr->band, ia, ja, a, (hipStream_t)stream, first_row);
+}
+
 int xt_prec_band_info(xt_prec pr, int *block, int *nblocks, int *bandwidth) {
     XT_ARG(pr && (pr->band || !pr->groups.empty()), "not a banded-factor preconditioner");
     if (pr->band) {

@@ -225,8 +225,15 @@ class BandPreconditioner(IlutPreconditioner):
                                         gblock.ctypes.data), "prec_band_groups")
         self.group_offsets, self.group_blocks = goff, gblock
 
-    def refactor(self, rowptr, colidx, vals):
-        """Refactor from K on the device (1-based CSR int32/FP64 tensors, same band)."""
+    def refactor(self, rowptr, colidx, vals, first_row=0):
+        """Refactor from K on the device (1-based CSR int32/FP64 tensors, same band).
+        first_row > 0: K changed only in rows and columns >= first_row since the last
+        factorization; the blocks before it are kept (xt_prec_refresh_band)."""
+        if first_row > 0:
+            check(lib().xt_prec_refresh_band(self.h, rowptr.data_ptr(), colidx.data_ptr(),
+                                             vals.data_ptr(), int(first_row), _stream()),
+                  "prec_refresh_band")
+            return
         check(lib().xt_prec_refactor_band(self.h, rowptr.data_ptr(), colidx.data_ptr(),
                                           vals.data_ptr(), _stream()), "prec_refactor_band")
 

@@ -17,7 +17,7 @@ state belongs to the specimen): the solve is ``xt_slab_solve``
 temporal-major layout (``xt_from_interleaved``), the element status is the device
 reduction, and a deletion event edits the matrices on the host (``ElementDeletion``)
 and puts their values into the device operator and banded factor (refactored on the
-device).  The damage law's
+device from the first changed row on, xt_prec_refresh_band).  The damage law's
 constants are parameters (the reference's by default, damage.h:9-17).
 """
 from __future__ import annotations
@@ -46,8 +46,11 @@ class FatigueSlabs:
     of the deleted elements' stiffness (True) or leave them removed (False)."""
 
     def __init__(self, mesh, K, M, Nxyz, bc, rext, amp, freq, dt, nip, Dc, ym, nu,
-                 params=None, mode=1, readd=False, xtol=0.0, im=30, eps=1e-8):
+                 params=None, mode=1, readd=False, xtol=0.0, im=30, eps=1e-8, refresh=True):
+        """refresh: after a deletion event refactor the banded factor from the first
+        changed row on (xt_prec_refresh_band); False: refactor it whole."""
         self.mesh, self.nip, self.Dc = mesh, int(nip), float(Dc)
+        self.refresh, self.first_row = bool(refresh), 0
         self.mode, self.xtol, self.im, self.eps = mode, xtol, im, eps
         self.p = 3 * mesh.nn
         self.amp, self.w, self.dt = float(amp), 2 * np.pi * float(freq), float(dt)
@@ -73,8 +76,8 @@ class FatigueSlabs:
         zeros included: the assembly and the 1-0 method drop entries that sum to zero,
         which a deletion can make nonzero): every matrix a deletion event leaves fits them,
         so after the first build the new values go into the existing handles
-        (xt_op_set_values, xt_prec_refactor_band from the device CSR) with no host-side
-        operator build or upload."""
+        (xt_op_set_values, and xt_prec_refresh_band from the device CSR: the factor from
+        the first changed row on) with no host-side operator build or upload."""
         self.fixed = torch.as_tensor((np.asarray(fixed) != 0).astype(np.uint8), device="cuda")
         if getattr(self, "op", None) is None:
             self.patK = fem.dof_pattern(self.mesh, "all")
@@ -85,11 +88,29 @@ class FatigueSlabs:
             self.pr = device.BandPreconditioner(self.patK + (kv,))
             self.patK_dev = tuple(torch.as_tensor(v, device="cuda") for v in self.patK)
             self.tj = self.op.tmat(self.J, False)
+            self.kv = kv
             return
-        kd = torch.as_tensor(api.csr_values_in_pattern(self.patK, K), device="cuda")
+        kv = api.csr_values_in_pattern(self.patK, K)
+        kd = torch.as_tensor(kv, device="cuda")
         md = torch.as_tensor(api.csr_values_in_pattern(self.patM, M), device="cuda")
         self.op.set_values(kd, md)
-        self.pr.refactor(self.patK_dev[0], self.patK_dev[1], kd)
+        # the deletion edits (element matrices removed, orphan diagonals, 1-0 method) are
+        # symmetric, so K is unchanged in the rows and columns before its first changed
+        # row: the factor is refreshed from there on
+        self.first_row = self.refresh_row(self.patK[0], self.kv, kv)
+        if self.first_row < self.p:
+            self.pr.refactor(self.patK_dev[0], self.patK_dev[1], kd,
+                             first_row=self.first_row if self.refresh else 0)
+        self.kv = kv
+
+    @staticmethod
+    def refresh_row(rowptr, old, new):
+        """First (0-based) row of a 1-based CSR whose values differ between old and new
+        (len(rowptr) - 1 when none does)."""
+        d = np.flatnonzero(np.asarray(old) != np.asarray(new))
+        if d.size == 0:
+            return len(rowptr) - 1
+        return int(np.searchsorted(np.asarray(rowptr), d[0] + 1, side="right") - 1)
 
     def step(self):
         """One slab: deletion check, right-hand side, solve, damage.  Returns the slab's
@@ -102,6 +123,7 @@ class FatigueSlabs:
         t0 = time.perf_counter()
         ev = self.deletion.step(self.damage)           # chkelesta + edits (main3d.f90:625-686)
         rec["failed"] = [] if ev is None else ev.nefail.tolist()
+        self.first_row = 0
         if ev is not None:
             try:
                 self._build(ev.K, ev.M, ev.fixed)
@@ -116,6 +138,8 @@ class FatigueSlabs:
                 return rec
         torch.cuda.synchronize()
         rec["deletion_ms"] = 1e3 * (time.perf_counter() - t0)
+        if ev is not None:
+            rec["refresh_row"] = self.first_row
         t0 = time.perf_counter()
         s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         self.op.apply(self.tj, self.xprev, self.b, 1)  # jump term (main3d.f90:818-819)

@@ -593,7 +593,7 @@ chol_panel_kernel(int b, int nt, int k, const double *__restrict__ L,
     // row per thread, then the rank-16 update of the columns to the right on the matrix
     // cores (v_mfma_f64_16x16x4f64, one 16 x 16 block per wave at a time).
     __shared__ double sA[kCT][kCT + 1], sX[kCT][kCT + 1];
-    __shared__ double rdiag[16];
+    __shared__ double rdiag[16], colb[16];
     __shared__ int sbad;
     const int tid = threadIdx.x, t = blockIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -623,15 +623,25 @@ chol_panel_kernel(int b, int nt, int k, const double *__restrict__ L,
             for (int j = 0; j < 16; ++j) {
                 const double pj = lane_bcast(a[j], j);
                 bad = (!(pj > 0.0) && !bad) ? j + 1 : bad;
-                const double d = sqrt(pj), rd = 1.0 / d;
+                // 1 / sqrt(pj): the hardware estimate refined by two Newton steps
+                // (y <- y + y (1 - pj y^2) / 2, full FP64 precision), then sqrt(pj) = pj y
+                double rd = __builtin_amdgcn_rsq(pj);
+#pragma unroll
+                for (int it = 0; it < 2; ++it) rd = fma(0.5 * rd, fma(-pj * rd, rd, 1.0), rd);
+                const double d = pj * rd;
                 const double col = r > j ? a[j] * rd : (r == j ? d : 0.0);
                 a[j] = r >= j ? col : a[j];
                 if (r == j) rdiag[j] = rd;
+                // the column to every lane through LDS (one write, broadcast reads: the
+                // lanes are one wave, whose LDS accesses complete in order)
+                colb[r] = col;
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int l = j + 1; l < 16; ++l) {
-                    const double cl = lane_bcast(col, l);
+                    const double cl = colb[l];
                     a[l] = l <= r ? a[l] - col * cl : a[l];
                 }
+                __builtin_amdgcn_wave_barrier();
             }
 #pragma unroll
             for (int c = 0; c < 16; ++c) sA[c0 + r][c0 + c] = a[c];

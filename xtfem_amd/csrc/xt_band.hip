@@ -49,8 +49,8 @@ constexpr int kLevels = 4;     // pairing levels of the block recurrences, at mo
 constexpr int kMinChain = 6;   // a level is added while its chain keeps this many steps
 constexpr int kKey = 6 + 2 * kLevels;
 struct SweepGraph {
-    int c = 0, pi = -1, tile = 0, smallc = 0, chain = 0, nl = 0;   // columns, precision,
-                                                                     // knobs, levels used
+    int c = 0, pi = -1, tile = 0, smallc = 0, nl = 0;   // columns, precision, knobs,
+    long long epoch = -1;                                // levels used, knob epoch
     const void *key[kKey] = {};
     hipGraphExec_t exec = nullptr;
 };
@@ -1327,7 +1327,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             if (cand.exec && cand.c == c && cand.pi == pi &&
                 cand.tile == tune_knob(XT_TUNE_BAND_TILE) &&
                 cand.smallc == tune_knob(XT_TUNE_BAND_SMALLC) &&
-                cand.chain == tune_knob(XT_TUNE_BAND_CHAIN) * 4 + tune_knob(XT_TUNE_BAND_CHAIN_FEW) &&
+                cand.epoch == tune_epoch() &&
                 cand.nl == nlev &&
                 std::equal(key, key + kKey, cand.key)) {
                 g = &cand;
@@ -1358,7 +1358,7 @@ int band_apply(BandPrec *bp, int R, int q, const T *minv, const T *x, T *y, hipS
             g->pi = pi;
             g->tile = tune_knob(XT_TUNE_BAND_TILE);
             g->smallc = tune_knob(XT_TUNE_BAND_SMALLC);
-            g->chain = tune_knob(XT_TUNE_BAND_CHAIN) * 4 + tune_knob(XT_TUNE_BAND_CHAIN_FEW);
+            g->epoch = tune_epoch();
             g->nl = nlev;
             std::copy(key, key + kKey, g->key);
         }

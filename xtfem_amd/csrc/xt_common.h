@@ -144,6 +144,7 @@ bool prof_on(int kind);
 bool graphs_on();
 // Kernel-variant knob (XT_TUNE_*): 0 = the built-in choice.
 int tune_knob(int knob);
+long long tune_epoch();   // changes whenever a knob does
 void prof_begin(int kind, hipStream_t s);
 // bytes / flops the launch performs; alg: they are also its algorithmic share (else the
 // launch site records the algorithmic totals with prof_alg)

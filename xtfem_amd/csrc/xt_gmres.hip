@@ -739,6 +739,7 @@ struct Solver {
             (const void *)(intptr_t)(hist != nullptr), (const void *)(intptr_t)hist_cap,
             (const void *)(intptr_t)(ws.split.ns), (const void *)(intptr_t)R,
             (const void *)(intptr_t)tune_knob(XT_TUNE_GMRES_REORTH), ws.reorth.ptr,
+            (const void *)(intptr_t)tune_epoch(),
             (const void *)(intptr_t)(cur_act() ? act_version : -1)};
         SolverWorkspace::StepGraph *g = nullptr;
         for (SolverWorkspace::StepGraph &cand : ws.steps)

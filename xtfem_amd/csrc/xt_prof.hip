@@ -7,6 +7,8 @@
 // state: enable, run, read from one host thread.
 #include <vector>
 
+#include <atomic>
+
 #include "xt_common.h"
 
 namespace xt {
@@ -39,6 +41,9 @@ int g_knob[XT_TUNE_NKNOBS] = {};   // 0 = the built-in choice
 std::atomic<unsigned long long> g_dev_epoch{0};
 bool graphs_on() { return g_graphs; }
 int tune_knob(int knob) { return (knob >= 0 && knob < XT_TUNE_NKNOBS) ? g_knob[knob] : 0; }
+// bumped by every knob change: captured graphs keyed on it never replay another variant
+static std::atomic<long long> g_knob_epoch{0};
+long long tune_epoch() { return g_knob_epoch.load(std::memory_order_relaxed); }
 
 void prof_begin(int kind, hipStream_t s) {
     if (!prof_on(kind)) return;
@@ -97,6 +102,7 @@ int xt_tune_set(int knob, int value) {
     bool ok = false;
     for (int v : allowed[knob]) ok = ok || v == value;
     XT_ARG(ok, "value not allowed for this knob");
+    if (g_knob[knob] != value) g_knob_epoch.fetch_add(1, std::memory_order_relaxed);
     g_knob[knob] = value;
     return XT_OK;
 }

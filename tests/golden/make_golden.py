@@ -88,7 +88,7 @@ def main():
     g["bc_mask"] = bc
     g["Kb_digest"] = np.array(digest(*Kb))
     g["Mb_digest"] = np.array(digest(*Mb))
-    # ILUT on the rod, lfil=20, droptol=1e-4 (BASELINE settings) and a tighter variant
+    # ILUT on the rod, lfil=20, droptol=1e-4 (the rod benchmark settings) and a tighter variant
     alu, jlu, ju, ierr = ref.ilut(Kb, 20, 1e-4)
     assert ierr == 0
     g["ilut20_digest"] = np.array(digest(alu, jlu, ju))

@@ -53,7 +53,27 @@ multidot_partial(int c, int k, const int2 *__restrict__ chunks, const double *__
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) acc[jj] = 0.0;
     if (lr < rpl && !off) {
-        for (int row = rbeg + lr; row < rend; row += rpl) {
+        // four rows' loads issued before their products (the row loop is otherwise one
+        // latency-bound load pair per iteration); the sums keep the row order
+        constexpr int U = 4;
+        int row = rbeg + lr;
+        for (; row + (U - 1) * rpl < rend; row += U * rpl) {
+            double wv[U], vv[U][JB];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const size_t o = (size_t)(row + u * rpl) * c + col;
+                wv[u] = w[o];
+#pragma unroll
+                for (int jj = 0; jj < JB; ++jj)
+                    vv[u][jj] = j0 + jj < k ? V[(size_t)(j0 + jj) * ldv + o] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int jj = 0; jj < JB; ++jj)
+                    if (j0 + jj < k) acc[jj] += vv[u][jj] * wv[u];
+        }
+        for (; row < rend; row += rpl) {
             const size_t o = (size_t)row * c + col;
             const double wv = w[o];
 #pragma unroll

@@ -2,8 +2,9 @@
 solve equals the launch-by-launch solve bit for bit, over repeated solves (captures on
 the second use of a step, replays after) with eager work and synchronisations between
 them, for the ILUT path (the rod, FP64, one load case) and the banded factor with the
-split preconditioner (a plate, FP32-IR, several load cases), and for the segmented
-multi-system solve."""
+split preconditioner (a plate, FP32-IR, several load cases), for the segmented
+multi-system solve, and for one-case solves whose splits map the case to different
+shifted factors."""
 import numpy as np
 import pytest
 
@@ -122,3 +123,40 @@ def test_step_graphs_across_precisions_and_reallocation(dev):
     finally:
         lib().xt_graphs_enable(1)
         tune.set_knob("ir_skip32", 0)
+
+
+def test_step_graphs_follow_each_solves_split(dev):
+    """One-case FP64 solves one after another on one operator and split preconditioner,
+    the cases' dominant temporal modes going to different shifted factors: the Arnoldi
+    step graphs are keyed on the split's groups, so a graph captured for one case's
+    split is never replayed for another's.  Each solve reaches the 1e-9 residual and
+    equals its launch-by-launch counterpart bit for bit, over repeated rounds."""
+    import torch
+    from xtfem_amd._lib import lib
+    S = structured_system(16, 4, 3, 32.0, 8.0, 6.0, rho=50 * 7.85e-9)
+    A, B, rhs = _rhs(S, [14.0, 47.0, 88.0], [50.0, 90.0, 30.0])
+    op = dev.KronOperator([S["Kb"], S["Mb"]], q=6)
+    pr0 = dev.BandPreconditioner(S["Kb"])
+    mu = dev.lowest_mode(pr0, S["Mb"], S["bc"])
+    pr = dev.BandPreconditioner(S["Kb"], M=S["Mb"],
+                                shifts=dev.shifts_for_cases(dev.dominant_temporal_mode(A, B), mu))
+    assert len(pr.shifts) >= 2, pr.shifts
+    p = S["p"]
+
+    def solve(on, r):
+        lib().xt_graphs_enable(int(on))
+        b = dev.to_interleaved(rhs[r:r + 1], p, 6)
+        x = torch.zeros_like(b)
+        out = dev.slab_solve_device(op, pr, A[r:r + 1], B[r:r + 1], b, x, 1, eps=1e-9, mode=0)
+        torch.cuda.synchronize()
+        return x.clone(), float(np.asarray(out[3]).ravel()[0])
+
+    try:
+        want = [solve(False, r) for r in range(3)]
+        for rep in range(3):
+            for r in range(3):
+                x, rr = solve(True, r)
+                assert rr <= 1e-9, (rep, r, rr)
+                assert torch.equal(x, want[r][0]), (rep, r)
+    finally:
+        lib().xt_graphs_enable(1)

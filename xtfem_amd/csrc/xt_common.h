@@ -163,6 +163,7 @@ void prof_alg(int kind, double bytes, double flops);
 struct PrecSplit {
     int ns = 0;
     std::vector<int> goff, gfac;   // host: column offsets of the groups, their factor index
+    unsigned long long sig = 0;    // hash of goff and gfac: the launches a split apply makes
     DevBuf dcase, dbase, dstride;  // device: int case[ns], long long base[ns], int stride[ns]
     DevBuf m2v[2];                 // device [ns][2q]: m2 (q) then v (q); FP64 [0], FP32 [1]
     // host sources of the uploads (kept until the next build: the copies are asynchronous)

@@ -382,6 +382,14 @@ static int build_split(PrecSplit &sp, int p, int q, int R, const std::vector<dou
         sp.gfac.push_back(g);
     }
     sp.ns = sp.goff.back();
+    // the group structure decides which shifted factors a split apply sweeps and on which
+    // columns: captured step graphs are keyed on it (FNV-1a over the two lists)
+    sp.sig = 1469598103934665603ULL;
+    for (const std::vector<int> *vec : {&sp.goff, &sp.gfac})
+        for (int v : *vec) {
+            sp.sig ^= (unsigned long long)(unsigned)v;
+            sp.sig *= 1099511628211ULL;
+        }
     if (!sp.ns) return XT_OK;
     sp.h32.assign(sp.h64.begin(), sp.h64.end());
     XT_TRY(sp.dcase.upload(sp.hcase.data(), sizeof(int) * sp.ns, s));
@@ -757,7 +765,8 @@ struct Solver {
             (const void *)(intptr_t)(flexible * 4 + f32 * 2 + f32prec),
             (const void *)(intptr_t)n, (const void *)(intptr_t)I, (const void *)(intptr_t)nblk,
             (const void *)(intptr_t)(hist != nullptr), (const void *)(intptr_t)hist_cap,
-            (const void *)(intptr_t)(ws.split.ns), (const void *)(intptr_t)R,
+            (const void *)(intptr_t)(ws.split.ns), (const void *)(uintptr_t)ws.split.sig,
+            (const void *)(intptr_t)R,
             (const void *)(intptr_t)tune_knob(XT_TUNE_GMRES_REORTH), ws.reorth.ptr,
             (const void *)(intptr_t)tune_epoch(),
             (const void *)(intptr_t)(cur_act() ? act_version : -1)};

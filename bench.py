@@ -47,6 +47,9 @@ if ROOT not in sys.path:
 
 PEAK_HBM_GBS = 8000.0
 PEAK_MFMA_TFLOPS = {"f32": 157.3, "f64": 78.6}   # dense matrix-core peaks
+# vector ALU: FP64 FMA, and FP32 FMA without the packed instructions (disabled in this
+# build), one per lane per clock: 78.6 TFLOP/s either way
+PEAK_VALU_TFLOPS = 78.6
 SEED = 20261202
 
 
@@ -366,11 +369,21 @@ def spmm_microbench(op, p, reps=20, warm=3, R=1):
     check(lib().xt_prof_enable(0), "prof_enable")
     med = float(np.median(times))
     csr = op.algorithmic_bytes(R, True, fmt="csr")
+    # which roof bounds the launch: HBM below the ridge intensity (peak flops / peak
+    # bytes), the vector ALU above it; attainable_frac = the bound's time / the time taken
+    inten = op.flops(R) / by.value
+    t_hbm = by.value / (PEAK_HBM_GBS * 1e9)
+    t_valu = op.flops(R) / (PEAK_VALU_TFLOPS * 1e12)
     return dict(R=R, dtype="f32", format=op.format, median_us=1e3 * med, bytes=by.value,
                 gbs=by.value / (med * 1e-3) / 1e9,
                 frac=by.value / (med * 1e-3) / 1e9 / PEAK_HBM_GBS,
                 csr_bytes=csr, csr_gbs=csr / (med * 1e-3) / 1e9,
-                tflops=op.flops(R) / (med * 1e-3) / 1e12)
+                tflops=op.flops(R) / (med * 1e-3) / 1e12,
+                valu_frac=op.flops(R) / (med * 1e-3) / 1e12 / PEAK_VALU_TFLOPS,
+                intensity_flop_per_byte=inten,
+                ridge_flop_per_byte=PEAK_VALU_TFLOPS * 1e3 / PEAK_HBM_GBS,
+                bound="valu" if t_valu > t_hbm else "hbm",
+                attainable_frac=max(t_hbm, t_valu) / (med * 1e-3))
 
 
 def spmm_legs(op, p):
@@ -1247,7 +1260,9 @@ def main():
           flush=True)
     spmm = spmm_legs(op, prob.p)
     print(f"# kron-spmm fp32: R=1 {spmm['R1']['gbs']:.0f} GB/s ({spmm['R1']['frac']:.2f}), "
-          f"R=32 {spmm['R32']['gbs']:.0f} GB/s ({spmm['R32']['frac']:.2f})", file=sys.stderr,
+          f"R=32 {spmm['R32']['gbs']:.0f} GB/s ({spmm['R32']['frac']:.2f}; "
+          f"{spmm['R32']['bound']}-bound, {spmm['R32']['attainable_frac']:.2f} of that roof)",
+          file=sys.stderr,
           flush=True)
     # per-kernel durations: the batched FP32-IR steps once more, every launch of the
     # operator and sweep kernels bracketed by HIP events on the launching stream

@@ -619,12 +619,14 @@ int bsr_set_values(BsrPlan &P, const double *kv, const double *mv, hipStream_t s
 static constexpr int kR1Split = 4;   // lanes per node row for one load case (8: slower)
 
 // Kernel variant of a launch: the tuning knobs, else the built-in choices (4 entries in
-// flight in FP32 and 2 in FP64; one wave per workgroup when four-wave workgroups would
-// leave CUs idle; kR1Split lanes per node for one load case).
+// flight for one load case in FP32, else 2: with several cases in FP32 two entries took
+// 849 us against 909 at 32 cases and 233 against 271 at 8 on a 435,969-DOF operator; one
+// wave per workgroup when four-wave workgroups would leave CUs idle; kR1Split lanes per
+// node for one load case).
 template <typename T>
-static int spmm_unr() {
+static int spmm_unr(int R) {
     const int k = tune_knob(XT_TUNE_SPMM_UNR);
-    return k ? k : (sizeof(T) == 4 ? 4 : 2);
+    return k ? k : (sizeof(T) == 4 && R == 1 ? 4 : 2);
 }
 
 // LDS budget of one staged workgroup (two fit a CU's 160 KB).
@@ -688,7 +690,7 @@ int bsr_apply(const BsrPlan &P, int R, const T *tmat, double alpha, double beta,
         const int wk = tune_knob(XT_TUNE_SPMM_WPB);
         const int wpb = wk ? wk : (waves >= 4 * 1024 ? 4 : 1);
         const int grid = (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
-        const int unr = spmm_unr<T>();
+        const int unr = spmm_unr<T>(R);
 #define XT_BSR_LAUNCH(N, U)                                                                   \
     hipLaunchKernelGGL((kron_bsr3_kernel<T, N, U>), dim3(grid), dim3(64 * wpb), 0, s,         \
                        P.nslices, P.slot_node.as<int>(), P.slice_base.as<int>(),              \

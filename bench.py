@@ -548,13 +548,16 @@ def multimesh_meshes(nmesh=64, target_dof=6.0e6):
     return out, f
 
 
-def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
+def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7, group_gb=48.0):
     """The uneven multi-mesh batch: every mesh's K and M stacked block-diagonally, a
     banded factor per run of meshes of similar bandwidth (FP32, xt_prec_create_band_seg),
     one batched FP32-IR slab solve of nmesh x R instances with
     per-instance convergence (xt_slab_solve_seg), timed over K solves; against the same
     meshes solved one at a time in FP64 on the GPU (each with its own operator and FP64
-    banded factor, R load cases per solve)."""
+    banded factor, R load cases per solve), and in FP64 batched by groups: consecutive
+    meshes (similar bandwidth) whose FP64 factors' four block sets take at most group_gb
+    GB together, one segmented FP64 solve per group (the FP64 factors of all meshes do
+    not fit in HBM at once)."""
     import torch
     from xtfem_amd import device, fem, slab
     t0 = time.perf_counter()
@@ -600,14 +603,16 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
                                                      xtol=xtol)
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / K
+    print(f"# multimesh: batched FP32-IR solve {1e3 * t32:.0f} ms", file=sys.stderr, flush=True)
     x32 = x.clone()
     del op, pr
     torch.cuda.empty_cache()
     # one mesh at a time in FP64 (setup excluded from the timed solves)
-    t64, its64, err = 0.0, [], []
+    t64, its64, err, x64, fbytes = 0.0, [], [], [], []
     for si, s in enumerate(systems):
         op1 = device.KronOperator([s["Kb"], s["Mb"]], q=6)
         pr1 = device.BandPreconditioner(s["Kb"])
+        fbytes.append(pr1.matrix_bytes(False))
         b1 = device.to_interleaved(s["rhs"], s["p"], 6)
         x1 = torch.zeros_like(b1)
         device.slab_solve_device(op1, pr1, A, B, b1, x1, R, mode=0)       # warm-up
@@ -624,7 +629,53 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
         for r in range(R):
             cols = slice(6 * r, 6 * r + 6)
             err.append(float(torch.linalg.norm(d[:, cols]) / torch.linalg.norm(ref[:, cols])))
+        x64.append(x1)
         del op1, pr1
+        if si % 16 == 15:
+            print(f"# multimesh: {si + 1} meshes solved one at a time", file=sys.stderr,
+                  flush=True)
+    del x32
+    torch.cuda.empty_cache()
+    # FP64 batched by groups of consecutive meshes within group_gb of factor blocks each
+    groups64, cur, acc = [], [], 0.0
+    for si, fb in enumerate(fbytes):
+        if cur and acc + fb > group_gb * 1e9:
+            groups64.append(cur)
+            cur, acc = [], 0.0
+        cur.append(si)
+        acc += fb
+    groups64.append(cur)
+    tg, itsg, relg, convg, errg = 0.0, [], [], True, []
+    for gi in groups64:
+        Kg, segg = device.block_diagonal([systems[i]["Kb"] for i in gi])
+        Mg, _ = device.block_diagonal([systems[i]["Mb"] for i in gi])
+        opg = device.KronOperator([Kg, Mg], q=6)
+        prg = device.BandPreconditioner(Kg, seg=segg)
+        bg = torch.cat([device.to_interleaved(systems[i]["rhs"], systems[i]["p"], 6)
+                        for i in gi]).contiguous()
+        xg = torch.zeros_like(bg)
+        device.slab_solve_seg(opg, prg, segg, A, B, bg, xg, R, mode=0)   # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        xg.zero_()
+        _, itg, ieg, rrg = device.slab_solve_seg(opg, prg, segg, A, B, bg, xg, R, mode=0)
+        torch.cuda.synchronize()
+        tg += time.perf_counter() - t1
+        itsg += np.asarray(itg).ravel().tolist()
+        relg += np.asarray(rrg).ravel().tolist()
+        convg = convg and bool(np.all(np.asarray(ieg) == 0))
+        for k, si in enumerate(gi):
+            xs = xg[segg[k] * 6 * R:segg[k + 1] * 6 * R]
+            d = (xs - x64[si]).view(-1, 6 * R)
+            ref = x64[si].view(-1, 6 * R)
+            for r in range(R):
+                cols = slice(6 * r, 6 * r + 6)
+                errg.append(float(torch.linalg.norm(d[:, cols]) /
+                                  torch.linalg.norm(ref[:, cols])))
+        del opg, prg, xg, bg
+        torch.cuda.empty_cache()
+        print(f"# multimesh: FP64 group of {len(gi)} meshes solved", file=sys.stderr,
+              flush=True)
     n_inst = nmesh * R
     elems = [s["ne"] for s in systems]
     return dict(workload=f"{nmesh} meshes ({sum(1 for s in systems if s['kind'] == 'rod')} "
@@ -640,6 +691,14 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7):
                 all_converged=bool(np.all(ierr == 0) and np.all(relres <= 1e-8)),
                 relres_max=float(relres.max()),
                 rel_err_vs_fp64_max=float(max(err)),
+                fp64_grouped=dict(groups=len(groups64), group_gb=group_gb,
+                                  meshes_per_group=[len(g) for g in groups64],
+                                  slabs_per_s=n_inst / tg,
+                                  speedup_vs_one_at_a_time=(n_inst / tg) / (n_inst / t64),
+                                  its_max=int(max(itsg)),
+                                  all_converged=bool(convg and max(relg) <= 1e-8),
+                                  relres_max=float(max(relg)),
+                                  rel_err_vs_one_at_a_time_max=float(max(errg))),
                 banded_factor=groups,
                 setup_s=dict(meshes=t_mesh, operator_and_factor=t_setup))
 

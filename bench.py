@@ -99,6 +99,9 @@ def parse():
     ap.add_argument("--cpu-matrices", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the setup and the timed FP32-IR steps (for rocprofv3)")
+    ap.add_argument("--spmm-only", action="store_true",
+                    help="only the workload's operator and the Kron-SpMM microbenchmark "
+                         "(FP32, 1 and 32 load cases), one JSON line (for rocprofv3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     return ap.parse_args()
@@ -1213,6 +1216,11 @@ def main():
         fp32_only = True      # FP64 shifted factors of the panel would not fit beside the rest
     else:
         prob = Problem(*(args.grid or [100, 46, 4]), args.notch, R, args.cases_kind)
+    if args.spmm_only:
+        from xtfem_amd import device
+        op = device.KronOperator([prob.Kb, prob.Mb], q=6)
+        print(json.dumps(dict(workload=prob.describe(), kron_spmm=spmm_legs(op, prob.p))))
+        return
         fp32_only = False
     t_asm = time.perf_counter() - t0
     op, pr, sinfo = setup_solver(prob, args, fp32_only)

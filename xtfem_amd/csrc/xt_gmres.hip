@@ -32,6 +32,7 @@ static constexpr int JB = 8;  // basis vectors per multidot block row
 
 // partial[b][j][col] = sum over the rows of chunk b of V_j[row][col] * w[row][col]; chunk b
 // covers rows chunks[b].x .. chunks[b].y - 1 (chunks never straddle two segments).
+// 256 / c row lanes of c columns; the block has at least those threads, at most 256.
 // skip (device flag, may be NULL): nonzero makes the launch a no-op.  frozen (device,
 // per instance, may be NULL): the columns of frozen instances are taken as zero.
 __global__ void __launch_bounds__(256)
@@ -609,7 +610,10 @@ struct Solver {
     int dots(int k, const double *Vbase, const double *w, double *out,
              const int *skip = nullptr, const int *frz = nullptr) {
         dim3 grid(nblk, ceil_div(k, JB));
-        hipLaunchKernelGGL(multidot_partial, grid, dim3(256), 0, s, c, k,
+        // the kernel's 256 / c row lanes of c columns, rounded up to whole waves: no wave
+        // that only waits at the barriers holds a slot (64 of 256 threads at c = 192)
+        const int nthr = ((256 / c) * c + 63) / 64 * 64;
+        hipLaunchKernelGGL(multidot_partial, grid, dim3(nthr), 0, s, c, k,
                            ws.chunks.as<const int2>(), Vbase, (size_t)n, w, partial.as<double>(),
                            skip, q, R, segof, frz);
         hipLaunchKernelGGL(reduce_partials, dim3(k, I), dim3(256), 0, s, k, c, q, R, I,

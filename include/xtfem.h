@@ -169,7 +169,8 @@ int xt_csrapplybc(int *ia, int *ja, double *a, int n, int *nz, const int *bc);
  * the matrices a deletion event leaves (delelement / adddiagonal / csrapplybc only drop
  * entries of the assembled pattern or set its diagonal), so xt_op_set_values and
  * xt_prec_refactor_band update the device handles in place.  No counterpart in the
- * reference, which rebuilds its MUMPS structure (main3d.f90:683-684). */
+ * reference, which applies the constraints to the edited K and M (csrapplybc,
+ * main3d.f90:684-685) and refactors with MUMPS (main3d.f90:687). */
 int xt_csr_values_in_pattern(int n, const int *pia, const int *pja, const int *ia,
                              const int *ja, const double *a, double *out);
 
@@ -272,7 +273,7 @@ int xt_prec_refactor_band(xt_prec pr, const int *rowptr, const int *colidx,
  * D_{j-1} and K (equal to a full refactorization up to rounding).  A factor kept in FP32
  * only is refactored whole.  Plain banded factors only (no shifted or grouped factors).
  * Replaces the refactorization after element deletion, mumps_solver(job = 1),
- * main3d.f90:686. */
+ * main3d.f90:687. */
 int xt_prec_refresh_band(xt_prec pr, const int *rowptr, const int *colidx,
                          const double *vals, int first_row, void *stream);
 /* Block size, number of blocks and bandwidth of a banded-factor preconditioner (of a

@@ -11,7 +11,7 @@ When some Gauss point reaches the critical damage Dc the reference, in order:
    delelement.f90:204-247) and fixes them in every temporal block
    (main3d.f90:666-675);
 5. applies the fixed set to M and K (csrapplybc) and refactors the
-   preconditioner (main3d.f90:681-686);
+   preconditioner (main3d.f90:683-687);
 6. shortens the slab: dt = dt0*n2, nip = nipc*n2 (main3d.f90:689-690).
 
 The sparse edits are host C++ in libxtfem_hip.so (bit-identical to the
@@ -75,7 +75,7 @@ class ElementDeletion:
         self.events = []
 
     def apply(self, nefail):
-        """Delete the elements ``nefail`` (1-based ids): main3d.f90:650-686.
+        """Delete the elements ``nefail`` (1-based ids): main3d.f90:650-687.
         Returns a DeletionEvent, or None when the list is empty."""
         nefail = np.asarray(nefail, dtype=np.int32)
         if nefail.size == 0:

@@ -121,7 +121,7 @@ class FatigueSlabs:
             raise RuntimeError("the specimen has separated: no further slabs")
         rec = {"slab": self.n}
         t0 = time.perf_counter()
-        ev = self.deletion.step(self.damage)           # chkelesta + edits (main3d.f90:625-686)
+        ev = self.deletion.step(self.damage)           # chkelesta + edits (main3d.f90:625-687)
         rec["failed"] = [] if ev is None else ev.nefail.tolist()
         self.first_row = 0
         if ev is not None:

@@ -403,6 +403,22 @@ int xt_slab_solve_seg(xt_op op, xt_prec pr, int nseg, const long long *seg_dof, 
                       double xtol, int maxits, int mode, int *its, int *ierr, double *relres,
                       void *stream);
 
+/* mpipgmres (kronecker_mumps.f90:186-415): the reference's live solve path, GMRES
+ * right-preconditioned by A^-1 (x) K^-1 with K^-1 from the MUMPS factor, on host arrays.
+ * Same arguments as the reference with the preconditioner handle pr in place of
+ * mumps_par (the driver's mumps_solver job 0, main3d.f90:550, becomes
+ * xt_prec_create_band; job 1 after re-assembly or deletion, main3d.f90:687 and 783,
+ * becomes xt_prec_refactor_band or xt_prec_refresh_band; any preconditioner handle
+ * works) and no MPI (one device holds all of K and M).  The operator is built from
+ * the CSR arrays per call, rhs and sol staged to the device and back; FP64 throughout,
+ * the reference's algorithm (xt_slab_solve mode 0).  Outputs and residual history as
+ * xt_kronpgmres. */
+int xt_mpipgmres(const double *a, const double *b, int q,
+                 const int *ki, const int *kj, const double *kv,
+                 const int *mi, const int *mj, const double *mv, int p,
+                 xt_prec pr, double *rhs, double *sol, int im, double eps, int maxits,
+                 double *hist, int hist_cap, int *nhist, int *its, int *ierr);
+
 /* Iterations of the last slab solve on op (any of the entry points above), summed over
  * its instances: it32 those whose preconditioner sweeps ran in FP32 (GMRES-IR's FP32
  * cycles, mode 2), it64 those in FP64 (mode 0, GMRES-IR's FP64 cycles).  No counterpart

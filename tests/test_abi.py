@@ -184,6 +184,37 @@ def test_fortran_binding_links_and_runs_ilut(oracle_lib, tmp_path):
     assert np.array_equal(alu, oa[:m])
 
 
+def test_fortran_binding_live_path(tmp_path):
+    """The live-path wrappers of the Fortran binding (mumps_solver_hip and mpipgmres_hip in
+    place of mumps_solver and mpipgmres, the factor as a handle) compile, link against
+    libxtfem_hip.so and reach it: without a device the factorisation stops with the
+    library's message through xt_check, as every fatal path of the binding does; with one
+    the solve of (K + 0.5 I) x = 1 (K the 1-D Laplacian of order 4) converges to it."""
+    if not os.path.exists(FLANG):
+        pytest.skip("flang not available")
+    if not os.path.exists(LIB):
+        pytest.skip("libxtfem_hip.so not built")
+    fdir = os.path.join(ROOT, "xtfem_amd", "fortran")
+    exe = str(tmp_path / "live_driver")
+    libdir = os.path.dirname(LIB)
+    subprocess.run([FLANG, "-c", os.path.join(fdir, "xtfem_hip.f90"), "-module-dir",
+                    str(tmp_path), "-o", str(tmp_path / "xtfem_hip.o")], check=True)
+    subprocess.run([FLANG, os.path.join(ROOT, "tests", "fortran", "live_driver.f90"),
+                    "-I", str(tmp_path), str(tmp_path / "xtfem_hip.o"), "-L", libdir,
+                    "-lxtfem_hip", f"-Wl,-rpath,{libdir}", "-Wl,-rpath-link,/opt/rocm/lib",
+                    "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120).stdout
+    if "failed" in out:
+        assert "libxtfem_hip: mumps_solver_hip failed" in out, out
+        assert "ierr" not in out, out
+        return
+    lines = out.split("\n")
+    assert lines[0].split() == ["ierr", "0"], out
+    sol = np.array(lines[1].split()[1:], dtype=np.float64)
+    K = 2.0 * np.eye(4) - np.eye(4, k=1) - np.eye(4, k=-1)
+    assert rel(sol, np.linalg.solve(K + 0.5 * np.eye(4), np.ones(4))) < 1e-12
+
+
 def test_fortran_binding_runs_the_deletion_sequence(tmp_path):
     """delelement / chkcnsta / chkndsta / adddiagonal called by their reference names
     from Fortran (tests/fortran/delete_driver.f90) give the same CSR, counts and orphan

@@ -36,6 +36,8 @@ _SIGS = {
     "xt_kronlusol": (ci, [vp, ci, vp, vp, vp, ci, vp, vp]),
     "xt_kronpgmres": (ci, [vp, vp, ci, vp, vp, vp, vp, vp, vp, ci, vp, vp, vp, vp, vp, ci, cd,
                            ci, vp, ci, vp, vp, vp]),
+    "xt_mpipgmres": (ci, [vp, vp, ci, vp, vp, vp, vp, vp, vp, ci, vp, vp, vp, ci, cd, ci, vp,
+                          ci, vp, vp, vp]),
     "xt_kokkos_host": (ci, [ci] * 9 + [vp] * 17),
     "xt_chkelesta": (ci, [ci, ci, cd, vp, vp, vp, vp, vp]),
     "xt_delelement": (ci, [ci] * 7 + [vp] * 7 + [ci, vp, vp, vp, vp, ci, vp]),

@@ -132,6 +132,27 @@ def kronpgmres(A, B, K, M, prec, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
     return sol, ierr.value, its.value, hist[:nh.value].copy()
 
 
+def mpipgmres(A, B, K, M, pr, rhs, sol0=None, im=30, eps=1e-8, maxits=500):
+    """The live path's GMRES, kronecker_mumps.f90:186-415, with the preconditioner handle
+    pr (e.g. device.BandPreconditioner(K), the exact factor MUMPS gives the reference) in
+    place of mumps_par.  Returns (sol, ierr, its, hist) as kronpgmres."""
+    q = A.shape[0]
+    ki, kj, kv = _i32(K[0]), _i32(K[1]), _f64(K[2])
+    mi, mj, mv = _i32(M[0]), _i32(M[1]), _f64(M[2])
+    p = len(ki) - 1
+    a, b = _fmat(A), _fmat(B)
+    rhs = _f64(rhs).copy()
+    sol = np.zeros(p * q) if sol0 is None else _f64(sol0).copy()
+    cap = maxits + 64
+    hist = np.zeros(cap)
+    nh, its, ierr = C.c_int(0), C.c_int(0), C.c_int(0)
+    check(lib().xt_mpipgmres(_P(a), _P(b), q, _P(ki), _P(kj), _P(kv), _P(mi), _P(mj), _P(mv), p,
+                             pr.h, _P(rhs), _P(sol), int(im), float(eps), int(maxits), _P(hist),
+                             cap, C.addressof(nh), C.addressof(its), C.addressof(ierr)),
+          "mpipgmres")
+    return sol, ierr.value, its.value, hist[:nh.value].copy()
+
+
 def kokkos_host(ne, nn, nip, nodes, Nx, Ny, Nz, Nt, sol, ym, nu, state):
     """Damage functor over all Gauss points.  kokkos_host.cpp:37-190 (kokkos_kernel.h:44-344).
     Returns a new state dict (eps_old, eps_pl, sig_eff, sig_back, p, D, ws, del_po, sig_d)."""

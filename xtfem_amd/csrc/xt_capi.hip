@@ -139,19 +139,19 @@ int xt_kronlusol(const double *a, int q, const double *alu, const int *jlu, cons
     return XT_OK;
 }
 
-int xt_kronpgmres(const double *a, const double *b, int q, const int *ki, const int *kj,
-                  const double *kv, const int *mi, const int *mj, const double *mv, int p,
-                  const double *alu, const int *jlu, const int *ju, double *rhs, double *sol,
-                  int im, double eps, int maxits, double *hist, int hist_cap, int *nhist,
-                  int *its, int *ierr) {
-    XT_ARG(a && b && q > 0 && q <= 8 && p > 0 && rhs && sol && ierr, "kronpgmres arguments");
+// One host-array GMRES solve of (A (x) K + B (x) M) sol = rhs with the preconditioner pr
+// (kronpgmres and mpipgmres share it): the operator is built from the CSR arrays, rhs and
+// sol staged into the interleaved device layout, the FP64 slab solve run with the
+// reference's algorithm (mode 0), sol and the residual history copied back.
+static int host_pgmres(const double *a, const double *b, int q, const int *ki, const int *kj,
+                       const double *kv, const int *mi, const int *mj, const double *mv, int p,
+                       xt_prec pr, double *rhs, double *sol, int im, double eps, int maxits,
+                       double *hist, int hist_cap, int *nhist, int *its, int *ierr) {
     OpGuard og;
-    PrecGuard pg;
     const int *rp[2] = {ki, mi};
     const int *ci[2] = {kj, mj};
     const double *vv[2] = {kv, mv};
     XT_TRY(xt_op_create(p, q, 2, rp, ci, vv, &og.op));
-    XT_TRY(xt_prec_create(p, alu, jlu, ju, &pg.pr));
     const size_t n = (size_t)p * q;
     std::vector<double> ab(2 * q * q);
     for (int k = 0; k < q * q; ++k) { ab[k] = a[k]; ab[q * q + k] = b[k]; }
@@ -166,7 +166,7 @@ int xt_kronpgmres(const double *a, const double *b, int q, const int *ki, const 
     std::vector<double> h;
     int itv = 0, ie = 0;
     double rr = 0.0;
-    XT_TRY(slab_solve_impl(og.op, pg.pr, 1, ab.data(), db.as<double>(), dxs.as<double>(), im,
+    XT_TRY(slab_solve_impl(og.op, pr, 1, ab.data(), db.as<double>(), dxs.as<double>(), im,
                            eps, maxits, 0, &itv, &ie, &rr, nullptr, &h));
     XT_TRY(xt_from_interleaved(p, q, 1, dxs.as<double>(), t.as<double>(), nullptr));
     XT_HIP(hipMemcpy(sol, t.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -178,6 +178,28 @@ int xt_kronpgmres(const double *a, const double *b, int q, const int *ki, const 
         *nhist = m;
     }
     return XT_OK;
+}
+
+int xt_kronpgmres(const double *a, const double *b, int q, const int *ki, const int *kj,
+                  const double *kv, const int *mi, const int *mj, const double *mv, int p,
+                  const double *alu, const int *jlu, const int *ju, double *rhs, double *sol,
+                  int im, double eps, int maxits, double *hist, int hist_cap, int *nhist,
+                  int *its, int *ierr) {
+    XT_ARG(a && b && q > 0 && q <= 8 && p > 0 && rhs && sol && ierr, "kronpgmres arguments");
+    PrecGuard pg;
+    XT_TRY(xt_prec_create(p, alu, jlu, ju, &pg.pr));
+    return host_pgmres(a, b, q, ki, kj, kv, mi, mj, mv, p, pg.pr, rhs, sol, im, eps, maxits,
+                       hist, hist_cap, nhist, its, ierr);
+}
+
+int xt_mpipgmres(const double *a, const double *b, int q, const int *ki, const int *kj,
+                 const double *kv, const int *mi, const int *mj, const double *mv, int p,
+                 xt_prec pr, double *rhs, double *sol, int im, double eps, int maxits,
+                 double *hist, int hist_cap, int *nhist, int *its, int *ierr) {
+    XT_ARG(a && b && q > 0 && q <= 8 && p > 0 && pr && rhs && sol && ierr,
+           "mpipgmres arguments");
+    return host_pgmres(a, b, q, ki, kj, kv, mi, mj, mv, p, pr, rhs, sol, im, eps, maxits, hist,
+                       hist_cap, nhist, its, ierr);
 }
 
 int xt_kokkos_host(int rank, int ne, int nn, int ndofn, int nnse, int nnte, int ngpe, int ncmp,

@@ -89,6 +89,32 @@ module xtfem_hip
       integer(c_int), intent(out) :: nhist, its, ierr
     end function
 
+    integer(c_int) function xt_mpipgmres(a, b, q, ki, kj, kv, mi, mj, mv, p, pr, rhs, sol, &
+        im, eps, maxits, hist, hist_cap, nhist, its, ierr) bind(C, name="xt_mpipgmres")
+      import :: c_int, c_double, c_ptr
+      real(c_double), intent(in) :: a(*), b(*), kv(*), mv(*)
+      integer(c_int), value :: q, p, im, maxits, hist_cap
+      real(c_double), value :: eps
+      integer(c_int), intent(in) :: ki(*), kj(*), mi(*), mj(*)
+      type(c_ptr), value :: pr
+      real(c_double), intent(inout) :: rhs(*), sol(*), hist(*)
+      integer(c_int), intent(out) :: nhist, its, ierr
+    end function
+
+    integer(c_int) function xt_prec_create_band(p, rowptr, colidx, vals, block, out) &
+        bind(C, name="xt_prec_create_band")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: p, block
+      integer(c_int), intent(in) :: rowptr(*), colidx(*)
+      real(c_double), intent(in) :: vals(*)
+      type(c_ptr), intent(out) :: out
+    end function
+
+    subroutine xt_prec_destroy(pr) bind(C, name="xt_prec_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: pr
+    end subroutine
+
     integer(c_int) function xt_chkelesta(ne, ngpe, dc, d, elesta, gpsta, nef, nefail) &
         bind(C, name="xt_chkelesta")
       import :: c_int, c_double
@@ -240,6 +266,48 @@ subroutine kronpgmres(a, b, q, ki, kj, kv, mi, mj, mv, p, alu, jlu, ju, rhs, sol
   deallocate(hist)
 199 format('   its =', i4, ' rel. res. norm =', d20.6)
 end subroutine kronpgmres
+
+! The live path (mumps.f90, kronecker_mumps.f90) with the exact banded factor of K as an
+! opaque handle in place of mumps_par (a MUMPS structure cannot cross the C ABI):
+! mumps_solver_hip(pr, p, ki, kj, kv, job) factors K as mumps_solver(..., job) does
+! (job 0: the first factor, main3d.f90:550; job 1: K changed, the handle's factor is
+! replaced, main3d.f90:687 and 783), and mpipgmres_hip takes the handle where
+! mpipgmres takes mumps_par, the other arguments as the reference's.
+subroutine mumps_solver_hip(pr, p, ki, kj, kv, job)
+  use xtfem_hip
+  implicit none
+  type(c_ptr), intent(inout) :: pr
+  integer, intent(in) :: p, ki(*), kj(*), job
+  real(8), intent(in) :: kv(*)
+  if (job /= 0 .and. c_associated(pr)) call xt_prec_destroy(pr)
+  pr = c_null_ptr
+  call xt_check(xt_prec_create_band(p, ki, kj, kv, 0, pr), 'mumps_solver_hip')
+end subroutine mumps_solver_hip
+
+subroutine mpipgmres_hip(a, b, q, ki, kj, kv, mi, mj, mv, p, pr, rhs, sol, im, eps, &
+    maxits, iout, ierr)
+  use xtfem_hip
+  implicit none
+  integer, intent(in) :: q, p, ki(*), kj(*), mi(*), mj(*), im, maxits, iout
+  real(8), intent(in) :: a(*), b(*), kv(*), mv(*), eps
+  type(c_ptr), intent(in) :: pr
+  real(8), intent(inout) :: rhs(*), sol(*)
+  integer, intent(out) :: ierr
+  real(8), allocatable :: hist(:)
+  integer :: nhist, its, k
+  allocate(hist(maxits + im + 2))
+  call xt_check(xt_mpipgmres(a, b, q, ki, kj, kv, mi, mj, mv, p, pr, rhs, sol, im, eps, &
+                             maxits, hist, size(hist), nhist, its, ierr), 'mpipgmres')
+  ! the residual log of kronecker_mumps.f90:270-271, 344, as kronpgmres writes it
+  if (iout >= 0 .and. nhist > 0) write(iout, 199) 0, hist(1)
+  if (iout > 0) then
+    do k = 2, nhist
+      write(iout, 199) k - 1, hist(k)
+    end do
+  end if
+  deallocate(hist)
+199 format('   its =', i4, ' rel. res. norm =', d20.6)
+end subroutine mpipgmres_hip
 
 ! Element deletion (delelement.f90): reference names and argument lists, so
 ! main3d.f90:632-691 links against these in place of delelement.o.  As in the

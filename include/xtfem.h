@@ -372,7 +372,8 @@ int xt_from_interleaved(int p, int q, int R, const double *src, double *dst, voi
  * ab: host [R][2][q][q] column-major A_r, B_r (as passed to kronpgmres).
  * b, x: device, interleaved, FP64.  x is the initial guess on entry.
  * Outputs per r (host arrays of length R): its, ierr, relres (true
- * ||b - A x|| / ||b||, FP64).  eps is relative to the initial residual. */
+ * ||b - A x|| / ||b||, FP64).  eps is relative to the initial residual.  pr must have
+ * op's p rows (XT_E_ARG otherwise, before any device work). */
 int xt_slab_solve(xt_op op, xt_prec pr, int R, const double *ab, const double *b,
                   double *x, int im, double eps, int maxits, int mode, int *its,
                   int *ierr, double *relres, void *stream);

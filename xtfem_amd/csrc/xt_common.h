@@ -135,6 +135,8 @@ int csr_upload_1based(int n, const int *ia, const int *ja, const double *a, DevC
 
 // Sizes of an operator handle.
 int op_dims(xt_op op, int *p, int *q, int *nterms);
+// Spatial rows of a preconditioner handle (-1 for none).
+int prec_rows(xt_prec pr);
 // q x q row-major inverse with partial pivoting; 0 on success, 1 if singular.
 int small_inverse_rowmajor(int q, const double *A, double *Ainv);
 

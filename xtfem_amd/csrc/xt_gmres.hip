@@ -1037,6 +1037,7 @@ int xt::slab_solve_impl(xt_op op, xt_prec pr, int R, const double *ab, const dou
         int p = 0, q = 0, nt = 0;
         XT_TRY(op_dims(op, &p, &q, &nt));
         XT_ARG(nt == 2, "slab_solve needs a two-term (K, M) operator");
+        XT_ARG(prec_rows(pr) == p, "the preconditioner and the operator differ in size");
         S.p = p;
         S.q = q;
     }

@@ -315,6 +315,8 @@ struct xt_prec_s {
 
 namespace xt {
 
+int prec_rows(xt_prec pr) { return pr ? pr->p : -1; }
+
 static int upload_tri(int n, const std::vector<int> &rp, const std::vector<int> &ci,
                       const std::vector<double> &v, bool forward, Tri &T) {
     T.m.n = n;

@@ -9,7 +9,9 @@
 ! program, as the reference's fatal paths do; the reference's own ierr conventions
 ! are passed through unchanged.  kronpgmres
 ! writes the residual history to unit iout in the reference's format
-! (kronecker.f90:258, 311, 354).
+! (kronecker.f90:258, 311, 354).  The live path (mumps.f90, kronecker_mumps.f90)
+! carries a MUMPS structure in its argument lists; mumps_solver_hip and mpipgmres_hip
+! take an opaque factor handle in its place.
 module xtfem_hip
   use iso_c_binding
   implicit none

@@ -54,6 +54,18 @@ def test_python_bindings_cover_the_header():
     assert set(declared()) == set(_lib.EXPORTS)
 
 
+def test_tuner_knob_ids_match_the_header():
+    """xtfem_amd.tune's knob names map to the XT_TUNE_* ids include/xtfem.h defines, and
+    every knob the header defines is tunable from Python."""
+    from xtfem_amd import tune
+    text = open(HEADER).read()
+    ids = {m.group(1).lower(): int(m.group(2))
+           for m in re.finditer(r"#define XT_TUNE_([A-Z0-9_]+) (\d+)", text)}
+    n = ids.pop("nknobs")
+    assert sorted(ids.values()) == list(range(n))
+    assert ids == tune.KNOBS
+
+
 def test_host_ilut_bit_exact_with_oracle(oracle_lib, rod, golden):
     """xt_ilut (host C++ in the product) reproduces ilut.f: same factors, bit for bit."""
     from xtfem_amd import api

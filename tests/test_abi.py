@@ -54,6 +54,40 @@ def test_python_bindings_cover_the_header():
     assert set(declared()) == set(_lib.EXPORTS)
 
 
+def _header_params():
+    """{name: [C type of each parameter]} of the functions include/xtfem.h declares."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    text = re.sub(r"#.*", "", text)
+    out = {}
+    for m in re.finditer(r"\b(xt_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        args = " ".join(m.group(2).split())
+        out[m.group(1)] = [] if args in ("", "void") else [a.strip() for a in args.split(",")]
+    return out
+
+
+def test_python_signatures_match_the_header():
+    """Every ctypes argument list in xtfem_amd/_lib.py has the header's parameter count
+    and kinds (pointers and handles as void *, int, unsigned, long long, double), so no
+    call through the binding shifts its arguments."""
+    import ctypes as C
+    from xtfem_amd import _lib
+
+    def kind(decl):
+        if "*" in decl or re.match(r"(const\s+)?(xt_op|xt_prec|xt_asm)\b", decl):
+            return C.c_void_p
+        base = re.sub(r"\b(const|signed)\b", "", decl).split()
+        t = " ".join(base[:-1])
+        return {"int": C.c_int, "unsigned": C.c_uint, "unsigned int": C.c_uint,
+                "long long": C.c_longlong, "double": C.c_double}[t]
+
+    params = _header_params()
+    assert set(params) == set(_lib._SIGS)
+    for name, (_, argtypes) in _lib._SIGS.items():
+        want = [kind(d) for d in params[name]]
+        got = [C.c_void_p if a in (C.c_char_p, _lib.pi, _lib.pd) else a for a in argtypes]
+        assert got == want, (name, params[name])
+
+
 def test_tuner_knob_ids_match_the_header():
     """xtfem_amd.tune's knob names map to the XT_TUNE_* ids include/xtfem.h defines, and
     every knob the header defines is tunable from Python."""

@@ -88,6 +88,30 @@ def test_python_signatures_match_the_header():
         assert got == want, (name, params[name])
 
 
+def test_fortran_interfaces_match_the_header():
+    """Each BIND(C) interface of the Fortran binding has the C declaration's parameters in
+    order, passed the same way: a scalar or handle by value in C has the VALUE attribute,
+    a pointer parameter is passed by reference."""
+    text = open(os.path.join(ROOT, "xtfem_amd", "fortran", "xtfem_hip.f90")).read()
+    text = re.sub(r"&\s*\n\s*", " ", text)            # join continuation lines
+    params = _header_params()
+    blocks = re.findall(r"(?:function|subroutine)\s+(xt_[a-z0-9_]+)\s*\(([^)]*)\)[^\n]*"
+                        r"bind\(C,\s*name=\"(xt_[a-z0-9_]+)\"\)(.*?)end (?:function|subroutine)",
+                        text, flags=re.S)
+    assert len(blocks) >= 15
+    for fname, args, cname, body in blocks:
+        assert fname == cname and cname in params, cname
+        names = [a.strip() for a in args.split(",") if a.strip()]
+        decls = params[cname]
+        assert len(names) == len(decls), (cname, names, decls)
+        byval = set()
+        for line in body.split("\n"):
+            if "::" in line and re.search(r"\bvalue\b", line.split("::")[0]):
+                byval |= {v.strip().split("(")[0] for v in line.split("::")[1].split(",")}
+        for nm, d in zip(names, decls):
+            assert (nm in byval) == ("*" not in d), (cname, nm, d)
+
+
 def test_tuner_knob_ids_match_the_header():
     """xtfem_amd.tune's knob names map to the XT_TUNE_* ids include/xtfem.h defines, and
     every knob the header defines is tunable from Python."""

@@ -186,6 +186,26 @@ def test_deletion_block_golden(rod, golden, readd):
         assert same(dl.K, api.csrapplybc(rod["K"], dl.fixed))
 
 
+def test_event_without_orphans_keeps_the_fixed_matrices(bar):
+    """An event that orphans no node leaves the fixed set as it was; the 1-0 method is
+    idempotent, so the edited M, and on re-assembly K and M, are the current matrices
+    unchanged: equal to the 1-0 method applied again, bit for bit."""
+    from xtfem_amd import api
+    from xtfem_amd.deletion import ElementDeletion
+    mesh = bar["mesh"]
+    single = failed_sets(mesh)[0]
+    for readd in (False, True):
+        dl = ElementDeletion(mesh, bar["K"], bar["M"], bar["bc"], YM_REF, NU_REF, readd=readd,
+                             edited=True)
+        M0 = dl.M
+        ev = dl.apply(single)
+        assert len(ev.ndfail) == 0
+        assert same(ev.M_edited, api.csrapplybc(M0, ev.fixed))
+        if readd:
+            assert same(dl.K, api.csrapplybc(bar["K"], ev.fixed))
+            assert same(dl.M, api.csrapplybc(bar["M"], ev.fixed))
+
+
 def test_deletion_block_vs_oracle(bar):
     """The product's deletion block against the numpy restatement (oracle) to rounding."""
     from xtfem_amd import fem

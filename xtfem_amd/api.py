@@ -228,7 +228,8 @@ def delelement(xconn, xcoord, dns, dnxyz, sd, stiff, wqs, nefail, K):
                               _P(dns), _P(dnxyz), _P(sd), _P(stiff), _P(wqs), len(nefail),
                               _P(nefail), _P(ki), _P(kj), _P(kv), ndof, C.addressof(knz)),
           "delelement")
-    return ki, kj[:knz.value].copy(), kv[:knz.value].copy()
+    # views of the staging arrays (a few element matrices' room beyond knz): no copy of K
+    return ki, kj[:knz.value], kv[:knz.value]
 
 
 def chkcnsta(nn, xconn, nefail, necnt):
@@ -254,12 +255,15 @@ def chkndsta(nn, xconn, nefail, necnt):
 
 
 def adddiagonal(ndofn, ndfail, K):
-    """K + I on the DOFs of the orphan nodes (delelement.f90:204-247)."""
+    """K + I on the DOFs of the orphan nodes (delelement.f90:204-247); K's own arrays
+    when there are none (the reference's loop then adds nothing)."""
     ndfail = _i32(ndfail)
+    if len(ndfail) == 0:
+        return _i32(K[0]), _i32(K[1]), np.ascontiguousarray(K[2], dtype=np.float64)
     ki, kj, kv, knz = _csr_in(K, extra=len(ndfail) * ndofn)
     check(lib().xt_adddiagonal(8, ndofn, len(ndfail), _P(ndfail), _P(ki), _P(kj), _P(kv),
                                len(ki) - 1, C.addressof(knz)), "adddiagonal")
-    return ki, kj[:knz.value].copy(), kv[:knz.value].copy()
+    return ki, kj[:knz.value], kv[:knz.value]
 
 
 def csrapplybc(S, bc):
@@ -268,7 +272,7 @@ def csrapplybc(S, bc):
     bc = _i32(bc)
     check(lib().xt_csrapplybc(_P(ia), _P(ja), _P(a), len(ia) - 1, C.addressof(nz), _P(bc)),
           "csrapplybc")
-    return ia, ja[:nz.value].copy(), a[:nz.value].copy()
+    return ia, ja[:nz.value], a[:nz.value]
 
 
 def csr_values_in_pattern(P, S):

@@ -88,18 +88,22 @@ class ElementDeletion:
         for node in ndfail:
             self.fixed[3 * (node - 1):3 * node] = 1
         t["node_status"] = time.perf_counter() - t0
+        # the 1-0 method is idempotent: without a new orphan node the fixed set is the one
+        # the current M (and, re-assembling, K) already carry, so they are their own result
+        new_fixed = len(ndfail) > 0
         K_ed = M_ed = None
         if self.edited:
             t0 = time.perf_counter()
             K1 = api.delelement(xconn, self.mesh.xcoord, self.dns, self.dnxyz, fem.SD,
                                 self.stiff, self.wqs, nefail, self.K)
             K_ed = api.csrapplybc(api.adddiagonal(3, ndfail, K1), self.fixed)
-            M_ed = api.csrapplybc(self.M, self.fixed)
+            M_ed = api.csrapplybc(self.M, self.fixed) if new_fixed else self.M
             t["edit"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         if self.readd:
-            self.K = api.csrapplybc(self.K_asm, self.fixed)
-            self.M = api.csrapplybc(self.M_asm, self.fixed)
+            if new_fixed:
+                self.K = api.csrapplybc(self.K_asm, self.fixed)
+                self.M = api.csrapplybc(self.M_asm, self.fixed)
         else:
             self.K, self.M = K_ed, M_ed
         t["next_matrices"] = time.perf_counter() - t0

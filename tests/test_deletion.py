@@ -189,18 +189,26 @@ def test_deletion_block_golden(rod, golden, readd):
 def test_event_without_orphans_keeps_the_fixed_matrices(bar):
     """An event that orphans no node leaves the fixed set as it was; the 1-0 method is
     idempotent, so the edited M, and on re-assembly K and M, are the current matrices
-    unchanged: equal to the 1-0 method applied again, bit for bit."""
-    from xtfem_amd import api
+    unchanged, and the edited K of elements with no fixed DOF needs no 1-0 pass: each
+    equal to the driver block's whole chain, bit for bit."""
+    from xtfem_amd import api, fem
     from xtfem_amd.deletion import ElementDeletion
     mesh = bar["mesh"]
     single = failed_sets(mesh)[0]
     for readd in (False, True):
         dl = ElementDeletion(mesh, bar["K"], bar["M"], bar["bc"], YM_REF, NU_REF, readd=readd,
                              edited=True)
-        M0 = dl.M
+        M0, K0 = dl.M, dl.K
         ev = dl.apply(single)
         assert len(ev.ndfail) == 0
         assert same(ev.M_edited, api.csrapplybc(M0, ev.fixed))
+        # the edited K without the 1-0 pass (no fixed DOF among the element's) against
+        # the whole chain of the driver block
+        nodes = mesh.xconn[single - 1].ravel() - 1
+        assert not ev.fixed[(3 * nodes[:, None] + np.arange(3)).ravel()].any()
+        K1 = api.delelement(mesh.xconn, mesh.xcoord, dl.dns, dl.dnxyz, fem.SD, dl.stiff,
+                            dl.wqs, single, K0)
+        assert same(ev.K_edited, api.csrapplybc(api.adddiagonal(3, ev.ndfail, K1), ev.fixed))
         if readd:
             assert same(dl.K, api.csrapplybc(bar["K"], ev.fixed))
             assert same(dl.M, api.csrapplybc(bar["M"], ev.fixed))

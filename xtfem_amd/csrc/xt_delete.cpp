@@ -34,37 +34,40 @@ struct Triplet {
     double v;
 };
 
-// C := alpha*A + beta*B over n rows (1-based CSR in and out).  B is given as
-// per-row runs of (col, value) pairs sorted by row; the result goes to
-// (ic, jc, c).
+// Whether row r of a 1-based CSR has strictly increasing columns and no exact zeros:
+// such a row comes out of the scatter of C := 1*A + beta*B unchanged where B has
+// nothing in it (1*a == a).
+bool clean_row(const int *ia, const int *ja, const double *a, int r) {
+    const int k0 = ia[r] - 1, k1 = ia[r + 1] - 1;
+    for (int k = k0; k < k1; ++k)
+        if (!(a[k] != 0.0 && (k == k0 || ja[k] > ja[k - 1]))) return false;
+    return true;
+}
+
+// C := alpha*A + beta*B over rows rbeg .. rend-1 of an n-row 1-based CSR A.  B is given
+// as per-row runs of (col, value) pairs sorted by row; the rows of C go to (ic, jc, c)
+// with ic[0] = 1 at row rbeg (ic has rend - rbeg + 1 entries).
 void csr_add_rows(int n, const int *ia, const int *ja, const double *a, double alpha,
                   const std::vector<int> &ib, const std::vector<int> &jb,
                   const std::vector<double> &b, double beta, std::vector<int> &ic,
-                  std::vector<int> &jc, std::vector<double> &c) {
+                  std::vector<int> &jc, std::vector<double> &c, int rbeg, int rend) {
     std::vector<int> mark(n, -1);
     std::vector<double> x(n);
     std::vector<int> cols;
     cols.reserve(128);
-    ic.assign(n + 1, 0);
+    ic.assign(rend - rbeg + 1, 0);
     jc.clear();
     c.clear();
-    jc.reserve((size_t)(ia[n] - 1) + jb.size());
-    c.reserve((size_t)(ia[n] - 1) + jb.size());
+    jc.reserve((size_t)(ia[rend] - ia[rbeg]) + jb.size());
+    c.reserve((size_t)(ia[rend] - ia[rbeg]) + jb.size());
     ic[0] = 1;
-    for (int r = 0; r < n; ++r) {
+    for (int r = rbeg; r < rend; ++r) {
         const int k0 = ia[r] - 1, k1 = ia[r + 1] - 1;
-        if (ib[r] == ib[r + 1] && alpha == 1.0) {
-            // B has nothing in this row: a row with strictly increasing columns
-            // and no zeros comes out of the scatter unchanged (1*a == a)
-            bool clean = true;
-            for (int k = k0; k < k1 && clean; ++k)
-                clean = a[k] != 0.0 && (k == k0 || ja[k] > ja[k - 1]);
-            if (clean) {
-                jc.insert(jc.end(), ja + k0, ja + k1);
-                c.insert(c.end(), a + k0, a + k1);
-                ic[r + 1] = (int)jc.size() + 1;
-                continue;
-            }
+        if (ib[r] == ib[r + 1] && alpha == 1.0 && clean_row(ia, ja, a, r)) {
+            jc.insert(jc.end(), ja + k0, ja + k1);
+            c.insert(c.end(), a + k0, a + k1);
+            ic[r - rbeg + 1] = (int)jc.size() + 1;
+            continue;
         }
         cols.clear();
         for (int k = k0; k < k1; ++k) {
@@ -80,8 +83,46 @@ void csr_add_rows(int n, const int *ia, const int *ja, const double *a, double a
         std::sort(cols.begin(), cols.end());
         for (int j : cols)
             if (x[j] != 0.0) { jc.push_back(j + 1); c.push_back(x[j]); }
-        ic[r + 1] = (int)jc.size() + 1;
+        ic[r - rbeg + 1] = (int)jc.size() + 1;
     }
+}
+
+// C := A + beta*B in A's own arrays (1-based, cap entries of storage), B as for
+// csr_add_rows.  Only the rows from B's first to its last nonempty row are merged when
+// every other row is clean (they are C's rows unchanged: the leading ones stay where
+// they are, the trailing ones move by the change in entry count); otherwise every row
+// is.  The result is the same either way.
+int csr_add_in_place(int n, int *ia, int *ja, double *a, int *nz, int cap,
+                     const std::vector<int> &ib, const std::vector<int> &jb,
+                     const std::vector<double> &b, double beta, const char *what) {
+    int r0 = 0, r1 = n;
+    while (r0 < n && ib[r0] == ib[r0 + 1]) ++r0;
+    while (r1 > r0 && ib[r1 - 1] == ib[r1]) --r1;   // B empty: r0 = r1 = n
+    bool outside_clean = true;
+    for (int r = 0; r < r0 && outside_clean; ++r) outside_clean = clean_row(ia, ja, a, r);
+    for (int r = r1; r < n && outside_clean; ++r) outside_clean = clean_row(ia, ja, a, r);
+    if (!outside_clean) { r0 = 0; r1 = n; }
+    std::vector<int> ic, jc;
+    std::vector<double> c;
+    csr_add_rows(n, ia, ja, a, 1.0, ib, jb, b, beta, ic, jc, c, r0, r1);
+    const long long base = ia[r0] - 1, old_len = ia[r1] - ia[r0];
+    const long long delta = (long long)jc.size() - old_len;
+    if ((long long)*nz + delta > (long long)cap) {
+        xt::set_error("%s: result has %lld entries, storage holds %d", what,
+                      (long long)*nz + delta, cap);
+        return XT_E_ARG;
+    }
+    const long long tail = (long long)*nz - (ia[r1] - 1);   // entries of rows r1 ..
+    if (delta != 0 && tail > 0) {
+        std::memmove(ja + base + (long long)jc.size(), ja + ia[r1] - 1, sizeof(int) * tail);
+        std::memmove(a + base + (long long)jc.size(), a + ia[r1] - 1, sizeof(double) * tail);
+    }
+    std::memcpy(ja + base, jc.data(), sizeof(int) * jc.size());
+    std::memcpy(a + base, c.data(), sizeof(double) * c.size());
+    for (int r = r0 + 1; r <= r1; ++r) ia[r] = (int)base + ic[r - r0];
+    for (int r = r1 + 1; r <= n; ++r) ia[r] += (int)delta;
+    *nz = (int)(*nz + delta);
+    return XT_OK;
 }
 
 // Triplets -> per-row runs (0-based columns), duplicates summed in sequence
@@ -114,21 +155,6 @@ bool valid_csr(int n, const int *ia, const int *ja, int nz) {
     for (int k = 0; k < nz; ++k)
         if (ja[k] < 1 || ja[k] > n) return false;
     return true;
-}
-
-// Write C back over the caller's arrays if it fits in cap entries.
-int store(int n, const std::vector<int> &ic, const std::vector<int> &jc,
-          const std::vector<double> &c, int *ki, int *kj, double *kv, int cap, int *knz,
-          const char *what) {
-    if ((long long)jc.size() > (long long)cap) {
-        xt::set_error("%s: result has %zu entries, storage holds %d", what, jc.size(), cap);
-        return XT_E_ARG;
-    }
-    std::memcpy(ki, ic.data(), sizeof(int) * (n + 1));
-    std::memcpy(kj, jc.data(), sizeof(int) * jc.size());
-    std::memcpy(kv, c.data(), sizeof(double) * c.size());
-    *knz = (int)jc.size();
-    return XT_OK;
 }
 
 // 3x3 determinant and cofactor inverse with the term order of procs.f90:300-310,
@@ -259,13 +285,13 @@ int xt_delelement(int nn, int ne, int ndofn, int nnse, int ngpe, int ncmp, int n
         for (int r = 0; r < nk; ++r)
             for (int c = 0; c < nk; ++c) t.push_back({dofs[r], dofs[c], f, ke[r + (size_t)nk * c]});
     }
-    std::vector<int> ib, jb, ic, jc;
-    std::vector<double> b, c;
+    std::vector<int> ib, jb;
+    std::vector<double> b;
     triplets_to_rows(ndof, t, ib, jb, b);
-    csr_add_rows(ndof, ki, kj, kv, 1.0, ib, jb, b, -1.0, ic, jc, c);
     // kj/kv have room for the result by the caller's contract (at most
     // knz + nef*(3*nnse)^2 entries)
-    return store(ndof, ic, jc, c, ki, kj, kv, *knz + nef * nk * nk, knz, "delelement");
+    return csr_add_in_place(ndof, ki, kj, kv, knz, *knz + nef * nk * nk, ib, jb, b, -1.0,
+                            "delelement");
 }
 
 int xt_chkcnsta(int nn, int ne, int nnse, const int *xconn, int nef, const int *nefail,
@@ -325,13 +351,13 @@ int xt_adddiagonal(int nnse, int ndofn, int ndf, const int *ndfail, int *ki, int
             XT_ARG(d >= 0 && d < ndof, "adddiagonal: DOF out of range");
             t.push_back({d, d, (int)t.size(), 1.0});
         }
-    std::vector<int> ib, jb, ic, jc;
-    std::vector<double> b, c;
+    std::vector<int> ib, jb;
+    std::vector<double> b;
     triplets_to_rows(ndof, t, ib, jb, b);
-    csr_add_rows(ndof, ki, kj, kv, 1.0, ib, jb, b, 1.0, ic, jc, c);
     // storage: the caller's arrays hold knz + ndf*ndofn entries (the reference
     // grows knz by that much before its addition)
-    return store(ndof, ic, jc, c, ki, kj, kv, *knz + ndf * ndofn, knz, "adddiagonal");
+    return csr_add_in_place(ndof, ki, kj, kv, knz, *knz + ndf * ndofn, ib, jb, b, 1.0,
+                            "adddiagonal");
 }
 
 int xt_csrapplybc(int *ia, int *ja, double *a, int n, int *nz, const int *bc) {

@@ -96,7 +96,15 @@ class ElementDeletion:
             t0 = time.perf_counter()
             K1 = api.delelement(xconn, self.mesh.xcoord, self.dns, self.dnxyz, fem.SD,
                                 self.stiff, self.wqs, nefail, self.K)
-            K_ed = api.csrapplybc(api.adddiagonal(3, ndfail, K1), self.fixed)
+            K1 = api.adddiagonal(3, ndfail, K1)
+            # delelement changes only the couplings among the failed elements' DOFs; when
+            # none of them is fixed and the fixed set is unchanged, K1's fixed rows and
+            # columns are the current K's, already in 1-0 form, and K1 holds no exact
+            # zero: the 1-0 method would return K1 as it is
+            nodes = xconn[nefail - 1].ravel() - 1
+            dofs = (3 * nodes[:, None] + np.arange(3)).ravel()
+            touches = bool(self.fixed[dofs].any())
+            K_ed = api.csrapplybc(K1, self.fixed) if (new_fixed or touches) else K1
             M_ed = api.csrapplybc(self.M, self.fixed) if new_fixed else self.M
             t["edit"] = time.perf_counter() - t0
         t0 = time.perf_counter()

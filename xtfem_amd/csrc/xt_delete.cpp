@@ -15,8 +15,11 @@
 // B are scattered (x = alpha*a, then x += beta*b for a repeated column), the
 // row is emitted in increasing column order and exact zeros are dropped.  The
 // element matrices accumulate in the order of the failed-element list, so the
-// result is bit-identical to the reference's.  Host work is O(nnz(K)) per
-// deletion event; the refactorisation that follows runs on the GPU.
+// result is bit-identical to the reference's.  The merge covers only the rows
+// from the first to the last one a failed element touches (the others, already
+// sorted and zero-free, are the result's own: kept in place or moved by the change
+// in entry count); the checks and that move are the O(nnz(K)) part.  The
+// refactorisation that follows runs on the GPU.
 #include <algorithm>
 #include <cmath>
 #include <cstring>

@@ -551,6 +551,21 @@ def multimesh_meshes(nmesh=64, target_dof=6.0e6):
     return out, f
 
 
+def consecutive_groups(sizes, budget):
+    """Runs of consecutive indices whose sizes sum to at most budget each (an index whose
+    size alone exceeds it forms its own group), in order; every index in exactly one."""
+    groups, cur, acc = [], [], 0.0
+    for i, sz in enumerate(sizes):
+        if cur and acc + sz > budget:
+            groups.append(cur)
+            cur, acc = [], 0.0
+        cur.append(i)
+        acc += sz
+    if cur:
+        groups.append(cur)
+    return groups
+
+
 def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7, group_gb=48.0):
     """The uneven multi-mesh batch: every mesh's K and M stacked block-diagonally, a
     banded factor per run of meshes of similar bandwidth (FP32, xt_prec_create_band_seg),
@@ -640,14 +655,7 @@ def multimesh_leg(K=3, R=4, nmesh=64, target_dof=6.0e6, xtol=1e-7, group_gb=48.0
     del x32
     torch.cuda.empty_cache()
     # FP64 batched by groups of consecutive meshes within group_gb of factor blocks each
-    groups64, cur, acc = [], [], 0.0
-    for si, fb in enumerate(fbytes):
-        if cur and acc + fb > group_gb * 1e9:
-            groups64.append(cur)
-            cur, acc = [], 0.0
-        cur.append(si)
-        acc += fb
-    groups64.append(cur)
+    groups64 = consecutive_groups(fbytes, group_gb * 1e9)
     tg, itsg, relg, convg, errg = 0.0, [], [], True, []
     for gi in groups64:
         Kg, segg = device.block_diagonal([systems[i]["Kb"] for i in gi])
